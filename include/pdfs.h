@@ -1,0 +1,128 @@
+/*
+ * pdfs.h — C-ABI of the MI355X-native parallel Fast Sweeping Method.
+ *
+ * Drop-in for the solver surface of GEM3D/PDFS:
+ *   run_fsm  replaces single_gpu_cuda/src/phi3D_fsm.h:69-71 (definition
+ *            phi3D_fsm.cu:64-106), called from single_gpu_cuda/src/phi3D.c:119.
+ *   runFSM   replaces mpi_openacc/src/phi3D_fsm.h:73 (definition
+ *            phi3D_fsm.c:67), called from mpi_openacc/src/block_fsm.c:298,332.
+ * The pdfs_* entry points are new (status codes, options, statistics,
+ * device-resident fields, the FP64 judge, synthetic fields).
+ *
+ * Plain C types and pointers only. Link with -lpdfs (pdfs_amd/lib).
+ */
+#ifndef PDFS_H
+#define PDFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field struct of the single-GPU build, same layout (x86-64 SysV: x,y,z at
+ * 0,4,8; dx,dy,dz at 16,24,32; location 40; distance 48; F 56; size 64).
+ * single_gpu_cuda/src/phi3D.h:45-55. `distance` is the padded
+ * (x+2)(y+2)(z+2) FP64 array, x fastest, ghost layer +inf on entry. */
+#ifndef PDFS_NO_PHI_TYPEDEF
+typedef struct {
+  int x, y, z;
+  double dx, dy, dz;
+  int *location;
+  double *distance;
+  double F;
+} Phi;
+#endif
+
+/* Field struct of the MPI/OpenACC build (no `location`; size 56).
+ * mpi_openacc/src/phi3D.h:45-54. */
+typedef struct {
+  int x, y, z;
+  double dx, dy, dz;
+  double *distance;
+  double F;
+} PhiACC;
+
+/* Reference semantics: exactly `iterations` passes of orderings sn = 1..8 in
+ * the reference's order, FP64, reference arithmetic, in place; ghost cells
+ * never written. Extension: iterations <= 0 runs passes until no value
+ * changes by more than 1e-12. Errors print a message and exit(EXIT_FAILURE)
+ * as cudaCheckError does (single_gpu_cuda/src/phi3D_fsm.h:39-46). Prints
+ * "Parallel FSM time: %f s." (phi3D_fsm.cu:104). */
+void run_fsm(Phi *pf, int iterations);
+
+/* One ordering sn (1..8) of the reference, FP64, in place on the padded
+ * array of an OpenACC-layout field. */
+void runFSM(PhiACC *pf, int sn);
+
+/* ---- status-returning API ---- */
+enum { PDFS_OK = 0, PDFS_EINVAL = -1, PDFS_EHIP = -2, PDFS_ENOMEM = -3, PDFS_ETIMEOUT = -4,
+       PDFS_ENOTCONV = -5 };
+enum { PDFS_F64_REF = 0, /* FP64, reference arithmetic and schedule */
+       PDFS_F32 = 1 };   /* FP32 difference form, column pipeline */
+
+typedef struct {
+  int precision;   /* PDFS_F64_REF or PDFS_F32 */
+  int max_passes;  /* cap on 8-ordering passes (default 100) */
+  int fixed_passes;/* > 0: run exactly this many passes, no convergence test */
+  double tol;      /* FP64: stop when a pass changes nothing by more than tol */
+  int judge;       /* FP32: run the FP64 residual judge after convergence */
+  int tile;        /* column tile variant, -1 = autotuned default */
+  int quiet;       /* no stdout */
+  int profile;     /* time every sweep launch with events on the solver stream */
+} pdfs_opts;
+
+typedef struct {
+  int passes, orderings;
+  long long point_updates;   /* interior points x orderings executed */
+  long long last_changed;    /* points changed in the last pass */
+  double last_maxdiff;       /* FP64: max |delta| of the last pass */
+  long long judge_flagged;   /* FP64 judge: points failing the residual */
+  double ms_sweeps, ms_judge, ms_total;
+  double ms_sweep_kernels;   /* profile: sum of per-launch sweep kernel times */
+  int tile;
+} pdfs_stats;
+
+void pdfs_default_opts(pdfs_opts *o);
+
+/* Solve on a host padded FP64 array (reference layout), in place. */
+int pdfs_solve(double *distance, int x, int y, int z, double dx, double dy, double dz,
+               const pdfs_opts *o, pdfs_stats *st);
+
+/* ---- device-resident API (pointers are HIP device memory) ---- */
+typedef struct pdfs_ctx pdfs_ctx;
+pdfs_ctx *pdfs_ctx_create(void);
+void pdfs_ctx_destroy(pdfs_ctx *c);
+const char *pdfs_last_error(void);
+
+/* Dense FP32 field [z][y][x] (interior only, ghosts implicit +inf), in place. */
+int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init_or_null, int x, int y,
+                       int z, double dx, double dy, double dz, const pdfs_opts *o,
+                       pdfs_stats *st);
+/* One ordering over a dense FP32 field; *changed += points changed. */
+int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx, double dy,
+                       double dz, int sn, int tile, long long *changed);
+/* Padded FP64 field, reference arithmetic, `passes` passes (<= 0: until
+ * max |delta| <= tol), in place. */
+int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx,
+                         double dy, double dz, int passes, double tol, pdfs_stats *st);
+/* FP64 residual judge; flags: [ceil(x*y*z/32)] words or NULL. */
+int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
+                       double dx, double dy, double dz, uint32_t *flags,
+                       long long *flagged);
+
+/* Synthetic initial fields; dense FP32 and/or padded FP64 outputs (NULL to
+ * skip). kind: 0 point source, 1 terrain, 2 spheres. */
+enum { PDFS_GEN_POINT = 0, PDFS_GEN_TERRAIN = 1, PDFS_GEN_SPHERES = 2 };
+int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
+                      float *dense, double *padded);
+
+/* Layout conversion between the padded FP64 and dense FP32 device arrays. */
+int pdfs_pack_dev(pdfs_ctx *c, const double *padded, float *dense, int x, int y, int z);
+int pdfs_unpack_dev(pdfs_ctx *c, const float *dense, double *padded, int x, int y, int z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

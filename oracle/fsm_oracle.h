@@ -1,0 +1,95 @@
+/*
+ * CPU oracle for the parallel Fast Sweeping Method (FSM) of GEM3D/PDFS.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
+ * product (pdfs_amd/) never links, loads or calls it.
+ *
+ * Layout everywhere: padded (X+2)(Y+2)(Z+2) arrays, x fastest, then y, then z,
+ * ghost layer = +inf during the solve (single_gpu_cuda/src/vti_parser.c:137-180,
+ * single_gpu_cuda/src/phi3D_fsm.cu:209-225, mpi_openacc/src/phi3D_fsm.c:112).
+ *
+ * Pinning of each function (what checks it besides its own restatement):
+ *   orc_godunov_ref      golden vectors from the compiled reference runFSM
+ *                        (tests/golden), closed forms, invariants.
+ *   orc_sweep_ref64      compiled reference runFSM (oracle/_ref), bitwise;
+ *                        golden vectors; lexicographic Gauss-Seidel (K4).
+ *   orc_sweep_lex        bitwise equal to orc_sweep_ref64 (schedule
+ *                        independence), brute force on tiny grids.
+ *   orc_godunov_f32/_d64 closed forms (1-D m+h, 2-D m+h/sqrt2, 3-D m+h/sqrt3),
+ *                        agreement with orc_godunov_ref to fp tolerance,
+ *                        shift invariance, monotonicity.
+ *   orc_solve_*          point-source axis exactness (n*h), fixed-point
+ *                        residual 0, agreement with the compiled reference.
+ *   orc_update_distance, orc_set_negative_inside, orc_adjust_boundary
+ *                        known-answer tests written from the reference's
+ *                        branch structure (0->90000, NaN->90000, ghosts).
+ *   generators           closed-form checks (sphere surface distances,
+ *                        terrain column heights) and determinism.
+ *   orc_judge_f32        planted defects (values below truth) are flagged;
+ *                        a converged field passes.
+ */
+#ifndef PDFS_FSM_ORACLE_H
+#define PDFS_FSM_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Godunov local solves. */
+double orc_godunov_ref(double cur, double mx, double my, double mz,
+                       double dx, double dy, double dz);
+double orc_godunov_d64(double cur, double mx, double my, double mz,
+                       double dx, double dy, double dz);
+float orc_godunov_f32(float cur, float mx, float my, float mz,
+                      double dx, double dy, double dz);
+
+/* One ordering sn (1..8) over the padded field, in place. */
+void orc_sweep_ref64(double *d, int X, int Y, int Z, double dx, double dy,
+                     double dz, int sn);
+void orc_sweep_lex64(double *d, int X, int Y, int Z, double dx, double dy,
+                     double dz, int sn);
+void orc_sweep_f32(float *d, int X, int Y, int Z, double dx, double dy,
+                   double dz, int sn);
+void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
+                   double dz, int sn);
+
+/* Passes of sn = 1..8 until converged (or max_passes). Return passes run. */
+int orc_solve_ref64(double *d, int X, int Y, int Z, double dx, double dy,
+                    double dz, double tol, int max_passes, double *last_maxdiff);
+int orc_solve_f32(float *d, int X, int Y, int Z, double dx, double dy,
+                  double dz, int max_passes, long *last_changed);
+int orc_solve_d64(double *d, int X, int Y, int Z, double dx, double dy,
+                  double dz, double tol, int max_passes);
+
+/* Pre/post processing of the CLI path. */
+void orc_update_distance(const int *loc, double *d, size_t n);
+void orc_set_negative_inside(const int *loc, double *d, size_t n);
+void orc_adjust_boundary(double *d, int X, int Y, int Z);
+void orc_fill_ghost(double *d, int X, int Y, int Z, double v);
+
+/* Synthetic initial fields (padded, ghost = +inf). */
+void orc_gen_point(double *d, int X, int Y, int Z, uint64_t seed);
+void orc_gen_terrain(double *d, int X, int Y, int Z, double h, uint64_t seed,
+                     int nhills, double amp_lo, double amp_hi, double sig_lo,
+                     double sig_hi);
+void orc_gen_spheres(double *d, int X, int Y, int Z, double h, uint64_t seed,
+                     int nspheres, double r_lo, double r_hi);
+
+/* FP64 residual judge of an FP32 field: flags[p] = 1 where
+ * |v - min(init, G64(v))| > tol. Returns the number of flagged points. */
+long orc_judge_f32(const float *v, const float *init, int X, int Y, int Z,
+                   double dx, double dy, double dz, uint8_t *flags);
+/* max |v - min(init, G(v))| over the interior, FP64 difference form. */
+double orc_residual_f64(const double *v, const double *init, int X, int Y,
+                        int Z, double dx, double dy, double dz);
+
+uint64_t orc_splitmix64(uint64_t *state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,254 @@
+"""Python host-side mirror of the MI355X-native FSM C-ABI (include/pdfs.h).
+
+Loads pdfs_amd/lib/libpdfs.so (built by pdfs_amd/Makefile; run
+`__graft_entry__.build()`); there is no fallback: if the library is missing
+every entry point raises. Device arrays are torch tensors (HIP memory);
+their data_ptr() is handed to the C-ABI.
+
+Reference interface mirrored:
+  run_fsm(Phi*, int)  single_gpu_cuda/src/phi3D_fsm.h:69-71
+  runFSM(Phi*, int)   mpi_openacc/src/phi3D_fsm.h:73
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpdfs.so")
+
+PDFS_OK, PDFS_EINVAL, PDFS_EHIP, PDFS_ENOMEM, PDFS_ETIMEOUT, PDFS_ENOTCONV = 0, -1, -2, -3, -4, -5
+PDFS_F64_REF, PDFS_F32 = 0, 1
+GEN_POINT, GEN_TERRAIN, GEN_SPHERES = 0, 1, 2
+TILE_16x16, TILE_32x8, TILE_16x8 = 0, 1, 2
+
+EXPORTED = [
+    "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
+    "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
+    "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_pack_dev",
+    "pdfs_unpack_dev",
+]
+
+
+class Phi(C.Structure):
+    """single_gpu_cuda/src/phi3D.h:45-55 layout (64 bytes)."""
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("location", C.POINTER(C.c_int)), ("distance", C.POINTER(C.c_double)),
+                ("F", C.c_double)]
+
+
+class PhiACC(C.Structure):
+    """mpi_openacc/src/phi3D.h:45-54 layout (56 bytes)."""
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("distance", C.POINTER(C.c_double)), ("F", C.c_double)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_passes", C.c_int), ("fixed_passes", C.c_int),
+                ("tol", C.c_double), ("judge", C.c_int), ("tile", C.c_int),
+                ("quiet", C.c_int), ("profile", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("passes", C.c_int), ("orderings", C.c_int), ("point_updates", C.c_longlong),
+                ("last_changed", C.c_longlong), ("last_maxdiff", C.c_double),
+                ("judge_flagged", C.c_longlong), ("ms_sweeps", C.c_double),
+                ("ms_judge", C.c_double), ("ms_total", C.c_double),
+                ("ms_sweep_kernels", C.c_double), ("tile", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PdfsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdfs error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C-ABI library. Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: the HIP extension must be built "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        vp, I, D, LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+        L.run_fsm.argtypes = [C.POINTER(Phi), I]
+        L.run_fsm.restype = None
+        L.runFSM.argtypes = [C.POINTER(PhiACC), I]
+        L.runFSM.restype = None
+        L.pdfs_default_opts.argtypes = [C.POINTER(Opts)]
+        L.pdfs_solve.argtypes = [vp, I, I, I, D, D, D, C.POINTER(Opts), C.POINTER(Stats)]
+        L.pdfs_solve.restype = I
+        L.pdfs_ctx_create.argtypes = []
+        L.pdfs_ctx_create.restype = vp
+        L.pdfs_ctx_destroy.argtypes = [vp]
+        L.pdfs_last_error.restype = C.c_char_p
+        L.pdfs_solve_f32_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, C.POINTER(Opts),
+                                         C.POINTER(Stats)]
+        L.pdfs_solve_f32_dev.restype = I
+        L.pdfs_sweep_f32_dev.argtypes = [vp, vp, I, I, I, D, D, D, I, I, C.POINTER(LL)]
+        L.pdfs_sweep_f32_dev.restype = I
+        L.pdfs_solve_ref64_dev.argtypes = [vp, vp, I, I, I, D, D, D, I, D, C.POINTER(Stats)]
+        L.pdfs_solve_ref64_dev.restype = I
+        L.pdfs_judge_f32_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, vp, C.POINTER(LL)]
+        L.pdfs_judge_f32_dev.restype = I
+        L.pdfs_generate_dev.argtypes = [vp, I, C.c_uint64, I, I, I, D, vp, vp]
+        L.pdfs_generate_dev.restype = I
+        L.pdfs_pack_dev.argtypes = [vp, vp, vp, I, I, I]
+        L.pdfs_pack_dev.restype = I
+        L.pdfs_unpack_dev.argtypes = [vp, vp, vp, I, I, I]
+        L.pdfs_unpack_dev.restype = I
+        _lib = L
+    return _lib
+
+
+def _check(rc, ok=(PDFS_OK,)):
+    if rc not in ok:
+        raise PdfsError(rc, lib().pdfs_last_error().decode(errors="replace"))
+    return rc
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().pdfs_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _dims(a):
+    z, y, x = a.shape
+    return x - 2, y - 2, z - 2
+
+
+def run_fsm(distance, iterations=1, h=(1.0, 1.0, 1.0)):
+    """Reference solver entry on a padded FP64 numpy array (in place)."""
+    assert distance.dtype == np.float64 and distance.flags.c_contiguous
+    X, Y, Z = _dims(distance)
+    pf = Phi(X, Y, Z, h[0], h[1], h[2], None,
+             distance.ctypes.data_as(C.POINTER(C.c_double)), 1.0)
+    lib().run_fsm(C.byref(pf), iterations)
+    return distance
+
+
+def runFSM(distance, sn, h=(1.0, 1.0, 1.0)):
+    """One reference ordering sn (1..8), OpenACC-build entry (in place)."""
+    assert distance.dtype == np.float64 and distance.flags.c_contiguous
+    X, Y, Z = _dims(distance)
+    pf = PhiACC(X, Y, Z, h[0], h[1], h[2], distance.ctypes.data_as(C.POINTER(C.c_double)), 1.0)
+    lib().runFSM(C.byref(pf), sn)
+    return distance
+
+
+def solve(distance, h=(1.0, 1.0, 1.0), precision=PDFS_F32, allow_notconv=False, **kw):
+    """pdfs_solve on a host padded FP64 array (in place). Returns stats dict."""
+    assert distance.dtype == np.float64 and distance.flags.c_contiguous
+    X, Y, Z = _dims(distance)
+    o = default_opts(precision=precision, **kw)
+    st = Stats()
+    rc = lib().pdfs_solve(distance.ctypes.data, X, Y, Z, h[0], h[1], h[2], C.byref(o),
+                          C.byref(st))
+    _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+    return st.as_dict()
+
+
+class Context:
+    """Owns a pdfs_ctx (device scratch, stream). Device arrays are torch tensors."""
+
+    def __init__(self):
+        self.c = lib().pdfs_ctx_create()
+        if not self.c:
+            raise PdfsError(PDFS_EHIP, lib().pdfs_last_error().decode())
+
+    def close(self):
+        if self.c:
+            lib().pdfs_ctx_destroy(self.c)
+            self.c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ptr(t):
+        if t is None:
+            return None
+        assert t.is_cuda and t.is_contiguous()
+        return t.data_ptr()
+
+    def solve_f32(self, field, init=None, h=(1.0, 1.0, 1.0), allow_notconv=False, **kw):
+        """field: dense float32 cuda tensor [Z, Y, X], solved in place."""
+        import torch
+        assert field.dtype == torch.float32 and field.dim() == 3
+        Z, Y, X = field.shape
+        o = default_opts(precision=PDFS_F32, **kw)
+        st = Stats()
+        torch.cuda.synchronize()
+        rc = lib().pdfs_solve_f32_dev(self.c, self._ptr(field), self._ptr(init), X, Y, Z, *h,
+                                      C.byref(o), C.byref(st))
+        _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+        return st.as_dict()
+
+    def sweep_f32(self, field, sn, h=(1.0, 1.0, 1.0), tile=-1):
+        import torch
+        Z, Y, X = field.shape
+        ch = C.c_longlong(0)
+        torch.cuda.synchronize()
+        _check(lib().pdfs_sweep_f32_dev(self.c, self._ptr(field), X, Y, Z, *h, sn, tile,
+                                        C.byref(ch)))
+        return ch.value
+
+    def solve_ref64(self, padded, passes=0, tol=1e-12, h=(1.0, 1.0, 1.0)):
+        import torch
+        assert padded.dtype == torch.float64 and padded.dim() == 3
+        Z, Y, X = (s - 2 for s in padded.shape)
+        st = Stats()
+        torch.cuda.synchronize()
+        _check(lib().pdfs_solve_ref64_dev(self.c, self._ptr(padded), X, Y, Z, *h, passes, tol,
+                                          C.byref(st)))
+        return st.as_dict()
+
+    def judge_f32(self, field, init, h=(1.0, 1.0, 1.0), flags=None):
+        import torch
+        Z, Y, X = field.shape
+        n = C.c_longlong(0)
+        torch.cuda.synchronize()
+        _check(lib().pdfs_judge_f32_dev(self.c, self._ptr(field), self._ptr(init), X, Y, Z, *h,
+                                        self._ptr(flags), C.byref(n)))
+        return n.value
+
+    def generate(self, kind, seed, X, Y, Z, h=1.0, dense=None, padded=None):
+        import torch
+        torch.cuda.synchronize()
+        _check(lib().pdfs_generate_dev(self.c, kind, seed, X, Y, Z, h, self._ptr(dense),
+                                       self._ptr(padded)))
+
+    def pack(self, padded, dense):
+        import torch
+        Z, Y, X = dense.shape
+        torch.cuda.synchronize()
+        _check(lib().pdfs_pack_dev(self.c, self._ptr(padded), self._ptr(dense), X, Y, Z))
+        self._sync()
+
+    def unpack(self, dense, padded):
+        import torch
+        Z, Y, X = dense.shape
+        torch.cuda.synchronize()
+        _check(lib().pdfs_unpack_dev(self.c, self._ptr(dense), self._ptr(padded), X, Y, Z))
+        self._sync()
+
+    def _sync(self):
+        # device-wide sync: drains the library's own stream as well
+        import torch
+        torch.cuda.synchronize()

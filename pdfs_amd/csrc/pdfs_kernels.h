@@ -1,0 +1,85 @@
+/*
+ * Thin C-ABI layer over the gfx950 HIP kernels (pdfs_kernels.hip).
+ * Called from the host C solver (pdfs_solver.c). Device pointers only.
+ *
+ * Internal FP32 layout ("dense"): interior only, [Z][Y][X], x fastest, the
+ * ghost layer implicit +inf. The FP64 reference-schedule path keeps the
+ * reference's padded (X+2)(Y+2)(Z+2) layout with explicit ghosts.
+ */
+#ifndef PDFS_KERNELS_H
+#define PDFS_KERNELS_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP32 spacing constants, computed on the host in IEEE single precision in
+ * the same operation order as the oracle's difference-form solve. */
+typedef struct {
+  float h, two_h2, three_h2, third; /* isotropic */
+  float d[3], w[3], W;               /* anisotropic: spacing, 1/d^2, sum */
+  int iso;
+} pdfs_sp32;
+
+/* Per-field scratch for the column-pipeline sweep. */
+typedef struct {
+  uint64_t *edge_x;    /* [ncols][Z][TY] {value, epoch} east-edge values */
+  uint64_t *edge_y;    /* [ncols][Z][TX] {value, epoch} north-edge values */
+  int *order;          /* [ncols][2] ticket -> column (a, b) */
+  unsigned *ticket;    /* monotonic work counter */
+  unsigned *changed;   /* changed-point counter (accumulates) */
+  unsigned *err;       /* bounded-spin timeout flag */
+} pdfsk_col_scratch;
+
+/* Tile shapes of the column pipeline (autotuner chooses one). */
+enum { PDFSK_TILE_16x16 = 0, PDFSK_TILE_32x8 = 1, PDFSK_TILE_16x8 = 2, PDFSK_TILE_COUNT = 3 };
+void pdfsk_tile_dims(int variant, int *tx, int *ty);
+
+/* Resident workgroups of the column kernel for a tile variant (0 on error). */
+int pdfsk_col_max_resident(int variant, int iso);
+
+/* One ordering sn (1..8) over the dense FP32 field, in place. Tickets
+ * [ticket_base, ticket_base + ncols + grid) are consumed. */
+int pdfsk_sweep_col_f32(float *field, int X, int Y, int Z, int sn,
+                        const pdfs_sp32 *sp, const pdfsk_col_scratch *s,
+                        int variant, uint32_t epoch, unsigned ticket_base,
+                        int grid, hipStream_t st);
+
+/* One ordering sn of the reference hyperplane schedule, FP64, on the padded
+ * array, one launch per level (reference arithmetic, bitwise parity).
+ * If maxdiff != NULL, atomically maxes |old - new| (as uint64 bits). */
+int pdfsk_sweep_ref64(double *d, int X, int Y, int Z, int sn, double dx,
+                      double dy, double dz, unsigned long long *maxdiff,
+                      hipStream_t st);
+
+/* FP64 residual judge of a dense FP32 field against its init. Sets bits of
+ * flags ([ceil(N/32)] words, must be zeroed) and adds to *count. */
+int pdfsk_judge_f32(const float *v, const float *init, int X, int Y, int Z,
+                    double dx, double dy, double dz, uint32_t *flags,
+                    unsigned long long *count, hipStream_t st);
+
+/* Layout conversions: padded FP64 <-> dense FP32 and dense copies. */
+int pdfsk_pack_f64(const double *padded, float *dense, int X, int Y, int Z,
+                   hipStream_t st);
+int pdfsk_unpack_f64(const float *dense, double *padded, int X, int Y, int Z,
+                     hipStream_t st);
+int pdfsk_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+
+/* Synthetic initial fields, dense FP32 and/or padded FP64 (either may be
+ * NULL). params: see pdfs_gen.c for the packing. */
+int pdfsk_gen_point(float *dense, double *padded, int X, int Y, int Z, int k,
+                    int j, int i, hipStream_t st);
+int pdfsk_gen_terrain(float *dense, double *padded, int X, int Y, int Z,
+                      double h, const double *dev_params, int nhills,
+                      hipStream_t st);
+int pdfsk_gen_spheres(float *dense, double *padded, int X, int Y, int Z,
+                      double h, const double *dev_params, int nspheres,
+                      hipStream_t st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,557 @@
+/*
+ * Host C orchestration of the MI355X-native FSM: the public C-ABI of
+ * include/pdfs.h. Calls the gfx950 kernels through the thin C layer of
+ * pdfs_kernels.h; owns device scratch, epochs, tickets and convergence.
+ */
+#include "pdfs.h"
+#include "pdfs_kernels.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct pdfs_ctx {
+  int dev;
+  hipStream_t st;
+  pdfsk_col_scratch col;
+  size_t cap_ex, cap_ey, cap_order; /* elements */
+  uint32_t epoch;
+  unsigned ticket_base;
+  int resident[PDFSK_TILE_COUNT][2];
+  unsigned long long *d_u64; /* [0] maxdiff bits, [1] judge count */
+  unsigned *h_pin;           /* pinned readback: changed, err */
+  unsigned long long *h_pin64;
+  uint32_t *flags;
+  size_t cap_flags;
+  double *d_params;
+  hipEvent_t ev0, ev1;
+  hipEvent_t kev[16];
+};
+
+static __thread char g_err[512];
+
+const char *pdfs_last_error(void) { return g_err; }
+
+static int fail(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(PDFS_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,              \
+                  hipGetErrorString(e_));                                            \
+  } while (0)
+
+#define KCHK(call)                                                                   \
+  do {                                                                               \
+    int r_ = (call);                                                                 \
+    if (r_ != 0)                                                                     \
+      return fail(PDFS_EHIP, "%s:%d %s: launch failed (%d: %s)", __FILE__, __LINE__, \
+                  #call, r_, hipGetErrorString((hipError_t)r_));                     \
+  } while (0)
+
+void pdfs_default_opts(pdfs_opts *o)
+{
+  memset(o, 0, sizeof *o);
+  o->precision = PDFS_F32;
+  o->max_passes = 100;
+  o->tol = 1e-12;
+  o->judge = 1;
+  o->tile = -1;
+}
+
+static int create_events(hipEvent_t *ev, int n)
+{
+  for (int i = 0; i < n; i++)
+    if (hipEventCreate(&ev[i]) != hipSuccess) return -1;
+  return 0;
+}
+
+pdfs_ctx *pdfs_ctx_create(void)
+{
+  pdfs_ctx *c = (pdfs_ctx *)calloc(1, sizeof *c);
+  if (!c) return NULL;
+  if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+      hipMalloc((void **)&c->col.ticket, 4 * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc((void **)&c->d_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_pin, 16 * sizeof(unsigned), 0) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_pin64, 4 * sizeof(unsigned long long), 0) != hipSuccess ||
+      hipMalloc((void **)&c->d_params, 1024 * sizeof(double)) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      create_events(c->kev, 16) != 0) {
+    fail(PDFS_EHIP, "pdfs_ctx_create: HIP initialisation failed");
+    free(c);
+    return NULL;
+  }
+  c->col.changed = c->col.ticket + 1;
+  c->col.err = c->col.ticket + 2;
+  if (hipMemset(c->col.ticket, 0, 4 * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(c->d_u64, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    free(c);
+    return NULL;
+  }
+  c->epoch = 0;
+  c->ticket_base = 0;
+  for (int v = 0; v < PDFSK_TILE_COUNT; v++)
+    for (int iso = 0; iso < 2; iso++) c->resident[v][iso] = pdfsk_col_max_resident(v, iso);
+  return c;
+}
+
+void pdfs_ctx_destroy(pdfs_ctx *c)
+{
+  if (!c) return;
+  hipStreamSynchronize(c->st);
+  hipFree(c->col.edge_x);
+  hipFree(c->col.edge_y);
+  hipFree(c->col.order);
+  hipFree(c->col.ticket);
+  hipFree(c->d_u64);
+  hipFree(c->flags);
+  hipFree(c->d_params);
+  hipHostFree(c->h_pin);
+  hipHostFree(c->h_pin64);
+  hipEventDestroy(c->ev0);
+  hipEventDestroy(c->ev1);
+  for (int i = 0; i < 16; i++) hipEventDestroy(c->kev[i]);
+  hipStreamDestroy(c->st);
+  free(c);
+}
+
+/* FP32 spacing constants in the oracle's operation order. */
+static void make_sp32(double dx, double dy, double dz, pdfs_sp32 *s)
+{
+  memset(s, 0, sizeof *s);
+  s->iso = (dx == dy && dy == dz);
+  float h = (float)dx;
+  float h2 = h * h;
+  s->h = h;
+  s->two_h2 = 2.0f * h2;
+  s->three_h2 = 3.0f * h2;
+  s->third = 1.0f / 3.0f;
+  s->d[0] = (float)dx;
+  s->d[1] = (float)dy;
+  s->d[2] = (float)dz;
+  for (int i = 0; i < 3; i++) s->w[i] = 1.0f / (s->d[i] * s->d[i]);
+  s->W = s->w[0] + s->w[1] + s->w[2];
+}
+
+static int default_tile(int x, int y)
+{
+  (void)x;
+  (void)y;
+  return PDFSK_TILE_16x16;
+}
+
+/* Grow the column-pipeline scratch for (x, y, z, tile) and rebuild the
+ * upwind-first ticket order (anti-diagonals a + b ascending). */
+static int ensure_col(pdfs_ctx *c, int x, int y, int z, int tile, int *ncols_out)
+{
+  int TX, TY;
+  pdfsk_tile_dims(tile, &TX, &TY);
+  int nTx = (x + TX - 1) / TX, nTy = (y + TY - 1) / TY;
+  size_t ncols = (size_t)nTx * nTy;
+  size_t nex = ncols * (size_t)z * TY, ney = ncols * (size_t)z * TX;
+  if (nex > c->cap_ex) {
+    hipFree(c->col.edge_x);
+    c->col.edge_x = NULL;
+    HIPCHK(hipMalloc((void **)&c->col.edge_x, nex * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->col.edge_x, 0, nex * sizeof(uint64_t), c->st));
+    c->cap_ex = nex;
+  }
+  if (ney > c->cap_ey) {
+    hipFree(c->col.edge_y);
+    c->col.edge_y = NULL;
+    HIPCHK(hipMalloc((void **)&c->col.edge_y, ney * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->col.edge_y, 0, ney * sizeof(uint64_t), c->st));
+    c->cap_ey = ney;
+  }
+  if (2 * ncols > c->cap_order) {
+    hipFree(c->col.order);
+    c->col.order = NULL;
+    HIPCHK(hipMalloc((void **)&c->col.order, 2 * ncols * sizeof(int)));
+    c->cap_order = 2 * ncols;
+  }
+  int *h = (int *)malloc(2 * ncols * sizeof(int));
+  if (!h) return fail(PDFS_ENOMEM, "ensure_col: host alloc");
+  size_t t = 0;
+  for (int d = 0; d <= nTx + nTy - 2; d++)
+    for (int a = 0; a < nTx; a++) {
+      int b = d - a;
+      if (b < 0 || b >= nTy) continue;
+      h[2 * t] = a;
+      h[2 * t + 1] = b;
+      t++;
+    }
+  hipError_t e = hipMemcpyAsync(c->col.order, h, 2 * ncols * sizeof(int), hipMemcpyHostToDevice,
+                                c->st);
+  hipError_t e2 = hipStreamSynchronize(c->st);
+  free(h);
+  if (e != hipSuccess || e2 != hipSuccess) return fail(PDFS_EHIP, "ensure_col: order upload");
+  *ncols_out = (int)ncols;
+  return PDFS_OK;
+}
+
+static int sweep_one(pdfs_ctx *c, float *field, int x, int y, int z, const pdfs_sp32 *sp, int sn,
+                     int tile, int ncols)
+{
+  int grid = c->resident[tile][sp->iso];
+  if (grid <= 0) return fail(PDFS_EHIP, "column kernel cannot be resident");
+  if (grid > ncols) grid = ncols;
+  c->epoch++;
+  KCHK(pdfsk_sweep_col_f32(field, x, y, z, sn, sp, &c->col, tile, c->epoch, c->ticket_base, grid,
+                           c->st));
+  c->ticket_base += (unsigned)ncols + (unsigned)grid;
+  return PDFS_OK;
+}
+
+/* Read (changed, err) counters; resets err. */
+static int read_counters(pdfs_ctx *c, unsigned *changed, unsigned *err)
+{
+  HIPCHK(hipMemcpyAsync(c->h_pin, c->col.changed, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                        c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  *changed = c->h_pin[0];
+  *err = c->h_pin[1];
+  return PDFS_OK;
+}
+
+int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx, double dy,
+                       double dz, int sn, int tile, long long *changed)
+{
+  if (!c || !field || x < 1 || y < 1 || z < 1 || sn < 1 || sn > 8)
+    return fail(PDFS_EINVAL, "pdfs_sweep_f32_dev: bad arguments");
+  if (tile < 0 || tile >= PDFSK_TILE_COUNT) tile = default_tile(x, y);
+  pdfs_sp32 sp;
+  make_sp32(dx, dy, dz, &sp);
+  int ncols, rc;
+  if ((rc = ensure_col(c, x, y, z, tile, &ncols))) return rc;
+  unsigned ch0, ch1, err;
+  if ((rc = read_counters(c, &ch0, &err))) return rc;
+  if ((rc = sweep_one(c, field, x, y, z, &sp, sn, tile, ncols))) return rc;
+  if ((rc = read_counters(c, &ch1, &err))) return rc;
+  if (err) {
+    hipMemsetAsync(c->col.err, 0, sizeof(unsigned), c->st);
+    return fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+  }
+  if (changed) *changed += (long long)(unsigned)(ch1 - ch0);
+  return PDFS_OK;
+}
+
+int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
+                       double dx, double dy, double dz, uint32_t *flags, long long *flagged)
+{
+  if (!c || !field || !init) return fail(PDFS_EINVAL, "pdfs_judge_f32_dev: bad arguments");
+  size_t n = (size_t)x * y * z, words = (n + 31) / 32;
+  uint32_t *fl = flags;
+  if (!fl) {
+    if (words > c->cap_flags) {
+      hipFree(c->flags);
+      c->flags = NULL;
+      HIPCHK(hipMalloc((void **)&c->flags, words * sizeof(uint32_t)));
+      c->cap_flags = words;
+    }
+    fl = c->flags;
+  }
+  KCHK(pdfsk_fill_u32(fl, 0u, words, c->st));
+  KCHK(pdfsk_fill_u32((uint32_t *)(c->d_u64 + 1), 0u, 2, c->st));
+  KCHK(pdfsk_judge_f32(field, init, x, y, z, dx, dy, dz, fl, c->d_u64 + 1, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_pin64, c->d_u64 + 1, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (flagged) *flagged = (long long)c->h_pin64[0];
+  return PDFS_OK;
+}
+
+int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
+                       double dx, double dy, double dz, const pdfs_opts *o, pdfs_stats *st)
+{
+  if (!c || !field || x < 1 || y < 1 || z < 1 || !(dx > 0) || !(dy > 0) || !(dz > 0))
+    return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
+  pdfs_opts dflt;
+  if (!o) {
+    pdfs_default_opts(&dflt);
+    o = &dflt;
+  }
+  pdfs_stats s;
+  memset(&s, 0, sizeof s);
+  int tile = (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) ? o->tile : default_tile(x, y);
+  s.tile = tile;
+  pdfs_sp32 sp;
+  make_sp32(dx, dy, dz, &sp);
+  int ncols, rc;
+  if ((rc = ensure_col(c, x, y, z, tile, &ncols))) return rc;
+  unsigned prev, err, cur;
+  if ((rc = read_counters(c, &prev, &err))) return rc;
+  long long npts = (long long)x * y * z;
+  int maxp = o->fixed_passes > 0 ? o->fixed_passes : (o->max_passes > 0 ? o->max_passes : 100);
+  float ms = 0.f;
+  for (int p = 0; p < maxp; p++) {
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    for (int sn = 1; sn <= 8; sn++) {
+      if (o->profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1)], c->st));
+      if ((rc = sweep_one(c, field, x, y, z, &sp, sn, tile, ncols))) return rc;
+      if (o->profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1) + 1], c->st));
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    if ((rc = read_counters(c, &cur, &err))) return rc;
+    float t = 0.f;
+    hipEventElapsedTime(&t, c->ev0, c->ev1);
+    ms += t;
+    if (o->profile)
+      for (int q = 0; q < 8; q++) {
+        float tk = 0.f;
+        hipEventElapsedTime(&tk, c->kev[2 * q], c->kev[2 * q + 1]);
+        s.ms_sweep_kernels += tk;
+      }
+    s.passes++;
+    s.orderings += 8;
+    s.point_updates += 8 * npts;
+    s.last_changed = (long long)(unsigned)(cur - prev);
+    prev = cur;
+    if (err) {
+      hipMemsetAsync(c->col.err, 0, sizeof(unsigned), c->st);
+      if (st) *st = s;
+      return fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+    }
+    if (o->fixed_passes <= 0 && s.last_changed == 0) break;
+  }
+  s.ms_sweeps = ms;
+  if (o->judge && init) {
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    if ((rc = pdfs_judge_f32_dev(c, field, init, x, y, z, dx, dy, dz, NULL, &s.judge_flagged)))
+      return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    hipEventElapsedTime(&t, c->ev0, c->ev1);
+    s.ms_judge = t;
+  }
+  s.ms_total = s.ms_sweeps + s.ms_judge;
+  if (st) *st = s;
+  if (o->fixed_passes <= 0 && s.last_changed != 0)
+    return fail(PDFS_ENOTCONV, "not converged after %d passes", s.passes);
+  return PDFS_OK;
+}
+
+int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx, double dy,
+                         double dz, int passes, double tol, pdfs_stats *st)
+{
+  if (!c || !padded || x < 1 || y < 1 || z < 1)
+    return fail(PDFS_EINVAL, "pdfs_solve_ref64_dev: bad arguments");
+  pdfs_stats s;
+  memset(&s, 0, sizeof s);
+  int maxp = passes > 0 ? passes : 1000;
+  long long npts = (long long)x * y * z;
+  float ms = 0.f;
+  for (int p = 0; p < maxp; p++) {
+    HIPCHK(hipMemsetAsync(c->d_u64, 0, sizeof(unsigned long long), c->st));
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    for (int sn = 1; sn <= 8; sn++)
+      KCHK(pdfsk_sweep_ref64(padded, x, y, z, sn, dx, dy, dz, c->d_u64, c->st));
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipMemcpyAsync(c->h_pin64, c->d_u64, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    float t = 0.f;
+    hipEventElapsedTime(&t, c->ev0, c->ev1);
+    ms += t;
+    double md;
+    unsigned long long b = c->h_pin64[0];
+    memcpy(&md, &b, sizeof md);
+    s.passes++;
+    s.orderings += 8;
+    s.point_updates += 8 * npts;
+    s.last_maxdiff = md;
+    if (passes <= 0 && md <= tol) break;
+  }
+  s.ms_sweeps = ms;
+  s.ms_total = ms;
+  if (st) *st = s;
+  return PDFS_OK;
+}
+
+/* ---- host padded FP64 entry points ---- */
+static pdfs_ctx *g_ctx;
+
+static pdfs_ctx *default_ctx(void)
+{
+  if (!g_ctx) g_ctx = pdfs_ctx_create();
+  return g_ctx;
+}
+
+int pdfs_solve(double *distance, int x, int y, int z, double dx, double dy, double dz,
+               const pdfs_opts *o, pdfs_stats *st)
+{
+  if (!distance || x < 1 || y < 1 || z < 1) return fail(PDFS_EINVAL, "pdfs_solve: bad arguments");
+  pdfs_opts dflt;
+  if (!o) {
+    pdfs_default_opts(&dflt);
+    o = &dflt;
+  }
+  pdfs_ctx *c = default_ctx();
+  if (!c) return fail(PDFS_EHIP, "pdfs_solve: no HIP device");
+  size_t np = (size_t)(x + 2) * (y + 2) * (z + 2), n = (size_t)x * y * z;
+  double *d = NULL;
+  float *f = NULL, *f0 = NULL;
+  int rc = PDFS_OK;
+  HIPCHK(hipMalloc((void **)&d, np * sizeof(double)));
+  if (hipMemcpy(d, distance, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail(PDFS_EHIP, "pdfs_solve: H2D copy");
+    goto out;
+  }
+  if (o->precision == PDFS_F64_REF) {
+    rc = pdfs_solve_ref64_dev(c, d, x, y, z, dx, dy, dz,
+                              o->fixed_passes > 0 ? o->fixed_passes : 0, o->tol, st);
+  } else {
+    if (hipMalloc((void **)&f, n * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&f0, n * sizeof(float)) != hipSuccess) {
+      rc = fail(PDFS_ENOMEM, "pdfs_solve: device alloc");
+      goto out;
+    }
+    if ((rc = pdfs_pack_dev(c, d, f, x, y, z))) goto out;
+    if (hipMemcpyAsync(f0, f, n * sizeof(float), hipMemcpyDeviceToDevice, c->st) != hipSuccess) {
+      rc = fail(PDFS_EHIP, "pdfs_solve: D2D copy");
+      goto out;
+    }
+    rc = pdfs_solve_f32_dev(c, f, f0, x, y, z, dx, dy, dz, o, st);
+    if (rc && rc != PDFS_ENOTCONV) goto out;
+    int rc2 = pdfs_unpack_dev(c, f, d, x, y, z);
+    if (rc2) {
+      rc = rc2;
+      goto out;
+    }
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess ||
+      hipMemcpy(distance, d, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(PDFS_EHIP, "pdfs_solve: D2H copy");
+out:
+  hipFree(d);
+  hipFree(f);
+  hipFree(f0);
+  return rc;
+}
+
+/* single_gpu_cuda/src/phi3D_fsm.cu:64-106: device copy in, `iterations`
+ * passes, copy out, print the elapsed time; errors exit like
+ * cudaCheckError. */
+void run_fsm(Phi *pf, int iterations)
+{
+  pdfs_opts o;
+  pdfs_default_opts(&o);
+  o.precision = PDFS_F64_REF;
+  o.fixed_passes = iterations > 0 ? iterations : 0;
+  o.tol = 1e-12;
+  pdfs_stats s;
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipEventRecord(a, 0);
+  int rc = pdfs_solve(pf->distance, pf->x, pf->y, pf->z, pf->dx, pf->dy, pf->dz, &o, &s);
+  hipEventRecord(b, 0);
+  hipEventSynchronize(b);
+  if (rc != PDFS_OK) {
+    printf("\nHIP failure in run_fsm: '%s'\n", pdfs_last_error());
+    exit(EXIT_FAILURE);
+  }
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  printf("Parallel FSM time: %f s.\n", ms / 1000.0);
+}
+
+/* mpi_openacc/src/phi3D_fsm.c:67: one ordering sn, in place. */
+void runFSM(PhiACC *pf, int sn)
+{
+  pdfs_ctx *c = default_ctx();
+  if (!c || sn < 1 || sn > 8) {
+    printf("runFSM: invalid call\n");
+    exit(EXIT_FAILURE);
+  }
+  size_t np = (size_t)(pf->x + 2) * (pf->y + 2) * (pf->z + 2);
+  double *d = NULL;
+  if (hipMalloc((void **)&d, np * sizeof(double)) != hipSuccess ||
+      hipMemcpy(d, pf->distance, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      pdfsk_sweep_ref64(d, pf->x, pf->y, pf->z, sn, pf->dx, pf->dy, pf->dz, NULL, c->st) != 0 ||
+      hipStreamSynchronize(c->st) != hipSuccess ||
+      hipMemcpy(pf->distance, d, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    printf("\nHIP failure in runFSM\n");
+    exit(EXIT_FAILURE);
+  }
+  hipFree(d);
+}
+
+int pdfs_pack_dev(pdfs_ctx *c, const double *padded, float *dense, int x, int y, int z)
+{
+  KCHK(pdfsk_pack_f64(padded, dense, x, y, z, c->st));
+  return PDFS_OK;
+}
+
+int pdfs_unpack_dev(pdfs_ctx *c, const float *dense, double *padded, int x, int y, int z)
+{
+  KCHK(pdfsk_unpack_f64(dense, padded, x, y, z, c->st));
+  return PDFS_OK;
+}
+
+/* ---- synthetic fields: host parameter draws (splitmix64, Steele, Lea &
+ * Flood 2014) in the documented draw order, per-cell evaluation on the GPU */
+static uint64_t splitmix64(uint64_t *s)
+{
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static double u01(uint64_t *s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
+
+int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
+                      float *dense, double *padded)
+{
+  if (!c || x < 1 || y < 1 || z < 1 || !(h > 0)) return fail(PDFS_EINVAL, "generate: bad args");
+  uint64_t s = seed;
+  double prm[1024];
+  if (kind == PDFS_GEN_POINT) {
+    int k = 1 + (int)(u01(&s) * x);
+    int j = 1 + (int)(u01(&s) * y);
+    int i = 1 + (int)(u01(&s) * z);
+    KCHK(pdfsk_gen_point(dense, padded, x, y, z, k, j, i, c->st));
+  } else if (kind == PDFS_GEN_TERRAIN) {
+    /* 8 hills, amplitude U(5,40) h, width U(10,60) h, centres uniform */
+    int nh = 8;
+    for (int q = 0; q < nh; q++) {
+      prm[4 * q] = (5.0 + (40.0 - 5.0) * u01(&s)) * h;
+      prm[4 * q + 1] = (10.0 + (60.0 - 10.0) * u01(&s)) * h;
+      prm[4 * q + 2] = u01(&s) * (double)(x - 1) * h;
+      prm[4 * q + 3] = u01(&s) * (double)(y - 1) * h;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_params, prm, 4 * nh * sizeof(double), hipMemcpyHostToDevice,
+                          c->st));
+    KCHK(pdfsk_gen_terrain(dense, padded, x, y, z, h, c->d_params, nh, c->st));
+  } else if (kind == PDFS_GEN_SPHERES) {
+    /* 64 spheres, radius U(4,32) h, centres uniform */
+    int ns = 64;
+    for (int q = 0; q < ns; q++) {
+      prm[4 * q] = u01(&s) * (double)(x - 1) * h;
+      prm[4 * q + 1] = u01(&s) * (double)(y - 1) * h;
+      prm[4 * q + 2] = u01(&s) * (double)(z - 1) * h;
+      prm[4 * q + 3] = (4.0 + (32.0 - 4.0) * u01(&s)) * h;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_params, prm, 4 * ns * sizeof(double), hipMemcpyHostToDevice,
+                          c->st));
+    KCHK(pdfsk_gen_spheres(dense, padded, x, y, z, h, c->d_params, ns, c->st));
+  } else {
+    return fail(PDFS_EINVAL, "generate: unknown kind %d", kind);
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return PDFS_OK;
+}

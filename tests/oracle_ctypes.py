@@ -1,0 +1,173 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle.so) and to the reference
+FSM core built from source (oracle/_ref/libpdfs_ref.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg. Arrays are padded (X+2)(Y+2)(Z+2), x fastest,
+numpy shape (Z+2, Y+2, X+2).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpdfs_ref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_I, _D = C.c_int, C.c_double
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise FileNotFoundError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        L = C.CDLL(ORACLE_SO)
+        G = [_D] * 7
+        L.orc_godunov_ref.argtypes = G
+        L.orc_godunov_ref.restype = _D
+        L.orc_godunov_d64.argtypes = G
+        L.orc_godunov_d64.restype = _D
+        L.orc_godunov_f32.argtypes = [C.c_float] * 4 + [_D] * 3
+        L.orc_godunov_f32.restype = C.c_float
+        for nm, p in (("orc_sweep_ref64", _dp), ("orc_sweep_lex64", _dp),
+                      ("orc_sweep_f32", _fp), ("orc_sweep_d64", _dp)):
+            getattr(L, nm).argtypes = [p, _I, _I, _I, _D, _D, _D, _I]
+            getattr(L, nm).restype = None
+        L.orc_solve_ref64.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _D, _I, C.POINTER(_D)]
+        L.orc_solve_ref64.restype = _I
+        L.orc_solve_d64.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _D, _I]
+        L.orc_solve_d64.restype = _I
+        L.orc_solve_f32.argtypes = [_fp, _I, _I, _I, _D, _D, _D, _I, C.POINTER(C.c_long)]
+        L.orc_solve_f32.restype = _I
+        L.orc_update_distance.argtypes = [_ip, _dp, C.c_size_t]
+        L.orc_set_negative_inside.argtypes = [_ip, _dp, C.c_size_t]
+        L.orc_adjust_boundary.argtypes = [_dp, _I, _I, _I]
+        L.orc_fill_ghost.argtypes = [_dp, _I, _I, _I, _D]
+        L.orc_gen_point.argtypes = [_dp, _I, _I, _I, C.c_uint64]
+        L.orc_gen_terrain.argtypes = [_dp, _I, _I, _I, _D, C.c_uint64, _I, _D, _D, _D, _D]
+        L.orc_gen_spheres.argtypes = [_dp, _I, _I, _I, _D, C.c_uint64, _I, _D, _D]
+        L.orc_judge_f32.argtypes = [_fp, _fp, _I, _I, _I, _D, _D, _D, C.c_void_p]
+        L.orc_judge_f32.restype = C.c_long
+        L.orc_residual_f64.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D]
+        L.orc_residual_f64.restype = _D
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_SO)
+        R.ref_run_sn.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _I]
+        R.ref_run_sn.restype = _I
+        R.ref_solve.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _D, _I, C.POINTER(_D), C.POINTER(_D)]
+        R.ref_solve.restype = _I
+        R.ref_time_orderings.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _I]
+        R.ref_time_orderings.restype = _D
+        _ref = R
+    return _ref
+
+
+def dims(a):
+    z, y, x = a.shape
+    return x - 2, y - 2, z - 2
+
+
+def padded(X, Y, Z, dtype=np.float64, fill=np.inf):
+    return np.full((Z + 2, Y + 2, X + 2), fill, dtype=dtype)
+
+
+def interior(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+# ---- convenience wrappers (all in place, return the array) ----
+def sweep_ref64(a, sn, h=(1.0, 1.0, 1.0)):
+    lib().orc_sweep_ref64(a, *dims(a), *h, sn)
+    return a
+
+
+def sweep_lex64(a, sn, h=(1.0, 1.0, 1.0)):
+    lib().orc_sweep_lex64(a, *dims(a), *h, sn)
+    return a
+
+
+def sweep_f32(a, sn, h=(1.0, 1.0, 1.0)):
+    lib().orc_sweep_f32(a, *dims(a), *h, sn)
+    return a
+
+
+def solve_ref64(a, h=(1.0, 1.0, 1.0), tol=1e-12, max_passes=100):
+    md = _D()
+    p = lib().orc_solve_ref64(a, *dims(a), *h, tol, max_passes, C.byref(md))
+    return p, md.value
+
+
+def solve_d64(a, h=(1.0, 1.0, 1.0), tol=0.0, max_passes=100):
+    return lib().orc_solve_d64(a, *dims(a), *h, tol, max_passes)
+
+
+def solve_f32(a, h=(1.0, 1.0, 1.0), max_passes=100):
+    ch = C.c_long()
+    p = lib().orc_solve_f32(a, *dims(a), *h, max_passes, C.byref(ch))
+    return p, ch.value
+
+
+def gen_point(X, Y, Z, seed):
+    a = padded(X, Y, Z)
+    lib().orc_gen_point(a, X, Y, Z, seed)
+    return a
+
+
+def gen_terrain(X, Y, Z, seed, h=1.0, nhills=8, amp=(5.0, 40.0), sig=(10.0, 60.0)):
+    a = padded(X, Y, Z)
+    lib().orc_gen_terrain(a, X, Y, Z, h, seed, nhills, amp[0], amp[1], sig[0], sig[1])
+    return a
+
+
+def gen_spheres(X, Y, Z, seed, h=1.0, n=64, r=(4.0, 32.0)):
+    a = padded(X, Y, Z)
+    lib().orc_gen_spheres(a, X, Y, Z, h, seed, n, r[0], r[1])
+    return a
+
+
+def judge_f32(v, init, h=(1.0, 1.0, 1.0), want_flags=False):
+    flags = np.zeros(v.shape, np.uint8) if want_flags else None
+    n = lib().orc_judge_f32(v, init, *dims(v), *h,
+                            flags.ctypes.data if want_flags else None)
+    return (n, flags) if want_flags else n
+
+
+def ref_run_sn(a, sn, h=(1.0, 1.0, 1.0)):
+    ref().ref_run_sn(a, *dims(a), *h, sn)
+    return a
+
+
+def ref_solve(a, h=(1.0, 1.0, 1.0), tol=1e-12, max_passes=100):
+    sec, md = _D(), _D()
+    p = ref().ref_solve(a, *dims(a), *h, tol, max_passes, C.byref(sec), C.byref(md))
+    return p, sec.value, md.value
+
+
+def random_field(X, Y, Z, seed, p_src=0.02, p_far=0.3, hi=20.0):
+    """Random padded init: sources (0), far cells (1e6), the rest U(0, hi)."""
+    rng = np.random.default_rng(seed)
+    a = padded(X, Y, Z)
+    v = rng.uniform(0.0, hi, size=(Z, Y, X))
+    r = rng.uniform(size=(Z, Y, X))
+    v[r < p_src] = 0.0
+    v[r > 1.0 - p_far] = 1e6
+    a[1:-1, 1:-1, 1:-1] = v
+    return a

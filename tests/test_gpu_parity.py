@@ -1,0 +1,170 @@
+"""GPU parity of the HIP FSM path against the CPU oracle, through the C-ABI.
+
+FP64 reference arithmetic: bit-exact (per ordering, per pass, converged).
+FP32 column pipeline: bit-exact against the oracle's FP32 difference form,
+and within the FP32 error budget of the FP64 oracle (tolerance stated below).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_ctypes as O
+import pdfs_amd
+
+pytestmark = pytest.mark.gpu
+
+# FP32 vs FP64 oracle tolerance for the small fields here (distances < 64 h):
+# storage rounding random walk, a few hundred FP32 ulps at most.
+F32_VS_F64_TOL = 1e-4
+
+SHAPES = [(5, 4, 3), (17, 13, 9), (16, 16, 16), (33, 40, 20), (48, 17, 31), (64, 64, 64)]
+SPACINGS = [(1.0, 1.0, 1.0), (0.5, 0.7, 1.3)]
+
+
+def _dense(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(O.interior(a)).astype(np.float32)).to(dev)
+
+
+def _bits32(x):
+    return np.ascontiguousarray(x).view(np.uint32)
+
+
+@pytest.mark.parametrize("shape", SHAPES[:4])
+@pytest.mark.parametrize("h", SPACINGS)
+def test_ref64_each_ordering_bitwise(shape, h):
+    X, Y, Z = shape
+    for sn in range(1, 9):
+        a = O.random_field(X, Y, Z, seed=100 + sn)
+        b = a.copy()
+        print(f"ref64 {shape} h={h} sn={sn}", flush=True)
+        O.sweep_ref64(a, sn, h)
+        pdfs_amd.runFSM(b, sn, h)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (shape, h, sn)
+
+
+def test_run_fsm_one_pass_and_converged_bitwise():
+    a = O.gen_point(24, 20, 16, seed=3)
+    b = a.copy()
+    for sn in range(1, 9):
+        O.sweep_ref64(a, sn)
+    pdfs_amd.run_fsm(b, 1)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    c = O.gen_spheres(30, 26, 22, seed=4, n=5, r=(2.0, 5.0))
+    d = c.copy()
+    p_o, _ = O.solve_ref64(c, tol=1e-12)
+    pdfs_amd.run_fsm(d, 0)  # until converged (1e-12)
+    assert np.array_equal(c.view(np.uint64), d.view(np.uint64)), p_o
+
+
+def test_c0_point_source_64cube_converged(ctx, dev):
+    """Config c0 shape: 64^3, h=1, FP64, point source, until a pass changes
+    nothing by more than 1e-12."""
+    a = O.gen_point(64, 64, 64, seed=12345)
+    p_o, md = O.solve_ref64(a.copy(), tol=1e-12)
+    want = a.copy()
+    O.solve_ref64(want, tol=1e-12)
+    t = torch.from_numpy(a).to(dev)
+    st = ctx.solve_ref64(t, passes=0, tol=1e-12)
+    got = t.cpu().numpy()
+    assert st["passes"] == p_o
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("h", SPACINGS)
+@pytest.mark.parametrize("tile", [pdfs_amd.TILE_16x16, pdfs_amd.TILE_32x8, pdfs_amd.TILE_16x8])
+def test_f32_each_ordering_bitwise(ctx, dev, shape, h, tile):
+    X, Y, Z = shape
+    a = O.random_field(X, Y, Z, seed=7 + X)
+    ref = a.astype(np.float32)
+    t = _dense(a, dev)
+    for sn in range(1, 9):
+        print(f"f32 {shape} h={h} tile={tile} sn={sn}", flush=True)
+        O.sweep_f32(ref, sn, h)
+        ctx.sweep_f32(t, sn, h, tile=tile)
+        got = t.cpu().numpy()
+        want = O.interior(ref)
+        assert np.array_equal(_bits32(got), _bits32(want)), (shape, h, tile, sn,
+                                                               float(np.abs(got - want).max()))
+
+
+@pytest.mark.parametrize("gen", ["point", "terrain", "spheres"])
+def test_f32_converged_bitwise_and_vs_f64(ctx, dev, gen):
+    X, Y, Z = 72, 56, 40
+    if gen == "point":
+        a = O.gen_point(X, Y, Z, seed=21)
+    elif gen == "terrain":
+        a = O.gen_terrain(X, Y, Z, seed=22, amp=(2.0, 10.0), sig=(3.0, 15.0))
+    else:
+        a = O.gen_spheres(X, Y, Z, seed=23, n=8, r=(2.0, 8.0))
+    ref32 = a.astype(np.float32)
+    p32, _ = O.solve_f32(ref32)
+    ref64 = a.copy()
+    O.solve_d64(ref64, tol=0.0)
+    t = _dense(a, dev)
+    init = t.clone()
+    st = ctx.solve_f32(t, init)
+    got = t.cpu().numpy()
+    assert st["passes"] == p32
+    assert np.array_equal(_bits32(got), _bits32(O.interior(ref32)))
+    err = np.abs(got.astype(np.float64) - O.interior(ref64))
+    print(f"{gen}: passes={p32} max={err.max():.3e} rms={np.sqrt((err**2).mean()):.3e}")
+    assert err.max() <= F32_VS_F64_TOL
+    assert st["judge_flagged"] == 0
+
+
+def test_judge_matches_oracle_and_flags_planted_defect(ctx, dev):
+    X, Y, Z = 40, 36, 28
+    a = O.gen_spheres(X, Y, Z, seed=31, n=6, r=(2.0, 6.0))
+    init32 = a.astype(np.float32)
+    v = init32.copy()
+    O.solve_f32(v)
+    assert O.judge_f32(v, init32) == 0
+    # plant a too-low region: every point in a box lowered by 0.5 h
+    bad = v.copy()
+    bad[10:14, 10:15, 12:20] -= 0.5
+    n_orc, fl = O.judge_f32(bad, init32, want_flags=True)
+    assert n_orc > 0
+    tv = torch.from_numpy(np.ascontiguousarray(O.interior(bad))).to(dev)
+    ti = torch.from_numpy(np.ascontiguousarray(O.interior(init32))).to(dev)
+    words = (X * Y * Z + 31) // 32
+    flags = torch.zeros(words, dtype=torch.int32, device=dev)
+    n_gpu = ctx.judge_f32(tv, ti, flags=flags)
+    assert n_gpu == n_orc
+    bits = np.unpackbits(flags.cpu().numpy().view(np.uint8), bitorder="little")[: X * Y * Z]
+    assert np.array_equal(bits.reshape(Z, Y, X), O.interior(fl))
+
+
+@pytest.mark.parametrize("kind", [pdfs_amd.GEN_POINT, pdfs_amd.GEN_TERRAIN, pdfs_amd.GEN_SPHERES])
+def test_generators_match_oracle(ctx, dev, kind):
+    X, Y, Z = 48, 40, 36
+    seed = 99
+    if kind == pdfs_amd.GEN_POINT:
+        want = O.gen_point(X, Y, Z, seed)
+    elif kind == pdfs_amd.GEN_TERRAIN:
+        want = O.gen_terrain(X, Y, Z, seed)
+    else:
+        want = O.gen_spheres(X, Y, Z, seed)
+    dense = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+    padded = torch.empty((Z + 2, Y + 2, X + 2), dtype=torch.float64, device=dev)
+    ctx.generate(kind, seed, X, Y, Z, 1.0, dense=dense, padded=padded)
+    gp = padded.cpu().numpy()
+    gd = dense.cpu().numpy()
+    if kind == pdfs_amd.GEN_TERRAIN:
+        # exp() differs between the GPU and glibc in the last bits
+        assert np.allclose(gp, want, rtol=0, atol=1e-12)
+        assert np.abs(gd - O.interior(want).astype(np.float32)).max() <= 1e-5
+    else:
+        assert np.array_equal(gp.view(np.uint64), want.view(np.uint64))
+        assert np.array_equal(_bits32(gd), _bits32(O.interior(want).astype(np.float32)))
+
+
+def test_host_padded_solve_f32_matches_oracle():
+    a = O.gen_spheres(34, 30, 26, seed=41, n=5, r=(2.0, 6.0))
+    ref32 = a.astype(np.float32)
+    O.solve_f32(ref32)
+    b = a.copy()
+    st = pdfs_amd.solve(b, precision=pdfs_amd.PDFS_F32)
+    assert np.array_equal(O.interior(b), O.interior(ref32).astype(np.float64))
+    assert np.all(np.isinf(b[0])) and np.all(np.isinf(b[:, 0])) and np.all(np.isinf(b[:, :, -1]))
+    assert st["judge_flagged"] == 0

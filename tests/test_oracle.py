@@ -1,0 +1,325 @@
+"""CPU tests: pin the oracle (golden vectors from the reference core, the
+compiled reference itself, closed forms, independent root finding, schedule
+independence, invariants) and check the host-side pieces."""
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref_fsm_vectors.npz")
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+# ---------------------------------------------------------------- golden vectors
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD, allow_pickle=False)
+
+
+@pytest.mark.parametrize("name", ["a", "b", "c", "d"])
+def test_oracle_matches_reference_golden_vectors(gold, name):
+    h = tuple(gold[f"{name}_h"])
+    inp = gold[f"{name}_input"]
+    for sn in range(1, 9):
+        a = inp.copy()
+        O.sweep_ref64(a, sn, h)
+        assert np.array_equal(bits(a), bits(gold[f"{name}_sn{sn}"])), sn
+    a = inp.copy()
+    for sn in range(1, 9):
+        O.sweep_ref64(a, sn, h)
+    assert np.array_equal(bits(a), bits(gold[f"{name}_pass1"]))
+    a = inp.copy()
+    passes, _ = O.solve_ref64(a, h, tol=1e-12, max_passes=200)
+    assert passes == int(gold[f"{name}_passes"])
+    assert np.array_equal(bits(a), bits(gold[f"{name}_converged"]))
+
+
+def test_c0_point_source_golden(gold):
+    a = O.gen_point(64, 64, 64, 12345)
+    passes, md = O.solve_ref64(a, tol=1e-12)
+    assert passes == int(gold["c0_passes"])
+    assert hashlib.sha256(a.tobytes()).digest() == gold["c0_sha256"].tobytes()
+    assert np.array_equal(a.reshape(-1)[gold["c0_sample_idx"]], gold["c0_sample_val"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference absent)")
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_compiled_reference(seed):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = (int(v) for v in rng.integers(2, 14, 3))
+    h = tuple(float(v) for v in rng.choice([0.5, 1.0, 1.7], 3))
+    a = O.random_field(X, Y, Z, 1000 + seed)
+    for sn in range(1, 9):
+        b, c = a.copy(), a.copy()
+        O.sweep_ref64(b, sn, h)
+        O.ref_run_sn(c, sn, h)
+        assert np.array_equal(bits(b), bits(c)), (X, Y, Z, h, sn)
+
+
+# ---------------------------------------------------------------- Godunov solves
+def quad_root(ms, ds):
+    """Largest root of sum(((u - m_i)/d_i)^2) = 1 by numpy.roots (independent
+    of the closed forms under test)."""
+    c2 = sum(1.0 / d ** 2 for d in ds)
+    c1 = sum(-2.0 * m / d ** 2 for m, d in zip(ms, ds))
+    c0 = sum(m * m / d ** 2 for m, d in zip(ms, ds)) - 1.0
+    r = np.roots([c2, c1, c0])
+    return float(np.max(r.real))
+
+
+def godunov_brute(mx, my, mz, dx, dy, dz):
+    """Godunov upwind value by trying the 1-, 2-, 3-axis solutions in order of
+    the sorted minima and keeping the first consistent one."""
+    pairs = sorted([(mx, dx), (my, dy), (mz, dz)], key=lambda p: p[0])
+    for k in (1, 2, 3):
+        ms = [p[0] for p in pairs[:k]]
+        ds = [p[1] for p in pairs[:k]]
+        if any(math.isinf(m) for m in ms):
+            break
+        u = ms[0] + ds[0] if k == 1 else quad_root(ms, ds)
+        if k == 3 or u <= pairs[k][0]:
+            return u
+    return math.inf
+
+
+@pytest.mark.parametrize("fn", ["ref", "d64", "f32"])
+def test_godunov_closed_forms(fn):
+    g = {"ref": O.lib().orc_godunov_ref, "d64": O.lib().orc_godunov_d64,
+         "f32": O.lib().orc_godunov_f32}[fn]
+    tol = 1e-12 if fn != "f32" else 2e-6
+    inf = math.inf
+    for m in (0.0, 3.0, 250.0):
+        for hh in (1.0, 0.5):
+            assert abs(g(1e9, m, inf, inf, hh, hh, hh) - (m + hh)) <= tol * max(1, m)
+            assert abs(g(1e9, m, m, inf, hh, hh, hh) - (m + hh / math.sqrt(2))) <= tol * max(1, m)
+            assert abs(g(1e9, m, m, m, hh, hh, hh) - (m + hh / math.sqrt(3))) <= tol * max(1, m)
+    # all neighbours infinite -> cur; cur below the candidate wins
+    assert g(7.0, inf, inf, inf, 1, 1, 1) == 7.0
+    assert g(0.25, 2.0, 3.0, 4.0, 1, 1, 1) == 0.25
+
+
+@pytest.mark.parametrize("fn", ["ref", "d64", "f32"])
+def test_godunov_general_roots_vs_numpy(fn):
+    """Random anisotropic cases against an independent root finder: a
+    dropped term, swapped spacing or wrong sign fails here."""
+    g = {"ref": O.lib().orc_godunov_ref, "d64": O.lib().orc_godunov_d64,
+         "f32": O.lib().orc_godunov_f32}[fn]
+    rng = np.random.default_rng(5)
+    for it in range(3000):
+        base = float(rng.uniform(0, 300))
+        m = [base + float(v) for v in rng.uniform(0, 2.0, 3)]
+        if it % 5 == 0:
+            m[int(rng.integers(3))] = math.inf
+        d = [float(v) for v in rng.uniform(0.3, 2.0, 3)]
+        if it % 2:
+            d = [d[0]] * 3
+        want = godunov_brute(m[0], m[1], m[2], *d)
+        got = g(1e9, m[0], m[1], m[2], *d)
+        tol = (1e-9 if fn != "f32" else 4e-5) * max(1.0, base)
+        assert abs(got - want) <= tol, (m, d, got, want)
+
+
+def test_godunov_invariants():
+    L = O.lib()
+    rng = np.random.default_rng(9)
+    for _ in range(2000):
+        m = rng.uniform(0, 5, 3)
+        d = rng.uniform(0.5, 1.5, 3)
+        u = L.orc_godunov_d64(1e9, *m, *d)
+        # shift invariance G(m + c) = G(m) + c
+        c = float(rng.uniform(-3, 100))
+        assert abs(L.orc_godunov_d64(1e9, *(m + c), *d) - (u + c)) < 1e-10
+        # monotone and non-expansive: raising one minimum never lowers G and
+        # raises it by at most that amount
+        k = int(rng.integers(3))
+        e = float(rng.uniform(0, 0.5))
+        m2 = m.copy()
+        m2[k] += e
+        u2 = L.orc_godunov_d64(1e9, *m2, *d)
+        assert u - 1e-12 <= u2 <= u + e + 1e-12
+        # result never above the current value, and above the smallest minimum
+        assert L.orc_godunov_d64(2.0, *m, *d) <= 2.0
+        assert u > min(m)
+
+
+# ---------------------------------------------------------------- sweeps
+@pytest.mark.parametrize("sn", range(1, 9))
+def test_hyperplane_equals_lexicographic_gauss_seidel(sn):
+    """Schedule independence (the hyperplane sweep reads only level L-1 new
+    and L+1 old values), bitwise, anisotropic spacing."""
+    for seed, (X, Y, Z) in enumerate([(2, 2, 2), (5, 3, 4), (9, 7, 6), (16, 4, 11)]):
+        a = O.random_field(X, Y, Z, 50 + seed + 10 * sn)
+        b = a.copy()
+        O.sweep_ref64(a, sn, (0.8, 1.1, 1.3))
+        O.sweep_lex64(b, sn, (0.8, 1.1, 1.3))
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_ordering_directions_match_reference_table():
+    """A single source at the centre of a 1-D line along each axis: ordering
+    sn reaches exactly the cells downstream of the source in its direction
+    ((x,y,z) signs: 1 +++, 2 -+-, 3 ++-, 4 -++, 5 ---, 6 +-+, 7 --+, 8 +--),
+    plus the one upstream neighbour, which reads the source as an old value."""
+    table = {1: (1, 1, 1), 2: (-1, 1, -1), 3: (1, 1, -1), 4: (-1, 1, 1),
+             5: (-1, -1, -1), 6: (1, -1, 1), 7: (-1, -1, 1), 8: (1, -1, -1)}
+    for sn, signs in table.items():
+        for axis in range(3):
+            dims = [1, 1, 1]
+            dims[axis] = 9
+            X, Y, Z = dims
+            a = O.padded(X, Y, Z)
+            a[1:-1, 1:-1, 1:-1] = 1e6
+            c = [1, 1, 1]
+            c[axis] = 5
+            a[c[2], c[1], c[0]] = 0.0
+            O.sweep_ref64(a, sn)
+            line = O.interior(a).reshape(-1)
+            s = signs[axis]
+            for p in range(9):
+                off = (p + 1) - 5
+                want = abs(off) if (abs(off) <= 1 or np.sign(off) == s) else 1e6
+                assert line[p] == want, (sn, axis, p)
+
+
+def test_point_source_axis_exactness_and_lipschitz():
+    a = O.gen_point(40, 36, 30, 77)
+    O.solve_ref64(a)
+    u = O.interior(a)
+    k, j, i = np.argwhere(u == 0)[0][::-1]
+    for n in range(1, 8):
+        for (dx, dy, dz) in ((n, 0, 0), (-n, 0, 0), (0, n, 0), (0, -n, 0), (0, 0, n), (0, 0, -n)):
+            x, y, z = k + dx, j + dy, i + dz
+            if 0 <= x < 40 and 0 <= y < 36 and 0 <= z < 30:
+                assert u[z, y, x] == float(n)
+    for ax in range(3):
+        assert np.abs(np.diff(u, axis=ax)).max() <= 1.0 + 1e-12
+    # converged field is a fixed point of v = min(init, G(v))
+    init = O.gen_point(40, 36, 30, 77)
+    assert O.lib().orc_residual_f64(a, init, 40, 36, 30, 1.0, 1.0, 1.0) < 1e-9
+
+
+def test_solution_is_distance_like():
+    """Two-source field: the converged value is within the known first-order
+    Godunov error of the Euclidean distance and never below it along axes."""
+    X = Y = Z = 24
+    a = O.padded(X, Y, Z)
+    a[1:-1, 1:-1, 1:-1] = 1e6
+    srcs = [(4, 5, 6), (18, 17, 15)]
+    for (x, y, z) in srcs:
+        a[z + 1, y + 1, x + 1] = 0.0
+    O.solve_ref64(a)
+    u = O.interior(a)
+    zz, yy, xx = np.mgrid[0:Z, 0:Y, 0:X]
+    eu = np.min([np.sqrt((xx - x) ** 2 + (yy - y) ** 2 + (zz - z) ** 2) for (x, y, z) in srcs],
+                axis=0)
+    # first-order upwind: over-estimates away from the sources, may dip a
+    # little below where the two fronts meet
+    assert np.min(u - eu) > -0.05
+    assert np.max(u - eu) < 0.1 * np.max(eu)
+    # the sources' own axes are exact
+    assert u[6, 5, 4 + 3] == 3.0 and u[15, 17 - 2, 18] == 2.0
+
+
+# ---------------------------------------------------------------- FP32 + judge
+def test_f32_oracle_close_to_f64_and_judge():
+    a = O.gen_spheres(48, 40, 36, 3, n=6, r=(2.0, 6.0))
+    f = a.astype(np.float32)
+    init32 = f.copy()
+    O.solve_f32(f)
+    b = a.copy()
+    O.solve_d64(b, tol=0.0)
+    err = np.abs(O.interior(f).astype(np.float64) - O.interior(b))
+    assert err.max() < 5e-5
+    assert O.judge_f32(f, init32) == 0
+    # a region lowered below the truth is flagged (at least at its border)
+    bad = f.copy()
+    bad[10:16, 10:16, 10:16] -= 0.75
+    n, fl = O.judge_f32(bad, init32, want_flags=True)
+    assert n > 0 and fl[10:16, 10:16, 10:16].any()
+    # a single raised point is flagged exactly there
+    bad = f.copy()
+    bad[20, 21, 22] += 0.5
+    n, fl = O.judge_f32(bad, init32, want_flags=True)
+    assert fl[20, 21, 22] == 1
+
+
+# ---------------------------------------------------------------- pre/post
+def test_update_distance_known_answers():
+    loc = np.array([-1, 0, 0, 0, 1, 1, 0, 0], np.int32)
+    d = np.array([0.0, 0.0, np.nan, np.inf, 0.0, 2.5, -3.0, 7.0])
+    O.lib().orc_update_distance(loc, d, d.size)
+    assert d[0] == 0.0  # ghost location untouched
+    assert d[1] == 90000 and d[2] == 90000 and d[4] == 90000
+    assert d[3] == np.inf and d[5] == 2.5 and d[6] == -3.0 and d[7] == 7.0
+
+
+def test_set_negative_inside_known_answers():
+    loc = np.array([-1, 1, 1, 0, 1], np.int32)
+    d = np.array([5.0, 3.0, np.inf, 4.0, 0.5])
+    O.lib().orc_set_negative_inside(loc, d, d.size)
+    assert list(d) == [5.0, -1.0, np.inf, 4.0, -1.0]
+
+
+def test_adjust_boundary_is_edge_padding():
+    X, Y, Z = 5, 4, 3
+    a = O.padded(X, Y, Z)
+    a[1:-1, 1:-1, 1:-1] = np.arange(X * Y * Z, dtype=np.float64).reshape(Z, Y, X)
+    O.lib().orc_adjust_boundary(a, X, Y, Z)
+    want = np.pad(O.interior(a), 1, mode="edge")
+    assert np.array_equal(a, want)
+
+
+# ---------------------------------------------------------------- generators
+def splitmix64(s):
+    s = (s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = s
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return s, z ^ (z >> 31)
+
+
+def u01_stream(seed):
+    s = seed
+    while True:
+        s, z = splitmix64(s)
+        yield (z >> 11) * 2.0 ** -53
+
+
+def test_generators_independent_recomputation():
+    X, Y, Z, seed = 30, 26, 22, 8
+    # spheres
+    g = u01_stream(seed)
+    sph = []
+    for _ in range(64):
+        cx, cy, cz = next(g) * (X - 1), next(g) * (Y - 1), next(g) * (Z - 1)
+        r = 4 + 28 * next(g)
+        sph.append((cx, cy, cz, r))
+    zz, yy, xx = np.mgrid[0:Z, 0:Y, 0:X].astype(np.float64)
+    s = np.min([np.sqrt((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) - r
+                for cx, cy, cz, r in sph], axis=0)
+    want = np.where(s <= 0, 0.0, np.where(s <= 1.5, s, 1e6))
+    got = O.interior(O.gen_spheres(X, Y, Z, seed))
+    assert np.allclose(got, want, rtol=0, atol=1e-12)
+    # terrain
+    g = u01_stream(seed)
+    hills = [(5 + 35 * next(g), 10 + 50 * next(g), next(g) * (X - 1), next(g) * (Y - 1))
+             for _ in range(8)]
+    yy2, xx2 = np.mgrid[0:Y, 0:X].astype(np.float64)
+    H = sum(A * np.exp(-((xx2 - cx) ** 2 + (yy2 - cy) ** 2) / (2 * S * S)) for A, S, cx, cy in hills)
+    off = zz - H[None]
+    want = np.where(off <= 0, 0.0, np.where(off <= 1.5, off, 1e6))
+    got = O.interior(O.gen_terrain(X, Y, Z, seed))
+    assert np.allclose(got, want, rtol=0, atol=1e-9)
+    # point
+    g = u01_stream(seed)
+    k, j, i = 1 + int(next(g) * X), 1 + int(next(g) * Y), 1 + int(next(g) * Z)
+    p = O.gen_point(X, Y, Z, seed)
+    assert p[i, j, k] == 0.0 and np.count_nonzero(O.interior(p) == 0) == 1
+    assert np.all(np.isinf(p[0])) and np.all(np.isinf(p[:, :, -1]))

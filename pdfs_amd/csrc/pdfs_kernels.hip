@@ -70,6 +70,10 @@ __device__ __forceinline__ double god_ref64(double cur, double mx, double my, do
   return cur < u ? cur : u;
 }
 
+// Square root in the operand's own precision (correctly rounded).
+__device__ __forceinline__ float sqrt_t(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrt_t(double x) { return __builtin_sqrt(x); }
+
 // Shift-invariant difference form u = m0 + t(m1 - m0, m2 - m0, spacing).
 // Isotropic spacing: sorting network + constant multiplies, no division.
 template <typename T>
@@ -85,13 +89,13 @@ __device__ __forceinline__ T god_diff_iso(T cur, T mx, T my, T mz, T h, T two_h2
   T u = m0 + h;
   if (u > m1) {
     T a = m1 - m0;
-    T r = __builtin_sqrt(two_h2 - a * a);
+    T r = sqrt_t(two_h2 - a * a);
     u = m0 + (T)0.5 * (a + r);
     if (u > m2) {
       T b = m2 - m0;
       T D = three_h2 - (T)2 * ((a * a + b * b) - a * b);
       D = D > (T)0 ? D : (T)0;
-      u = m0 + ((a + b) + __builtin_sqrt(D)) * third;
+      u = m0 + ((a + b) + sqrt_t(D)) * third;
     }
   }
   return cur < u ? cur : u;
@@ -151,7 +155,7 @@ __device__ __forceinline__ T god_diff_aniso(T cur, T mx, T my, T mz, const T dd[
   if (u > m[1]) {
     T a = m[1] - m[0];
     T d20 = d[0] * d[0], d21 = d[1] * d[1];
-    T r = __builtin_sqrt(d20 + d21 - a * a);
+    T r = sqrt_t(d20 + d21 - a * a);
     u = m[0] + (a * d20 + d[0] * d[1] * r) / (d20 + d21);
     if (u > m[2]) {
       T b = m[2] - m[0];
@@ -159,7 +163,7 @@ __device__ __forceinline__ T god_diff_aniso(T cur, T mx, T my, T mz, const T dd[
       T Q = a * a * w[1] + b * b * w[2] - (T)1;
       T D = P * P - W * Q;
       D = D > (T)0 ? D : (T)0;
-      u = m[0] + (P + __builtin_sqrt(D)) / W;
+      u = m[0] + (P + sqrt_t(D)) / W;
     }
   }
   return cur < u ? cur : u;

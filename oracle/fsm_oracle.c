@@ -75,9 +75,12 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
 /* roots are written as u = m0 + t(m1 - m0, m2 - m0, spacing), so FP32 keeps */
 /* the O(h) radicand at any distance magnitude. Isotropic spacing uses      */
 /* constant multiplies; anisotropic spacing the general weighted form.      */
+/* FP32: a 3-D radicand at or below 2^-96 is flushed to 0 (its root is below */
+/* 2^-48 and cannot move any value >= 2^-24), so the GPU square root never  */
+/* sees a denormal or tiny input. FP64 clamps at 0.                         */
 /* The HIP kernels evaluate exactly these operations in the same order.     */
 /* ------------------------------------------------------------------------ */
-#define DIFF_FORM_BODY(T, SQRT)                                                 \
+#define DIFF_FORM_BODY(T, SQRT, FLUSH)                                          \
   T m[3] = {mx, my, mz};                                                        \
   T d[3] = {(T)dx, (T)dy, (T)dz};                                               \
   if (dx == dy && dy == dz) {                                                   \
@@ -99,7 +102,7 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
       if (u > m2) {                                                             \
         T b = m2 - m0;                                                          \
         T D = three_h2 - (T)2 * ((a * a + b * b) - a * b);                      \
-        D = D > (T)0 ? D : (T)0;                                                \
+        D = D > (T)FLUSH ? D : (T)0;                                            \
         u = m0 + ((a + b) + SQRT(D)) * ((T)1 / (T)3);                           \
       }                                                                         \
     }                                                                           \
@@ -132,7 +135,7 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
         T P = a * w[1] + b * w[2];                                              \
         T Q = a * a * w[1] + b * b * w[2] - (T)1;                               \
         T D = P * P - W * Q;                                                    \
-        D = D > (T)0 ? D : (T)0;                                                \
+        D = D > (T)FLUSH ? D : (T)0;                                            \
         u = m[0] + (P + SQRT(D)) / W;                                           \
       }                                                                         \
     }                                                                           \
@@ -142,13 +145,13 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
 float orc_godunov_f32(float cur, float mx, float my, float mz, double dx,
                       double dy, double dz)
 {
-  DIFF_FORM_BODY(float, sqrtf)
+  DIFF_FORM_BODY(float, sqrtf, 0x1p-96f)
 }
 
 double orc_godunov_d64(double cur, double mx, double my, double mz, double dx,
                        double dy, double dz)
 {
-  DIFF_FORM_BODY(double, sqrt)
+  DIFF_FORM_BODY(double, sqrt, 0.0)
 }
 
 /* ------------------------------------------------------------------------ */

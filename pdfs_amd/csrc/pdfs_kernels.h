@@ -24,10 +24,19 @@ typedef struct {
   int iso;
 } pdfs_sp32;
 
+/* Entries of padding before and after each edge array: prefetch indices of
+ * the column kernel stay inside the allocation. */
+#define PDFSK_EDGE_PAD 4096
+
+/* Largest dense FP32 field of the column kernel: byte offsets are 32-bit
+ * and out-of-range planes must map past the buffer end (x*y*z*4 < 2^31). */
+#define PDFSK_MAX_POINTS ((1ll << 31) / 4 - 1)
+
 /* Per-field scratch for the column-pipeline sweep. */
 typedef struct {
-  uint64_t *edge_x;    /* [ncols][Z][TY] {value, epoch} east-edge values */
-  uint64_t *edge_y;    /* [ncols][Z][TX] {value, epoch} north-edge values */
+  uint64_t *edge_x;    /* [ncols][Z][TY] {value, epoch} east-edge values (base + PAD) */
+  uint64_t *edge_y;    /* [ncols][Z][TX] {value, epoch} north-edge values (base + PAD) */
+  size_t edge_x_n, edge_y_n; /* entries excluding the padding */
   int *order;          /* [ncols][2] ticket -> column (a, b) */
   unsigned *ticket;    /* monotonic work counter */
   unsigned *changed;   /* changed-point counter (accumulates) */

@@ -73,6 +73,11 @@ __device__ __forceinline__ double god_ref64(double cur, double mx, double my, do
 // Square root in the operand's own precision (correctly rounded).
 __device__ __forceinline__ float sqrt_t(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return __builtin_sqrt(x); }
+// Lower limit of a kept 3-D radicand: FP32 flushes radicands <= 2^-96 to 0
+// (their root cannot move a value >= 2^-24), FP64 clamps at 0.
+template <typename T> __device__ __forceinline__ T radicand_floor();
+template <> __device__ __forceinline__ float radicand_floor<float>() { return 0x1p-96f; }
+template <> __device__ __forceinline__ double radicand_floor<double>() { return 0.0; }
 
 // Shift-invariant difference form u = m0 + t(m1 - m0, m2 - m0, spacing).
 // Isotropic spacing: sorting network + constant multiplies, no division.
@@ -94,36 +99,53 @@ __device__ __forceinline__ T god_diff_iso(T cur, T mx, T my, T mz, T h, T two_h2
     if (u > m2) {
       T b = m2 - m0;
       T D = three_h2 - (T)2 * ((a * a + b * b) - a * b);
-      D = D > (T)0 ? D : (T)0;
+      D = D > radicand_floor<T>() ? D : (T)0;
       u = m0 + ((a + b) + sqrt_t(D)) * third;
     }
   }
   return cur < u ? cur : u;
 }
 
-__device__ __forceinline__ float god_diff_iso32(float cur, float mx, float my, float mz,
-                                                float h, float two_h2, float three_h2,
-                                                float third) {
+// Correctly rounded FP32 square root for x == 0 or x >= 2^-96: v_sqrt_f32
+// (within one ulp) followed by a one-ulp correction from the two FMA
+// residuals x - (s -/+ ulp) s. Inputs here never reach the denormal range
+// (see radicand_floor), so no rescaling is needed.
+__device__ __forceinline__ float sqrt_cr(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = __builtin_fmaf(-sm, s, x);
+  const float rp = __builtin_fmaf(-sp, s, x);
+  float r = rm <= 0.0f ? sm : s;
+  r = rp > 0.0f ? sp : r;
+  return x > 0.0f ? r : 0.0f;
+}
+
+// Branch-free isotropic FP32 solve: the same operations as the branching
+// difference form, with the 1-D / 2-D / 3-D choice made by selects.
+// Inputs are non-negative or +inf (no NaN, no -0), where fminf/fmaxf equal
+// the '<' / '>' selects of the oracle.
+__device__ __forceinline__ float god_iso32(float c, float xm, float xp, float ym, float yp,
+                                           float zm, float zp, float h, float two_h2,
+                                           float three_h2, float third) {
 #pragma clang fp contract(off)
-  float lo = mx < my ? mx : my;
-  float hi = mx > my ? mx : my;
-  float m0 = lo < mz ? lo : mz;
-  float mid = lo > mz ? lo : mz;
-  float m1 = hi < mid ? hi : mid;
-  float m2 = hi > mid ? hi : mid;
-  float u = m0 + h;
-  if (u > m1) {
-    float a = m1 - m0;
-    float r = __builtin_sqrtf(two_h2 - a * a);
-    u = m0 + 0.5f * (a + r);
-    if (u > m2) {
-      float b = m2 - m0;
-      float D = three_h2 - 2.0f * ((a * a + b * b) - a * b);
-      D = D > 0.0f ? D : 0.0f;
-      u = m0 + ((a + b) + __builtin_sqrtf(D)) * third;
-    }
-  }
-  return cur < u ? cur : u;
+  const float mx = __builtin_fminf(xm, xp), my = __builtin_fminf(ym, yp);
+  const float mz = __builtin_fminf(zm, zp);
+  const float lo = __builtin_fminf(mx, my), hi = __builtin_fmaxf(mx, my);
+  const float m0 = __builtin_fminf(lo, mz), mid = __builtin_fmaxf(lo, mz);
+  const float m1 = __builtin_fminf(hi, mid), m2 = __builtin_fmaxf(hi, mid);
+  const float u1 = m0 + h;
+  const float a = m1 - m0;
+  const float aa = a * a;
+  const float r = sqrt_cr(two_h2 - aa);
+  const float u2 = m0 + 0.5f * (a + r);
+  const float b = m2 - m0;
+  float D = three_h2 - 2.0f * ((aa + b * b) - a * b);
+  D = D > 0x1p-96f ? D : 0.0f;
+  const float u3 = m0 + ((a + b) + sqrt_cr(D)) * third;
+  const float t = u2 > m2 ? u3 : u2;
+  const float u = u1 > m1 ? t : u1;
+  return c < u ? c : u;
 }
 
 // Anisotropic difference form: bubble sort of (m, d, 1/d^2) triples, 2-D
@@ -162,7 +184,7 @@ __device__ __forceinline__ T god_diff_aniso(T cur, T mx, T my, T mz, const T dd[
       T P = a * w[1] + b * w[2];
       T Q = a * a * w[1] + b * b * w[2] - (T)1;
       T D = P * P - W * Q;
-      D = D > (T)0 ? D : (T)0;
+      D = D > radicand_floor<T>() ? D : (T)0;
       u = m[0] + (P + sqrt_t(D)) / W;
     }
   }
@@ -170,10 +192,11 @@ __device__ __forceinline__ T god_diff_aniso(T cur, T mx, T my, T mz, const T dd[
 }
 
 template <bool ISO>
-__device__ __forceinline__ float god32(float cur, float mx, float my, float mz,
-                                       const pdfs_sp32 &s) {
-  if (ISO) return god_diff_iso32(cur, mx, my, mz, s.h, s.two_h2, s.three_h2, s.third);
-  return god_diff_aniso<float>(cur, mx, my, mz, s.d, s.w, s.W);
+__device__ __forceinline__ float god32(float c, float xm, float xp, float ym, float yp, float zm,
+                                       float zp, const pdfs_sp32 &s) {
+  if (ISO) return god_iso32(c, xm, xp, ym, yp, zm, zp, s.h, s.two_h2, s.three_h2, s.third);
+  return god_diff_aniso<float>(c, xm < xp ? xm : xp, ym < yp ? ym : yp, zm < zp ? zm : zp, s.d,
+                               s.w, s.W);
 }
 
 // FP64 difference form (judge). Spacing constants in double, same op order
@@ -191,6 +214,11 @@ __device__ __forceinline__ double god_d64(double cur, double mx, double my, doub
   return god_diff_aniso<double>(cur, mx, my, mz, dd, ww, W);
 }
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// Buffer-op cache policy bit sc1 (gfx940+): the access is coherent at agent
+// scope (all XCDs); 64-bit aligned accesses are single-copy atomic.
+#define PDFSK_SC1 16
+
 // ---------------------------------------------------------------------------
 // Relaxed agent-scope 64-bit atomics: single-copy atomic, coherent across the
 // XCDs, no L2 writeback or invalidate.
@@ -206,9 +234,9 @@ __device__ __forceinline__ void st_edge(uint64_t *p, float v, uint32_t epoch) {
 // Bounded wait for an edge entry of the current epoch (2 s of the 100 MHz
 // constant clock). On timeout the error flag is raised and the stale value
 // used, so every wave still exits.
-__device__ __noinline__ float wait_edge(const uint64_t *p, uint64_t e, uint32_t epoch,
-                                        unsigned *err) {
+__device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsigned *err) {
   uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  uint64_t e = ld_edge(p);
   while ((uint32_t)(e >> 32) != epoch) {
     __builtin_amdgcn_s_sleep(1);
     e = ld_edge(p);
@@ -223,35 +251,72 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint64_t e, uint32_t 
 
 // ---------------------------------------------------------------------------
 // Column-pipeline sweep, FP32, one ordering.
+//
+// Virtual coordinates make every ordering (+,+,+): virtual v in [1, dim]
+// maps to physical flip ? dim - v : v - 1. Workgroup = one TX x TY column
+// tile; thread (tx, ty) updates virtual z = l + 1 - tx - ty at step l, so a
+// step sweeps one i+j+k hyperplane of the tile. The LDS ring holds R =
+// TX+TY+2 z-planes of the tile plus its east column and north row halos
+// (old values, loaded with the plane); plane l+2 enters the ring at step l
+// and plane l-(TX+TY-2) leaves it (coalesced store) at step l. West and
+// south halos are the upwind columns' edge values, read from epoch-tagged
+// 64-bit entries U steps ahead. All field accesses are buffer loads/stores
+// with a range-checked descriptor: lanes outside the field or outside
+// [1, Z] read 0 / drop their store, so the step has no lane branches.
 // ---------------------------------------------------------------------------
 struct ColArgs {
   float *f;
   int X, Y, Z;
-  int fx, fy, fz;  // physical = flip ? dim - v : v - 1 (v = virtual 1-based)
+  int fx, fy, fz;
   int nTx, nTy;
   const int *order;
   unsigned *ticket;
   unsigned ticket_base;
-  uint64_t *edge_x, *edge_y;
+  uint64_t *edge_x, *edge_y;  // allocation bases (entry 0 = index -PDFSK_EDGE_PAD)
+  uint32_t edge_x_bytes, edge_y_bytes;  // allocation sizes incl. both paddings
   uint32_t epoch;
   unsigned *changed;
   unsigned *err;
   pdfs_sp32 sp;
 };
 
+template <int TX, int TY>
+struct ColGeom {
+  static constexpr int NT = TX * TY;
+  static constexpr int R = TX + TY + 2;     // ring depth (z-planes)
+  static constexpr int PW = TX + 1;         // row pitch: x = 0..TX-1, east halo at TX
+  static constexpr int PA = PW * (TY + 1);  // with the north halo row at y = TY
+  static constexpr int PS = PA + ((2 - PA % 32) + 32) % 32;  // plane stride = 2 mod 32 banks
+  static constexpr int LEAD = PW + 1;       // guard so tx-1 / ty-1 of slot 0 stay in range
+  static constexpr int DUMMY = LEAD + R * PS;
+  static constexpr int LDS_FLOATS = DUMMY + NT;
+};
+
 template <int TX, int TY, int U, bool ISO>
 __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
-  constexpr int NT = TX * TY;
-  constexpr int R = TX + TY + 2;  // ring depth (z-planes)
-  __shared__ float ring[R * NT];
+  using G = ColGeom<TX, TY>;
+  constexpr int NT = G::NT, R = G::R, PW = G::PW, PS = G::PS;
+  __shared__ float lds[G::LDS_FLOATS];
   __shared__ int s_col;
   __shared__ unsigned s_changed;
+  float *const ring = lds + G::LEAD;
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int X = a.X, Y = a.Y, Z = a.Z;
-  const long XY = (long)X * Y;
   const int ncols = a.nTx * a.nTy;
+  const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
+  const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
+  const __amdgpu_buffer_rsrc_t fr =
+      __builtin_amdgcn_make_buffer_rsrc(a.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
+  const float INF = __builtin_inff();
+  const uint32_t epoch = a.epoch;
+  // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
+  auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
+  auto xyb = [&](int vx, int vy) -> uint32_t {
+    const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
+    return (uint32_t)(py * X + px) * 4u;
+  };
 
   for (;;) {
     if (tid == 0) {
@@ -263,101 +328,174 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
     if ((unsigned)col >= (unsigned)ncols) break;
     const int ca = a.order[2 * col], cb = a.order[2 * col + 1];
     const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
-    const bool inx = vx <= X, iny = vy <= Y, active = inx && iny;
-    const int px = a.fx ? X - vx : vx - 1;
-    const int py = a.fy ? Y - vy : vy - 1;
-    const long poff = (long)py * X + px;
-    const int sxp = a.fx ? -1 : 1;             // physical step of virtual x+1
-    const long syp = a.fy ? -(long)X : (long)X; // physical step of virtual y+1
-    const bool east_ok = (tx == TX - 1) && iny && (vx + 1 <= X);
-    const bool north_ok = (ty == TY - 1) && inx && (vy + 1 <= Y);
-    const bool west_edge = (tx == 0) && active && ca > 0;
-    const bool south_edge = (ty == 0) && active && cb > 0;
-    const uint64_t *ew = a.edge_x + ((long)((ca - 1) * a.nTy + cb) * Z) * TY + ty;
-    const uint64_t *es = a.edge_y + ((long)(ca * a.nTy + cb - 1) * Z) * TX + tx;
-    uint64_t *mx_out = a.edge_x + ((long)(ca * a.nTy + cb) * Z) * TY + ty;
-    uint64_t *my_out = a.edge_y + ((long)(ca * a.nTy + cb) * Z) * TX + tx;
-    const bool pub_x = (tx == TX - 1) && active && ca + 1 < a.nTx;
-    const bool pub_y = (ty == TY - 1) && active && cb + 1 < a.nTy;
-    auto pz = [&](int vz) -> long { return (long)(a.fz ? Z - vz : vz - 1); };
-
-    // Queues of in-flight loads, U steps deep (fully unrolled indices).
-    float qP[U], qE[U], qN[U];
-    uint64_t qW[U], qS[U];
+    const bool dom = vx <= X && vy <= Y;
     const int skew = tx + ty;
+    const int lo = ty * PW + tx;  // ring cell of this thread
 
-    auto load_plane = [&](int vz) -> float {
-      return (active && vz >= 1 && vz <= Z) ? a.f[pz(vz) * XY + poff] : PDFS_INF;
-    };
-    auto issue = [&](int j, int l) {
-      // plane needed at step l: z = l + 2; halos needed at step l: z = l + 1 - skew
-      qP[j] = load_plane(l + 2);
-      const int vz = l + 1 - skew;
-      const bool zok = vz >= 1 && vz <= Z;
-      qE[j] = (east_ok && zok) ? a.f[pz(vz) * XY + poff + sxp] : PDFS_INF;
-      qN[j] = (north_ok && zok) ? a.f[pz(vz) * XY + poff + syp] : PDFS_INF;
-      qW[j] = (west_edge && zok) ? ld_edge(ew + (long)(vz - 1) * TY) : 0ull;
-      qS[j] = (south_edge && zok) ? ld_edge(es + (long)(vz - 1) * TX) : 0ull;
-    };
+    // interior plane loads / stores
+    const uint32_t lxy = dom ? xyb(vx, vy) : 0x80000000u;
+    uint32_t ld = zb(2) + lxy;                  // next plane to load: 2
+    uint32_t st = dom ? zb(1) + lxy : 0xFFFFFFF0u;
+    const uint32_t st_inc = dom ? dZ : 0u;
+    // halo loaders: lanes [0, TY) east column (row tid), [TY, TY+TX) north row;
+    // the LDS cell of slot s is lds[hcell + s * hmul] (other lanes: a fixed
+    // dummy cell, hmul = 0)
+    int hcell = G::DUMMY + tid;
+    int hmul = 0;
+    bool hval = false;
+    uint32_t hxy = 0x80000000u;
+    if (tid < TY) {
+      const int hx = ca * TX + TX + 1, hy = cb * TY + tid + 1;
+      hcell = G::LEAD + tid * PW + TX;
+      hmul = PS;
+      hval = hx <= X && hy <= Y;
+      if (hval) hxy = xyb(hx, hy);
+    } else if (tid < TY + TX) {
+      const int q = tid - TY;
+      const int hx = ca * TX + q + 1, hy = cb * TY + TY + 1;
+      hcell = G::LEAD + TY * PW + q;
+      hmul = PS;
+      hval = hx <= X && hy <= Y;
+      if (hval) hxy = xyb(hx, hy);
+    }
+    uint32_t hl = zb(2) + hxy;
+    // west / south edge entries (upwind columns), byte offsets from the
+    // allocation base; entry of step l is (vz - 1) with vz = l + 1 - skew
+    const bool useW = tx == 0, useS = ty == 0;
+    const bool hasW = ca > 0, hasS = cb > 0;
+    constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
+    uint32_t wo = (useW && hasW)
+                      ? PADB + (uint32_t)((((ca - 1) * a.nTy + cb) * Z - skew) * TY + ty) * 8u
+                      : 0u;
+    uint32_t so = (useS && hasS)
+                      ? PADB + (uint32_t)(((ca * a.nTy + cb - 1) * Z - skew) * TX + tx) * 8u
+                      : 0u;
+    const uint32_t wo_inc = (useW && hasW) ? TY * 8u : 0u, so_inc = (useS && hasS) ? TX * 8u : 0u;
+    // own east / north edge entries (published for the downstream columns)
+    const bool pubE = tx == TX - 1, pubN = ty == TY - 1;
+    uint32_t pe = PADB + (uint32_t)(((ca * a.nTy + cb) * Z - skew) * TY + ty) * 8u;
+    uint32_t pn = PADB + (uint32_t)(((ca * a.nTy + cb) * Z - skew) * TX + tx) * 8u;
+    const __amdgpu_buffer_rsrc_t rx =
+        __builtin_amdgcn_make_buffer_rsrc(a.edge_x, (short)0, (int)a.edge_x_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ry =
+        __builtin_amdgcn_make_buffer_rsrc(a.edge_y, (short)0, (int)a.edge_y_bytes, 0x00020000);
 
-    // Prologue: ghost plane 0 and plane 1 in the ring; queue steps 0..U-1.
-    ring[0 * NT + tid] = PDFS_INF;
-    ring[1 * NT + tid] = load_plane(1);
+    // prologue: plane 0 (ghost) and plane 1 into the ring
+    ring[lo] = INF;
+    lds[hcell] = INF;
+    {
+      const float v1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + lxy, 0, 0));
+      const float h1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + hxy, 0, 0));
+      ring[PS + lo] = dom ? v1 : INF;
+      lds[hcell + hmul] = hval ? h1 : INF;
+    }
+    float qP[U], qH[U];
+    u32x2 qW[U], qS[U];
 #pragma unroll
-    for (int j = 0; j < U; j++) issue(j, j);
+    for (int j = 0; j < U; j++) {
+      qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
+      qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+      ld += dZ;
+      hl += dZ;
+      qW[j] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
+      qS[j] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
+      wo += wo_inc;
+      so += so_inc;
+    }
+    __syncthreads();
+    // slot byte offset of this thread's current plane vz = 1 - skew
+    constexpr uint32_t PS4 = PS * 4, RPS4 = R * PS4;
+    uint32_t sv = (uint32_t)(((1 - skew) % R + R) % R) * PS4;
+    float c = ring[sv / 4 + lo];
+    float zm = INF;
+    unsigned nchg = 0;
+    const int NL = Z + TX + TY - 2;  // steps 0..NL; the last one only stores
+    int slot_w = 2, slot_s = (R - (TX + TY - 2) % R) % R;
 
-    unsigned my_changed = 0;
-    const int NL = Z + TX + TY - 2;  // steps 0..NL (last step only stores)
-    int slot_l2 = 2;                 // (l + 2) % R
-    for (int l0 = 0; l0 <= NL; l0 += U) {
+    // U steps; the first block is peeled so that in the steady-state loop
+    // every queued load is U steps old on entry as on the back edge (the
+    // compiler's wait counts are then exact instead of draining each step)
+    auto block = [&](const int l0, const bool guard) __attribute__((always_inline)) {
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int l = l0 + j;
-        if (l <= NL) {
-          // (1) plane l+2 into the ring, then refill this queue entry
-          ring[slot_l2 * NT + tid] = qP[j];
-          const float hE = qE[j], hN = qN[j];
-          const uint64_t eW = qW[j], eS = qS[j];
-          issue(j, l + U);
+        if (!guard || l <= NL) {
+          // (1) plane l+2 (+ halos) into the ring, then refill the queues
+          {
+            const bool live = l + 2 <= Z;
+            ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
+            lds[hcell + slot_w * hmul] = (hval && live) ? qH[j] : INF;
+          }
+          qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
+          qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+          ld += dZ;
+          hl += dZ;
           __syncthreads();
-          // (2) store the finished plane zs
-          const int zs = l - (TX + TY - 2);
-          if (zs >= 1 && active) {
-            int s = zs % R;
-            a.f[pz(zs) * XY + poff] = ring[s * NT + tid];
+          // (2) store the finished plane zs = l - (TX+TY-2); before the first
+          // finished plane the store goes out of range (dropped), so every
+          // step issues the same memory operations
+          {
+            const bool sto = l >= TX + TY - 1;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ring[slot_s * PS + lo]), fr,
+                                                  sto ? st : 0xFFFFFFF0u, 0, 0);
+            st += sto ? st_inc : 0u;
           }
-          // (3) the hyperplane point of this thread
+          // (3) this thread's hyperplane point
+          uint32_t svp = sv + PS4;
+          svp = svp >= RPS4 ? svp - RPS4 : svp;
+          const float *pc = ring + sv / 4 + lo;
+          const float zp = ring[svp / 4 + lo];
+          float xm = pc[-1], xp = pc[1], ym = pc[-PW], yp = pc[PW];
           const int vz = l + 1 - skew;
-          if (active && vz >= 1 && vz <= Z) {
-            const int sc = vz % R;
-            const int sm = sc == 0 ? R - 1 : sc - 1;
-            const int sp = sc == R - 1 ? 0 : sc + 1;
-            const float c = ring[sc * NT + tid];
-            const float zm = ring[sm * NT + tid];
-            const float zp = ring[sp * NT + tid];
-            float xm, xp, ym, yp;
-            if (tx > 0) xm = ring[sc * NT + tid - 1];
-            else xm = (ca > 0) ? wait_edge(ew + (long)(vz - 1) * TY, eW, a.epoch, a.err)
-                               : PDFS_INF;
-            xp = (tx < TX - 1) ? ring[sc * NT + tid + 1] : hE;
-            if (ty > 0) ym = ring[sc * NT + tid - TX];
-            else ym = (cb > 0) ? wait_edge(es + (long)(vz - 1) * TX, eS, a.epoch, a.err)
-                               : PDFS_INF;
-            yp = (ty < TY - 1) ? ring[sc * NT + tid + TX] : hN;
-            const float mxv = xm < xp ? xm : xp;
-            const float myv = ym < yp ? ym : yp;
-            const float mzv = zm < zp ? zm : zp;
-            const float u = god32<ISO>(c, mxv, myv, mzv, a.sp);
-            my_changed += (u != c);
-            ring[sc * NT + tid] = u;
-            if (pub_x) st_edge(mx_out + (long)(vz - 1) * TY, u, a.epoch);
-            if (pub_y) st_edge(my_out + (long)(vz - 1) * TX, u, a.epoch);
+          const bool act = dom && (unsigned)(vz - 1) < (unsigned)Z;
+          // edge entries of this step (issued U steps ago; wo/so point U ahead)
+          if (useW) {
+            float w = __uint_as_float(qW[j].x);
+            if (hasW && act && qW[j].y != epoch)
+              w = wait_edge(a.edge_x + (wo - U * wo_inc) / 8u, epoch, a.err);
+            xm = hasW ? w : INF;
           }
-          slot_l2 = slot_l2 == R - 1 ? 0 : slot_l2 + 1;
+          if (useS) {
+            float v = __uint_as_float(qS[j].x);
+            if (hasS && act && qS[j].y != epoch)
+              v = wait_edge(a.edge_y + (so - U * so_inc) / 8u, epoch, a.err);
+            ym = hasS ? v : INF;
+          }
+          const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
+          nchg += (act && u != c) ? 1u : 0u;
+          // result into the ring (inactive lanes: own dummy cell) and the
+          // published edges (other lanes: out of range, dropped)
+          ring[act ? sv / 4 + lo : G::DUMMY - G::LEAD + tid] = u;
+          {
+            const u32x2 e = {__float_as_uint(u), epoch};
+            __builtin_amdgcn_raw_buffer_store_b64(e, rx, (act && pubE) ? pe : 0xFFFFFFF0u, 0,
+                                                  PDFSK_SC1);
+            __builtin_amdgcn_raw_buffer_store_b64(e, ry, (act && pubN) ? pn : 0xFFFFFFF0u, 0,
+                                                  PDFSK_SC1);
+          }
+          pe += TY * 8u;
+          pn += TX * 8u;
+          // refill this step's edge queue entries with step l + U
+          qW[j] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
+          qS[j] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
+          wo += wo_inc;
+          so += so_inc;
+          zm = act ? u : INF;
+          c = zp;
+          sv = svp;
+          slot_w = slot_w == R - 1 ? 0 : slot_w + 1;
+          slot_s = slot_s == R - 1 ? 0 : slot_s + 1;
         }
       }
-    }
-    if (my_changed) atomicAdd(&s_changed, my_changed);
+    };
+    // peeled first block, unguarded full blocks, guarded tail: the steady
+    // state issues identical memory operations every step
+    static_assert(TX + TY - 2 >= U - 1, "first block must be full (NL >= U - 1)");
+    block(0, false);
+    int l0 = U;
+    for (; l0 + U - 1 <= NL; l0 += U) block(l0, false);
+    if (l0 <= NL) block(l0, true);
+    if (nchg) atomicAdd(&s_changed, nchg);
     __syncthreads();
     if (tid == 0 && s_changed) atomicAdd(a.changed, s_changed);
   }
@@ -581,7 +719,7 @@ extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
   }
 }
 
-#define PDFSK_U 4
+#define PDFSK_U 8
 
 template <int TX, int TY, bool ISO>
 static int max_resident() {
@@ -637,8 +775,10 @@ extern "C" int pdfsk_sweep_col_f32(float *field, int X, int Y, int Z, int sn,
   a.order = s->order;
   a.ticket = s->ticket;
   a.ticket_base = ticket_base;
-  a.edge_x = s->edge_x;
-  a.edge_y = s->edge_y;
+  a.edge_x = s->edge_x - PDFSK_EDGE_PAD;
+  a.edge_y = s->edge_y - PDFSK_EDGE_PAD;
+  a.edge_x_bytes = (uint32_t)((s->edge_x_n + 2 * PDFSK_EDGE_PAD) * 8);
+  a.edge_y_bytes = (uint32_t)((s->edge_y_n + 2 * PDFSK_EDGE_PAD) * 8);
   a.epoch = epoch;
   a.changed = s->changed;
   a.err = s->err;

@@ -16,6 +16,7 @@ struct pdfs_ctx {
   int dev;
   hipStream_t st;
   pdfsk_col_scratch col;
+  uint64_t *edge_x_base, *edge_y_base;
   size_t cap_ex, cap_ey, cap_order; /* elements */
   uint32_t epoch;
   unsigned ticket_base;
@@ -110,8 +111,8 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
 {
   if (!c) return;
   hipStreamSynchronize(c->st);
-  hipFree(c->col.edge_x);
-  hipFree(c->col.edge_y);
+  hipFree(c->edge_x_base);
+  hipFree(c->edge_y_base);
   hipFree(c->col.order);
   hipFree(c->col.ticket);
   hipFree(c->d_u64);
@@ -155,23 +156,32 @@ static int default_tile(int x, int y)
  * upwind-first ticket order (anti-diagonals a + b ascending). */
 static int ensure_col(pdfs_ctx *c, int x, int y, int z, int tile, int *ncols_out)
 {
+  if ((long long)x * y * z > PDFSK_MAX_POINTS)
+    return fail(PDFS_EINVAL, "field of %lld points exceeds the FP32 sweep limit %lld",
+                (long long)x * y * z, (long long)PDFSK_MAX_POINTS);
   int TX, TY;
   pdfsk_tile_dims(tile, &TX, &TY);
   int nTx = (x + TX - 1) / TX, nTy = (y + TY - 1) / TY;
   size_t ncols = (size_t)nTx * nTy;
   size_t nex = ncols * (size_t)z * TY, ney = ncols * (size_t)z * TX;
   if (nex > c->cap_ex) {
-    hipFree(c->col.edge_x);
-    c->col.edge_x = NULL;
-    HIPCHK(hipMalloc((void **)&c->col.edge_x, nex * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->col.edge_x, 0, nex * sizeof(uint64_t), c->st));
+    size_t n = nex + 2 * PDFSK_EDGE_PAD;
+    hipFree(c->edge_x_base);
+    c->edge_x_base = c->col.edge_x = NULL;
+    HIPCHK(hipMalloc((void **)&c->edge_x_base, n * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->edge_x_base, 0, n * sizeof(uint64_t), c->st));
+    c->col.edge_x = c->edge_x_base + PDFSK_EDGE_PAD;
+    c->col.edge_x_n = nex;
     c->cap_ex = nex;
   }
   if (ney > c->cap_ey) {
-    hipFree(c->col.edge_y);
-    c->col.edge_y = NULL;
-    HIPCHK(hipMalloc((void **)&c->col.edge_y, ney * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->col.edge_y, 0, ney * sizeof(uint64_t), c->st));
+    size_t n = ney + 2 * PDFSK_EDGE_PAD;
+    hipFree(c->edge_y_base);
+    c->edge_y_base = c->col.edge_y = NULL;
+    HIPCHK(hipMalloc((void **)&c->edge_y_base, n * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->edge_y_base, 0, n * sizeof(uint64_t), c->st));
+    c->col.edge_y = c->edge_y_base + PDFSK_EDGE_PAD;
+    c->col.edge_y_n = ney;
     c->cap_ey = ney;
   }
   if (2 * ncols > c->cap_order) {

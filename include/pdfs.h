@@ -122,6 +122,16 @@ int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z,
 int pdfs_pack_dev(pdfs_ctx *c, const double *padded, float *dense, int x, int y, int z);
 int pdfs_unpack_dev(pdfs_ctx *c, const float *dense, double *padded, int x, int y, int z);
 
+/* CLI pipeline of single_gpu_cuda/src/phi3D.c:107-126 (calc_dist_field) on
+ * the device: update_distance (0/NaN -> 90000), `passes` sweep passes
+ * (reference: 1; <= 0: until converged), location 1 -> -1, ghosts <- nearest
+ * interior. precision: PDFS_F64_REF (reference arithmetic) or PDFS_F32. */
+int pdfs_calc_dist_field(Phi *pf, int passes, int precision);
+/* One pre/post step on host padded arrays: op 0 update_distance
+ * (phi3D.c:165-180), 1 set_distance_negative_inside (phi3D.c:190-205),
+ * 2 adjust_boundary (phi3D.c:221-242). */
+int pdfs_prepost(int op, const int *location, double *distance, int x, int y, int z);
+
 #ifdef __cplusplus
 }
 #endif

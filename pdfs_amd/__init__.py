@@ -26,8 +26,10 @@ EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
     "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
     "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_pack_dev",
-    "pdfs_unpack_dev",
+    "pdfs_unpack_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
+    "pdfs_vti_write", "pdfs_vti_free",
 ]
+CLI_PATH = os.path.join(HERE, "bin", "PDFS")
 
 
 class Phi(C.Structure):
@@ -107,6 +109,10 @@ def lib():
         L.pdfs_pack_dev.restype = I
         L.pdfs_unpack_dev.argtypes = [vp, vp, vp, I, I, I]
         L.pdfs_unpack_dev.restype = I
+        L.pdfs_calc_dist_field.argtypes = [C.POINTER(Phi), I, I]
+        L.pdfs_calc_dist_field.restype = I
+        L.pdfs_prepost.argtypes = [I, vp, vp, I, I, I]
+        L.pdfs_prepost.restype = I
         _lib = L
     return _lib
 
@@ -146,6 +152,27 @@ def runFSM(distance, sn, h=(1.0, 1.0, 1.0)):
     X, Y, Z = _dims(distance)
     pf = PhiACC(X, Y, Z, h[0], h[1], h[2], distance.ctypes.data_as(C.POINTER(C.c_double)), 1.0)
     lib().runFSM(C.byref(pf), sn)
+    return distance
+
+
+def calc_dist_field(location, distance, h=(1.0, 1.0, 1.0), passes=1, precision=PDFS_F64_REF):
+    """single_gpu_cuda/src/phi3D.c:107-126 on padded host arrays (in place):
+    update_distance, `passes` passes (<= 0: until converged), location 1 ->
+    -1, ghosts <- nearest interior."""
+    assert distance.dtype == np.float64 and location.dtype == np.int32
+    X, Y, Z = _dims(distance)
+    pf = Phi(X, Y, Z, h[0], h[1], h[2], location.ctypes.data_as(C.POINTER(C.c_int)),
+             distance.ctypes.data_as(C.POINTER(C.c_double)), 1.0)
+    _check(lib().pdfs_calc_dist_field(C.byref(pf), passes, precision))
+    return distance
+
+
+def prepost(op, location, distance):
+    """One CLI pre/post step on padded host arrays (in place): 0
+    update_distance, 1 set_distance_negative_inside, 2 adjust_boundary."""
+    X, Y, Z = _dims(distance)
+    _check(lib().pdfs_prepost(op, None if location is None else location.ctypes.data,
+                              distance.ctypes.data, X, Y, Z))
     return distance
 
 

@@ -88,6 +88,11 @@ int pdfsk_gen_spheres(float *dense, double *padded, int X, int Y, int Z,
                       double h, const double *dev_params, int nspheres,
                       hipStream_t st);
 
+/* CLI pre/post processing on device arrays (padded FP64 + int location). */
+int pdfsk_update_distance(const int *loc, double *d, long n, hipStream_t st);
+int pdfsk_set_negative_inside(const int *loc, double *d, long n, hipStream_t st);
+int pdfsk_adjust_boundary(double *d, int X, int Y, int Z, hipStream_t st);
+
 #ifdef __cplusplus
 }
 #endif

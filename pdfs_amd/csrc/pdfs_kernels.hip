@@ -702,6 +702,47 @@ __global__ void k_gen_spheres(float *dn, double *pd, int X, int Y, int Z, double
 }
 
 // ---------------------------------------------------------------------------
+// CLI pre/post processing on the padded FP64 field and int location array.
+// ---------------------------------------------------------------------------
+// single_gpu_cuda/src/phi3D.c:165-180: non-ghost values that are not +inf are
+// kept when nonzero, and 0 or NaN becomes 90000 (the location==1 && +inf
+// branch of the reference is unreachable under its outer test).
+__global__ void k_update_distance(const int *__restrict__ loc, double *__restrict__ d, long n) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    const double v = d[p];
+    if (loc[p] != -1 && v != __builtin_inf()) d[p] = (v > 0.0 || v < 0.0) ? v : 90000.0;
+  }
+}
+
+// single_gpu_cuda/src/phi3D.c:190-205: location 1 (inside) becomes -1.
+__global__ void k_set_negative_inside(const int *__restrict__ loc, double *__restrict__ d,
+                                      long n) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    if (loc[p] != -1 && d[p] != __builtin_inf() && loc[p] == 1) d[p] = -1.0;
+  }
+}
+
+// single_gpu_cuda/src/phi3D.c:221-242: every ghost cell takes the value of
+// the interior cell reached by clamping each index into [1, dim]. Reads
+// only interior cells and writes only ghost cells, so one pass is exact.
+__global__ void k_adjust_boundary(double *d, int X, int Y, int Z) {
+  const int x = X + 2, y = Y + 2, z = Z + 2;
+  const long n = (long)x * y * z;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(p % x);
+    const long q = p / x;
+    const int j = (int)(q % y), i = (int)(q / y);
+    const int I = i == z - 1 ? i - 1 : (i == 0 ? i + 1 : i);
+    const int J = j == y - 1 ? j - 1 : (j == 0 ? j + 1 : j);
+    const int K = k == x - 1 ? k - 1 : (k == 0 ? k + 1 : k);
+    if (i != I || j != J || k != K) d[p] = d[((long)I * y + J) * x + K];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI launchers
 // ---------------------------------------------------------------------------
 static inline int hip_rc(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
@@ -879,5 +920,21 @@ extern "C" int pdfsk_gen_spheres(float *dense, double *padded, int X, int Y, int
   if (gen_ghost(padded, X, Y, Z, st)) return -1;
   hipLaunchKernelGGL(k_gen_spheres, dim3(grid_for((long)X * Y * Z, 256)), dim3(256), 0, st,
                      dense, padded, X, Y, Z, h, prm, ns);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_update_distance(const int *loc, double *d, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_update_distance, dim3(grid_for(n, 256)), dim3(256), 0, st, loc, d, n);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_set_negative_inside(const int *loc, double *d, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_negative_inside, dim3(grid_for(n, 256)), dim3(256), 0, st, loc, d, n);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_adjust_boundary(double *d, int X, int Y, int Z, hipStream_t st) {
+  hipLaunchKernelGGL(k_adjust_boundary, dim3(grid_for((long)(X + 2) * (Y + 2) * (Z + 2), 256)),
+                     dim3(256), 0, st, d, X, Y, Z);
   return hip_rc(hipGetLastError());
 }

@@ -565,3 +565,98 @@ int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z,
   HIPCHK(hipStreamSynchronize(c->st));
   return PDFS_OK;
 }
+
+/* ---- CLI pipeline: single_gpu_cuda/src/phi3D.c:107-126 (calc_dist_field) ----
+ * update_distance -> `passes` passes of the sweeps -> set_distance_negative_
+ * inside -> adjust_boundary, all on the device. */
+int pdfs_calc_dist_field(Phi *pf, int passes, int precision)
+{
+  if (!pf || !pf->distance || !pf->location || pf->x < 1 || pf->y < 1 || pf->z < 1)
+    return fail(PDFS_EINVAL, "pdfs_calc_dist_field: bad arguments");
+  pdfs_ctx *c = default_ctx();
+  if (!c) return fail(PDFS_EHIP, "pdfs_calc_dist_field: no HIP device");
+  const int x = pf->x, y = pf->y, z = pf->z;
+  size_t np = (size_t)(x + 2) * (y + 2) * (z + 2), n = (size_t)x * y * z;
+  double *d = NULL;
+  int *loc = NULL;
+  float *f = NULL;
+  int rc = PDFS_OK;
+  if (hipMalloc((void **)&d, np * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&loc, np * sizeof(int)) != hipSuccess) {
+    rc = fail(PDFS_ENOMEM, "pdfs_calc_dist_field: device alloc");
+    goto out;
+  }
+  if (hipMemcpy(d, pf->distance, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(loc, pf->location, np * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail(PDFS_EHIP, "pdfs_calc_dist_field: H2D copy");
+    goto out;
+  }
+  if (pdfsk_update_distance(loc, d, (long)np, c->st)) {
+    rc = fail(PDFS_EHIP, "update_distance launch");
+    goto out;
+  }
+  if (precision == PDFS_F64_REF) {
+    if ((rc = pdfs_solve_ref64_dev(c, d, x, y, z, pf->dx, pf->dy, pf->dz, passes, 1e-12, NULL)))
+      goto out;
+  } else {
+    pdfs_opts o;
+    pdfs_default_opts(&o);
+    o.fixed_passes = passes > 0 ? passes : 0;
+    o.judge = 0;
+    if (hipMalloc((void **)&f, n * sizeof(float)) != hipSuccess) {
+      rc = fail(PDFS_ENOMEM, "pdfs_calc_dist_field: device alloc");
+      goto out;
+    }
+    if ((rc = pdfs_pack_dev(c, d, f, x, y, z))) goto out;
+    rc = pdfs_solve_f32_dev(c, f, NULL, x, y, z, pf->dx, pf->dy, pf->dz, &o, NULL);
+    if (rc && rc != PDFS_ENOTCONV) goto out;
+    if ((rc = pdfs_unpack_dev(c, f, d, x, y, z))) goto out;
+  }
+  if (pdfsk_set_negative_inside(loc, d, (long)np, c->st) ||
+      pdfsk_adjust_boundary(d, x, y, z, c->st)) {
+    rc = fail(PDFS_EHIP, "post-processing launch");
+    goto out;
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess ||
+      hipMemcpy(pf->distance, d, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(PDFS_EHIP, "pdfs_calc_dist_field: D2H copy");
+out:
+  hipFree(d);
+  hipFree(loc);
+  hipFree(f);
+  return rc;
+}
+
+/* Single pre/post steps on host padded arrays (used by tests and callers
+ * that keep their own pipeline). op: 0 update_distance, 1 set_negative_
+ * inside, 2 adjust_boundary (loc unused). */
+int pdfs_prepost(int op, const int *location, double *distance, int x, int y, int z)
+{
+  pdfs_ctx *c = default_ctx();
+  if (!c || !distance || (op != 2 && !location) || op < 0 || op > 2)
+    return fail(PDFS_EINVAL, "pdfs_prepost: bad arguments");
+  size_t np = (size_t)(x + 2) * (y + 2) * (z + 2);
+  double *d = NULL;
+  int *loc = NULL;
+  int rc = PDFS_OK;
+  if (hipMalloc((void **)&d, np * sizeof(double)) != hipSuccess ||
+      (op != 2 && hipMalloc((void **)&loc, np * sizeof(int)) != hipSuccess)) {
+    rc = fail(PDFS_ENOMEM, "pdfs_prepost: device alloc");
+    goto out;
+  }
+  if (hipMemcpy(d, distance, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      (loc && hipMemcpy(loc, location, np * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+    rc = fail(PDFS_EHIP, "pdfs_prepost: H2D copy");
+    goto out;
+  }
+  int k = op == 0   ? pdfsk_update_distance(loc, d, (long)np, c->st)
+          : op == 1 ? pdfsk_set_negative_inside(loc, d, (long)np, c->st)
+                    : pdfsk_adjust_boundary(d, x, y, z, c->st);
+  if (k || hipStreamSynchronize(c->st) != hipSuccess ||
+      hipMemcpy(distance, d, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(PDFS_EHIP, "pdfs_prepost: kernel or copy");
+out:
+  hipFree(d);
+  hipFree(loc);
+  return rc;
+}

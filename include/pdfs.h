@@ -71,6 +71,7 @@ typedef struct {
   int tile;        /* column tile variant, -1 = autotuned default */
   int quiet;       /* no stdout */
   int profile;     /* time every sweep launch with events on the solver stream */
+  int graph;       /* replay each pass (8 persistent launches) from a captured hipGraph */
 } pdfs_opts;
 
 typedef struct {
@@ -100,6 +101,15 @@ const char *pdfs_last_error(void);
 int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init_or_null, int x, int y,
                        int z, double dx, double dy, double dz, const pdfs_opts *o,
                        pdfs_stats *st);
+/* Batch of independent dense FP32 fields (uneven sizes), one persistent
+ * launch per ordering covering every field still changing; each field stops
+ * when a pass changes none of its values. dims: 3 ints per field; inits
+ * (or NULL) for the judge; field_passes / field_ms (or NULL) receive each
+ * field's pass count and completion time since the batch start. */
+int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nfields, float *const *fields,
+                             const float *const *inits, const int *dims, double dx, double dy,
+                             double dz, const pdfs_opts *o, pdfs_stats *st, int *field_passes,
+                             double *field_ms);
 /* One ordering over a dense FP32 field; *changed += points changed. */
 int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx, double dy,
                        double dz, int sn, int tile, long long *changed);

@@ -26,7 +26,7 @@ EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
     "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
     "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_pack_dev",
-    "pdfs_unpack_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
+    "pdfs_unpack_dev", "pdfs_solve_batch_f32_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
     "pdfs_vti_write", "pdfs_vti_free",
 ]
 CLI_PATH = os.path.join(HERE, "bin", "PDFS")
@@ -50,7 +50,7 @@ class PhiACC(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_passes", C.c_int), ("fixed_passes", C.c_int),
                 ("tol", C.c_double), ("judge", C.c_int), ("tile", C.c_int),
-                ("quiet", C.c_int), ("profile", C.c_int)]
+                ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -109,6 +109,9 @@ def lib():
         L.pdfs_pack_dev.restype = I
         L.pdfs_unpack_dev.argtypes = [vp, vp, vp, I, I, I]
         L.pdfs_unpack_dev.restype = I
+        L.pdfs_solve_batch_f32_dev.argtypes = [vp, I, vp, vp, vp, D, D, D, C.POINTER(Opts),
+                                               C.POINTER(Stats), vp, vp]
+        L.pdfs_solve_batch_f32_dev.restype = I
         L.pdfs_calc_dist_field.argtypes = [C.POINTER(Phi), I, I]
         L.pdfs_calc_dist_field.restype = I
         L.pdfs_prepost.argtypes = [I, vp, vp, I, I, I]
@@ -226,6 +229,30 @@ class Context:
                                       C.byref(o), C.byref(st))
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
         return st.as_dict()
+
+    def solve_batch_f32(self, fields, inits=None, h=(1.0, 1.0, 1.0), allow_notconv=False,
+                        **kw):
+        """fields: list of dense float32 cuda tensors [Z, Y, X] (uneven
+        sizes), solved in place in shared launches. Returns (stats,
+        per-field passes, per-field completion ms)."""
+        import torch
+        n = len(fields)
+        ptrs = (C.c_void_p * n)(*[self._ptr(f) for f in fields])
+        iptrs = (C.c_void_p * n)(*[self._ptr(t) for t in inits]) if inits else None
+        dims = (C.c_int * (3 * n))()
+        for i, f in enumerate(fields):
+            assert f.dtype == torch.float32 and f.dim() == 3
+            Z, Y, X = f.shape
+            dims[3 * i], dims[3 * i + 1], dims[3 * i + 2] = X, Y, Z
+        passes = (C.c_int * n)()
+        ms = (C.c_double * n)()
+        o = default_opts(precision=PDFS_F32, **kw)
+        st = Stats()
+        torch.cuda.synchronize()
+        rc = lib().pdfs_solve_batch_f32_dev(self.c, n, ptrs, iptrs, dims, *h, C.byref(o),
+                                            C.byref(st), passes, ms)
+        _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+        return st.as_dict(), list(passes), list(ms)
 
     def sweep_f32(self, field, sn, h=(1.0, 1.0, 1.0), tile=-1):
         import torch

@@ -32,16 +32,30 @@ typedef struct {
  * and out-of-range planes must map past the buffer end (x*y*z*4 < 2^31). */
 #define PDFSK_MAX_POINTS ((1ll << 31) / 4 - 1)
 
-/* Per-field scratch for the column-pipeline sweep. */
+/* Field table entry of a batched sweep (one per field; a single solve is a
+ * batch of one). Dense FP32 field [Z][Y][X]; its east/north edge blocks
+ * start at entry ex_off / ey_off of the edge allocations. */
 typedef struct {
-  uint64_t *edge_x;    /* [ncols][Z][TY] {value, epoch} east-edge values (base + PAD) */
-  uint64_t *edge_y;    /* [ncols][Z][TX] {value, epoch} north-edge values (base + PAD) */
-  size_t edge_x_n, edge_y_n; /* entries excluding the padding */
-  int *order;          /* [ncols][2] ticket -> column (a, b) */
-  unsigned *ticket;    /* monotonic work counter */
-  unsigned *changed;   /* changed-point counter (accumulates) */
-  unsigned *err;       /* bounded-spin timeout flag */
-} pdfsk_col_scratch;
+  float *f;
+  int X, Y, Z;
+  int nTx, nTy;
+  uint32_t ex_off, ey_off;
+  int pad;
+} pdfsk_field;
+
+/* Device state of the batched column sweep. state: [0] ticket counter,
+ * [1] epoch, [2] ticket base, [3] finished workgroups, [4] column tickets;
+ * advanced on the device by the last workgroup of each launch, so launches
+ * (and graph replays) take no per-launch host arguments. */
+typedef struct {
+  pdfsk_field *fields;   /* [nfields] */
+  int *cols;             /* [ncols][3] ticket -> (field, a, b), upwind-first */
+  unsigned *state;       /* [8] */
+  unsigned *changed;     /* [nfields] changed-point counters (accumulate) */
+  unsigned *err;         /* bounded-spin timeout flag */
+  uint64_t *edge_x, *edge_y;        /* allocation bases (index -PDFSK_EDGE_PAD) */
+  size_t edge_x_bytes, edge_y_bytes;
+} pdfsk_batch;
 
 /* Tile shapes of the column pipeline (autotuner chooses one). */
 enum { PDFSK_TILE_16x16 = 0, PDFSK_TILE_32x8 = 1, PDFSK_TILE_16x8 = 2, PDFSK_TILE_COUNT = 3 };
@@ -50,12 +64,10 @@ void pdfsk_tile_dims(int variant, int *tx, int *ty);
 /* Resident workgroups of the column kernel for a tile variant (0 on error). */
 int pdfsk_col_max_resident(int variant, int iso);
 
-/* One ordering sn (1..8) over the dense FP32 field, in place. Tickets
- * [ticket_base, ticket_base + ncols + grid) are consumed. */
-int pdfsk_sweep_col_f32(float *field, int X, int Y, int Z, int sn,
-                        const pdfs_sp32 *sp, const pdfsk_col_scratch *s,
-                        int variant, uint32_t epoch, unsigned ticket_base,
-                        int grid, hipStream_t st);
+/* One ordering sn (1..8) over every field of the batch, in place; grid =
+ * persistent workgroups (at most the resident count). */
+int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp32 *sp, int variant,
+                          int grid, hipStream_t st);
 
 /* One ordering sn of the reference hyperplane schedule, FP64, on the padded
  * array, one launch per level (reference arithmetic, bitwise parity).

@@ -264,18 +264,31 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
 // with a range-checked descriptor: lanes outside the field or outside
 // [1, Z] read 0 / drop their store, so the step has no lane branches.
 // ---------------------------------------------------------------------------
-struct ColArgs {
+// Field table entry (one per field of a batch; a single solve is a batch
+// of one). Dense FP32 field [Z][Y][X]; edge blocks start at ex_off / ey_off
+// (entries from the edge allocations' index 0, padding included).
+struct FieldDesc {
   float *f;
   int X, Y, Z;
-  int fx, fy, fz;
   int nTx, nTy;
-  const int *order;
-  unsigned *ticket;
-  unsigned ticket_base;
+  uint32_t ex_off, ey_off;
+  int pad;
+};
+
+// Launch state in device memory, advanced by the last workgroup of each
+// launch so that consecutive launches (and graph replays) need no host
+// arguments: [0] ticket counter, [1] epoch, [2] ticket base, [3] finished
+// workgroups, [4] number of column tickets of the batch.
+enum { ST_TICKET = 0, ST_EPOCH = 1, ST_BASE = 2, ST_DONE = 3, ST_NCOLS = 4 };
+
+struct ColArgs {
+  const FieldDesc *fields;
+  const int *cols;  // ticket -> (field, a, b)
+  unsigned *state;
+  int fx, fy, fz;   // the ordering's reflections (same for every field)
   uint64_t *edge_x, *edge_y;  // allocation bases (entry 0 = index -PDFSK_EDGE_PAD)
-  uint32_t edge_x_bytes, edge_y_bytes;  // allocation sizes incl. both paddings
-  uint32_t epoch;
-  unsigned *changed;
+  uint32_t edge_x_bytes, edge_y_bytes;  // allocation sizes incl. paddings
+  unsigned *changed;  // per field
   unsigned *err;
   pdfs_sp32 sp;
 };
@@ -303,30 +316,39 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int X = a.X, Y = a.Y, Z = a.Z;
-  const int ncols = a.nTx * a.nTy;
-  const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
-  const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
-  const __amdgpu_buffer_rsrc_t fr =
-      __builtin_amdgcn_make_buffer_rsrc(a.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
   const float INF = __builtin_inff();
-  const uint32_t epoch = a.epoch;
-  // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
-  auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
-  auto xyb = [&](int vx, int vy) -> uint32_t {
-    const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
-    return (uint32_t)(py * X + px) * 4u;
-  };
+  const unsigned ncols = __hip_atomic_load(a.state + ST_NCOLS, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t epoch = __hip_atomic_load(a.state + ST_EPOCH, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned tbase = __hip_atomic_load(a.state + ST_BASE, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  const __amdgpu_buffer_rsrc_t rx =
+      __builtin_amdgcn_make_buffer_rsrc(a.edge_x, (short)0, (int)a.edge_x_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t ry =
+      __builtin_amdgcn_make_buffer_rsrc(a.edge_y, (short)0, (int)a.edge_y_bytes, 0x00020000);
 
   for (;;) {
     if (tid == 0) {
-      s_col = (int)(atomicAdd(a.ticket, 1u) - a.ticket_base);
+      s_col = (int)(atomicAdd(a.state + ST_TICKET, 1u) - tbase);
       s_changed = 0;
     }
     __syncthreads();
     const int col = s_col;
-    if ((unsigned)col >= (unsigned)ncols) break;
-    const int ca = a.order[2 * col], cb = a.order[2 * col + 1];
+    if ((unsigned)col >= ncols) break;
+    const int fi = a.cols[3 * col], ca = a.cols[3 * col + 1], cb = a.cols[3 * col + 2];
+    const FieldDesc fd = a.fields[fi];
+    const int X = fd.X, Y = fd.Y, Z = fd.Z, nTy = fd.nTy;
+    const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
+    const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
+    const __amdgpu_buffer_rsrc_t fr =
+        __builtin_amdgcn_make_buffer_rsrc(fd.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
+    // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
+    auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
+    auto xyb = [&](int vx, int vy) -> uint32_t {
+      const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
+      return (uint32_t)(py * X + px) * 4u;
+    };
     const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
     const bool dom = vx <= X && vy <= Y;
     const int skew = tx + ty;
@@ -364,21 +386,18 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
     const bool useW = tx == 0, useS = ty == 0;
     const bool hasW = ca > 0, hasS = cb > 0;
     constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
+    const uint32_t exb = PADB + fd.ex_off * 8u, eyb = PADB + fd.ey_off * 8u;
     uint32_t wo = (useW && hasW)
-                      ? PADB + (uint32_t)((((ca - 1) * a.nTy + cb) * Z - skew) * TY + ty) * 8u
+                      ? exb + (uint32_t)((((ca - 1) * nTy + cb) * Z - skew) * TY + ty) * 8u
                       : 0u;
     uint32_t so = (useS && hasS)
-                      ? PADB + (uint32_t)(((ca * a.nTy + cb - 1) * Z - skew) * TX + tx) * 8u
+                      ? eyb + (uint32_t)(((ca * nTy + cb - 1) * Z - skew) * TX + tx) * 8u
                       : 0u;
     const uint32_t wo_inc = (useW && hasW) ? TY * 8u : 0u, so_inc = (useS && hasS) ? TX * 8u : 0u;
     // own east / north edge entries (published for the downstream columns)
     const bool pubE = tx == TX - 1, pubN = ty == TY - 1;
-    uint32_t pe = PADB + (uint32_t)(((ca * a.nTy + cb) * Z - skew) * TY + ty) * 8u;
-    uint32_t pn = PADB + (uint32_t)(((ca * a.nTy + cb) * Z - skew) * TX + tx) * 8u;
-    const __amdgpu_buffer_rsrc_t rx =
-        __builtin_amdgcn_make_buffer_rsrc(a.edge_x, (short)0, (int)a.edge_x_bytes, 0x00020000);
-    const __amdgpu_buffer_rsrc_t ry =
-        __builtin_amdgcn_make_buffer_rsrc(a.edge_y, (short)0, (int)a.edge_y_bytes, 0x00020000);
+    uint32_t pe = exb + (uint32_t)(((ca * nTy + cb) * Z - skew) * TY + ty) * 8u;
+    uint32_t pn = eyb + (uint32_t)(((ca * nTy + cb) * Z - skew) * TX + tx) * 8u;
 
     // prologue: plane 0 (ghost) and plane 1 into the ring
     ring[lo] = INF;
@@ -497,7 +516,13 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
     if (l0 <= NL) block(l0, true);
     if (nchg) atomicAdd(&s_changed, nchg);
     __syncthreads();
-    if (tid == 0 && s_changed) atomicAdd(a.changed, s_changed);
+    if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);
+  }
+  // the last workgroup to finish advances the launch state for the next launch
+  if (tid == 0 && atomicAdd(a.state + ST_DONE, 1u) == gridDim.x - 1) {
+    a.state[ST_EPOCH] = epoch + 1u;
+    a.state[ST_BASE] = tbase + ncols + gridDim.x;
+    a.state[ST_DONE] = 0u;
   }
 }
 
@@ -791,38 +816,28 @@ static void launch_col(const ColArgs &a, int grid, hipStream_t st, int iso) {
     hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, false>), dim3(grid), dim3(TX * TY), 0, st, a);
 }
 
-extern "C" int pdfsk_sweep_col_f32(float *field, int X, int Y, int Z, int sn,
-                                   const pdfs_sp32 *sp, const pdfsk_col_scratch *s,
-                                   int variant, uint32_t epoch, unsigned ticket_base,
-                                   int grid, hipStream_t st) {
-  if (sn < 1 || sn > 8 || X < 1 || Y < 1 || Z < 1) return -1;
-  int TX, TY;
-  pdfsk_tile_dims(variant, &TX, &TY);
+extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp32 *sp,
+                                     int variant, int grid, hipStream_t st) {
+  if (sn < 1 || sn > 8 || grid < 1) return -1;
+  static_assert(sizeof(FieldDesc) == sizeof(pdfsk_field), "field table layout");
   // Ordering sn as (+,+,+) in virtual coordinates: physical direction per
   // axis (mpi_openacc/src/phi3D_fsm.c:206-218 reflections x{4,8} y{2,6}
   // z{3,7}, descending levels for {2,5,7,8}).
   const int desc = (sn == 2 || sn == 5 || sn == 7 || sn == 8);
   const int rx = (sn == 4 || sn == 8), ry = (sn == 2 || sn == 6), rz = (sn == 3 || sn == 7);
   ColArgs a;
-  a.f = field;
-  a.X = X;
-  a.Y = Y;
-  a.Z = Z;
+  a.fields = (const FieldDesc *)b->fields;
+  a.cols = b->cols;
+  a.state = b->state;
   a.fx = desc ^ rx;
   a.fy = desc ^ ry;
   a.fz = desc ^ rz;
-  a.nTx = (X + TX - 1) / TX;
-  a.nTy = (Y + TY - 1) / TY;
-  a.order = s->order;
-  a.ticket = s->ticket;
-  a.ticket_base = ticket_base;
-  a.edge_x = s->edge_x - PDFSK_EDGE_PAD;
-  a.edge_y = s->edge_y - PDFSK_EDGE_PAD;
-  a.edge_x_bytes = (uint32_t)((s->edge_x_n + 2 * PDFSK_EDGE_PAD) * 8);
-  a.edge_y_bytes = (uint32_t)((s->edge_y_n + 2 * PDFSK_EDGE_PAD) * 8);
-  a.epoch = epoch;
-  a.changed = s->changed;
-  a.err = s->err;
+  a.edge_x = b->edge_x;
+  a.edge_y = b->edge_y;
+  a.edge_x_bytes = (uint32_t)b->edge_x_bytes;
+  a.edge_y_bytes = (uint32_t)b->edge_y_bytes;
+  a.changed = b->changed;
+  a.err = b->err;
   a.sp = *sp;
   switch (variant) {
     case PDFSK_TILE_32x8: launch_col<32, 8>(a, grid, st, sp->iso); break;

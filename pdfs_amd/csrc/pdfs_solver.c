@@ -15,14 +15,19 @@
 struct pdfs_ctx {
   int dev;
   hipStream_t st;
-  pdfsk_col_scratch col;
-  uint64_t *edge_x_base, *edge_y_base;
-  size_t cap_ex, cap_ey, cap_order; /* elements */
-  uint32_t epoch;
-  unsigned ticket_base;
   int resident[PDFSK_TILE_COUNT][2];
+  /* batched column sweep: persistent device state, edge allocations, plan */
+  pdfsk_batch bt;
+  size_t cap_ex, cap_ey;           /* edge entries excluding padding */
+  pdfsk_field *h_fields;           /* host copy of the field table */
+  int *h_cols;
+  int nf, ncols, tile, cap_f, cap_cols;
+  unsigned *h_changed, *prev_changed; /* pinned readback, last values */
+  unsigned *h_err;
+  hipGraphExec_t gexec;            /* captured pass (8 launches) */
+  int g_tile, g_iso, g_grid, g_sig[8];
+  /* FP64 path / judge */
   unsigned long long *d_u64; /* [0] maxdiff bits, [1] judge count */
-  unsigned *h_pin;           /* pinned readback: changed, err */
   unsigned long long *h_pin64;
   uint32_t *flags;
   size_t cap_flags;
@@ -68,6 +73,7 @@ void pdfs_default_opts(pdfs_opts *o)
   o->tol = 1e-12;
   o->judge = 1;
   o->tile = -1;
+  o->graph = 0;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -82,9 +88,9 @@ pdfs_ctx *pdfs_ctx_create(void)
   pdfs_ctx *c = (pdfs_ctx *)calloc(1, sizeof *c);
   if (!c) return NULL;
   if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
-      hipMalloc((void **)&c->col.ticket, 4 * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc((void **)&c->bt.state, 16 * sizeof(unsigned)) != hipSuccess ||
       hipMalloc((void **)&c->d_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_pin, 16 * sizeof(unsigned), 0) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_err, 16 * sizeof(unsigned), 0) != hipSuccess ||
       hipHostMalloc((void **)&c->h_pin64, 4 * sizeof(unsigned long long), 0) != hipSuccess ||
       hipMalloc((void **)&c->d_params, 1024 * sizeof(double)) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
@@ -93,17 +99,18 @@ pdfs_ctx *pdfs_ctx_create(void)
     free(c);
     return NULL;
   }
-  c->col.changed = c->col.ticket + 1;
-  c->col.err = c->col.ticket + 2;
-  if (hipMemset(c->col.ticket, 0, 4 * sizeof(unsigned)) != hipSuccess ||
+  c->bt.err = c->bt.state + 8;
+  /* epochs start at 1 (zeroed edge entries carry epoch 0) and only grow for
+   * the lifetime of the context */
+  unsigned init[16] = {0, 1, 0, 0, 0};
+  if (hipMemcpy(c->bt.state, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->d_u64, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
     free(c);
     return NULL;
   }
-  c->epoch = 0;
-  c->ticket_base = 0;
   for (int v = 0; v < PDFSK_TILE_COUNT; v++)
     for (int iso = 0; iso < 2; iso++) c->resident[v][iso] = pdfsk_col_max_resident(v, iso);
+  c->tile = -1;
   return c;
 }
 
@@ -111,15 +118,22 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
 {
   if (!c) return;
   hipStreamSynchronize(c->st);
-  hipFree(c->edge_x_base);
-  hipFree(c->edge_y_base);
-  hipFree(c->col.order);
-  hipFree(c->col.ticket);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->bt.edge_x) hipFree(c->bt.edge_x);
+  if (c->bt.edge_y) hipFree(c->bt.edge_y);
+  hipFree(c->bt.fields);
+  hipFree(c->bt.cols);
+  hipFree(c->bt.changed);
+  hipFree(c->bt.state);
   hipFree(c->d_u64);
   hipFree(c->flags);
   hipFree(c->d_params);
-  hipHostFree(c->h_pin);
+  hipHostFree(c->h_changed);
+  hipHostFree(c->h_err);
   hipHostFree(c->h_pin64);
+  free(c->prev_changed);
+  free(c->h_fields);
+  free(c->h_cols);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   for (int i = 0; i < 16; i++) hipEventDestroy(c->kev[i]);
@@ -152,85 +166,206 @@ static int default_tile(int x, int y)
   return PDFSK_TILE_16x16;
 }
 
-/* Grow the column-pipeline scratch for (x, y, z, tile) and rebuild the
- * upwind-first ticket order (anti-diagonals a + b ascending). */
-static int ensure_col(pdfs_ctx *c, int x, int y, int z, int tile, int *ncols_out)
+/* ---- batched column sweep: plans ---------------------------------------
+ * A plan is a list of dense FP32 fields swept by the same launches. Column
+ * tickets run anti-diagonal by anti-diagonal (a + b ascending) across all
+ * active fields, so every column's upwind neighbours hold earlier tickets
+ * of the same field; the persistent grid therefore never waits on a column
+ * that no running workgroup has taken. */
+static int grow_dev(void **p, int *cap, int need, size_t elem)
 {
-  if ((long long)x * y * z > PDFSK_MAX_POINTS)
-    return fail(PDFS_EINVAL, "field of %lld points exceeds the FP32 sweep limit %lld",
-                (long long)x * y * z, (long long)PDFSK_MAX_POINTS);
+  if (need <= *cap) return 0;
+  hipFree(*p);
+  *p = NULL;
+  if (hipMalloc(p, (size_t)need * elem) != hipSuccess) return -1;
+  *cap = need;
+  return 0;
+}
+
+static int plan_upload_cols(pdfs_ctx *c, const int *active)
+{
+  int TX = 0, TY = 0;
+  pdfsk_tile_dims(c->tile, &TX, &TY);
+  int maxd = 0;
+  for (int f = 0; f < c->nf; f++) {
+    int d = c->h_fields[f].nTx + c->h_fields[f].nTy - 2;
+    if (d > maxd) maxd = d;
+  }
+  int t = 0;
+  for (int d = 0; d <= maxd; d++)
+    for (int f = 0; f < c->nf; f++) {
+      if (active && !active[f]) continue;
+      const pdfsk_field *fd = &c->h_fields[f];
+      for (int a = 0; a < fd->nTx; a++) {
+        int b = d - a;
+        if (b < 0 || b >= fd->nTy) continue;
+        c->h_cols[3 * t] = f;
+        c->h_cols[3 * t + 1] = a;
+        c->h_cols[3 * t + 2] = b;
+        t++;
+      }
+    }
+  unsigned n = (unsigned)t;
+  HIPCHK(hipMemcpyAsync(c->bt.cols, c->h_cols, (size_t)(3 * (t > 0 ? t : 1)) * sizeof(int),
+                        hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->bt.state + 4, &n, sizeof n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->ncols = t;
+  return PDFS_OK;
+}
+
+static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile)
+{
+  if (nf < 1) return fail(PDFS_EINVAL, "empty batch");
   int TX, TY;
   pdfsk_tile_dims(tile, &TX, &TY);
-  int nTx = (x + TX - 1) / TX, nTy = (y + TY - 1) / TY;
-  size_t ncols = (size_t)nTx * nTy;
-  size_t nex = ncols * (size_t)z * TY, ney = ncols * (size_t)z * TX;
-  if (nex > c->cap_ex) {
-    size_t n = nex + 2 * PDFSK_EDGE_PAD;
-    hipFree(c->edge_x_base);
-    c->edge_x_base = c->col.edge_x = NULL;
-    HIPCHK(hipMalloc((void **)&c->edge_x_base, n * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->edge_x_base, 0, n * sizeof(uint64_t), c->st));
-    c->col.edge_x = c->edge_x_base + PDFSK_EDGE_PAD;
-    c->col.edge_x_n = nex;
-    c->cap_ex = nex;
+  if (nf > c->nf || !c->h_fields) {
+    free(c->h_fields);
+    free(c->prev_changed);
+    c->h_fields = (pdfsk_field *)calloc((size_t)nf, sizeof(pdfsk_field));
+    c->prev_changed = (unsigned *)calloc((size_t)nf, sizeof(unsigned));
+    hipHostFree(c->h_changed);
+    c->h_changed = NULL;
+    if (!c->h_fields || !c->prev_changed ||
+        hipHostMalloc((void **)&c->h_changed, (size_t)nf * sizeof(unsigned), 0) != hipSuccess)
+      return fail(PDFS_ENOMEM, "plan: host alloc");
   }
-  if (ney > c->cap_ey) {
-    size_t n = ney + 2 * PDFSK_EDGE_PAD;
-    hipFree(c->edge_y_base);
-    c->edge_y_base = c->col.edge_y = NULL;
-    HIPCHK(hipMalloc((void **)&c->edge_y_base, n * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->edge_y_base, 0, n * sizeof(uint64_t), c->st));
-    c->col.edge_y = c->edge_y_base + PDFSK_EDGE_PAD;
-    c->col.edge_y_n = ney;
-    c->cap_ey = ney;
+  size_t ex = 0, ey = 0;
+  int ncols = 0;
+  for (int f = 0; f < nf; f++) {
+    int x = dims[3 * f], y = dims[3 * f + 1], z = dims[3 * f + 2];
+    if (!fields[f] || x < 1 || y < 1 || z < 1)
+      return fail(PDFS_EINVAL, "field %d: bad pointer or dims", f);
+    if ((long long)x * y * z > PDFSK_MAX_POINTS)
+      return fail(PDFS_EINVAL, "field %d: %lld points exceed the FP32 sweep limit %lld", f,
+                  (long long)x * y * z, (long long)PDFSK_MAX_POINTS);
+    pdfsk_field *fd = &c->h_fields[f];
+    fd->f = fields[f];
+    fd->X = x;
+    fd->Y = y;
+    fd->Z = z;
+    fd->nTx = (x + TX - 1) / TX;
+    fd->nTy = (y + TY - 1) / TY;
+    fd->ex_off = (uint32_t)ex;
+    fd->ey_off = (uint32_t)ey;
+    fd->pad = 0;
+    size_t nc = (size_t)fd->nTx * fd->nTy;
+    ex += nc * (size_t)z * TY;
+    ey += nc * (size_t)z * TX;
+    ncols += (int)nc;
   }
-  if (2 * ncols > c->cap_order) {
-    hipFree(c->col.order);
-    c->col.order = NULL;
-    HIPCHK(hipMalloc((void **)&c->col.order, 2 * ncols * sizeof(int)));
-    c->cap_order = 2 * ncols;
+  if ((ex + 2 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31) || (ey + 2 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31))
+    return fail(PDFS_EINVAL, "batch too large for one sweep (edge buffers exceed 2 GiB)");
+  if (ex > c->cap_ex) {
+    if (c->bt.edge_x) hipFree(c->bt.edge_x);
+    c->bt.edge_x = NULL;
+    size_t n = ex + 2 * PDFSK_EDGE_PAD;
+    HIPCHK(hipMalloc((void **)&c->bt.edge_x, n * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->bt.edge_x, 0, n * sizeof(uint64_t), c->st));
+    c->bt.edge_x_bytes = n * sizeof(uint64_t);
+    c->cap_ex = ex;
   }
-  int *h = (int *)malloc(2 * ncols * sizeof(int));
-  if (!h) return fail(PDFS_ENOMEM, "ensure_col: host alloc");
-  size_t t = 0;
-  for (int d = 0; d <= nTx + nTy - 2; d++)
-    for (int a = 0; a < nTx; a++) {
-      int b = d - a;
-      if (b < 0 || b >= nTy) continue;
-      h[2 * t] = a;
-      h[2 * t + 1] = b;
-      t++;
-    }
-  hipError_t e = hipMemcpyAsync(c->col.order, h, 2 * ncols * sizeof(int), hipMemcpyHostToDevice,
-                                c->st);
-  hipError_t e2 = hipStreamSynchronize(c->st);
-  free(h);
-  if (e != hipSuccess || e2 != hipSuccess) return fail(PDFS_EHIP, "ensure_col: order upload");
-  *ncols_out = (int)ncols;
-  return PDFS_OK;
+  if (ey > c->cap_ey) {
+    if (c->bt.edge_y) hipFree(c->bt.edge_y);
+    c->bt.edge_y = NULL;
+    size_t n = ey + 2 * PDFSK_EDGE_PAD;
+    HIPCHK(hipMalloc((void **)&c->bt.edge_y, n * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->bt.edge_y, 0, n * sizeof(uint64_t), c->st));
+    c->bt.edge_y_bytes = n * sizeof(uint64_t);
+    c->cap_ey = ey;
+  }
+  int cap_f = c->cap_f;
+  if (grow_dev((void **)&c->bt.fields, &c->cap_f, nf, sizeof(pdfsk_field)) ||
+      grow_dev((void **)&c->bt.changed, &cap_f, nf, sizeof(unsigned)) ||
+      grow_dev((void **)&c->bt.cols, &c->cap_cols, 3 * ncols, sizeof(int)))
+    return fail(PDFS_ENOMEM, "plan: device alloc");
+  free(c->h_cols);
+  c->h_cols = (int *)malloc((size_t)3 * ncols * sizeof(int));
+  if (!c->h_cols) return fail(PDFS_ENOMEM, "plan: host alloc");
+  c->nf = nf;
+  c->tile = tile;
+  HIPCHK(hipMemcpyAsync(c->bt.fields, c->h_fields, (size_t)nf * sizeof(pdfsk_field),
+                        hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemsetAsync(c->bt.changed, 0, (size_t)nf * sizeof(unsigned), c->st));
+  memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = NULL;
+  }
+  return plan_upload_cols(c, NULL);
 }
 
-static int sweep_one(pdfs_ctx *c, float *field, int x, int y, int z, const pdfs_sp32 *sp, int sn,
-                     int tile, int ncols)
+static int launch_grid(pdfs_ctx *c, int iso)
 {
-  int grid = c->resident[tile][sp->iso];
+  int grid = c->resident[c->tile][iso];
+  if (grid <= 0) return -1;
+  return grid > c->ncols ? (c->ncols > 0 ? c->ncols : 1) : grid;
+}
+
+/* One pass (orderings sn0..sn1) of the current plan, optionally profiled
+ * per launch or replayed from a captured graph. */
+static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int profile,
+                         int graph, double *kernel_ms)
+{
+  int grid = launch_grid(c, sp->iso);
   if (grid <= 0) return fail(PDFS_EHIP, "column kernel cannot be resident");
-  if (grid > ncols) grid = ncols;
-  c->epoch++;
-  KCHK(pdfsk_sweep_col_f32(field, x, y, z, sn, sp, &c->col, tile, c->epoch, c->ticket_base, grid,
-                           c->st));
-  c->ticket_base += (unsigned)ncols + (unsigned)grid;
+  if (graph && sn0 == 1 && sn1 == 8 && !profile) {
+    if (!c->gexec || c->g_tile != c->tile || c->g_iso != sp->iso || c->g_grid != grid) {
+      if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = NULL;
+      }
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      for (int sn = 1; sn <= 8; sn++)
+        if (pdfsk_sweep_batch_f32(&c->bt, sn, sp, c->tile, grid, c->st)) {
+          hipStreamEndCapture(c->st, &g);
+          return fail(PDFS_EHIP, "launch during capture failed");
+        }
+      HIPCHK(hipStreamEndCapture(c->st, &g));
+      hipError_t e = hipGraphInstantiate(&c->gexec, g, NULL, NULL, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(PDFS_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+      c->g_tile = c->tile;
+      c->g_iso = sp->iso;
+      c->g_grid = grid;
+    }
+    HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    return PDFS_OK;
+  }
+  for (int sn = sn0; sn <= sn1; sn++) {
+    if (profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1)], c->st));
+    KCHK(pdfsk_sweep_batch_f32(&c->bt, sn, sp, c->tile, grid, c->st));
+    if (profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1) + 1], c->st));
+  }
+  if (profile) {
+    HIPCHK(hipEventSynchronize(c->kev[2 * (sn1 - 1) + 1]));
+    for (int sn = sn0; sn <= sn1; sn++) {
+      float tk = 0.f;
+      hipEventElapsedTime(&tk, c->kev[2 * (sn - 1)], c->kev[2 * (sn - 1) + 1]);
+      *kernel_ms += tk;
+    }
+  }
   return PDFS_OK;
 }
 
-/* Read (changed, err) counters; resets err. */
-static int read_counters(pdfs_ctx *c, unsigned *changed, unsigned *err)
+/* Per-field points changed since the last read; returns the timeout flag. */
+static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
 {
-  HIPCHK(hipMemcpyAsync(c->h_pin, c->col.changed, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
-                        c->st));
+  HIPCHK(hipMemcpyAsync(c->h_changed, c->bt.changed, (size_t)c->nf * sizeof(unsigned),
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_err, c->bt.err, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  *changed = c->h_pin[0];
-  *err = c->h_pin[1];
+  for (int f = 0; f < c->nf; f++) {
+    delta[f] = (long long)(unsigned)(c->h_changed[f] - c->prev_changed[f]);
+    c->prev_changed[f] = c->h_changed[f];
+  }
+  *err = c->h_err[0];
+  if (*err) {
+    unsigned z = 0;
+    hipMemcpyAsync(c->bt.err, &z, sizeof z, hipMemcpyHostToDevice, c->st);
+    hipStreamSynchronize(c->st);
+  }
   return PDFS_OK;
 }
 
@@ -242,17 +377,14 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
   if (tile < 0 || tile >= PDFSK_TILE_COUNT) tile = default_tile(x, y);
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
-  int ncols, rc;
-  if ((rc = ensure_col(c, x, y, z, tile, &ncols))) return rc;
-  unsigned ch0, ch1, err;
-  if ((rc = read_counters(c, &ch0, &err))) return rc;
-  if ((rc = sweep_one(c, field, x, y, z, &sp, sn, tile, ncols))) return rc;
-  if ((rc = read_counters(c, &ch1, &err))) return rc;
-  if (err) {
-    hipMemsetAsync(c->col.err, 0, sizeof(unsigned), c->st);
-    return fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
-  }
-  if (changed) *changed += (long long)(unsigned)(ch1 - ch0);
+  int dims[3] = {x, y, z}, rc;
+  if ((rc = plan_build(c, 1, &field, dims, tile))) return rc;
+  if ((rc = run_orderings(c, &sp, sn, sn, 0, 0, NULL))) return rc;
+  long long d = 0;
+  unsigned err = 0;
+  if ((rc = read_changed(c, &d, &err))) return rc;
+  if (err) return fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+  if (changed) *changed += d;
   return PDFS_OK;
 }
 
@@ -281,11 +413,13 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
   return PDFS_OK;
 }
 
-int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
-                       double dx, double dy, double dz, const pdfs_opts *o, pdfs_stats *st)
+int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
+                             const float *const *inits, const int *dims, double dx, double dy,
+                             double dz, const pdfs_opts *o, pdfs_stats *st, int *field_passes,
+                             double *field_ms)
 {
-  if (!c || !field || x < 1 || y < 1 || z < 1 || !(dx > 0) || !(dy > 0) || !(dz > 0))
-    return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
+  if (!c || nf < 1 || !fields || !dims || !(dx > 0) || !(dy > 0) || !(dz > 0))
+    return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
   pdfs_opts dflt;
   if (!o) {
     pdfs_default_opts(&dflt);
@@ -293,52 +427,78 @@ int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int 
   }
   pdfs_stats s;
   memset(&s, 0, sizeof s);
-  int tile = (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) ? o->tile : default_tile(x, y);
+  int tile = (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) ? o->tile
+                                                           : default_tile(dims[0], dims[1]);
   s.tile = tile;
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
-  int ncols, rc;
-  if ((rc = ensure_col(c, x, y, z, tile, &ncols))) return rc;
-  unsigned prev, err, cur;
-  if ((rc = read_counters(c, &prev, &err))) return rc;
-  long long npts = (long long)x * y * z;
+  int rc;
+  if ((rc = plan_build(c, nf, fields, dims, tile))) return rc;
+  long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
+  int *active = (int *)malloc((size_t)nf * sizeof(int));
+  if (!delta || !active) {
+    free(delta);
+    free(active);
+    return fail(PDFS_ENOMEM, "batch: host alloc");
+  }
+  for (int f = 0; f < nf; f++) {
+    active[f] = 1;
+    if (field_passes) field_passes[f] = 0;
+    if (field_ms) field_ms[f] = 0.0;
+  }
   int maxp = o->fixed_passes > 0 ? o->fixed_passes : (o->max_passes > 0 ? o->max_passes : 100);
-  float ms = 0.f;
-  for (int p = 0; p < maxp; p++) {
-    HIPCHK(hipEventRecord(c->ev0, c->st));
-    for (int sn = 1; sn <= 8; sn++) {
-      if (o->profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1)], c->st));
-      if ((rc = sweep_one(c, field, x, y, z, &sp, sn, tile, ncols))) return rc;
-      if (o->profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1) + 1], c->st));
-    }
+  int nactive = nf;
+  double ms = 0.0, kms = 0.0;
+  long long last_total = 0;
+  rc = PDFS_OK;
+  HIPCHK(hipEventRecord(c->ev0, c->st));
+  for (int p = 0; p < maxp && nactive > 0; p++) {
+    long long npts = 0;
+    for (int f = 0; f < nf; f++)
+      if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+    if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph, &kms))) break;
     HIPCHK(hipEventRecord(c->ev1, c->st));
-    if ((rc = read_counters(c, &cur, &err))) return rc;
+    unsigned err = 0;
+    if ((rc = read_changed(c, delta, &err))) break;
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
-    ms += t;
-    if (o->profile)
-      for (int q = 0; q < 8; q++) {
-        float tk = 0.f;
-        hipEventElapsedTime(&tk, c->kev[2 * q], c->kev[2 * q + 1]);
-        s.ms_sweep_kernels += tk;
-      }
+    ms = t;
     s.passes++;
     s.orderings += 8;
     s.point_updates += 8 * npts;
-    s.last_changed = (long long)(unsigned)(cur - prev);
-    prev = cur;
     if (err) {
-      hipMemsetAsync(c->col.err, 0, sizeof(unsigned), c->st);
-      if (st) *st = s;
-      return fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+      rc = fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+      break;
     }
-    if (o->fixed_passes <= 0 && s.last_changed == 0) break;
+    last_total = 0;
+    int changed_set = 0;
+    for (int f = 0; f < nf; f++) {
+      if (!active[f]) continue;
+      last_total += delta[f];
+      if (field_passes) field_passes[f] = p + 1;
+      if (o->fixed_passes <= 0 && delta[f] == 0) {
+        active[f] = 0;
+        nactive--;
+        changed_set = 1;
+        if (field_ms) field_ms[f] = ms;
+      } else if (field_ms) {
+        field_ms[f] = ms;
+      }
+    }
+    if (changed_set && nactive > 0 && (rc = plan_upload_cols(c, active))) break;
   }
+  s.last_changed = last_total;
   s.ms_sweeps = ms;
-  if (o->judge && init) {
+  s.ms_sweep_kernels = kms;
+  if (rc == PDFS_OK && o->judge && inits) {
     HIPCHK(hipEventRecord(c->ev0, c->st));
-    if ((rc = pdfs_judge_f32_dev(c, field, init, x, y, z, dx, dy, dz, NULL, &s.judge_flagged)))
-      return rc;
+    for (int f = 0; f < nf && rc == PDFS_OK; f++) {
+      if (!inits[f]) continue;
+      long long fl = 0;
+      rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
+                              dims[3 * f + 2], dx, dy, dz, NULL, &fl);
+      s.judge_flagged += fl;
+    }
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     float t = 0.f;
@@ -347,9 +507,20 @@ int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int 
   }
   s.ms_total = s.ms_sweeps + s.ms_judge;
   if (st) *st = s;
-  if (o->fixed_passes <= 0 && s.last_changed != 0)
-    return fail(PDFS_ENOTCONV, "not converged after %d passes", s.passes);
-  return PDFS_OK;
+  if (rc == PDFS_OK && o->fixed_passes <= 0 && nactive > 0)
+    rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, s.passes);
+  free(delta);
+  free(active);
+  return rc;
+}
+
+int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
+                       double dx, double dy, double dz, const pdfs_opts *o, pdfs_stats *st)
+{
+  if (!c || !field || x < 1 || y < 1 || z < 1)
+    return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
+  int dims[3] = {x, y, z};
+  return pdfs_solve_batch_f32_dev(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL);
 }
 
 int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx, double dy,

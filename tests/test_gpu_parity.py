@@ -168,3 +168,46 @@ def test_host_padded_solve_f32_matches_oracle():
     assert np.array_equal(O.interior(b), O.interior(ref32).astype(np.float64))
     assert np.all(np.isinf(b[0])) and np.all(np.isinf(b[:, 0])) and np.all(np.isinf(b[:, :, -1]))
     assert st["judge_flagged"] == 0
+
+
+def _batch_fields(seed, n=9):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        X, Y, Z = (int(v) for v in rng.integers(3, 40, 3))
+        kind = i % 3
+        if kind == 0:
+            a = O.gen_point(X, Y, Z, seed + i)
+        elif kind == 1:
+            a = O.gen_terrain(X, Y, Z, seed + i, amp=(1.0, 6.0), sig=(2.0, 9.0))
+        else:
+            a = O.gen_spheres(X, Y, Z, seed + i, n=4, r=(1.5, 5.0))
+        out.append(a)
+    return out
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("tile", [pdfs_amd.TILE_16x16, pdfs_amd.TILE_32x8])
+def test_batch_uneven_fields_bitwise(ctx, dev, graph, tile):
+    """Uneven fields in shared persistent launches give each field exactly
+    its own converged FP32 result and pass count."""
+    fields = _batch_fields(500 + graph + 10 * tile)
+    wants, wpasses = [], []
+    for a in fields:
+        r = a.astype(np.float32)
+        p, _ = O.solve_f32(r)
+        wants.append(O.interior(r))
+        wpasses.append(p)
+    ts = [_dense(a, dev) for a in fields]
+    inits = [t.clone() for t in ts]
+    st, passes, ms = ctx.solve_batch_f32(ts, inits, graph=graph, tile=tile)
+    for i, (t, w) in enumerate(zip(ts, wants)):
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w)), i
+    assert passes == wpasses
+    assert st["judge_flagged"] == 0
+    assert all(m > 0 for m in ms)
+    # a second solve through the same (graph-cached) plan state stays exact
+    ts2 = [_dense(a, dev) for a in fields]
+    ctx.solve_batch_f32(ts2, None, graph=graph, tile=tile, judge=0)
+    for t, w in zip(ts2, wants):
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))

@@ -38,6 +38,7 @@ def main():
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 reference-arithmetic check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--cpu-orderings", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=512, help="fields of the c4 batch leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -116,6 +117,9 @@ def main():
             f"rms {extra['f32_vs_f64_rms_err']:.3e}")
         del padded, inner, err
 
+    if args.batch > 0:
+        extra["batch"] = batch_leg(ctx, args.batch)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, args, ref_passes, X, Y, Z)
@@ -145,6 +149,53 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def batch_leg(ctx, nfields):
+    """c4: uneven fields in one persistent launch per ordering (graph-replayed
+    passes, tile tuned online), against each field solved by its own
+    launches, with per-field completion times (ms since the batch start)."""
+    import numpy as np
+    import torch
+
+    import pdfs_amd
+    from pdfs_amd.workloads import c4_specs, generate_c4
+
+    specs = c4_specs(seed=4, nfields=nfields)
+    inits = generate_c4(ctx, specs)
+    npts = sum(t.numel() for t in inits)
+    log(f"[bench] c4 batch: {nfields} fields, {npts / 1e6:.1f} M points")
+    # warm the tuner / graph once, then time
+    fields = [t.clone() for t in inits]
+    ctx.solve_batch_f32(fields, None, tile=pdfs_amd.TILE_AUTO, graph=1, judge=0)
+    fields = [t.clone() for t in inits]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, passes, ms = ctx.solve_batch_f32(fields, None, tile=pdfs_amd.TILE_AUTO, graph=1, judge=0)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    batched = [f.clone() for f in fields]
+    # each field in its own launches, one after another
+    fields = [t.clone() for t in inits]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = []
+    for f in fields:
+        ctx.solve_f32(f, None, judge=0)
+        done.append(1000.0 * (time.perf_counter() - t0))
+    ts = time.perf_counter() - t0
+    same = all(torch.equal(a, b) for a, b in zip(batched, fields))
+    out = {"fields": nfields, "points": npts, "seconds": round(tb, 4),
+           "fields_per_s": round(nfields / tb, 2), "passes_max": max(passes),
+           "p50_ms": round(float(np.percentile(ms, 50)), 3),
+           "p99_ms": round(float(np.percentile(ms, 99)), 3), "tile": st["tile"],
+           "per_field_launches": {"seconds": round(ts, 4), "fields_per_s": round(nfields / ts, 2),
+                                  "p50_ms": round(float(np.percentile(done, 50)), 3),
+                                  "p99_ms": round(float(np.percentile(done, 99)), 3)},
+           "speedup": round(ts / tb, 2), "bitwise_equal_to_per_field": bool(same)}
+    log(f"[bench] c4 batched {tb * 1e3:.1f} ms ({out['fields_per_s']} fields/s, p50 "
+        f"{out['p50_ms']} p99 {out['p99_ms']} ms) vs per-field {ts * 1e3:.1f} ms; equal={same}")
+    return out
 
 
 def cpu_baseline(ctx, args, ref_passes, X, Y, Z):

@@ -59,6 +59,7 @@ void runFSM(PhiACC *pf, int sn);
 /* ---- status-returning API ---- */
 enum { PDFS_OK = 0, PDFS_EINVAL = -1, PDFS_EHIP = -2, PDFS_ENOMEM = -3, PDFS_ETIMEOUT = -4,
        PDFS_ENOTCONV = -5 };
+enum { PDFS_TILE_AUTO = -2 };
 enum { PDFS_F64_REF = 0, /* FP64, reference arithmetic and schedule */
        PDFS_F32 = 1 };   /* FP32 difference form, column pipeline */
 
@@ -68,7 +69,8 @@ typedef struct {
   int fixed_passes;/* > 0: run exactly this many passes, no convergence test */
   double tol;      /* FP64: stop when a pass changes nothing by more than tol */
   int judge;       /* FP32: run the FP64 residual judge after convergence */
-  int tile;        /* column tile variant, -1 = autotuned default */
+  int tile;        /* column tile: 0 16x16, 1 32x8, 2 16x8; -1 default; PDFS_TILE_AUTO tuned
+                      online on the first passes and cached per size class */
   int quiet;       /* no stdout */
   int profile;     /* time every sweep launch with events on the solver stream */
   int graph;       /* replay each pass (8 persistent launches) from a captured hipGraph */
@@ -127,6 +129,13 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
 enum { PDFS_GEN_POINT = 0, PDFS_GEN_TERRAIN = 1, PDFS_GEN_SPHERES = 2 };
 int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
                       float *dense, double *padded);
+
+/* Synthetic field with explicit object parameters: terrain (nobj hills,
+ * amplitude U(p_lo,p_hi) h, width U(q_lo,q_hi) h), spheres (nobj, radius
+ * U(p_lo,p_hi) h), point source (parameters unused). */
+int pdfs_generate_ex_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
+                         int nobj, double p_lo, double p_hi, double q_lo, double q_hi,
+                         float *dense, double *padded);
 
 /* Layout conversion between the padded FP64 and dense FP32 device arrays. */
 int pdfs_pack_dev(pdfs_ctx *c, const double *padded, float *dense, int x, int y, int z);

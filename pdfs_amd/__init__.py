@@ -20,12 +20,13 @@ LIB_PATH = os.path.join(HERE, "lib", "libpdfs.so")
 PDFS_OK, PDFS_EINVAL, PDFS_EHIP, PDFS_ENOMEM, PDFS_ETIMEOUT, PDFS_ENOTCONV = 0, -1, -2, -3, -4, -5
 PDFS_F64_REF, PDFS_F32 = 0, 1
 GEN_POINT, GEN_TERRAIN, GEN_SPHERES = 0, 1, 2
-TILE_16x16, TILE_32x8, TILE_16x8 = 0, 1, 2
+TILE_16x16, TILE_32x8, TILE_16x8, TILE_AUTO = 0, 1, 2, -2
 
 EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
     "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
-    "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_pack_dev",
+    "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_generate_ex_dev",
+    "pdfs_pack_dev",
     "pdfs_unpack_dev", "pdfs_solve_batch_f32_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
     "pdfs_vti_write", "pdfs_vti_free",
 ]
@@ -105,6 +106,8 @@ def lib():
         L.pdfs_judge_f32_dev.restype = I
         L.pdfs_generate_dev.argtypes = [vp, I, C.c_uint64, I, I, I, D, vp, vp]
         L.pdfs_generate_dev.restype = I
+        L.pdfs_generate_ex_dev.argtypes = [vp, I, C.c_uint64, I, I, I, D, I, D, D, D, D, vp, vp]
+        L.pdfs_generate_ex_dev.restype = I
         L.pdfs_pack_dev.argtypes = [vp, vp, vp, I, I, I]
         L.pdfs_pack_dev.restype = I
         L.pdfs_unpack_dev.argtypes = [vp, vp, vp, I, I, I]
@@ -287,6 +290,13 @@ class Context:
         torch.cuda.synchronize()
         _check(lib().pdfs_generate_dev(self.c, kind, seed, X, Y, Z, h, self._ptr(dense),
                                        self._ptr(padded)))
+
+    def generate_ex(self, kind, seed, X, Y, Z, h=1.0, nobj=0, p=(0.0, 0.0), q=(0.0, 0.0),
+                    dense=None, padded=None):
+        import torch
+        torch.cuda.synchronize()
+        _check(lib().pdfs_generate_ex_dev(self.c, kind, seed, X, Y, Z, h, nobj, p[0], p[1],
+                                          q[0], q[1], self._ptr(dense), self._ptr(padded)))
 
     def pack(self, padded, dense):
         import torch

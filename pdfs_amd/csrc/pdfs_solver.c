@@ -25,6 +25,8 @@ struct pdfs_ctx {
   unsigned *h_changed, *prev_changed; /* pinned readback, last values */
   unsigned *h_err;
   hipGraphExec_t gexec;            /* captured pass (8 launches) */
+  long long tune_keys[32];         /* tile autotuner cache: size class -> tile */
+  int tune_tiles[32], ntune;
   int g_tile, g_iso, g_grid, g_sig[8];
   /* FP64 path / judge */
   unsigned long long *d_u64; /* [0] maxdiff bits, [1] judge count */
@@ -159,11 +161,40 @@ static void make_sp32(double dx, double dy, double dz, pdfs_sp32 *s)
   s->W = s->w[0] + s->w[1] + s->w[2];
 }
 
+/* Size class of a batch for the tile cache: field count and log2 buckets of
+ * the mean extents. */
+static long long tune_key(int nf, const int *dims)
+{
+  double sx = 0, sy = 0, sz = 0;
+  for (int f = 0; f < nf; f++) {
+    sx += dims[3 * f];
+    sy += dims[3 * f + 1];
+    sz += dims[3 * f + 2];
+  }
+  int bx = (int)log2(sx / nf + 1), by = (int)log2(sy / nf + 1), bz = (int)log2(sz / nf + 1);
+  int bn = (int)log2((double)nf + 1);
+  return ((long long)bn << 24) | (bx << 16) | (by << 8) | bz;
+}
+
 static int default_tile(int x, int y)
 {
   (void)x;
   (void)y;
   return PDFSK_TILE_16x16;
+}
+
+static int tune_lookup(pdfs_ctx *c, long long key)
+{
+  for (int i = 0; i < c->ntune; i++)
+    if (c->tune_keys[i] == key) return c->tune_tiles[i];
+  return -1;
+}
+
+static void tune_store(pdfs_ctx *c, long long key, int tile)
+{
+  int i = c->ntune < 32 ? c->ntune++ : (int)(key & 31);
+  c->tune_keys[i] = key;
+  c->tune_tiles[i] = tile;
 }
 
 /* ---- batched column sweep: plans ---------------------------------------
@@ -427,9 +458,24 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   }
   pdfs_stats s;
   memset(&s, 0, sizeof s);
-  int tile = (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) ? o->tile
-                                                           : default_tile(dims[0], dims[1]);
-  s.tile = tile;
+  /* tile: explicit, default, or tuned online (PDFS_TILE_AUTO): the first
+   * passes use one candidate each (results do not depend on the tile), the
+   * fastest is kept for the rest and cached for the batch's size class */
+  long long key = tune_key(nf, dims);
+  int tuning = 0, cand = 0;
+  float cand_ms[PDFSK_TILE_COUNT];
+  int tile;
+  if (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) {
+    tile = o->tile;
+  } else if (o->tile == PDFS_TILE_AUTO) {
+    tile = tune_lookup(c, key);
+    if (tile < 0) {
+      tuning = 1;
+      tile = 0;
+    }
+  } else {
+    tile = default_tile(dims[0], dims[1]);
+  }
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
   int rc;
@@ -456,13 +502,32 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     long long npts = 0;
     for (int f = 0; f < nf; f++)
       if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
-    if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph, &kms))) break;
+    float t_prev = (float)ms;
+    if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph && !tuning, &kms))) break;
     HIPCHK(hipEventRecord(c->ev1, c->st));
     unsigned err = 0;
     if ((rc = read_changed(c, delta, &err))) break;
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
     ms = t;
+    if (tuning) {
+      /* only full passes (every field active) are comparable */
+      cand_ms[cand] = nactive == nf ? t - t_prev : 1e30f;
+      if (++cand == PDFSK_TILE_COUNT || nactive < nf) {
+        int best = 0;
+        for (int v = 1; v < cand; v++)
+          if (cand_ms[v] < cand_ms[best]) best = v;
+        tuning = 0;
+        if (nactive == nf) tune_store(c, key, best);
+        tile = best;
+      } else {
+        tile = cand;
+      }
+      if (tile != c->tile) {
+        /* carry the convergence bookkeeping across the re-plan */
+        if ((rc = plan_build(c, nf, fields, dims, tile))) break;
+      }
+    }
     s.passes++;
     s.orderings += 8;
     s.point_updates += 8 * npts;
@@ -487,6 +552,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     }
     if (changed_set && nactive > 0 && (rc = plan_upload_cols(c, active))) break;
   }
+  s.tile = c->tile;
   s.last_changed = last_total;
   s.ms_sweeps = ms;
   s.ms_sweep_kernels = kms;
@@ -695,10 +761,15 @@ static uint64_t splitmix64(uint64_t *s)
 
 static double u01(uint64_t *s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
 
-int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
-                      float *dense, double *padded)
+/* Synthetic field with explicit object parameters: terrain (nobj hills,
+ * amplitude U(p_lo, p_hi) h, width U(q_lo, q_hi) h), spheres (nobj spheres,
+ * radius U(p_lo, p_hi) h) or a point source (parameters unused). */
+int pdfs_generate_ex_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
+                         int nobj, double p_lo, double p_hi, double q_lo, double q_hi,
+                         float *dense, double *padded)
 {
   if (!c || x < 1 || y < 1 || z < 1 || !(h > 0)) return fail(PDFS_EINVAL, "generate: bad args");
+  if (nobj < 0 || nobj > 256) return fail(PDFS_EINVAL, "generate: 0..256 objects");
   uint64_t s = seed;
   double prm[1024];
   if (kind == PDFS_GEN_POINT) {
@@ -707,34 +778,45 @@ int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z,
     int i = 1 + (int)(u01(&s) * z);
     KCHK(pdfsk_gen_point(dense, padded, x, y, z, k, j, i, c->st));
   } else if (kind == PDFS_GEN_TERRAIN) {
-    /* 8 hills, amplitude U(5,40) h, width U(10,60) h, centres uniform */
-    int nh = 8;
-    for (int q = 0; q < nh; q++) {
-      prm[4 * q] = (5.0 + (40.0 - 5.0) * u01(&s)) * h;
-      prm[4 * q + 1] = (10.0 + (60.0 - 10.0) * u01(&s)) * h;
+    if (nobj > 64) return fail(PDFS_EINVAL, "generate: at most 64 hills");
+    for (int q = 0; q < nobj; q++) {
+      prm[4 * q] = (p_lo + (p_hi - p_lo) * u01(&s)) * h;
+      prm[4 * q + 1] = (q_lo + (q_hi - q_lo) * u01(&s)) * h;
       prm[4 * q + 2] = u01(&s) * (double)(x - 1) * h;
       prm[4 * q + 3] = u01(&s) * (double)(y - 1) * h;
     }
-    HIPCHK(hipMemcpyAsync(c->d_params, prm, 4 * nh * sizeof(double), hipMemcpyHostToDevice,
-                          c->st));
-    KCHK(pdfsk_gen_terrain(dense, padded, x, y, z, h, c->d_params, nh, c->st));
+    if (nobj) HIPCHK(hipMemcpyAsync(c->d_params, prm, 4 * nobj * sizeof(double),
+                                    hipMemcpyHostToDevice, c->st));
+    KCHK(pdfsk_gen_terrain(dense, padded, x, y, z, h, c->d_params, nobj, c->st));
   } else if (kind == PDFS_GEN_SPHERES) {
-    /* 64 spheres, radius U(4,32) h, centres uniform */
-    int ns = 64;
-    for (int q = 0; q < ns; q++) {
+    for (int q = 0; q < nobj; q++) {
       prm[4 * q] = u01(&s) * (double)(x - 1) * h;
       prm[4 * q + 1] = u01(&s) * (double)(y - 1) * h;
       prm[4 * q + 2] = u01(&s) * (double)(z - 1) * h;
-      prm[4 * q + 3] = (4.0 + (32.0 - 4.0) * u01(&s)) * h;
+      prm[4 * q + 3] = (p_lo + (p_hi - p_lo) * u01(&s)) * h;
     }
-    HIPCHK(hipMemcpyAsync(c->d_params, prm, 4 * ns * sizeof(double), hipMemcpyHostToDevice,
-                          c->st));
-    KCHK(pdfsk_gen_spheres(dense, padded, x, y, z, h, c->d_params, ns, c->st));
+    if (nobj) HIPCHK(hipMemcpyAsync(c->d_params, prm, 4 * nobj * sizeof(double),
+                                    hipMemcpyHostToDevice, c->st));
+    KCHK(pdfsk_gen_spheres(dense, padded, x, y, z, h, c->d_params, nobj, c->st));
   } else {
     return fail(PDFS_EINVAL, "generate: unknown kind %d", kind);
   }
   HIPCHK(hipStreamSynchronize(c->st));
   return PDFS_OK;
+}
+
+/* Configuration defaults: terrain 8 hills, amplitude U(5,40) h, width
+ * U(10,60) h; spheres 64, radius U(4,32) h. */
+int pdfs_generate_dev(pdfs_ctx *c, int kind, uint64_t seed, int x, int y, int z, double h,
+                      float *dense, double *padded)
+{
+  if (kind == PDFS_GEN_TERRAIN)
+    return pdfs_generate_ex_dev(c, kind, seed, x, y, z, h, 8, 5.0, 40.0, 10.0, 60.0, dense,
+                                padded);
+  if (kind == PDFS_GEN_SPHERES)
+    return pdfs_generate_ex_dev(c, kind, seed, x, y, z, h, 64, 4.0, 32.0, 0.0, 0.0, dense,
+                                padded);
+  return pdfs_generate_ex_dev(c, kind, seed, x, y, z, h, 0, 0.0, 0.0, 0.0, 0.0, dense, padded);
 }
 
 /* ---- CLI pipeline: single_gpu_cuda/src/phi3D.c:107-126 (calc_dist_field) ----

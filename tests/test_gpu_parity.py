@@ -211,3 +211,19 @@ def test_batch_uneven_fields_bitwise(ctx, dev, graph, tile):
     ctx.solve_batch_f32(ts2, None, graph=graph, tile=tile, judge=0)
     for t, w in zip(ts2, wants):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
+
+
+def test_batch_autotune_same_result(ctx, dev):
+    """Online tile tuning switches tiles between passes; the result is the
+    same as any fixed tile (schedule independence)."""
+    fields = _batch_fields(900, n=6)
+    wants = []
+    for a in fields:
+        r = a.astype(np.float32)
+        O.solve_f32(r)
+        wants.append(O.interior(r))
+    ts = [_dense(a, dev) for a in fields]
+    st, passes, ms = ctx.solve_batch_f32(ts, None, tile=pdfs_amd.TILE_AUTO, judge=0)
+    assert st["tile"] in (0, 1, 2)
+    for t, w in zip(ts, wants):
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))

@@ -74,6 +74,8 @@ typedef struct {
   int quiet;       /* no stdout */
   int profile;     /* time every sweep launch with events on the solver stream */
   int graph;       /* replay each pass (8 persistent launches) from a captured hipGraph */
+  int certify;     /* FP32: also stop a field once the FP64 residual judge passes on it */
+  int certify_from;/* first pass after which the judge is run (default 1) */
 } pdfs_opts;
 
 typedef struct {
@@ -85,6 +87,9 @@ typedef struct {
   double ms_sweeps, ms_judge, ms_total;
   double ms_sweep_kernels;   /* profile: sum of per-launch sweep kernel times */
   int tile;
+  int judge_runs;            /* judge launches (certification checks + final) */
+  int fallbacks;             /* warm start: regions reset to the init and re-solved */
+  double ms_warm;            /* warm start: network + upsample time */
 } pdfs_stats;
 
 void pdfs_default_opts(pdfs_opts *o);

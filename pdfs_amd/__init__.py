@@ -51,7 +51,8 @@ class PhiACC(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_passes", C.c_int), ("fixed_passes", C.c_int),
                 ("tol", C.c_double), ("judge", C.c_int), ("tile", C.c_int),
-                ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int)]
+                ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int),
+                ("certify", C.c_int), ("certify_from", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -59,7 +60,8 @@ class Stats(C.Structure):
                 ("last_changed", C.c_longlong), ("last_maxdiff", C.c_double),
                 ("judge_flagged", C.c_longlong), ("ms_sweeps", C.c_double),
                 ("ms_judge", C.c_double), ("ms_total", C.c_double),
-                ("ms_sweep_kernels", C.c_double), ("tile", C.c_int)]
+                ("ms_sweep_kernels", C.c_double), ("tile", C.c_int), ("judge_runs", C.c_int),
+                ("fallbacks", C.c_int), ("ms_warm", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -76,6 +76,8 @@ void pdfs_default_opts(pdfs_opts *o)
   o->judge = 1;
   o->tile = -1;
   o->graph = 0;
+  o->certify = 0;
+  o->certify_from = 1;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -537,11 +539,30 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     }
     last_total = 0;
     int changed_set = 0;
+    /* certified stop: the FP64 residual judge passes on a field from pass
+     * certify_from on (needs its init); plain stop: a pass changed nothing */
+    const int judge_now = o->certify && inits && o->fixed_passes <= 0 &&
+                          p + 1 >= (o->certify_from > 0 ? o->certify_from : 1);
+    float tj0 = 0.f;
+    if (judge_now) {
+      HIPCHK(hipEventRecord(c->ev1, c->st));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      hipEventElapsedTime(&tj0, c->ev0, c->ev1);
+    }
     for (int f = 0; f < nf; f++) {
       if (!active[f]) continue;
       last_total += delta[f];
       if (field_passes) field_passes[f] = p + 1;
-      if (o->fixed_passes <= 0 && delta[f] == 0) {
+      int done = o->fixed_passes <= 0 && delta[f] == 0;
+      if (!done && judge_now && inits[f]) {
+        long long fl = -1;
+        if ((rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
+                                     dims[3 * f + 2], dx, dy, dz, NULL, &fl)))
+          break;
+        s.judge_runs++;
+        done = fl == 0;
+      }
+      if (done) {
         active[f] = 0;
         nactive--;
         changed_set = 1;
@@ -550,13 +571,23 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
         field_ms[f] = ms;
       }
     }
+    if (rc) break;
+    if (judge_now) {
+      HIPCHK(hipEventRecord(c->ev1, c->st));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float tj1 = 0.f;
+      hipEventElapsedTime(&tj1, c->ev0, c->ev1);
+      s.ms_judge += tj1 - tj0;
+      ms = tj1;
+    }
     if (changed_set && nactive > 0 && (rc = plan_upload_cols(c, active))) break;
   }
   s.tile = c->tile;
   s.last_changed = last_total;
-  s.ms_sweeps = ms;
+  s.ms_sweeps = ms - s.ms_judge;
   s.ms_sweep_kernels = kms;
   if (rc == PDFS_OK && o->judge && inits) {
+    float tj_before = (float)s.ms_judge;
     HIPCHK(hipEventRecord(c->ev0, c->st));
     for (int f = 0; f < nf && rc == PDFS_OK; f++) {
       if (!inits[f]) continue;
@@ -569,7 +600,8 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     HIPCHK(hipEventSynchronize(c->ev1));
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
-    s.ms_judge = t;
+    s.ms_judge = tj_before + t;
+    s.judge_runs++;
   }
   s.ms_total = s.ms_sweeps + s.ms_judge;
   if (st) *st = s;

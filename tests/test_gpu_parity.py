@@ -227,3 +227,53 @@ def test_batch_autotune_same_result(ctx, dev):
     assert st["tile"] in (0, 1, 2)
     for t, w in zip(ts, wants):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
+
+
+@pytest.mark.parametrize("gen", ["point", "spheres"])
+def test_certified_stop_fewer_passes_within_tolerance(ctx, dev, gen):
+    """Certified stop: a field stops once the FP64 residual judge passes on it.
+    It needs no more passes than the plain stop (a pass that changes nothing),
+    stays within the FP32 tolerance of the converged FP64 oracle, and the
+    final judge agrees."""
+    X, Y, Z = 72, 56, 40
+    if gen == "point":
+        a = O.gen_point(X, Y, Z, seed=61)
+    else:
+        a = O.gen_spheres(X, Y, Z, seed=62, n=8, r=(2.0, 8.0))
+    ref64 = a.copy()
+    O.solve_d64(ref64, tol=0.0)
+    t = _dense(a, dev)
+    init = t.clone()
+    plain = t.clone()
+    sp = ctx.solve_f32(plain, init)
+    sc = ctx.solve_f32(t, init, certify=1)
+    got = t.cpu().numpy()
+    err = np.abs(got.astype(np.float64) - O.interior(ref64))
+    print(f"{gen}: plain {sp['passes']} passes, certified {sc['passes']} "
+          f"(judge runs {sc['judge_runs']}), max err {err.max():.3e}")
+    assert sc["passes"] <= sp["passes"]
+    assert sc["judge_flagged"] == 0
+    assert err.max() <= F32_VS_F64_TOL
+    # the oracle's judge on the certified field agrees
+    full = np.full((Z + 2, Y + 2, X + 2), np.inf, np.float32)
+    full[1:-1, 1:-1, 1:-1] = got
+    assert O.judge_f32(full, a.astype(np.float32)) == 0
+
+
+def test_certified_batch_matches_single(ctx, dev):
+    """Certified stop in a batch: each field stops at the same pass and with
+    the same bits as when solved alone."""
+    fields = _batch_fields(1300, n=6)
+    singles, sp = [], []
+    for a in fields:
+        t = _dense(a, dev)
+        st = ctx.solve_f32(t, t.clone(), certify=1)
+        singles.append(t.cpu().numpy())
+        sp.append(st["passes"])
+    ts = [_dense(a, dev) for a in fields]
+    inits = [t.clone() for t in ts]
+    st, passes, _ = ctx.solve_batch_f32(ts, inits, certify=1)
+    assert passes == sp
+    for t, w in zip(ts, singles):
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
+    assert st["judge_flagged"] == 0

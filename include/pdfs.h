@@ -103,6 +103,8 @@ typedef struct pdfs_ctx pdfs_ctx;
 pdfs_ctx *pdfs_ctx_create(void);
 void pdfs_ctx_destroy(pdfs_ctx *c);
 const char *pdfs_last_error(void);
+/* The context's HIP stream (hipStream_t): every *_dev call runs on it. */
+void *pdfs_ctx_stream(pdfs_ctx *c);
 
 /* Dense FP32 field [z][y][x] (interior only, ghosts implicit +inf), in place. */
 int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init_or_null, int x, int y,
@@ -155,6 +157,47 @@ int pdfs_calc_dist_field(Phi *pf, int passes, int precision);
  * (phi3D.c:165-180), 1 set_distance_negative_inside (phi3D.c:190-205),
  * 2 adjust_boundary (phi3D.c:221-242). */
 int pdfs_prepost(int op, const int *location, double *distance, int x, int y, int z);
+
+/* ---- learned warm start (no counterpart in the reference) ----
+ * A 3-D conv encoder-decoder (4 levels, 16/32/64/64 channels, 3x3x3, BF16
+ * weights and activations, FP32 accumulation on the matrix cores) maps the
+ * source mask at 1/4 resolution to the distance at the coarse points
+ * (4z, 4y, 4x). Coarse tensors are channels-last [nb][Dz][Dy][Dx][16], each
+ * coarse side ceil(side/4) rounded up to a multiple of 8 (pdfs_net_grid).
+ * A network belongs to the context it was created with and runs on its
+ * stream. */
+typedef struct pdfs_net pdfs_net;
+pdfs_net *pdfs_net_create(pdfs_ctx *c, unsigned long long seed); /* He-initialised */
+void pdfs_net_destroy(pdfs_net *n);
+int pdfs_net_load(pdfs_net *n, const char *path);
+int pdfs_net_save(pdfs_net *n, const char *path);
+long long pdfs_net_num_params(const pdfs_net *n);
+/* which: 0 FP32 master parameters, 1 gradients of the last train step */
+int pdfs_net_get(pdfs_net *n, int which, float *host);
+int pdfs_net_set(pdfs_net *n, const float *host);
+/* target scale (prediction = scale * distance in cells) and the upper-bound
+ * inflation of the warm start w = min(init, a * d + b * hmax) */
+void pdfs_net_get_calibration(const pdfs_net *n, float *scale, float *a, float *b);
+void pdfs_net_set_calibration(pdfs_net *n, float scale, float a, float b);
+void pdfs_net_grid(int x, int y, int z, int *Dz, int *Dy, int *Dx);
+/* mask (BF16 [Dz][Dy][Dx][16]) of a dense init; target (FP32 [Dz][Dy][Dx]) of
+ * a converged field */
+int pdfs_net_features_dev(pdfs_net *n, const float *init, int x, int y, int z, uint16_t *mask);
+int pdfs_net_target_dev(pdfs_net *n, const float *u, int x, int y, int z, float *target);
+/* pred (optional): FP32 [nb][Dz][Dy][Dx][16], channel 0 the prediction */
+int pdfs_net_forward_dev(pdfs_net *n, const uint16_t *mask, int nb, int Dz, int Dy, int Dx,
+                         float *pred);
+/* forward, MSE loss, backward, Adam (lr > 0; lr = 0 leaves the gradients) */
+int pdfs_net_train_step_dev(pdfs_net *n, const uint16_t *mask, const float *target, int nb,
+                            int Dz, int Dy, int Dx, float lr, float *loss);
+/* w = min(init, a * d + b * hmax), d the upsampled prediction */
+int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int y, int z,
+                            double hmax);
+/* warm start + certified FP32 solve; on a judge failure the field is reset to
+ * its init and solved plainly (stats.fallbacks = 1) */
+int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float *init, int x,
+                            int y, int z, double dx, double dy, double dz, const pdfs_opts *o,
+                            pdfs_stats *st);
 
 #ifdef __cplusplus
 }

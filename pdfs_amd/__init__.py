@@ -28,8 +28,13 @@ EXPORTED = [
     "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_generate_ex_dev",
     "pdfs_pack_dev",
     "pdfs_unpack_dev", "pdfs_solve_batch_f32_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
-    "pdfs_vti_write", "pdfs_vti_free",
+    "pdfs_vti_write", "pdfs_vti_free", "pdfs_ctx_stream", "pdfs_net_create", "pdfs_net_destroy",
+    "pdfs_net_load", "pdfs_net_save", "pdfs_net_num_params", "pdfs_net_get", "pdfs_net_set",
+    "pdfs_net_get_calibration", "pdfs_net_set_calibration", "pdfs_net_grid",
+    "pdfs_net_features_dev", "pdfs_net_target_dev", "pdfs_net_forward_dev",
+    "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_solve_warm_f32_dev",
 ]
+WEIGHTS_PATH = os.path.join(HERE, "weights", "warm_unet.bin")
 CLI_PATH = os.path.join(HERE, "bin", "PDFS")
 
 
@@ -121,6 +126,42 @@ def lib():
         L.pdfs_calc_dist_field.restype = I
         L.pdfs_prepost.argtypes = [I, vp, vp, I, I, I]
         L.pdfs_prepost.restype = I
+        F = C.c_float
+        L.pdfs_ctx_stream.argtypes = [vp]
+        L.pdfs_ctx_stream.restype = vp
+        L.pdfs_net_create.argtypes = [vp, C.c_uint64]
+        L.pdfs_net_create.restype = vp
+        L.pdfs_net_destroy.argtypes = [vp]
+        L.pdfs_net_destroy.restype = None
+        L.pdfs_net_load.argtypes = [vp, C.c_char_p]
+        L.pdfs_net_load.restype = I
+        L.pdfs_net_save.argtypes = [vp, C.c_char_p]
+        L.pdfs_net_save.restype = I
+        L.pdfs_net_num_params.argtypes = [vp]
+        L.pdfs_net_num_params.restype = LL
+        L.pdfs_net_get.argtypes = [vp, I, vp]
+        L.pdfs_net_get.restype = I
+        L.pdfs_net_set.argtypes = [vp, vp]
+        L.pdfs_net_set.restype = I
+        L.pdfs_net_get_calibration.argtypes = [vp, C.POINTER(F), C.POINTER(F), C.POINTER(F)]
+        L.pdfs_net_get_calibration.restype = None
+        L.pdfs_net_set_calibration.argtypes = [vp, F, F, F]
+        L.pdfs_net_set_calibration.restype = None
+        L.pdfs_net_grid.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        L.pdfs_net_grid.restype = None
+        L.pdfs_net_features_dev.argtypes = [vp, vp, I, I, I, vp]
+        L.pdfs_net_features_dev.restype = I
+        L.pdfs_net_target_dev.argtypes = [vp, vp, I, I, I, vp]
+        L.pdfs_net_target_dev.restype = I
+        L.pdfs_net_forward_dev.argtypes = [vp, vp, I, I, I, I, vp]
+        L.pdfs_net_forward_dev.restype = I
+        L.pdfs_net_train_step_dev.argtypes = [vp, vp, vp, I, I, I, I, F, C.POINTER(F)]
+        L.pdfs_net_train_step_dev.restype = I
+        L.pdfs_warm_start_f32_dev.argtypes = [vp, vp, vp, I, I, I, D]
+        L.pdfs_warm_start_f32_dev.restype = I
+        L.pdfs_solve_warm_f32_dev.argtypes = [vp, vp, vp, vp, I, I, I, D, D, D, C.POINTER(Opts),
+                                              C.POINTER(Stats)]
+        L.pdfs_solve_warm_f32_dev.restype = I
         _lib = L
     return _lib
 
@@ -317,4 +358,120 @@ class Context:
     def _sync(self):
         # device-wide sync: drains the library's own stream as well
         import torch
+        torch.cuda.synchronize()
+
+    def solve_warm_f32(self, net, field, init, h=(1.0, 1.0, 1.0), allow_notconv=False, **kw):
+        """Warm-started certified solve (in place); falls back to a plain
+        solve from init when the judge flags the warm-started result."""
+        import torch
+        assert field.dtype == torch.float32 and field.dim() == 3
+        Z, Y, X = field.shape
+        o = default_opts(precision=PDFS_F32, **kw)
+        st = Stats()
+        torch.cuda.synchronize()
+        rc = lib().pdfs_solve_warm_f32_dev(self.c, net.n, self._ptr(field), self._ptr(init), X, Y,
+                                           Z, *h, C.byref(o), C.byref(st))
+        _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+        return st.as_dict()
+
+
+def net_grid(X, Y, Z):
+    """Coarse grid (Dz, Dy, Dx) of the warm-start network for a field."""
+    d = [C.c_int(0) for _ in range(3)]
+    lib().pdfs_net_grid(X, Y, Z, *(C.byref(v) for v in d))
+    return tuple(v.value for v in d)
+
+
+class Net:
+    """The warm-start network (pdfs_net) of a Context. Tensors are torch
+    cuda tensors: masks BF16 [nb, Dz, Dy, Dx, 16] (as int16 or bfloat16),
+    targets FP32 [nb, Dz, Dy, Dx]."""
+
+    def __init__(self, ctx, seed=1, path=None):
+        self.ctx = ctx
+        self.n = lib().pdfs_net_create(ctx.c, seed)
+        if not self.n:
+            raise PdfsError(PDFS_EHIP, lib().pdfs_last_error().decode())
+        if path is not None:
+            self.load(path)
+
+    def close(self):
+        if self.n:
+            lib().pdfs_net_destroy(self.n)
+            self.n = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, path=None):
+        _check(lib().pdfs_net_load(self.n, (path or WEIGHTS_PATH).encode()))
+
+    def save(self, path):
+        _check(lib().pdfs_net_save(self.n, path.encode()))
+
+    @property
+    def num_params(self):
+        return lib().pdfs_net_num_params(self.n)
+
+    def params(self, which=0):
+        import numpy as np
+        out = np.empty(self.num_params, np.float32)
+        _check(lib().pdfs_net_get(self.n, which, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().pdfs_net_set(self.n, p.ctypes.data))
+
+    @property
+    def calibration(self):
+        v = [C.c_float(0) for _ in range(3)]
+        lib().pdfs_net_get_calibration(self.n, *(C.byref(x) for x in v))
+        return tuple(x.value for x in v)
+
+    def set_calibration(self, scale, a, b):
+        lib().pdfs_net_set_calibration(self.n, scale, a, b)
+
+    def features(self, init, mask):
+        import torch
+        Z, Y, X = init.shape
+        torch.cuda.synchronize()
+        _check(lib().pdfs_net_features_dev(self.n, init.data_ptr(), X, Y, Z, mask.data_ptr()))
+
+    def target(self, u, t):
+        import torch
+        Z, Y, X = u.shape
+        torch.cuda.synchronize()
+        _check(lib().pdfs_net_target_dev(self.n, u.data_ptr(), X, Y, Z, t.data_ptr()))
+
+    def forward(self, mask, pred=None):
+        import torch
+        nb, Dz, Dy, Dx, ch = mask.shape
+        assert ch == 16 and mask.is_contiguous()
+        torch.cuda.synchronize()
+        _check(lib().pdfs_net_forward_dev(self.n, mask.data_ptr(), nb, Dz, Dy, Dx,
+                                          None if pred is None else pred.data_ptr()))
+        return pred
+
+    def train_step(self, mask, target, lr):
+        import torch
+        nb, Dz, Dy, Dx, ch = mask.shape
+        assert ch == 16 and mask.is_contiguous() and target.is_contiguous()
+        assert tuple(target.shape) == (nb, Dz, Dy, Dx) and target.dtype == torch.float32
+        loss = C.c_float(0)
+        torch.cuda.synchronize()
+        _check(lib().pdfs_net_train_step_dev(self.n, mask.data_ptr(), target.data_ptr(), nb, Dz,
+                                             Dy, Dx, lr, C.byref(loss)))
+        return loss.value
+
+    def warm_start(self, init, w, hmax=1.0):
+        import torch
+        Z, Y, X = init.shape
+        torch.cuda.synchronize()
+        _check(lib().pdfs_warm_start_f32_dev(self.n, init.data_ptr(), w.data_ptr(), X, Y, Z, hmax))
         torch.cuda.synchronize()

@@ -1,0 +1,61 @@
+/* Launchers of the warm-start network's gfx950 kernels (pdfs_nn.hip).
+ * Tensors are channels-last [n][z][y][x][C] with C a multiple of 16;
+ * BF16 tensors are uint16_t (upper half of an IEEE float), weights are
+ * [27][Cout][Cin] with tap = (dz*3 + dy)*3 + dx. All launches are
+ * asynchronous on stream st and return 0 or a HIP error code (-1: shape
+ * outside what the kernel supports). */
+#ifndef PDFS_NN_H
+#define PDFS_NN_H
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out = act(conv3x3x3(in, W) + bias), zero padding; flip = 1 mirrors the taps
+ * (with transposed weights [27][Cin][Cout] this is the input gradient).
+ * Cin <= 128, Cout <= 128; out (BF16) and out_f32 each optional. */
+int pdfsnn_conv3(const uint16_t *in, int Cin, const uint16_t *W, const float *bias, int Cout,
+                 uint16_t *out, float *out_f32, int n, int Dz, int Dy, int Dx, int relu, int flip,
+                 hipStream_t st);
+/* dW[tap][co][ci] += sum_p dY[p][co] * X[p + shift(tap)][ci]; (Cout/16)(Cin/16) <= 32 */
+int pdfsnn_conv3_wgrad(const uint16_t *X, int Cin, const uint16_t *dY, int Cout, float *dW, int n,
+                       int Dz, int Dy, int Dx, hipStream_t st);
+int pdfsnn_bias_grad(const uint16_t *dY, int Cout, long M, float *db, hipStream_t st);
+int pdfsnn_pool2(const uint16_t *in, uint16_t *out, int n, int Dz, int Dy, int Dx, int C,
+                 hipStream_t st);
+/* din (fine dims Dz,Dy,Dx) = or += dout / 8 */
+int pdfsnn_pool2_bwd(const float *dout, float *din, int n, int Dz, int Dy, int Dx, int C,
+                     int accumulate, hipStream_t st);
+/* dst (fine, Cs+Ck channels) = [nearest-up2(src) | skip] */
+int pdfsnn_up2_concat(const uint16_t *src, int Cs, const uint16_t *skip, int Ck, uint16_t *dst,
+                      int n, int Dz, int Dy, int Dx, hipStream_t st);
+/* dsrc = sum of the 8 children's first Cs channels; dskip = channels [Cs, Cs+Ck) */
+int pdfsnn_up2_concat_bwd(const float *ddst, int Cs, int Ck, float *dsrc, float *dskip, int n,
+                          int Dz, int Dy, int Dx, hipStream_t st);
+/* out = y ? (y > 0 ? g : 0) : g, as BF16 */
+int pdfsnn_relu_bwd(const float *g, const uint16_t *y, uint16_t *out, long N, hipStream_t st);
+/* loss += mean (o[i*C] - t[i])^2 over the N samples; grad channel 0 = 2 (o - t) / N */
+int pdfsnn_mse(const float *o, int C, const float *t, long N, uint16_t *grad, float *loss,
+               hipStream_t st);
+/* Adam step `step` (1-based) on FP32 masters; refreshes BF16 weights and their
+ * [27][Ci][Co] transpose when wb != NULL */
+int pdfsnn_adam(float *w, const float *gw, float *m, float *v, long N, float lr, float b1, float b2,
+                float eps, int step, uint16_t *wb, uint16_t *wt, int Co, int Ci, hipStream_t st);
+int pdfsnn_f32_to_bf16(const float *x, uint16_t *y, long N, hipStream_t st);
+int pdfsnn_transpose_w(const uint16_t *wb, uint16_t *wt, int Co, int Ci, hipStream_t st);
+/* fraction of known cells (init < far) per 4x4x4 block -> channel 0 of [Dz][Dy][Dx][16] */
+int pdfsnn_mask_down4(const float *f, int X, int Y, int Z, uint16_t *m, int Dz, int Dy, int Dx,
+                      float far, hipStream_t st);
+/* t[z][y][x] = scale * u(4z, 4y, 4x) (clamped to the field) */
+int pdfsnn_target_down4(const float *u, int X, int Y, int Z, float *t, int Dz, int Dy, int Dx,
+                        float scale, hipStream_t st);
+/* w = min(init, a * max(trilinear(pred) * inv_scale, 0) + b) on the dense field */
+int pdfsnn_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_scale, float a,
+                    float b, const float *init, float *w, int X, int Y, int Z, hipStream_t st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

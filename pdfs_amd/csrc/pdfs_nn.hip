@@ -1,0 +1,586 @@
+// gfx950 kernels of the learned warm start: a 3-D conv encoder-decoder in
+// BF16 with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).
+//
+// Activations: channels-last [n][z][y][x][C] BF16, C a multiple of 16.
+// Weights: [27][Cout][Cin] BF16 (tap = (dz*3 + dy)*3 + dx, cin fastest),
+// an FP32 master copy lives with the optimizer.
+//
+// MFMA 16x16x16 operand layout (wave64): lane l = c + 16 g (c = l % 16,
+// g = l / 16). A (16 x 16, M x K): lane holds A[c][4g .. 4g+3]. B (K x N):
+// lane holds B[4g .. 4g+3][c]. D (M x N, FP32): lane holds D[4g .. 4g+3][c].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pdfs_nn.h"
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+static inline int hip_rc(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
+static inline unsigned grid1(long n, int bs) {
+  long g = (n + bs - 1) / bs;
+  if (g > 1048576) g = 1048576;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3x3 convolution, stride 1, zero padding 1, as an implicit GEMM:
+// M = positions, N = Cout, K = 27 taps x Cin. One wave: 16 positions x all
+// Cout (up to 8 tiles of 16); the block: 4 waves = 64 positions.
+// out = act(conv + bias); act: 0 linear, 1 ReLU. out_f32 optional (head).
+// With flip = 1 the taps are mirrored (dz,dy,dx) -> (2-dz, 2-dy, 2-dx):
+// with transposed weights this is the input gradient.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t *__restrict__ in, int Cin,
+                                                   const uint16_t *__restrict__ W,
+                                                   const float *__restrict__ bias, int Cout,
+                                                   uint16_t *__restrict__ out,
+                                                   float *__restrict__ out_f32, int n, int Dz,
+                                                   int Dy, int Dx, int relu, int flip) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const long M = (long)n * Dz * Dy * Dx;
+  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
+  if (p0 >= M) return;
+  // this lane's A row = position p0 + c
+  const long pa = p0 + c;
+  const bool rowok = pa < M;
+  int ax = 0, ay = 0, az = 0;
+  long ab = 0;
+  if (rowok) {
+    ax = (int)(pa % Dx);
+    long q = pa / Dx;
+    ay = (int)(q % Dy);
+    q /= Dy;
+    az = (int)(q % Dz);
+    ab = q / Dz;
+  }
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int nchunk = Cin / 16;
+  for (int tap = 0; tap < 27; tap++) {
+    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+    const int sz = flip ? 1 - dz : dz - 1, sy = flip ? 1 - dy : dy - 1,
+              sx = flip ? 1 - dx : dx - 1;
+    const int zz = az + sz, yy = ay + sy, xx = ax + sx;
+    const bool ok = rowok && (unsigned)zz < (unsigned)Dz && (unsigned)yy < (unsigned)Dy &&
+                    (unsigned)xx < (unsigned)Dx;
+    const uint16_t *ip = in + ((((ab * Dz + zz) * Dy + yy) * Dx + xx) * (long)Cin) + 4 * g;
+    const uint16_t *wp = W + ((long)tap * Cout + c) * Cin + 4 * g;
+    for (int k = 0; k < nchunk; k++) {
+      bf16x4 a = {0, 0, 0, 0};
+      if (ok) a = *(const bf16x4 *)(ip + 16 * k);
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        if (16 * t < Cout) {
+          const bf16x4 b = *(const bf16x4 *)(wp + (long)16 * t * Cin + 16 * k);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    if (16 * t >= Cout) continue;
+    const int co = 16 * t + c;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const long p = p0 + 4 * g + e;
+      if (p >= M) continue;
+      float v = acc[t][e] + bv;
+      if (relu) v = v > 0.f ? v : 0.f;
+      if (out) out[p * Cout + co] = f2bf(v);
+      if (out_f32) out_f32[p * Cout + co] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[tap][co][ci] += sum_p dY[p][co] X[p + s(tap)][ci].
+// Block: one tap, a run of positions; 64-position chunks staged in LDS,
+// then every wave accumulates its (co, ci) tiles with MFMA (M = co, N = ci,
+// K = positions), and adds them to the FP32 dW at the end.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict__ X, int Cin,
+                                                     const uint16_t *__restrict__ dY, int Cout,
+                                                     float *__restrict__ dW, int n, int Dz,
+                                                     int Dy, int Dx, int chunks_per_block) {
+  __shared__ uint16_t sY[64 * 64];    // [pos][co]
+  __shared__ uint16_t sX[64 * 128];   // [pos][ci]
+  const int tap = blockIdx.y;
+  const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const long M = (long)n * Dz * Dy * Dx;
+  const int nco = Cout / 16, nci = Cin / 16, ntile = nco * nci;
+  // up to 32 tiles (Cout <= 64, Cin <= 128) over 4 waves: 8 per wave
+  f32x4 acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) acc[i] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int ch = 0; ch < chunks_per_block; ch++) {
+    const long pbase = ((long)blockIdx.x * chunks_per_block + ch) * 64;
+    if (pbase >= M) break;
+    __syncthreads();
+    // stage dY[pbase .. +64][0..Cout) and X[shifted][0..Cin)
+    for (int i = threadIdx.x; i < 64 * Cout; i += 256) {
+      const int r = i / Cout, co = i % Cout;
+      const long p = pbase + r;
+      sY[r * Cout + co] = p < M ? dY[p * Cout + co] : (uint16_t)0;
+    }
+    for (int i = threadIdx.x; i < 64 * Cin; i += 256) {
+      const int r = i / Cin, ci = i % Cin;
+      const long p = pbase + r;
+      uint16_t v = 0;
+      if (p < M) {
+        const int x = (int)(p % Dx);
+        long q = p / Dx;
+        const int y = (int)(q % Dy);
+        q /= Dy;
+        const int z = (int)(q % Dz);
+        const long b = q / Dz;
+        const int zz = z + dz - 1, yy = y + dy - 1, xx = x + dx - 1;
+        if ((unsigned)zz < (unsigned)Dz && (unsigned)yy < (unsigned)Dy &&
+            (unsigned)xx < (unsigned)Dx)
+          v = X[(((b * Dz + zz) * Dy + yy) * Dx + xx) * (long)Cin + ci];
+      }
+      sX[r * Cin + ci] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int tl = wave + 4 * i;
+      if (tl >= ntile) break;
+      const int tco = tl / nci, tci = tl % nci;
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int r = 16 * kk + 4 * g + e;
+          a[e] = (short)sY[r * Cout + 16 * tco + c];
+          b[e] = (short)sX[r * Cin + 16 * tci + c];
+        }
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int tl = wave + 4 * i;
+    if (tl >= ntile) break;
+    const int tco = tl / nci, tci = tl % nci;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int co = 16 * tco + 4 * g + e, ci = 16 * tci + c;
+      atomicAdd(&dW[((long)tap * Cout + co) * Cin + ci], acc[i][e]);
+    }
+  }
+}
+
+// bias gradient: db[co] += sum_p dY[p][co]
+__global__ void k_bias_grad(const uint16_t *__restrict__ dY, int Cout, long M, float *db) {
+  __shared__ float part[256];
+  const int co = blockIdx.y;
+  float s = 0.f;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < M;
+       p += (long)gridDim.x * blockDim.x)
+    s += bf2f(dY[p * Cout + co]);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(&db[co], part[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Pooling, upsampling, concat, ReLU backward, loss, optimizer, conversions.
+// ---------------------------------------------------------------------------
+// 2x2x2 average pool: out[b][z][y][x][c] = mean of in[b][2z..][2y..][2x..][c]
+__global__ void k_pool2(const uint16_t *in, uint16_t *out, int n, int Dz, int Dy, int Dx,
+                        int C) {
+  const int oz = Dz / 2, oy = Dy / 2, ox = Dx / 2;
+  const long N = (long)n * oz * oy * ox * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long q = i / C;
+    const int x = (int)(q % ox);
+    q /= ox;
+    const int y = (int)(q % oy);
+    q /= oy;
+    const int z = (int)(q % oz);
+    const long b = q / oz;
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+      const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+      s += bf2f(in[(((b * Dz + zz) * Dy + yy) * Dx + xx) * (long)C + c]);
+    }
+    out[i] = f2bf(0.125f * s);
+  }
+}
+
+// gradient of the average pool: din = dout / 8 broadcast (din is fine-level)
+__global__ void k_pool2_bwd(const float *dout, float *din, int n, int Dz, int Dy, int Dx,
+                            int C, int accumulate) {
+  const long N = (long)n * Dz * Dy * Dx * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long q = i / C;
+    const int x = (int)(q % Dx);
+    q /= Dx;
+    const int y = (int)(q % Dy);
+    q /= Dy;
+    const int z = (int)(q % Dz);
+    const long b = q / Dz;
+    const float v = 0.125f * dout[((((b * (Dz / 2) + z / 2) * (Dy / 2) + y / 2) * (Dx / 2)) +
+                                   x / 2) * (long)C + c];
+    din[i] = accumulate ? din[i] + v : v;
+  }
+}
+
+// nearest x2 upsample of src (coarse, Cs channels) into channels [0, Cs) of
+// dst (fine, Cd channels), and copy skip (fine, Ck channels) into [Cs, Cs+Ck)
+__global__ void k_up2_concat(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
+                             uint16_t *dst, int n, int Dz, int Dy, int Dx) {
+  const int Cd = Cs + Ck;
+  const long N = (long)n * Dz * Dy * Dx * Cd;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Cd);
+    const long p = i / Cd;
+    if (c < Cs) {
+      const int x = (int)(p % Dx);
+      long q = p / Dx;
+      const int y = (int)(q % Dy);
+      q /= Dy;
+      const int z = (int)(q % Dz);
+      const long b = q / Dz;
+      dst[i] = src[((((b * (Dz / 2) + z / 2) * (Dy / 2) + y / 2) * (Dx / 2)) + x / 2) * (long)Cs +
+                   c];
+    } else {
+      dst[i] = skip[p * Ck + (c - Cs)];
+    }
+  }
+}
+
+// backward of k_up2_concat: dsrc (coarse) = sum of the 8 children's first Cs
+// channels; dskip (fine) = channels [Cs, Cs+Ck)
+__global__ void k_up2_concat_bwd(const float *ddst, int Cs, int Ck, float *dsrc,
+                                 float *dskip, int n, int Dz, int Dy, int Dx) {
+  const int Cd = Cs + Ck;
+  const int oz = Dz / 2, oy = Dy / 2, ox = Dx / 2;
+  const long Nc = (long)n * oz * oy * ox * Cs;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < Nc;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Cs);
+    long q = i / Cs;
+    const int x = (int)(q % ox);
+    q /= ox;
+    const int y = (int)(q % oy);
+    q /= oy;
+    const int z = (int)(q % oz);
+    const long b = q / oz;
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+      const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+      s += ddst[(((b * Dz + zz) * Dy + yy) * Dx + xx) * (long)Cd + c];
+    }
+    dsrc[i] = s;
+  }
+  const long Nf = (long)n * Dz * Dy * Dx * Ck;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < Nf;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Ck);
+    const long p = i / Ck;
+    dskip[i] = ddst[p * Cd + Cs + c];
+  }
+}
+
+// dy_pre (BF16) = g (FP32) masked by the ReLU output y (y > 0), or unmasked
+__global__ void k_relu_bwd(const float *g, const uint16_t *y, uint16_t *out, long N) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = (y == nullptr || bf2f(y[i]) > 0.f) ? f2bf(g[i]) : (uint16_t)0;
+}
+
+// head output channel 0 (FP32, C channels) vs target: loss += sum (o-t)^2 / N,
+// grad (BF16, C channels, zero except channel 0) = 2 (o - t) / N
+__global__ void k_mse(const float *o, int C, const float *t, long N, uint16_t *grad,
+                      float *loss) {
+  __shared__ float part[256];
+  float s = 0.f;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const float d = o[i * C] - t[i];
+    s += d * d;
+    for (int c = 0; c < C; c++) grad[i * C + c] = c == 0 ? f2bf(2.f * d / (float)N) : (uint16_t)0;
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) part[threadIdx.x] += part[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0] / (float)N);
+}
+
+// Adam (Kingma & Ba 2015) on FP32 masters; refresh the BF16 weights [27][Co][Ci]
+// and the flipped-transposed copy [27][Ci][Co] used by the input gradient.
+__global__ void k_adam(float *w, const float *gw, float *m, float *v, long N, float lr, float b1,
+                       float b2, float eps, float c1, float c2, uint16_t *wb, uint16_t *wt,
+                       int Co, int Ci) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const float gi = gw[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) {
+      const uint16_t h = f2bf(wi);
+      wb[i] = h;
+      const int ci = (int)(i % Ci);
+      const long q = i / Ci;
+      const int co = (int)(q % Co);
+      const int tap = (int)(q / Co);
+      wt[((long)tap * Ci + ci) * Co + co] = h;
+    }
+  }
+}
+
+__global__ void k_f32_to_bf16(const float *x, uint16_t *y, long N) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+
+__global__ void k_transpose_w(const uint16_t *wb, uint16_t *wt, int Co, int Ci) {
+  const long N = 27L * Co * Ci;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % Ci);
+    const long q = i / Ci;
+    const int co = (int)(q % Co);
+    const int tap = (int)(q / Co);
+    wt[((long)tap * Ci + ci) * Co + co] = wb[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Warm-start input and output.
+// ---------------------------------------------------------------------------
+// Source mask at 1/4 resolution: fraction of known cells (init < far) in
+// each 4x4x4 block of the dense field [Z][Y][X]; channel 0 of a 16-channel
+// BF16 tensor [Dz][Dy][Dx][16]. The coarse grid may extend past the field
+// (rounded up for pooling): blocks outside it are empty.
+__global__ void k_mask_down4(const float *f, int X, int Y, int Z, uint16_t *m, int Dz, int Dy,
+                             int Dx, float far) {
+  const long N = (long)Dz * Dy * Dx;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % Dx);
+    const long q = i / Dx;
+    const int y = (int)(q % Dy), z = (int)(q / Dy);
+    int known = 0, cnt = 0;
+    for (int k = 0; k < 64; k++) {
+      const int zz = 4 * z + (k >> 4), yy = 4 * y + ((k >> 2) & 3), xx = 4 * x + (k & 3);
+      if (zz < Z && yy < Y && xx < X) {
+        cnt++;
+        known += f[((long)zz * Y + yy) * X + xx] < far;
+      }
+    }
+    for (int c = 0; c < 16; c++) m[i * 16 + c] = c == 0 ? f2bf(cnt ? (float)known / 64.f : 0.f)
+                                                         : (uint16_t)0;
+  }
+}
+
+// Training target at 1/4 resolution: the converged field at block corners
+// (4z, 4y, 4x), scaled.
+__global__ void k_target_down4(const float *u, int X, int Y, int Z, float *t, int Dz, int Dy,
+                               int Dx, float scale) {
+  const long N = (long)Dz * Dy * Dx;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % Dx);
+    const long q = i / Dx;
+    const int y = (int)(q % Dy), z = (int)(q / Dy);
+    const int zz = min(4 * z, Z - 1), yy = min(4 * y, Y - 1), xx = min(4 * x, X - 1);
+    t[i] = scale * u[((long)zz * Y + yy) * X + xx];
+  }
+}
+
+// Warm start: trilinear interpolation of the coarse prediction (corner
+// samples at 4i, FP32 channel 0 of a C-channel tensor) back to the dense
+// field, inflated to an upper bound w = a * d + b, capped by the init.
+__global__ void k_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_scale,
+                           float a, float b, const float *init, float *w, int X, int Y, int Z) {
+#pragma clang fp contract(off)
+  const long N = (long)X * Y * Z;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % X);
+    const long q = i / X;
+    const int y = (int)(q % Y), z = (int)(q / Y);
+    const float fx = 0.25f * x, fy = 0.25f * y, fz = 0.25f * z;
+    const int x0 = min((int)fx, Dx - 1), y0 = min((int)fy, Dy - 1), z0 = min((int)fz, Dz - 1);
+    const int x1 = min(x0 + 1, Dx - 1), y1 = min(y0 + 1, Dy - 1), z1 = min(z0 + 1, Dz - 1);
+    const float tx = fx - x0, ty = fy - y0, tz = fz - z0;
+    auto P = [&](int zz, int yy, int xx) { return pred[(((long)zz * Dy + yy) * Dx + xx) * C]; };
+    const float c00 = P(z0, y0, x0) * (1 - tx) + P(z0, y0, x1) * tx;
+    const float c01 = P(z0, y1, x0) * (1 - tx) + P(z0, y1, x1) * tx;
+    const float c10 = P(z1, y0, x0) * (1 - tx) + P(z1, y0, x1) * tx;
+    const float c11 = P(z1, y1, x0) * (1 - tx) + P(z1, y1, x1) * tx;
+    const float c0 = c00 * (1 - ty) + c01 * ty, c1 = c10 * (1 - ty) + c11 * ty;
+    const float d = (c0 * (1 - tz) + c1 * tz) * inv_scale;
+    const float up = a * (d > 0.f ? d : 0.f) + b;
+    const float f0 = init[i];
+    w[i] = f0 < up ? f0 : up;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI launchers
+// ---------------------------------------------------------------------------
+extern "C" int pdfsnn_conv3(const uint16_t *in, int Cin, const uint16_t *W, const float *bias,
+                            int Cout, uint16_t *out, float *out_f32, int n, int Dz, int Dy,
+                            int Dx, int relu, int flip, hipStream_t st) {
+  if (Cin % 16 || Cout % 16 || Cout > 128 || Cin > 128 || Cin < 16 || Cout < 16) return -1;
+  const long M = (long)n * Dz * Dy * Dx;
+  const unsigned blocks = (unsigned)((M + 63) / 64);
+  const int nt = Cout / 16;
+  if (nt == 1)
+    hipLaunchKernelGGL(k_conv3_fwd<1>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
+                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+  else if (nt == 2)
+    hipLaunchKernelGGL(k_conv3_fwd<2>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
+                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+  else if (nt <= 4)
+    hipLaunchKernelGGL(k_conv3_fwd<4>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
+                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+  else
+    hipLaunchKernelGGL(k_conv3_fwd<8>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
+                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_conv3_wgrad(const uint16_t *X, int Cin, const uint16_t *dY, int Cout,
+                                  float *dW, int n, int Dz, int Dy, int Dx, hipStream_t st) {
+  if (Cin % 16 || Cout % 16 || Cout > 64 || Cin > 128 || (Cout / 16) * (Cin / 16) > 32) return -1;
+  const long M = (long)n * Dz * Dy * Dx;
+  const long nchunks = (M + 63) / 64;
+  const int per = 16;
+  const unsigned bx = (unsigned)((nchunks + per - 1) / per);
+  hipLaunchKernelGGL(k_conv3_wgrad, dim3(bx, 27), dim3(256), 0, st, X, Cin, dY, Cout, dW, n, Dz,
+                     Dy, Dx, per);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_bias_grad(const uint16_t *dY, int Cout, long M, float *db, hipStream_t st) {
+  hipLaunchKernelGGL(k_bias_grad, dim3(64, Cout), dim3(256), 0, st, dY, Cout, M, db);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_pool2(const uint16_t *in, uint16_t *out, int n, int Dz, int Dy, int Dx,
+                            int C, hipStream_t st) {
+  hipLaunchKernelGGL(k_pool2, dim3(grid1((long)n * Dz * Dy * Dx * C / 8, 256)), dim3(256), 0, st,
+                     in, out, n, Dz, Dy, Dx, C);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_pool2_bwd(const float *dout, float *din, int n, int Dz, int Dy, int Dx,
+                                int C, int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL(k_pool2_bwd, dim3(grid1((long)n * Dz * Dy * Dx * C, 256)), dim3(256), 0,
+                     st, dout, din, n, Dz, Dy, Dx, C, accumulate);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_up2_concat(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
+                                 uint16_t *dst, int n, int Dz, int Dy, int Dx, hipStream_t st) {
+  hipLaunchKernelGGL(k_up2_concat, dim3(grid1((long)n * Dz * Dy * Dx * (Cs + Ck), 256)),
+                     dim3(256), 0, st, src, Cs, skip, Ck, dst, n, Dz, Dy, Dx);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_up2_concat_bwd(const float *ddst, int Cs, int Ck, float *dsrc,
+                                     float *dskip, int n, int Dz, int Dy, int Dx,
+                                     hipStream_t st) {
+  hipLaunchKernelGGL(k_up2_concat_bwd, dim3(grid1((long)n * Dz * Dy * Dx * Ck, 256)), dim3(256),
+                     0, st, ddst, Cs, Ck, dsrc, dskip, n, Dz, Dy, Dx);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_relu_bwd(const float *g, const uint16_t *y, uint16_t *out, long N,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(N, 256)), dim3(256), 0, st, g, y, out, N);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_mse(const float *o, int C, const float *t, long N, uint16_t *grad,
+                          float *loss, hipStream_t st) {
+  hipLaunchKernelGGL(k_mse, dim3(grid1(N, 256) < 256 ? grid1(N, 256) : 256), dim3(256), 0, st, o,
+                     C, t, N, grad, loss);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_adam(float *w, const float *gw, float *m, float *v, long N, float lr,
+                           float b1, float b2, float eps, int step, uint16_t *wb, uint16_t *wt,
+                           int Co, int Ci, hipStream_t st) {
+  float c1 = 1.f, c2 = 1.f;
+  for (int i = 0; i < step; i++) {
+    c1 *= b1;
+    c2 *= b2;
+  }
+  hipLaunchKernelGGL(k_adam, dim3(grid1(N, 256)), dim3(256), 0, st, w, gw, m, v, N, lr, b1, b2,
+                     eps, 1.f - c1, 1.f - c2, wb, wt, Co, Ci);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_f32_to_bf16(const float *x, uint16_t *y, long N, hipStream_t st) {
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid1(N, 256)), dim3(256), 0, st, x, y, N);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_transpose_w(const uint16_t *wb, uint16_t *wt, int Co, int Ci,
+                                  hipStream_t st) {
+  hipLaunchKernelGGL(k_transpose_w, dim3(grid1(27L * Co * Ci, 256)), dim3(256), 0, st, wb, wt,
+                     Co, Ci);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_mask_down4(const float *f, int X, int Y, int Z, uint16_t *m, int Dz,
+                                 int Dy, int Dx, float far, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_down4, dim3(grid1((long)Dz * Dy * Dx, 256)), dim3(256), 0, st, f, X,
+                     Y, Z, m, Dz, Dy, Dx, far);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_target_down4(const float *u, int X, int Y, int Z, float *t, int Dz,
+                                   int Dy, int Dx, float scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_target_down4, dim3(grid1((long)Dz * Dy * Dx, 256)), dim3(256), 0, st, u,
+                     X, Y, Z, t, Dz, Dy, Dx, scale);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_scale,
+                               float a, float b, const float *init, float *w, int X, int Y,
+                               int Z, hipStream_t st) {
+  hipLaunchKernelGGL(k_warm_up4, dim3(grid1((long)X * Y * Z, 256)), dim3(256), 0, st, pred, C,
+                     Dz, Dy, Dx, inv_scale, a, b, init, w, X, Y, Z);
+  return hip_rc(hipGetLastError());
+}

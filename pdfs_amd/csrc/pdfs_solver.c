@@ -4,6 +4,7 @@
  * pdfs_kernels.h; owns device scratch, epochs, tickets and convergence.
  */
 #include "pdfs.h"
+#include "pdfs_internal.h"
 #include "pdfs_kernels.h"
 
 #include <math.h>
@@ -50,6 +51,17 @@ static int fail(int code, const char *fmt, ...)
   va_end(ap);
   return code;
 }
+
+int pdfsi_fail(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void *pdfs_ctx_stream(pdfs_ctx *c) { return c ? (void *)c->st : NULL; }
 
 #define HIPCHK(call)                                                                 \
   do {                                                                               \

@@ -1,0 +1,198 @@
+"""GPU checks of the warm-start network (BF16 MFMA kernels) against its numpy
+restatement, and of the warm-started solve against the FSM oracle.
+
+Tolerances: activations are stored in BF16 (8 significant bits) after every
+layer on both sides; the GPU accumulates in FP32 in another order, so a
+stored value can round to the neighbouring BF16 number and the difference
+propagates. Forward: max |gpu - ref| <= 3e-2 * max |ref|. Gradients (the
+GPU also rounds the ReLU-masked output gradients to BF16): per layer,
+relative L2 error <= 6e-2 and cosine >= 0.995.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_ctypes as O
+import pdfs_amd
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import nn_ref as R  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 3e-2
+GRAD_REL = 6e-2
+GRAD_COS = 0.995
+
+
+def _bf16_bits(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _mask_tensor(m, dev):
+    """[nb, Dz, Dy, Dx] float mask -> BF16 bits [nb, Dz, Dy, Dx, 16] (channel 0)."""
+    x = np.zeros(m.shape + (16,), np.float32)
+    x[..., 0] = m
+    bits = _bf16_bits(x)
+    return torch.from_numpy(bits.view(np.int16)).to(dev), R.bf16_bits_to_f64(bits)
+
+
+@pytest.fixture(scope="module")
+def net(ctx):
+    n = pdfs_amd.Net(ctx, seed=11)
+    yield n
+    n.close()
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    woff, boff, n = R.offsets()
+    p = np.zeros(n, np.float32)
+    for i, (ci, co) in enumerate(zip(R.CIN, R.COUT)):
+        p[woff[i]:woff[i] + 27 * ci * co] = rng.normal(0, np.sqrt(2.0 / (27 * ci)), 27 * ci * co)
+        p[boff[i]:boff[i] + co] = rng.normal(0, 0.1, co)
+    return p
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 16, 8, 24)])
+def test_forward_matches_numpy(net, dev, shape):
+    p = _random_params(3 + shape[0])
+    net.set_params(p)
+    rng = np.random.default_rng(7)
+    m = (rng.random(shape) < 0.3) * rng.random(shape)
+    xt, xr = _mask_tensor(m, dev)
+    pred = torch.empty(shape + (16,), dtype=torch.float32, device=dev)
+    net.forward(xt, pred)
+    got = pred.cpu().numpy().astype(np.float64)
+    want, _ = R.forward(p, xr, bf16=True)
+    err = np.abs(got - want).max()
+    scale = np.abs(want).max()
+    print(f"forward {shape}: max err {err:.3e} of max {scale:.3e}")
+    assert err <= FWD_TOL * scale
+    assert np.corrcoef(got[..., 0].ravel(), want[..., 0].ravel())[0, 1] > 0.999
+
+
+def test_gradients_match_numpy(net, dev):
+    shape = (2, 8, 8, 16)
+    p = _random_params(21)
+    net.set_params(p)
+    rng = np.random.default_rng(8)
+    m = (rng.random(shape) < 0.3) * rng.random(shape)
+    xt, xr = _mask_tensor(m, dev)
+    t = rng.normal(size=shape).astype(np.float32)
+    loss = net.train_step(xt, torch.from_numpy(t).to(dev), lr=0.0)
+    g = net.params(which=1).astype(np.float64)
+    out, cache = R.forward(p, xr, bf16=True)
+    L, g0 = R.mse(out, t.astype(np.float64))
+    gw = R.backward(cache, g0)
+    assert abs(loss - L) <= 1e-2 * abs(L)
+    woff, boff, _ = R.offsets()
+    for i, name in enumerate(R.NAMES):
+        for lo, n in ((woff[i], 27 * R.CIN[i] * R.COUT[i]), (boff[i], R.COUT[i])):
+            a, b = g[lo:lo + n], gw[lo:lo + n]
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+            print(f"{name}[{n}]: rel {rel:.3e} cos {cos:.6f}")
+            assert rel <= GRAD_REL and cos >= GRAD_COS, (name, n, rel, cos)
+
+
+def test_adam_reduces_loss(ctx, dev):
+    n = pdfs_amd.Net(ctx, seed=5)
+    shape = (2, 8, 8, 8)
+    rng = np.random.default_rng(9)
+    m = (rng.random(shape) < 0.2).astype(np.float64)
+    xt, _ = _mask_tensor(m, dev)
+    t = torch.from_numpy(rng.random(shape).astype(np.float32)).to(dev)
+    l0 = n.train_step(xt, t, lr=0.0)
+    for _ in range(60):
+        n.train_step(xt, t, lr=2e-3)
+    l1 = n.train_step(xt, t, lr=0.0)
+    print(f"loss {l0:.4f} -> {l1:.4f}")
+    assert l1 < 0.5 * l0
+    n.close()
+
+
+def test_features_and_warm_start_match_numpy(net, dev):
+    X, Y, Z = 45, 38, 29
+    a = O.gen_spheres(X, Y, Z, seed=77, n=4, r=(2.0, 5.0))
+    init = np.ascontiguousarray(O.interior(a)).astype(np.float32)
+    ti = torch.from_numpy(init).to(dev)
+    Dz, Dy, Dx = pdfs_amd.net_grid(X, Y, Z)
+    assert (Dz, Dy, Dx) == (8, 16, 16)
+    mask = torch.empty((1, Dz, Dy, Dx, 16), dtype=torch.int16, device=dev)
+    net.features(ti, mask)
+    mb = mask.cpu().numpy().view(np.uint16)
+    want = R.mask_down4(init, Dz, Dy, Dx)
+    assert np.array_equal(R.bf16_bits_to_f64(mb[0, ..., 0]), R.bf16_round(want))
+    assert not mb[..., 1:].any()
+    net.set_params(_random_params(31))
+    net.set_calibration(1.0 / 32, 1.25, 2.0)
+    pred = torch.empty((1, Dz, Dy, Dx, 16), dtype=torch.float32, device=dev)
+    net.forward(mask, pred)
+    w = torch.empty_like(ti)
+    net.warm_start(ti, w, hmax=1.5)
+    want_w = R.warm_up4(pred.cpu().numpy()[0, ..., 0], 1.5 * 32, 1.25, 2.0 * 1.5, init)
+    assert np.array_equal(w.cpu().numpy().view(np.uint32), want_w.view(np.uint32))
+
+
+def _spheres(seed, X=64, Y=56, Z=48):
+    return O.gen_spheres(X, Y, Z, seed=seed, n=5, r=(3.0, 8.0))
+
+
+def test_warm_solve_good_bound_certifies_without_fallback(ctx, net, dev):
+    """A loose upper bound (large inflation) always certifies without a
+    fallback; the result is within the FP32 tolerance of the FP64 oracle."""
+    a = _spheres(90)
+    ref64 = a.copy()
+    O.solve_d64(ref64, tol=0.0)
+    init = torch.from_numpy(np.ascontiguousarray(O.interior(a)).astype(np.float32)).to(dev)
+    net.set_params(_random_params(41))
+    net.set_calibration(1.0 / 32, 0.0, 1000.0)   # w = min(init, 1000 h): above everything
+    f = init.clone()
+    st = ctx.solve_warm_f32(net, f, init)
+    err = np.abs(f.cpu().numpy().astype(np.float64) - O.interior(ref64)).max()
+    assert st["fallbacks"] == 0 and st["judge_flagged"] == 0
+    assert err <= 1e-4
+
+
+def test_warm_solve_bad_bound_falls_back(ctx, net, dev):
+    """An underestimating warm start (everything 0.5 h) leaves a wrong field;
+    the judge flags it and the solver falls back to the plain solve, whose
+    result equals the plain FP32 oracle bit for bit."""
+    a = _spheres(91)
+    ref32 = a.astype(np.float32)
+    O.solve_f32(ref32)
+    init = torch.from_numpy(np.ascontiguousarray(O.interior(a)).astype(np.float32)).to(dev)
+    net.set_calibration(1.0 / 32, 0.0, 0.5)
+    f = init.clone()
+    st = ctx.solve_warm_f32(net, f, init)
+    assert st["fallbacks"] == 1 and st["judge_flagged"] == 0
+    assert np.array_equal(f.cpu().numpy().view(np.uint32), O.interior(ref32).view(np.uint32))
+
+
+@pytest.mark.skipif(not os.path.exists(pdfs_amd.WEIGHTS_PATH), reason="no trained weights")
+def test_trained_weights_warm_start_is_an_upper_bound(ctx, dev):
+    """The committed network with its calibration: on fresh fields of each
+    generator kind the warm start lies above the converged field (no
+    fallback) and the warm-started solve certifies."""
+    n = pdfs_amd.Net(ctx, path=pdfs_amd.WEIGHTS_PATH)
+    X = Y = Z = 96
+    for kind in (pdfs_amd.GEN_POINT, pdfs_amd.GEN_TERRAIN, pdfs_amd.GEN_SPHERES):
+        init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+        ctx.generate(kind, 7000 + kind, X, Y, Z, 1.0, dense=init)
+        plain = init.clone()
+        sp = ctx.solve_f32(plain, init)
+        f = init.clone()
+        st = ctx.solve_warm_f32(n, f, init)
+        w = torch.empty_like(init)
+        n.warm_start(init, w)
+        below = int((w < plain).sum())
+        print(f"kind {kind}: plain {sp['passes']} passes, warm {st['passes']} "
+              f"(fallbacks {st['fallbacks']}), points below the solution {below}")
+        assert st["judge_flagged"] == 0
+        assert (f - plain).abs().max().item() <= 1e-4
+    n.close()

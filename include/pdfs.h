@@ -69,7 +69,8 @@ typedef struct {
   int fixed_passes;/* > 0: run exactly this many passes, no convergence test */
   double tol;      /* FP64: stop when a pass changes nothing by more than tol */
   int judge;       /* FP32: run the FP64 residual judge after convergence */
-  int tile;        /* column tile: 0 16x16, 1 32x8, 2 16x8; -1 default; PDFS_TILE_AUTO tuned
+  int tile;        /* column variant: 0 16x16, 1 32x8, 2 16x8, 3 32x16, 4 16x16 one
+                      workgroup per CU, 5 16x8 two per CU; -1 default; PDFS_TILE_AUTO tuned
                       online on the first passes and cached per size class */
   int quiet;       /* no stdout */
   int profile;     /* time every sweep launch with events on the solver stream */
@@ -198,6 +199,12 @@ int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int
 int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float *init, int x,
                             int y, int z, double dx, double dy, double dz, const pdfs_opts *o,
                             pdfs_stats *st);
+/* batch form: warm starts, one certified batch solve, per-field judge, and a
+ * plain batch re-solve of flagged fields (stats.fallbacks = their count) */
+int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const *fields,
+                                  const float *const *inits, const int *dims, double dx,
+                                  double dy, double dz, const pdfs_opts *o, pdfs_stats *st,
+                                  int *field_passes, int *field_fallback);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,8 @@ PDFS_OK, PDFS_EINVAL, PDFS_EHIP, PDFS_ENOMEM, PDFS_ETIMEOUT, PDFS_ENOTCONV = 0, 
 PDFS_F64_REF, PDFS_F32 = 0, 1
 GEN_POINT, GEN_TERRAIN, GEN_SPHERES = 0, 1, 2
 TILE_16x16, TILE_32x8, TILE_16x8, TILE_AUTO = 0, 1, 2, -2
+TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU = 3, 4, 5
+TILES = (TILE_16x16, TILE_32x8, TILE_16x8, TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU)
 
 EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
@@ -33,6 +35,7 @@ EXPORTED = [
     "pdfs_net_get_calibration", "pdfs_net_set_calibration", "pdfs_net_grid",
     "pdfs_net_features_dev", "pdfs_net_target_dev", "pdfs_net_forward_dev",
     "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_solve_warm_f32_dev",
+    "pdfs_solve_batch_warm_f32_dev",
 ]
 WEIGHTS_PATH = os.path.join(HERE, "weights", "warm_unet.bin")
 CLI_PATH = os.path.join(HERE, "bin", "PDFS")
@@ -162,6 +165,9 @@ def lib():
         L.pdfs_solve_warm_f32_dev.argtypes = [vp, vp, vp, vp, I, I, I, D, D, D, C.POINTER(Opts),
                                               C.POINTER(Stats)]
         L.pdfs_solve_warm_f32_dev.restype = I
+        L.pdfs_solve_batch_warm_f32_dev.argtypes = [vp, vp, I, vp, vp, vp, D, D, D,
+                                                    C.POINTER(Opts), C.POINTER(Stats), vp, vp]
+        L.pdfs_solve_batch_warm_f32_dev.restype = I
         _lib = L
     return _lib
 
@@ -373,6 +379,30 @@ class Context:
                                            Z, *h, C.byref(o), C.byref(st))
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
         return st.as_dict()
+
+
+    def solve_batch_warm_f32(self, net, fields, inits, h=(1.0, 1.0, 1.0), allow_notconv=False,
+                             **kw):
+        """Batched warm-started certified solve (in place). Returns (stats,
+        per-field passes, per-field fallback flags)."""
+        import torch
+        n = len(fields)
+        ptrs = (C.c_void_p * n)(*[self._ptr(f) for f in fields])
+        iptrs = (C.c_void_p * n)(*[self._ptr(t) for t in inits])
+        dims = (C.c_int * (3 * n))()
+        for i, f in enumerate(fields):
+            assert f.dtype == torch.float32 and f.dim() == 3
+            Z, Y, X = f.shape
+            dims[3 * i], dims[3 * i + 1], dims[3 * i + 2] = X, Y, Z
+        passes = (C.c_int * n)()
+        fb = (C.c_int * n)()
+        o = default_opts(precision=PDFS_F32, **kw)
+        st = Stats()
+        torch.cuda.synchronize()
+        rc = lib().pdfs_solve_batch_warm_f32_dev(self.c, net.n, n, ptrs, iptrs, dims, *h,
+                                                 C.byref(o), C.byref(st), passes, fb)
+        _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+        return st.as_dict(), list(passes), list(fb)
 
 
 def net_grid(X, Y, Z):

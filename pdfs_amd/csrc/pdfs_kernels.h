@@ -57,8 +57,17 @@ typedef struct {
   size_t edge_x_bytes, edge_y_bytes;
 } pdfsk_batch;
 
-/* Tile shapes of the column pipeline (autotuner chooses one). */
-enum { PDFSK_TILE_16x16 = 0, PDFSK_TILE_32x8 = 1, PDFSK_TILE_16x8 = 2, PDFSK_TILE_COUNT = 3 };
+/* Variants of the column pipeline (autotuner chooses one): tile shape and
+ * workgroups per CU (max = as many as fit). */
+enum {
+  PDFSK_TILE_16x16 = 0,     /* 256 threads, max per CU */
+  PDFSK_TILE_32x8 = 1,      /* 256 threads, max per CU */
+  PDFSK_TILE_16x8 = 2,      /* 128 threads, max per CU */
+  PDFSK_TILE_32x16 = 3,     /* 512 threads, 1 per CU (LDS) */
+  PDFSK_TILE_16x16_1CU = 4, /* 256 threads, 1 per CU */
+  PDFSK_TILE_16x8_2CU = 5,  /* 128 threads, 2 per CU */
+  PDFSK_TILE_COUNT = 6
+};
 void pdfsk_tile_dims(int variant, int *tx, int *ty);
 
 /* Resident workgroups of the column kernel for a tile variant (0 on error). */

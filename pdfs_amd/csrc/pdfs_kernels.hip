@@ -121,28 +121,44 @@ __device__ __forceinline__ float sqrt_cr(float x) {
   return x > 0.0f ? r : 0.0f;
 }
 
+// sqrt_cr without the x > 0 guard: equal to sqrtf for x >= 0 (x = 0 gives
+// 0: the s - 1 candidate is a NaN pattern whose residual test fails, the
+// s + 1 residual is 0); NaN for x < 0.
+__device__ __forceinline__ float sqrt_cr_nonneg(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = __builtin_fmaf(-sm, s, x);
+  const float rp = __builtin_fmaf(-sp, s, x);
+  float r = rm <= 0.0f ? sm : s;
+  return rp > 0.0f ? sp : r;
+}
+
 // Branch-free isotropic FP32 solve: the same operations as the branching
 // difference form, with the 1-D / 2-D / 3-D choice made by selects.
-// Inputs are non-negative or +inf (no NaN, no -0), where fminf/fmaxf equal
-// the '<' / '>' selects of the oracle.
+// Inputs are non-negative or +inf (no NaN, no -0): the per-axis minima and
+// the 3-value sort (min3 / med3 / max3) equal the '<' / '>' selects of the
+// oracle. A negative 2-D radicand (a^2 > 2h^2) implies a > h, so u1 <= m1
+// and the 2-D / 3-D candidates (NaN then) are not selected.
 __device__ __forceinline__ float god_iso32(float c, float xm, float xp, float ym, float yp,
                                            float zm, float zp, float h, float two_h2,
                                            float three_h2, float third) {
 #pragma clang fp contract(off)
-  const float mx = __builtin_fminf(xm, xp), my = __builtin_fminf(ym, yp);
-  const float mz = __builtin_fminf(zm, zp);
-  const float lo = __builtin_fminf(mx, my), hi = __builtin_fmaxf(mx, my);
-  const float m0 = __builtin_fminf(lo, mz), mid = __builtin_fmaxf(lo, mz);
-  const float m1 = __builtin_fminf(hi, mid), m2 = __builtin_fmaxf(hi, mid);
+  const float mx = __builtin_elementwise_minimum(xm, xp);
+  const float my = __builtin_elementwise_minimum(ym, yp);
+  const float mz = __builtin_elementwise_minimum(zm, zp);
+  const float m0 = __builtin_elementwise_minimum(__builtin_elementwise_minimum(mx, my), mz);
+  const float m2 = __builtin_elementwise_maximum(__builtin_elementwise_maximum(mx, my), mz);
+  const float m1 = __builtin_amdgcn_fmed3f(mx, my, mz);
   const float u1 = m0 + h;
   const float a = m1 - m0;
   const float aa = a * a;
-  const float r = sqrt_cr(two_h2 - aa);
+  const float r = sqrt_cr_nonneg(two_h2 - aa);
   const float u2 = m0 + 0.5f * (a + r);
   const float b = m2 - m0;
   float D = three_h2 - 2.0f * ((aa + b * b) - a * b);
   D = D > 0x1p-96f ? D : 0.0f;
-  const float u3 = m0 + ((a + b) + sqrt_cr(D)) * third;
+  const float u3 = m0 + ((a + b) + sqrt_cr_nonneg(D)) * third;
   const float t = u2 > m2 ? u3 : u2;
   const float u = u1 > m1 ? t : u1;
   return c < u ? c : u;
@@ -250,6 +266,10 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
 }
 
 // ---------------------------------------------------------------------------
+#ifndef PDFSK_UE
+#define PDFSK_UE 4
+#endif
+
 // Column-pipeline sweep, FP32, one ordering.
 //
 // Virtual coordinates make every ordering (+,+,+): virtual v in [1, dim]
@@ -307,6 +327,10 @@ struct ColGeom {
 
 template <int TX, int TY, int U, bool ISO>
 __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
+  // queue depths: interior/halo plane loads U steps ahead (HBM latency),
+  // edge entries UE steps ahead; GATE = start lag beyond the inherent skew
+  constexpr int UE = PDFSK_UE, GATE = PDFSK_UE + 4;
+  static_assert(U % UE == 0, "edge queue depth must divide the plane queue depth");
   using G = ColGeom<TX, TY>;
   constexpr int NT = G::NT, R = G::R, PW = G::PW, PS = G::PS;
   __shared__ float lds[G::LDS_FLOATS];
@@ -334,10 +358,27 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       s_changed = 0;
     }
     __syncthreads();
-    const int col = s_col;
+    // the ticket and everything derived from it are wave-uniform: keep them
+    // in scalar registers (a divergent buffer descriptor would make every
+    // buffer operation a waterfall loop)
+    const int col = __builtin_amdgcn_readfirstlane(s_col);
     if ((unsigned)col >= ncols) break;
-    const int fi = a.cols[3 * col], ca = a.cols[3 * col + 1], cb = a.cols[3 * col + 2];
-    const FieldDesc fd = a.fields[fi];
+    const int fi = __builtin_amdgcn_readfirstlane(a.cols[3 * col]);
+    const int ca = __builtin_amdgcn_readfirstlane(a.cols[3 * col + 1]);
+    const int cb = __builtin_amdgcn_readfirstlane(a.cols[3 * col + 2]);
+    FieldDesc fd = a.fields[fi];
+    {
+      const uint64_t fp = (uint64_t)fd.f;
+      const uint32_t flo = __builtin_amdgcn_readfirstlane((uint32_t)fp);
+      const uint32_t fhi = __builtin_amdgcn_readfirstlane((uint32_t)(fp >> 32));
+      fd.f = (float *)(((uint64_t)fhi << 32) | flo);
+      fd.X = __builtin_amdgcn_readfirstlane(fd.X);
+      fd.Y = __builtin_amdgcn_readfirstlane(fd.Y);
+      fd.Z = __builtin_amdgcn_readfirstlane(fd.Z);
+      fd.nTy = __builtin_amdgcn_readfirstlane(fd.nTy);
+      fd.ex_off = __builtin_amdgcn_readfirstlane(fd.ex_off);
+      fd.ey_off = __builtin_amdgcn_readfirstlane(fd.ey_off);
+    }
     const int X = fd.X, Y = fd.Y, Z = fd.Z, nTy = fd.nTy;
     const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
     const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
@@ -381,10 +422,12 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       if (hval) hxy = xyb(hx, hy);
     }
     uint32_t hl = zb(2) + hxy;
+    const int hmask = hmul ? -1 : 0;  // slot offset applies to halo lanes only
     // west / south edge entries (upwind columns), byte offsets from the
     // allocation base; entry of step l is (vz - 1) with vz = l + 1 - skew
     const bool useW = tx == 0, useS = ty == 0;
     const bool hasW = ca > 0, hasS = cb > 0;
+    const bool waitW = useW && hasW, waitS = useS && hasS;
     constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
     const uint32_t exb = PADB + fd.ex_off * 8u, eyb = PADB + fd.ey_off * 8u;
     uint32_t wo = (useW && hasW)
@@ -408,14 +451,37 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       ring[PS + lo] = dom ? v1 : INF;
       lds[hcell + hmul] = hval ? h1 : INF;
     }
+    // Start gate: wait until the upwind columns have published this column's
+    // entries of step GATE. Edge entries are prefetched UE steps ahead; a
+    // column that starts at least GATE > UE steps behind its producers finds
+    // them published in the steady state instead of catching up and paying a
+    // memory round trip on every step. (Upwind columns never wait on this one,
+    // so the wait ends; the spin is bounded like every edge wait.)
+    {
+      const int nl = Z + TX + TY - 2;  // last step
+      const int gl = nl < GATE ? nl : GATE;
+      const int gz = gl + 1 - skew;
+      const bool gact = dom && (unsigned)(gz - 1) < (unsigned)Z;
+      if (waitW && gact) {
+        const uint64_t *p = a.edge_x + (wo + (uint32_t)gl * wo_inc) / 8u;
+        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge(p, epoch, a.err);
+      }
+      if (waitS && gact) {
+        const uint64_t *p = a.edge_y + (so + (uint32_t)gl * so_inc) / 8u;
+        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge(p, epoch, a.err);
+      }
+    }
     float qP[U], qH[U];
-    u32x2 qW[U], qS[U];
+    u32x2 qW[UE], qS[UE];
 #pragma unroll
     for (int j = 0; j < U; j++) {
       qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
       qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
       ld += dZ;
       hl += dZ;
+    }
+#pragma unroll
+    for (int j = 0; j < UE; j++) {
       qW[j] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
       qS[j] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
       wo += wo_inc;
@@ -443,7 +509,7 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
           {
             const bool live = l + 2 <= Z;
             ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
-            lds[hcell + slot_w * hmul] = (hval && live) ? qH[j] : INF;
+            lds[hcell + ((slot_w * PS) & hmask)] = (hval && live) ? qH[j] : INF;
           }
           qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
           qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
@@ -467,18 +533,20 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
           float xm = pc[-1], xp = pc[1], ym = pc[-PW], yp = pc[PW];
           const int vz = l + 1 - skew;
           const bool act = dom && (unsigned)(vz - 1) < (unsigned)Z;
-          // edge entries of this step (issued U steps ago; wo/so point U ahead)
-          if (useW) {
-            float w = __uint_as_float(qW[j].x);
-            if (hasW && act && qW[j].y != epoch)
-              w = wait_edge(a.edge_x + (wo - U * wo_inc) / 8u, epoch, a.err);
-            xm = hasW ? w : INF;
-          }
-          if (useS) {
-            float v = __uint_as_float(qS[j].x);
-            if (hasS && act && qS[j].y != epoch)
-              v = wait_edge(a.edge_y + (so - U * so_inc) / 8u, epoch, a.err);
-            ym = hasS ? v : INF;
+          // edge entries of this step (issued U steps ago; wo/so point U ahead);
+          // a stale entry (its upwind column is behind) is waited for on a
+          // rare path shared by both directions
+          {
+            const int je = j % UE;
+            float w = __uint_as_float(qW[je].x), v = __uint_as_float(qS[je].x);
+            const bool sw = waitW && act && qW[je].y != epoch;
+            const bool ss = waitS && act && qS[je].y != epoch;
+            if (__builtin_expect(sw || ss, 0)) {
+              if (sw) w = wait_edge(a.edge_x + (wo - UE * wo_inc) / 8u, epoch, a.err);
+              if (ss) v = wait_edge(a.edge_y + (so - UE * so_inc) / 8u, epoch, a.err);
+            }
+            xm = useW ? (hasW ? w : INF) : xm;
+            ym = useS ? (hasS ? v : INF) : ym;
           }
           const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
           nchg += (act && u != c) ? 1u : 0u;
@@ -494,9 +562,9 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
           }
           pe += TY * 8u;
           pn += TX * 8u;
-          // refill this step's edge queue entries with step l + U
-          qW[j] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
-          qS[j] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
+          // refill this step's edge queue entries with step l + UE
+          qW[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
+          qS[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
           wo += wo_inc;
           so += so_inc;
           zm = act ? u : INF;
@@ -777,18 +845,26 @@ static inline int grid_for(long n, int bs) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Column pipeline variants: tile shape and a cap on workgroups per CU (0 =
+// as many as fit). A step of one column is latency-bound (barrier, LDS
+// round trip, dependent Godunov chain); several columns per CU mostly
+// compete for the same issue slots, so capped variants can be faster.
+struct ColVariant {
+  int tx, ty, per_cu;
+};
+static const ColVariant kVariants[PDFSK_TILE_COUNT] = {
+    {16, 16, 0}, {32, 8, 0}, {16, 8, 0}, {32, 16, 0}, {16, 16, 1}, {16, 8, 2}};
+
 extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
-  switch (variant) {
-    case PDFSK_TILE_32x8: *tx = 32; *ty = 8; break;
-    case PDFSK_TILE_16x8: *tx = 16; *ty = 8; break;
-    default: *tx = 16; *ty = 16; break;
-  }
+  const ColVariant &v = kVariants[(unsigned)variant < PDFSK_TILE_COUNT ? variant : 0];
+  *tx = v.tx;
+  *ty = v.ty;
 }
 
 #define PDFSK_U 8
 
 template <int TX, int TY, bool ISO>
-static int max_resident() {
+static int max_resident(int per_cu) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, PDFSK_U, ISO>,
                                                    TX * TY, 0) != hipSuccess)
@@ -797,15 +873,20 @@ static int max_resident() {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
+  if (per_cu > 0 && per_cu < nb) nb = per_cu;
   return nb * ncu;
 }
 
 extern "C" int pdfsk_col_max_resident(int variant, int iso) {
-  switch (variant) {
-    case PDFSK_TILE_32x8: return iso ? max_resident<32, 8, true>() : max_resident<32, 8, false>();
-    case PDFSK_TILE_16x8: return iso ? max_resident<16, 8, true>() : max_resident<16, 8, false>();
-    default: return iso ? max_resident<16, 16, true>() : max_resident<16, 16, false>();
-  }
+  if ((unsigned)variant >= PDFSK_TILE_COUNT) return 0;
+  const ColVariant &v = kVariants[variant];
+  if (v.tx == 32 && v.ty == 8)
+    return iso ? max_resident<32, 8, true>(v.per_cu) : max_resident<32, 8, false>(v.per_cu);
+  if (v.tx == 16 && v.ty == 8)
+    return iso ? max_resident<16, 8, true>(v.per_cu) : max_resident<16, 8, false>(v.per_cu);
+  if (v.tx == 32 && v.ty == 16)
+    return iso ? max_resident<32, 16, true>(v.per_cu) : max_resident<32, 16, false>(v.per_cu);
+  return iso ? max_resident<16, 16, true>(v.per_cu) : max_resident<16, 16, false>(v.per_cu);
 }
 
 template <int TX, int TY>
@@ -839,11 +920,16 @@ extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp
   a.changed = b->changed;
   a.err = b->err;
   a.sp = *sp;
-  switch (variant) {
-    case PDFSK_TILE_32x8: launch_col<32, 8>(a, grid, st, sp->iso); break;
-    case PDFSK_TILE_16x8: launch_col<16, 8>(a, grid, st, sp->iso); break;
-    default: launch_col<16, 16>(a, grid, st, sp->iso); break;
-  }
+  int tx = 16, ty = 16;
+  pdfsk_tile_dims(variant, &tx, &ty);
+  if (tx == 32 && ty == 8)
+    launch_col<32, 8>(a, grid, st, sp->iso);
+  else if (tx == 16 && ty == 8)
+    launch_col<16, 8>(a, grid, st, sp->iso);
+  else if (tx == 32 && ty == 16)
+    launch_col<32, 16>(a, grid, st, sp->iso);
+  else
+    launch_col<16, 16>(a, grid, st, sp->iso);
   return hip_rc(hipGetLastError());
 }
 

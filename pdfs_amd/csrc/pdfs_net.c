@@ -612,3 +612,128 @@ int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float 
   if (st) *st = sw;
   return PDFS_OK;
 }
+
+/* Batched warm-started certified solve: every field gets its warm start, the
+ * batch runs in shared persistent launches with certified stop, then each
+ * field is judged; flagged (or unconverged) fields are reset to their init
+ * and solved plainly in a second batch. field_passes / field_fallback
+ * (optional) report per field. */
+int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const *fields,
+                                  const float *const *inits, const int *dims, double dx,
+                                  double dy, double dz, const pdfs_opts *o, pdfs_stats *st,
+                                  int *field_passes, int *field_fallback)
+{
+  if (!c || !n || nf < 1 || !fields || !inits || !dims)
+    return pdfsi_fail(PDFS_EINVAL, "pdfs_solve_batch_warm_f32_dev: bad arguments");
+  pdfs_opts oo;
+  if (o)
+    oo = *o;
+  else
+    pdfs_default_opts(&oo);
+  hipStream_t s = (hipStream_t)pdfs_ctx_stream(c);
+  double hmax = dx > dy ? dx : dy;
+  if (dz > hmax) hmax = dz;
+  hipEvent_t e0, e1;
+  HCK(hipEventCreate(&e0));
+  HCK(hipEventCreate(&e1));
+  HCK(hipEventRecord(e0, s));
+  int rc = 0;
+  for (int f = 0; f < nf && !rc; f++)
+    rc = pdfs_warm_start_f32_dev(n, inits[f], fields[f], dims[3 * f], dims[3 * f + 1],
+                                 dims[3 * f + 2], hmax);
+  if (rc) return rc;
+  HCK(hipEventRecord(e1, s));
+  HCK(hipEventSynchronize(e1));
+  float wms = 0.f;
+  hipEventElapsedTime(&wms, e0, e1);
+  pdfs_opts ow = oo;
+  ow.certify = 1;
+  ow.judge = 0;
+  pdfs_stats sw;
+  int *passes = (int *)calloc((size_t)nf, sizeof(int));
+  int *bad = (int *)calloc((size_t)nf, sizeof(int));
+  float **rf = (float **)calloc((size_t)nf, sizeof(float *));
+  const float **ri = (const float **)calloc((size_t)nf, sizeof(float *));
+  int *rd = (int *)calloc((size_t)3 * nf, sizeof(int));
+  if (!passes || !bad || !rf || !ri || !rd) {
+    rc = pdfsi_fail(PDFS_ENOMEM, "pdfs_solve_batch_warm_f32_dev: out of memory");
+    goto out;
+  }
+  rc = pdfs_solve_batch_f32_dev(c, nf, fields, inits, dims, dx, dy, dz, &ow, &sw, passes, NULL);
+  if (rc && rc != PDFS_ENOTCONV) goto out;
+  int notconv = rc == PDFS_ENOTCONV;
+  rc = 0;
+  /* judge every field (certified ones pass by construction; this also
+   * catches fields that stopped at the pass cap) */
+  HCK(hipEventRecord(e0, s));
+  int nbad = 0;
+  for (int f = 0; f < nf; f++) {
+    long long fl = 0;
+    if ((rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
+                                 dims[3 * f + 2], dx, dy, dz, NULL, &fl)))
+      goto out;
+    sw.judge_runs++;
+    if (fl > 0 || notconv) {
+      bad[f] = 1;
+      nbad++;
+    }
+  }
+  HCK(hipEventRecord(e1, s));
+  HCK(hipEventSynchronize(e1));
+  float jms = 0.f;
+  hipEventElapsedTime(&jms, e0, e1);
+  sw.ms_judge += jms;
+  sw.fallbacks = 0;
+  sw.judge_flagged = 0;
+  if (nbad) {
+    int k = 0;
+    for (int f = 0; f < nf; f++) {
+      if (!bad[f]) continue;
+      HCK(hipMemcpyAsync(fields[f], inits[f],
+                         (size_t)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2] * sizeof(float),
+                         hipMemcpyDeviceToDevice, s));
+      rf[k] = fields[f];
+      ri[k] = inits[f];
+      memcpy(rd + 3 * k, dims + 3 * f, 3 * sizeof(int));
+      k++;
+    }
+    pdfs_opts op = oo;
+    op.judge = 1;
+    pdfs_stats sp;
+    int *pp = (int *)calloc((size_t)k, sizeof(int));
+    if (!pp) {
+      rc = pdfsi_fail(PDFS_ENOMEM, "pdfs_solve_batch_warm_f32_dev: out of memory");
+      goto out;
+    }
+    rc = pdfs_solve_batch_f32_dev(c, k, rf, ri, rd, dx, dy, dz, &op, &sp, pp, NULL);
+    k = 0;
+    for (int f = 0; f < nf; f++)
+      if (bad[f]) passes[f] += pp[k++];
+    free(pp);
+    if (rc && rc != PDFS_ENOTCONV) goto out;
+    sw.passes += sp.passes;
+    sw.orderings += sp.orderings;
+    sw.point_updates += sp.point_updates;
+    sw.ms_sweeps += sp.ms_sweeps;
+    sw.ms_judge += sp.ms_judge;
+    sw.judge_runs += sp.judge_runs;
+    sw.judge_flagged = sp.judge_flagged;
+    sw.fallbacks = nbad;
+  }
+  sw.ms_warm = wms;
+  sw.ms_total = sw.ms_sweeps + sw.ms_judge + wms;
+  if (st) *st = sw;
+  for (int f = 0; f < nf; f++) {
+    if (field_passes) field_passes[f] = passes[f];
+    if (field_fallback) field_fallback[f] = bad[f];
+  }
+out:
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  free(passes);
+  free(bad);
+  free(rf);
+  free(ri);
+  free(rd);
+  return rc;
+}

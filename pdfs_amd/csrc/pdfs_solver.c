@@ -190,10 +190,13 @@ static long long tune_key(int nf, const int *dims)
   return ((long long)bn << 24) | (bx << 16) | (by << 8) | bz;
 }
 
-static int default_tile(int x, int y)
+/* Measured on MI355X (spheres, FP32): 32x16 tiles (512 threads, one
+ * workgroup per CU) sweep 512^3 in 1.67 ms per ordering against 2.14 ms for
+ * 16x16 at three per CU, and a 512-field batch in 160 ms against 166 ms;
+ * at 256^3 and below 16x16 is as fast or faster. */
+static int default_tile(int nf, int x, int y)
 {
-  (void)x;
-  (void)y;
+  if (nf > 1 || (long long)x * y >= 512LL * 512) return PDFSK_TILE_32x16;
   return PDFSK_TILE_16x16;
 }
 
@@ -419,7 +422,7 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
 {
   if (!c || !field || x < 1 || y < 1 || z < 1 || sn < 1 || sn > 8)
     return fail(PDFS_EINVAL, "pdfs_sweep_f32_dev: bad arguments");
-  if (tile < 0 || tile >= PDFSK_TILE_COUNT) tile = default_tile(x, y);
+  if (tile < 0 || tile >= PDFSK_TILE_COUNT) tile = default_tile(1, x, y);
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
   int dims[3] = {x, y, z}, rc;
@@ -488,7 +491,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
       tile = 0;
     }
   } else {
-    tile = default_tile(dims[0], dims[1]);
+    tile = default_tile(nf, dims[0], dims[1]);
   }
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);

@@ -1,14 +1,20 @@
 """Benchmark: converged fields/s of the FP32 FSM on one MI355X, plus the
 sweep kernel's achieved HBM bandwidth against the 8 TB/s peak.
 
-Workload: multi-sphere distance field, 512^3, h = 1, 64 seeded spheres of
-radius U(4,32) (interior 0, shell within 1.5h exact distance, rest 1e6),
-generated on the GPU. A step = reset the device field from the resident init
-(device copy) + one converged plain FSM solve (passes of the 8 orderings
-until a pass changes no FP32 value) + the FP64 residual judge.
+Workload (the `value`): multi-sphere distance field, 512^3, h = 1, 64 seeded
+spheres of radius U(4,32) (interior 0, shell within 1.5h exact distance,
+rest 1e6), generated on the GPU. A step = reset the device field from the
+resident init (device copy) + one converged plain FSM solve (passes of the 8
+orderings until a pass changes no FP32 value) + the FP64 residual judge.
+
+Also reported (in "extra"): the certified stop (passes until the FP64 judge
+passes) and the learned warm start on the same field and on a fresh
+256x256x128 terrain field; the 512-field uneven batch solved in shared
+persistent launches (plain, certified, warm-started) against each field in
+its own launches; FP32 vs FP64 error; the reference CPU core's rate.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
-       [--no-oracle] [--no-cpu]
+       [--no-oracle] [--no-cpu] [--batch N] [--no-warm]
 Prints one JSON line.
 """
 import argparse
@@ -39,7 +45,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--cpu-orderings", type=int, default=3)
     ap.add_argument("--batch", type=int, default=512, help="fields of the c4 batch leg (0: skip)")
+    ap.add_argument("--no-warm", action="store_true", help="skip the certified / warm-start legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run plain 1 GiB copies (for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the single-field solve does not shard across GPUs in this build.", file=sys.stderr)
@@ -96,6 +108,18 @@ def main():
     extra = {"passes": st["passes"], "orderings": st["orderings"],
              "judge_flagged": int(st["judge_flagged"]), "ms_judge": round(pst["ms_judge"], 4),
              "sweep_pass_gbs": round(sweep_gbs_total, 1)}
+    net = None
+    if not args.no_warm:
+        net = pdfs_amd.Net(ctx, path=pdfs_amd.WEIGHTS_PATH) if os.path.exists(pdfs_amd.WEIGHTS_PATH) else None
+        extra["modes_512"] = solve_modes(ctx, net, init, field, args.tile, ms_per_step)
+        tinit = torch.empty((128, 256, 256), dtype=torch.float32, device=dev)
+        ctx.generate_ex(pdfs_amd.GEN_TERRAIN, 777, 256, 256, 128, 1.0, 8, (5.0, 40.0), (10.0, 60.0),
+                        dense=tinit)
+        extra["modes_terrain_256x256x128"] = solve_modes(ctx, net, tinit, torch.empty_like(tinit),
+                                                         args.tile, None)
+        del tinit
+        field.copy_(init)
+        ctx.solve_f32(field, init, tile=args.tile)
 
     # FP64 reference-arithmetic solve of the same field on the GPU: the
     # oracle at this size (bitwise-equal to the reference core at small sizes)
@@ -118,7 +142,9 @@ def main():
         del padded, inner, err
 
     if args.batch > 0:
-        extra["batch"] = batch_leg(ctx, args.batch)
+        extra["batch"] = batch_leg(ctx, args.batch, net)
+    if net is not None:
+        net.close()
 
     cpu = None
     if not args.no_cpu:
@@ -141,7 +167,8 @@ def main():
                                "FP32, until a pass changes nothing, + FP64 judge",
                    "grid": [X, Y, Z], "seed": args.seed, "solver": "plain", "tile": st["tile"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(),
                      "kernel": "k_sweep_col", "avg_launch_ms": round(avg_launch_ms, 4),
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
@@ -151,10 +178,78 @@ def main():
     ctx.close()
 
 
-def batch_leg(ctx, nfields):
+def copy_calibration():
+    """Plain copies of known size for the HBM counter calibration: 1 GiB read
+    + 1 GiB written by torch's copy (runtime blit kernel) and by an
+    element-wise kernel (x + 0), three times each."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    for _ in range(3):
+        torch.add(a, 0.0, out=b)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {n * 4} bytes read and written per kernel", flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per k_sweep_col launch on this workload from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    counter's reading for a plain copy of known size), or None."""
+    p = os.path.join(ROOT, "profiles", "k_sweep_col_spheres512.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def solve_modes(ctx, net, init, field, tile, plain_ms):
+    """Plain, certified and warm-started solves of one field: passes, time,
+    judge result, fallbacks (best of 3 for the timed modes)."""
+    import torch
+    out = {}
+
+    def timed(fn, reps=3):
+        best, st = None, None
+        for _ in range(reps):
+            field.copy_(init)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s = fn()
+            torch.cuda.synchronize()
+            dt = 1000.0 * (time.perf_counter() - t0)
+            if best is None or dt < best:
+                best, st = dt, s
+        return best, st
+
+    if plain_ms is None:
+        plain_ms, sp = timed(lambda: ctx.solve_f32(field, init, tile=tile))
+    else:
+        field.copy_(init)
+        sp = ctx.solve_f32(field, init, tile=tile)
+    out["plain"] = {"passes": sp["passes"], "ms": round(plain_ms, 3),
+                    "judge_flagged": int(sp["judge_flagged"])}
+    cms, sc = timed(lambda: ctx.solve_f32(field, init, tile=tile, certify=1))
+    out["certified"] = {"passes": sc["passes"], "ms": round(cms, 3), "judge_runs": sc["judge_runs"],
+                        "judge_flagged": int(sc["judge_flagged"]),
+                        "speedup_vs_plain": round(plain_ms / cms, 3)}
+    if net is not None:
+        wms, sw = timed(lambda: ctx.solve_warm_f32(net, field, init, tile=tile))
+        out["warm"] = {"passes": sw["passes"], "ms": round(wms, 3), "ms_network": round(sw["ms_warm"], 3),
+                       "fallbacks": sw["fallbacks"], "judge_flagged": int(sw["judge_flagged"]),
+                       "speedup_vs_plain": round(plain_ms / wms, 3)}
+    log(f"[bench] modes {tuple(init.shape)}: {out}")
+    return out
+
+
+def batch_leg(ctx, nfields, net=None):
     """c4: uneven fields in one persistent launch per ordering (graph-replayed
     passes, tile tuned online), against each field solved by its own
-    launches, with per-field completion times (ms since the batch start)."""
+    launches, with per-field completion times (ms since the batch start);
+    the same batch with certified stop and with the learned warm start."""
     import numpy as np
     import torch
 
@@ -193,6 +288,28 @@ def batch_leg(ctx, nfields):
                                   "p50_ms": round(float(np.percentile(done, 50)), 3),
                                   "p99_ms": round(float(np.percentile(done, 99)), 3)},
            "speedup": round(ts / tb, 2), "bitwise_equal_to_per_field": bool(same)}
+    # certified stop in the batch (per-field FP64 judge from pass 1)
+    fields = [t.clone() for t in inits]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stc, pc, _ = ctx.solve_batch_f32(fields, inits, tile=pdfs_amd.TILE_AUTO, graph=1, certify=1)
+    torch.cuda.synchronize()
+    tc = time.perf_counter() - t0
+    out["certified"] = {"seconds": round(tc, 4), "fields_per_s": round(nfields / tc, 2),
+                        "passes_max": max(pc), "judge_flagged": int(stc["judge_flagged"]),
+                        "speedup_vs_per_field_plain": round(ts / tc, 2)}
+    if net is not None:
+        fields = [t.clone() for t in inits]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stw, pw, fb = ctx.solve_batch_warm_f32(net, fields, inits, tile=pdfs_amd.TILE_AUTO, graph=1)
+        torch.cuda.synchronize()
+        tw = time.perf_counter() - t0
+        out["warm"] = {"seconds": round(tw, 4), "fields_per_s": round(nfields / tw, 2),
+                       "ms_network": round(stw["ms_warm"], 3), "fallbacks": int(sum(fb)),
+                       "passes_max": max(pw), "judge_flagged": int(stw["judge_flagged"]),
+                       "speedup_vs_per_field_plain": round(ts / tw, 2)}
+    log(f"[bench] c4 certified / warm: {out.get('certified')} {out.get('warm')}")
     log(f"[bench] c4 batched {tb * 1e3:.1f} ms ({out['fields_per_s']} fields/s, p50 "
         f"{out['p50_ms']} p99 {out['p99_ms']} ms) vs per-field {ts * 1e3:.1f} ms; equal={same}")
     return out

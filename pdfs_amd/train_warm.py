@@ -15,8 +15,12 @@ Calibration: on held-out fields, for each inflation a in a small set, b(a)
 mean slack a d + b - u is kept, so that w = min(init, a d + b) lies above the
 solution on every held-out cell.
 
+Calibration fields also include larger grids (256x256x128 terrain, 512^3
+spheres by default), whose distances exceed the 128^3 training fields'.
+
 Usage: python -m pdfs_amd.train_warm --out weights.bin [--fields 2000]
        [--steps 3000] [--seed 1]
+       python -m pdfs_amd.train_warm --out weights.bin --recalibrate old.bin
 """
 import argparse
 import json
@@ -36,6 +40,9 @@ def draw_spec(rng, i, S):
     if i % 2 == 0:
         return (1, 8, (5.0, 40.0), (10.0, 60.0))
     return (2, int(rng.integers(1, 9)), (4.0, 32.0), (0.0, 0.0))
+
+
+CAL_A = (1.0, 1.1, 1.2, 1.35, 1.5, 1.75, 2.0)
 
 
 def make_set(ctx, net, n, seed, S=128, chunk=125, dev="cuda:0"):
@@ -78,26 +85,50 @@ def raw_prediction(ctx, net, init):
     return d
 
 
-def calibrate(ctx, net, n, seed, S=128, dev="cuda:0", margin=1.0):
+LARGE_CALIB = (  # (kind, X, Y, Z, nobj, p, q)
+    (1, 256, 256, 128, 8, (5.0, 40.0), (10.0, 60.0)),
+    (1, 256, 256, 128, 8, (5.0, 40.0), (10.0, 60.0)),
+    (2, 512, 512, 512, 64, (4.0, 32.0), (0.0, 0.0)),
+    (2, 512, 512, 512, 64, (4.0, 32.0), (0.0, 0.0)),
+)
+
+
+def calibrate(ctx, net, n, seed, S=128, dev="cuda:0", margin=1.0, large=LARGE_CALIB):
     import torch
 
     import pdfs_amd
     rng = np.random.default_rng(seed)
     stats = []
-    for i in range(n):
-        kind, nobj, p, q = draw_spec(rng, i, S)
-        init = torch.empty((S, S, S), dtype=torch.float32, device=dev)
-        ctx.generate_ex(kind, seed * 1000003 + i, S, S, S, 1.0, nobj, p, q, dense=init)
+
+    def add(init):
         u = init.clone()
         ctx.solve_f32(u, None, judge=0)
         d = raw_prediction(ctx, net, init)
         unk = init >= 1.0e6
-        stats.append((u[unk].double(), d[unk].double()))
+        # a subsample of the unknown cells keeps the host memory small
+        idx = torch.nonzero(unk.flatten()).flatten()
+        if idx.numel() > 4_000_000:
+            idx = idx[:: idx.numel() // 4_000_000 + 1]
+        uu, dd = u.flatten()[idx].double(), d.flatten()[idx].double()
+        # the worst cell over all unknowns is kept exactly
+        worst = {a: float((u[unk] - a * d[unk].clamp(min=0)).max()) for a in CAL_A}
+        stats.append((uu, dd, worst))
+
+    for i in range(n):
+        kind, nobj, p, q = draw_spec(rng, i, S)
+        init = torch.empty((S, S, S), dtype=torch.float32, device=dev)
+        ctx.generate_ex(kind, seed * 1000003 + i, S, S, S, 1.0, nobj, p, q, dense=init)
+        add(init)
+    for j, (kind, X, Y, Z, nobj, p, q) in enumerate(large):
+        init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+        ctx.generate_ex(kind, seed * 7919 + j, X, Y, Z, 1.0, nobj, p, q, dense=init)
+        add(init)
+        del init
     cands = []
-    for a in (1.0, 1.1, 1.2, 1.35, 1.5, 1.75, 2.0):
-        worst = max(float((u - a * d.clamp(min=0)).max()) for u, d in stats)
+    for a in CAL_A:
+        worst = max(w[a] for _, _, w in stats)
         b = max(0.0, worst + margin)
-        slack = float(np.mean([float((a * d.clamp(min=0) + b - u).mean()) for u, d in stats]))
+        slack = float(np.mean([float((a * d.clamp(min=0) + b - u).mean()) for u, d, _ in stats]))
         cands.append({"a": a, "b": b, "mean_slack": slack})
         log(f"[calib] a={a:.2f} b={b:.2f} mean slack {slack:.2f} h")
     best = min(cands, key=lambda c: c["mean_slack"])
@@ -113,6 +144,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--recalibrate", default=None,
+                    help="load these weights, only recompute the calibration, save to --out")
     args = ap.parse_args()
 
     import torch
@@ -124,6 +157,22 @@ def main():
     ctx = pdfs_amd.Context()
     net = pdfs_amd.Net(ctx, seed=args.seed)
     log(f"[train] network: {net.num_params} parameters")
+    if args.recalibrate:
+        net.load(args.recalibrate)
+        scale = net.calibration[0]
+        best, cands = calibrate(ctx, net, args.val, args.seed + 104729)
+        net.set_calibration(scale, best["a"], best["b"])
+        net.save(args.out)
+        jp = os.path.splitext(args.recalibrate)[0] + ".json"
+        summary = json.load(open(jp)) if os.path.exists(jp) else {}
+        summary["calibration"] = {"scale": scale, "a": best["a"], "b": best["b"], "candidates": cands,
+                                  "large_fields": [list(map(str, c)) for c in LARGE_CALIB]}
+        with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+            json.dump(summary, f, indent=1)
+        log(json.dumps(summary["calibration"]))
+        net.close()
+        ctx.close()
+        return
     t0 = time.perf_counter()
     masks, targets, passes = make_set(ctx, net, args.fields, args.seed)
     torch.cuda.synchronize()

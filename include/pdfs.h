@@ -169,6 +169,8 @@ int pdfs_prepost(int op, const int *location, double *distance, int x, int y, in
  * stream. */
 typedef struct pdfs_net pdfs_net;
 pdfs_net *pdfs_net_create(pdfs_ctx *c, unsigned long long seed); /* He-initialised */
+/* May be called before or after pdfs_ctx_destroy of its context; every other
+ * pdfs_net_* call needs the context alive. */
 void pdfs_net_destroy(pdfs_net *n);
 int pdfs_net_load(pdfs_net *n, const char *path);
 int pdfs_net_save(pdfs_net *n, const char *path);

@@ -247,12 +247,16 @@ class Context:
     """Owns a pdfs_ctx (device scratch, stream). Device arrays are torch tensors."""
 
     def __init__(self):
+        import weakref
         self.c = lib().pdfs_ctx_create()
         if not self.c:
             raise PdfsError(PDFS_EHIP, lib().pdfs_last_error().decode())
+        self._nets = weakref.WeakSet()
 
     def close(self):
         if self.c:
+            for net in list(self._nets):
+                net.close()
             lib().pdfs_ctx_destroy(self.c)
             self.c = None
 
@@ -422,6 +426,7 @@ class Net:
         self.n = lib().pdfs_net_create(ctx.c, seed)
         if not self.n:
             raise PdfsError(PDFS_EHIP, lib().pdfs_last_error().decode())
+        ctx._nets.add(self)  # closed before its context
         if path is not None:
             self.load(path)
 

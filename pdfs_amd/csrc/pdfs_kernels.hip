@@ -269,6 +269,9 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
 #ifndef PDFSK_UE
 #define PDFSK_UE 4
 #endif
+#ifndef PDFSK_GATE_EXTRA
+#define PDFSK_GATE_EXTRA 4
+#endif
 
 // Column-pipeline sweep, FP32, one ordering.
 //
@@ -329,7 +332,7 @@ template <int TX, int TY, int U, bool ISO>
 __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
   // queue depths: interior/halo plane loads U steps ahead (HBM latency),
   // edge entries UE steps ahead; GATE = start lag beyond the inherent skew
-  constexpr int UE = PDFSK_UE, GATE = PDFSK_UE + 4;
+  constexpr int UE = PDFSK_UE, GATE = PDFSK_UE + PDFSK_GATE_EXTRA;
   static_assert(U % UE == 0, "edge queue depth must divide the plane queue depth");
   using G = ColGeom<TX, TY>;
   constexpr int NT = G::NT, R = G::R, PW = G::PW, PS = G::PS;
@@ -515,22 +518,19 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
           qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
           ld += dZ;
           hl += dZ;
-          __syncthreads();
-          // (2) store the finished plane zs = l - (TX+TY-2); before the first
-          // finished plane the store goes out of range (dropped), so every
-          // step issues the same memory operations
-          {
-            const bool sto = l >= TX + TY - 1;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ring[slot_s * PS + lo]), fr,
-                                                  sto ? st : 0xFFFFFFF0u, 0, 0);
-            st += sto ? st_inc : 0u;
-          }
-          // (3) this thread's hyperplane point
+          // the old values above, east and north of this thread's point
+          // (plane vz + 1 of its own cell, plane vz of the x+1 / y+1 cells)
+          // entered the ring in earlier steps and are updated only after this
+          // step: read them before the barrier, off the critical path
           uint32_t svp = sv + PS4;
           svp = svp >= RPS4 ? svp - RPS4 : svp;
           const float *pc = ring + sv / 4 + lo;
           const float zp = ring[svp / 4 + lo];
-          float xm = pc[-1], xp = pc[1], ym = pc[-PW], yp = pc[PW];
+          const float xp = pc[1], yp = pc[PW];
+          __syncthreads();
+          // (2) this thread's hyperplane point; west / south are the values
+          // the neighbouring threads wrote in the previous step
+          float xm = pc[-1], ym = pc[-PW];
           const int vz = l + 1 - skew;
           const bool act = dom && (unsigned)(vz - 1) < (unsigned)Z;
           // edge entries of this step (issued U steps ago; wo/so point U ahead);
@@ -562,6 +562,16 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
           }
           pe += TY * 8u;
           pn += TX * 8u;
+          // (3) store the finished plane zs = l - (TX+TY-2) (complete since the
+          // barrier; its ring slot is refilled only at step l + 2); before the
+          // first finished plane the store goes out of range (dropped), so
+          // every step issues the same memory operations
+          {
+            const bool sto = l >= TX + TY - 1;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ring[slot_s * PS + lo]), fr,
+                                                  sto ? st : 0xFFFFFFF0u, 0, 0);
+            st += sto ? st_inc : 0u;
+          }
           // refill this step's edge queue entries with step l + UE
           qW[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
           qS[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
@@ -648,15 +658,17 @@ __global__ __launch_bounds__(256) void k_judge_f32(const float *__restrict__ v,
                                                    double dz, uint32_t *flags,
                                                    unsigned long long *count) {
 #pragma clang fp contract(off)
-  const long N = (long)X * Y * Z;
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // FP32 fields hold fewer than 2^29 points: 32-bit index arithmetic
+  const uint32_t N = (uint32_t)X * (uint32_t)Y * (uint32_t)Z;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   bool bad = false;
   if (p < N) {
-    const long XY = (long)X * Y;
-    const int x = (int)(p % X);
-    const long q = p / X;
-    const int y = (int)(q % Y);
-    const int z = (int)(q / Y);
+    const uint32_t XY = (uint32_t)X * (uint32_t)Y;
+    const uint32_t q = p / (uint32_t)X;
+    const int x = (int)(p - q * (uint32_t)X);
+    const uint32_t zq = q / (uint32_t)Y;
+    const int y = (int)(q - zq * (uint32_t)Y);
+    const int z = (int)zq;
     const float c = v[p];
     const double xm = x > 0 ? (double)v[p - 1] : __builtin_inf();
     const double xp = x < X - 1 ? (double)v[p + 1] : __builtin_inf();
@@ -673,7 +685,7 @@ __global__ __launch_bounds__(256) void k_judge_f32(const float *__restrict__ v,
   }
   const unsigned long long m = __ballot(bad);
   const int lane = threadIdx.x & 63;
-  const long w0 = (((long)blockIdx.x * blockDim.x + threadIdx.x) - lane) >> 5;
+  const uint32_t w0 = (p - (uint32_t)lane) >> 5;
   if (lane == 0) {
     if ((w0 << 5) < N) flags[w0] |= (uint32_t)m;
     if (((w0 + 1) << 5) < N) flags[w0 + 1] |= (uint32_t)(m >> 32);
@@ -967,6 +979,7 @@ extern "C" int pdfsk_judge_f32(const float *v, const float *init, int X, int Y, 
                                double dx, double dy, double dz, uint32_t *flags,
                                unsigned long long *count, hipStream_t st) {
   const long N = (long)X * Y * Z;
+  if (N >= (1L << 31)) return -1;
   const int bs = 256;
   const long g = (N + bs - 1) / bs;
   hipLaunchKernelGGL(k_judge_f32, dim3((unsigned)g), dim3(bs), 0, st, v, init, X, Y, Z, dx, dy,

@@ -225,7 +225,8 @@ pdfs_net *pdfs_net_create(pdfs_ctx *c, unsigned long long seed)
 void pdfs_net_destroy(pdfs_net *n)
 {
   if (!n) return;
-  if (n->st) hipStreamSynchronize(n->st);
+  /* no call on the context's stream here (the context may already be gone);
+   * hipFree waits for the device's outstanding work itself */
   free_ws(n);
   hipFree(n->P);
   hipFree(n->G);
@@ -634,10 +635,25 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   double hmax = dx > dy ? dx : dy;
   if (dz > hmax) hmax = dz;
   hipEvent_t e0, e1;
+  /* size the network workspace and mask once, for the largest coarse grid */
+  long long vmax = 0;
+  for (int f = 0; f < nf; f++) {
+    const long long v = (long long)coarse(dims[3 * f]) * coarse(dims[3 * f + 1]) *
+                        coarse(dims[3 * f + 2]);
+    if (v > vmax) vmax = v;
+  }
+  int rc = ensure_ws(n, vmax, 0);
+  if (rc) return rc;
+  if (vmax * 16 > n->cap_mask) {
+    HCK(hipStreamSynchronize(n->st));
+    hipFree(n->mask);
+    n->mask = NULL;
+    HCK(hipMalloc((void **)&n->mask, (size_t)vmax * 16 * sizeof(uint16_t)));
+    n->cap_mask = vmax * 16;
+  }
   HCK(hipEventCreate(&e0));
   HCK(hipEventCreate(&e1));
   HCK(hipEventRecord(e0, s));
-  int rc = 0;
   for (int f = 0; f < nf && !rc; f++)
     rc = pdfs_warm_start_f32_dev(n, inits[f], fields[f], dims[3 * f], dims[3 * f + 1],
                                  dims[3 * f + 2], hmax);

@@ -74,13 +74,14 @@ def make_set(ctx, net, n, seed, S=128, chunk=125, dev="cuda:0"):
 
 
 def raw_prediction(ctx, net, init):
-    """Upsampled prediction d (cells) without cap or inflation."""
+    """Upsampled prediction d (cells) without inflation: the warm start with
+    a = 1, b = 0 is min(init, d), which is d on every unknown cell (init
+    1e6); the network's input is the mask of this init."""
     import torch
     scale, a, b = net.calibration
     net.set_calibration(scale, 1.0, 0.0)
-    big = torch.full_like(init, 3.0e38)
     d = torch.empty_like(init)
-    net.warm_start(big, d)
+    net.warm_start(init, d)
     net.set_calibration(scale, a, b)
     return d
 

@@ -277,3 +277,69 @@ def test_certified_batch_matches_single(ctx, dev):
     for t, w in zip(ts, singles):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
     assert st["judge_flagged"] == 0
+
+
+DEGENERATE = [(1, 1, 1), (1, 7, 1), (64, 1, 1), (1, 1, 40), (3, 1, 5), (1, 33, 17)]
+
+
+@pytest.mark.parametrize("shape", DEGENERATE)
+@pytest.mark.parametrize("tile", pdfs_amd.TILES)
+def test_f32_degenerate_shapes_bitwise(ctx, dev, shape, tile):
+    """Single points, rows and planes: every ordering bit-exact against the
+    oracle."""
+    X, Y, Z = shape
+    a = O.random_field(X, Y, Z, seed=3 + X + 7 * Y + 31 * Z)
+    ref = a.astype(np.float32)
+    t = _dense(a, dev)
+    for sn in range(1, 9):
+        O.sweep_f32(ref, sn)
+        ctx.sweep_f32(t, sn, tile=tile)
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(O.interior(ref))), (shape, tile, sn)
+
+
+def test_f32_no_sources_field_unchanged(ctx, dev):
+    """A field without sources (all 1e6) stays as it is: one pass, nothing
+    changed, the judge passes."""
+    t = torch.full((20, 24, 28), 1.0e6, dtype=torch.float32, device=dev)
+    init = t.clone()
+    st = ctx.solve_f32(t, init)
+    assert st["passes"] == 1 and st["last_changed"] == 0 and st["judge_flagged"] == 0
+    assert torch.equal(t, init)
+
+
+def test_batch_with_single_point_field(ctx, dev):
+    fields = [O.gen_point(1, 1, 1, seed=1), O.gen_spheres(40, 31, 22, seed=5, n=3, r=(2.0, 5.0)),
+              O.gen_point(2, 1, 3, seed=2), O.gen_terrain(17, 19, 15, seed=6)]
+    wants = []
+    for a in fields:
+        r = a.astype(np.float32)
+        O.solve_f32(r)
+        wants.append(O.interior(r))
+    ts = [_dense(a, dev) for a in fields]
+    ctx.solve_batch_f32(ts, [t.clone() for t in ts])
+    for t, w in zip(ts, wants):
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
+
+
+def test_largest_f32_field_properties(ctx, dev):
+    """The largest dense FP32 field the column kernel accepts (2^29 - 1
+    points; 1023 x 1024 x 512 here): a converged solve passes the FP64 judge,
+    never raises a value above the init, keeps the sources and fills every
+    cell; one point more is rejected."""
+    X, Y, Z = 1023, 1024, 512
+    assert X * Y * Z <= (1 << 29) - 1
+    init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+    ctx.generate(pdfs_amd.GEN_SPHERES, 77, X, Y, Z, 1.0, dense=init)
+    f = init.clone()
+    st = ctx.solve_f32(f, init, certify=1)
+    print(f"largest field: passes {st['passes']} ms {st['ms_total']:.1f}")
+    assert st["judge_flagged"] == 0
+    assert bool((f <= init).all())
+    src = init == 0
+    assert bool((f[src] == 0).all())
+    assert float(f.max()) < 1.0e6
+    del f, init
+    torch.cuda.empty_cache()
+    big = torch.empty((2, 1 << 14, 1 << 14), dtype=torch.float32, device=dev)  # 2^29 points
+    with pytest.raises(pdfs_amd.PdfsError):
+        ctx.solve_f32(big, None, judge=0)

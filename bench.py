@@ -206,6 +206,21 @@ def measured_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
+# warm-start network convolutions: (Cin, Cout, level); a level-l grid has 1/8^l
+# of the coarse points (the coarse grid is 1/4 of the field per axis)
+NET_CONVS = ((16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+             (64, 64, 3), (64, 64, 3), (128, 64, 2), (96, 32, 1), (48, 16, 0), (16, 16, 0))
+
+
+def network_gflop(X, Y, Z):
+    """Forward FLOPs of the warm-start network on a field: sum over its 3x3x3
+    convolutions of 2 * 27 * Cin * Cout * points."""
+    import pdfs_amd
+    Dz, Dy, Dx = pdfs_amd.net_grid(X, Y, Z)
+    v0 = Dz * Dy * Dx
+    return sum(2.0 * 27 * ci * co * v0 / 8 ** lvl for ci, co, lvl in NET_CONVS) / 1e9
+
+
 def solve_modes(ctx, net, init, field, tile, plain_ms):
     """Plain, certified and warm-started solves of one field: passes, time,
     judge result, fallbacks (best of 3 for the timed modes)."""
@@ -238,7 +253,11 @@ def solve_modes(ctx, net, init, field, tile, plain_ms):
                         "speedup_vs_plain": round(plain_ms / cms, 3)}
     if net is not None:
         wms, sw = timed(lambda: ctx.solve_warm_f32(net, field, init, tile=tile))
+        Z, Y, X = init.shape
+        gflop = network_gflop(X, Y, Z)
         out["warm"] = {"passes": sw["passes"], "ms": round(wms, 3), "ms_network": round(sw["ms_warm"], 3),
+                       "network_gflop": round(gflop, 1),
+                       "network_tflops": round(gflop / sw["ms_warm"], 2) if sw["ms_warm"] > 0 else None,
                        "fallbacks": sw["fallbacks"], "judge_flagged": int(sw["judge_flagged"]),
                        "speedup_vs_plain": round(plain_ms / wms, 3)}
     log(f"[bench] modes {tuple(init.shape)}: {out}")

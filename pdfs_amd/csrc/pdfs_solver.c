@@ -499,9 +499,12 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   if ((rc = plan_build(c, nf, fields, dims, tile))) return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
   int *active = (int *)malloc((size_t)nf * sizeof(int));
-  if (!delta || !active) {
+  /* fields whose passes ended on a passing judge: the final judge skips them */
+  char *certified = (char *)calloc((size_t)nf, 1);
+  if (!delta || !active || !certified) {
     free(delta);
     free(active);
+    free(certified);
     return fail(PDFS_ENOMEM, "batch: host alloc");
   }
   for (int f = 0; f < nf; f++) {
@@ -576,6 +579,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
           break;
         s.judge_runs++;
         done = fl == 0;
+        certified[f] = (char)done;
       }
       if (done) {
         active[f] = 0;
@@ -603,20 +607,22 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   s.ms_sweep_kernels = kms;
   if (rc == PDFS_OK && o->judge && inits) {
     float tj_before = (float)s.ms_judge;
+    int judged = 0;
     HIPCHK(hipEventRecord(c->ev0, c->st));
     for (int f = 0; f < nf && rc == PDFS_OK; f++) {
-      if (!inits[f]) continue;
+      if (!inits[f] || certified[f]) continue;
       long long fl = 0;
       rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
                               dims[3 * f + 2], dx, dy, dz, NULL, &fl);
       s.judge_flagged += fl;
+      judged = 1;
     }
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipEventSynchronize(c->ev1));
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
     s.ms_judge = tj_before + t;
-    s.judge_runs++;
+    s.judge_runs += judged;
   }
   s.ms_total = s.ms_sweeps + s.ms_judge;
   if (st) *st = s;
@@ -624,6 +630,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, s.passes);
   free(delta);
   free(active);
+  free(certified);
   return rc;
 }
 

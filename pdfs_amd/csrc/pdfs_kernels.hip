@@ -873,7 +873,9 @@ extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
   *ty = v.ty;
 }
 
+#ifndef PDFSK_U
 #define PDFSK_U 8
+#endif
 
 template <int TX, int TY, bool ISO>
 static int max_resident(int per_cu) {

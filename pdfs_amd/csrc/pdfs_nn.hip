@@ -33,75 +33,103 @@ static inline unsigned grid1(long n, int bs) {
 
 // ---------------------------------------------------------------------------
 // 3x3x3 convolution, stride 1, zero padding 1, as an implicit GEMM:
-// M = positions, N = Cout, K = 27 taps x Cin. One wave: 16 positions x all
-// Cout (up to 8 tiles of 16); the block: 4 waves = 64 positions.
-// out = act(conv + bias); act: 0 linear, 1 ReLU. out_f32 optional (head).
-// With flip = 1 the taps are mirrored (dz,dy,dx) -> (2-dz, 2-dy, 2-dx):
-// with transposed weights this is the input gradient.
+// M = positions, N = Cout, K = 27 taps x Cin, on LDS-staged input tiles.
+// Block (4 waves): an 8 x 8 x 4 tile of output positions of one sample, as
+// 16 groups of 16 positions (8 along x by 2 along y); wave w owns groups
+// 4w .. 4w+3, each against all Cout (NT tiles of 16). Per chunk of 16 input
+// channels the input tile with a one-point halo, 10 x 10 x 6 positions x 16
+// BF16 = 19.2 KB, is staged in LDS (zero outside the grid); all 27 taps then
+// read their A fragments from LDS, the weight fragments come from L2.
+// out = act(conv + bias); act: 0 linear, 1 ReLU. out (BF16) and out_f32
+// each optional. With flip = 1 the taps are mirrored (dz,dy,dx) ->
+// (2-dz, 2-dy, 2-dx): with transposed weights this is the input gradient.
 // ---------------------------------------------------------------------------
 template <int NT>
-__global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t *__restrict__ in, int Cin,
-                                                   const uint16_t *__restrict__ W,
-                                                   const float *__restrict__ bias, int Cout,
-                                                   uint16_t *__restrict__ out,
-                                                   float *__restrict__ out_f32, int n, int Dz,
-                                                   int Dy, int Dx, int relu, int flip) {
+__global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t *__restrict__ in, int Cin,
+                                                    const uint16_t *__restrict__ W,
+                                                    const float *__restrict__ bias, int Cout,
+                                                    uint16_t *__restrict__ out,
+                                                    float *__restrict__ out_f32, int n, int Dz,
+                                                    int Dy, int Dx, int relu, int flip) {
+  constexpr int HX = 10, HY = 10, HZ = 6, HP = HX * HY * HZ;
+  __shared__ bf16x4 tile[HP * 4];  // [z][y][x][channel quad]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
-  const long M = (long)n * Dz * Dy * Dx;
-  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
-  if (p0 >= M) return;
-  // this lane's A row = position p0 + c
-  const long pa = p0 + c;
-  const bool rowok = pa < M;
-  int ax = 0, ay = 0, az = 0;
-  long ab = 0;
-  if (rowok) {
-    ax = (int)(pa % Dx);
-    long q = pa / Dx;
-    ay = (int)(q % Dy);
-    q /= Dy;
-    az = (int)(q % Dz);
-    ab = q / Dz;
+  const int ntx = (Dx + 7) >> 3, nty = (Dy + 7) >> 3, ntz = (Dz + 3) >> 2;
+  int bid = blockIdx.x;
+  const int bx = bid % ntx;
+  bid /= ntx;
+  const int by = bid % nty;
+  bid /= nty;
+  const int bz = bid % ntz;
+  const long b = bid / ntz;
+  const int x0 = bx * 8, y0 = by * 8, z0 = bz * 4;
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[i][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  // this lane's A row in each of its groups: x = c & 7, y = 2 (q & 3) + (c >> 3), z = q >> 2
+  int arow[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int q = 4 * wave + i;
+    arow[i] = ((q >> 2) * HY + 2 * (q & 3) + (c >> 3)) * HX + (c & 7);
   }
-  f32x4 acc[NT];
+  const int nchunk = Cin >> 4;
+  for (int k = 0; k < nchunk; k++) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < HP * 4; i += 256) {
+      const int v = i & 3, p = i >> 2;
+      const int hx = p % HX, hy = (p / HX) % HY, hz = p / (HX * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      bf16x4 val = {0, 0, 0, 0};
+      if ((unsigned)gx < (unsigned)Dx && (unsigned)gy < (unsigned)Dy && (unsigned)gz < (unsigned)Dz)
+        val = *(const bf16x4 *)(in + (((b * Dz + gz) * Dy + gy) * (long)Dx + gx) * Cin + 16 * k +
+                                4 * v);
+      tile[i] = val;
+    }
+    __syncthreads();
+    for (int tap = 0; tap < 27; tap++) {
+      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+      const int sh = flip ? ((2 - dz) * HY + (2 - dy)) * HX + (2 - dx) : (dz * HY + dy) * HX + dx;
+      // output tiles beyond Cout (Cout = 48 or 96 with NT = 4 or 8) are skipped
+      bf16x4 bw[NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  const int nchunk = Cin / 16;
-  for (int tap = 0; tap < 27; tap++) {
-    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-    const int sz = flip ? 1 - dz : dz - 1, sy = flip ? 1 - dy : dy - 1,
-              sx = flip ? 1 - dx : dx - 1;
-    const int zz = az + sz, yy = ay + sy, xx = ax + sx;
-    const bool ok = rowok && (unsigned)zz < (unsigned)Dz && (unsigned)yy < (unsigned)Dy &&
-                    (unsigned)xx < (unsigned)Dx;
-    const uint16_t *ip = in + ((((ab * Dz + zz) * Dy + yy) * Dx + xx) * (long)Cin) + 4 * g;
-    const uint16_t *wp = W + ((long)tap * Cout + c) * Cin + 4 * g;
-    for (int k = 0; k < nchunk; k++) {
-      bf16x4 a = {0, 0, 0, 0};
-      if (ok) a = *(const bf16x4 *)(ip + 16 * k);
+      for (int t = 0; t < NT; t++)
+        bw[t] = 16 * t < Cout
+                    ? *(const bf16x4 *)(W + ((long)tap * Cout + 16 * t + c) * Cin + 16 * k + 4 * g)
+                    : (bf16x4){0, 0, 0, 0};
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        if (16 * t < Cout) {
-          const bf16x4 b = *(const bf16x4 *)(wp + (long)16 * t * Cin + 16 * k);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-        }
+      for (int i = 0; i < 4; i++) {
+        const bf16x4 a = tile[(arow[i] + sh) * 4 + g];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          if (16 * t < Cout)
+            acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[i][t], 0, 0, 0);
       }
     }
   }
+  // D: lane (c, g) holds rows 4g .. 4g+3 (positions of the group) of column c
 #pragma unroll
-  for (int t = 0; t < NT; t++) {
-    if (16 * t >= Cout) continue;
-    const int co = 16 * t + c;
-    const float bv = bias ? bias[co] : 0.f;
+  for (int i = 0; i < 4; i++) {
+    const int q = 4 * wave + i;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const long p = p0 + 4 * g + e;
-      if (p >= M) continue;
-      float v = acc[t][e] + bv;
-      if (relu) v = v > 0.f ? v : 0.f;
-      if (out) out[p * Cout + co] = f2bf(v);
-      if (out_f32) out_f32[p * Cout + co] = v;
+    for (int t = 0; t < NT; t++) {
+      if (16 * t >= Cout) continue;
+      const int co = 16 * t + c;
+      const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int r = 4 * g + e;
+        const int x = x0 + (r & 7), y = y0 + 2 * (q & 3) + (r >> 3), z = z0 + (q >> 2);
+        if (x >= Dx || y >= Dy || z >= Dz) continue;
+        const long pos = ((b * Dz + z) * Dy + y) * (long)Dx + x;
+        float v = acc[i][t][e] + bv;
+        if (relu) v = v > 0.f ? v : 0.f;
+        if (out) out[pos * Cout + co] = f2bf(v);
+        if (out_f32) out_f32[pos * Cout + co] = v;
+      }
     }
   }
 }
@@ -461,21 +489,23 @@ extern "C" int pdfsnn_conv3(const uint16_t *in, int Cin, const uint16_t *W, cons
                             int Cout, uint16_t *out, float *out_f32, int n, int Dz, int Dy,
                             int Dx, int relu, int flip, hipStream_t st) {
   if (Cin % 16 || Cout % 16 || Cout > 128 || Cin > 128 || Cin < 16 || Cout < 16) return -1;
-  const long M = (long)n * Dz * Dy * Dx;
-  const unsigned blocks = (unsigned)((M + 63) / 64);
+  if (n < 1 || Dz < 1 || Dy < 1 || Dx < 1) return -1;
+  const long blocks = (long)n * ((Dz + 3) / 4) * ((Dy + 7) / 8) * ((Dx + 7) / 8);
+  if (blocks >= (1L << 31)) return -1;
   const int nt = Cout / 16;
+  const dim3 grid((unsigned)blocks), blk(256);
   if (nt == 1)
-    hipLaunchKernelGGL(k_conv3_fwd<1>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
-                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+    hipLaunchKernelGGL(k_conv3_tile<1>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
+                       Dz, Dy, Dx, relu, flip);
   else if (nt == 2)
-    hipLaunchKernelGGL(k_conv3_fwd<2>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
-                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+    hipLaunchKernelGGL(k_conv3_tile<2>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
+                       Dz, Dy, Dx, relu, flip);
   else if (nt <= 4)
-    hipLaunchKernelGGL(k_conv3_fwd<4>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
-                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+    hipLaunchKernelGGL(k_conv3_tile<4>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
+                       Dz, Dy, Dx, relu, flip);
   else
-    hipLaunchKernelGGL(k_conv3_fwd<8>, dim3(blocks), dim3(256), 0, st, in, Cin, W, bias, Cout,
-                       out, out_f32, n, Dz, Dy, Dx, relu, flip);
+    hipLaunchKernelGGL(k_conv3_tile<8>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
+                       Dz, Dy, Dx, relu, flip);
   return hip_rc(hipGetLastError());
 }
 

@@ -51,6 +51,9 @@ typedef struct {
  * as cudaCheckError does (single_gpu_cuda/src/phi3D_fsm.h:39-46). Prints
  * "Parallel FSM time: %f s." (phi3D_fsm.cu:104). */
 void run_fsm(Phi *pf, int iterations);
+/* run_fsm, runFSM, pdfs_solve and pdfs_calc_dist_field share one
+ * process-wide context (created on first use, kept until exit): like the
+ * reference, they are not to be called from several threads at once. */
 
 /* One ordering sn (1..8) of the reference, FP64, in place on the padded
  * array of an OpenACC-layout field. */

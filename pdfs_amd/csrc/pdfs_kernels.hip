@@ -109,21 +109,9 @@ __device__ __forceinline__ T god_diff_iso(T cur, T mx, T my, T mz, T h, T two_h2
 // Correctly rounded FP32 square root for x == 0 or x >= 2^-96: v_sqrt_f32
 // (within one ulp) followed by a one-ulp correction from the two FMA
 // residuals x - (s -/+ ulp) s. Inputs here never reach the denormal range
-// (see radicand_floor), so no rescaling is needed.
-__device__ __forceinline__ float sqrt_cr(float x) {
-  const float s = __builtin_amdgcn_sqrtf(x);
-  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-  const float rm = __builtin_fmaf(-sm, s, x);
-  const float rp = __builtin_fmaf(-sp, s, x);
-  float r = rm <= 0.0f ? sm : s;
-  r = rp > 0.0f ? sp : r;
-  return x > 0.0f ? r : 0.0f;
-}
-
-// sqrt_cr without the x > 0 guard: equal to sqrtf for x >= 0 (x = 0 gives
-// 0: the s - 1 candidate is a NaN pattern whose residual test fails, the
-// s + 1 residual is 0); NaN for x < 0.
+// (see radicand_floor), so no rescaling is needed. x = 0 gives 0 (the s - 1
+// candidate is a NaN pattern whose residual test fails, the s + 1 residual is
+// 0); x < 0 gives NaN, which the callers discard.
 __device__ __forceinline__ float sqrt_cr_nonneg(float x) {
   const float s = __builtin_amdgcn_sqrtf(x);
   const float sm = __uint_as_float(__float_as_uint(s) - 1u);
@@ -241,10 +229,6 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t ld_edge(const uint64_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_edge(uint64_t *p, float v, uint32_t epoch) {
-  uint64_t e = ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(v);
-  __hip_atomic_store(p, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Bounded wait for an edge entry of the current epoch (2 s of the 100 MHz

@@ -77,6 +77,18 @@ struct pdfs_net {
                         __FILE__, __LINE__, #call, r_);                              \
   } while (0)
 
+/* HIP call in a function that owns host allocations: record and leave via
+ * its `out` label */
+#define HGO(call)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      rc = pdfsi_fail(PDFS_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,          \
+                      hipGetErrorString(e_));                                        \
+      goto out;                                                                      \
+    }                                                                                \
+  } while (0)
+
 static unsigned long long sm64(unsigned long long *s)
 {
   unsigned long long z = (*s += 0x9E3779B97F4A7C15ull);
@@ -681,7 +693,7 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   rc = 0;
   /* judge every field (certified ones pass by construction; this also
    * catches fields that stopped at the pass cap) */
-  HCK(hipEventRecord(e0, s));
+  HGO(hipEventRecord(e0, s));
   int nbad = 0;
   for (int f = 0; f < nf; f++) {
     long long fl = 0;
@@ -694,8 +706,8 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
       nbad++;
     }
   }
-  HCK(hipEventRecord(e1, s));
-  HCK(hipEventSynchronize(e1));
+  HGO(hipEventRecord(e1, s));
+  HGO(hipEventSynchronize(e1));
   float jms = 0.f;
   hipEventElapsedTime(&jms, e0, e1);
   sw.ms_judge += jms;
@@ -705,7 +717,7 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
     int k = 0;
     for (int f = 0; f < nf; f++) {
       if (!bad[f]) continue;
-      HCK(hipMemcpyAsync(fields[f], inits[f],
+      HGO(hipMemcpyAsync(fields[f], inits[f],
                          (size_t)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2] * sizeof(float),
                          hipMemcpyDeviceToDevice, s));
       rf[k] = fields[f];

@@ -79,6 +79,18 @@ void *pdfs_ctx_stream(pdfs_ctx *c) { return c ? (void *)c->st : NULL; }
                   #call, r_, hipGetErrorString((hipError_t)r_));                     \
   } while (0)
 
+/* HIP call inside a function that owns host allocations: record the error
+ * and leave through its `out` label */
+#define HIPGO(call)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      rc = fail(PDFS_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,                \
+                hipGetErrorString(e_));                                              \
+      goto out;                                                                      \
+    }                                                                                \
+  } while (0)
+
 void pdfs_default_opts(pdfs_opts *o)
 {
   memset(o, 0, sizeof *o);
@@ -517,14 +529,14 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   double ms = 0.0, kms = 0.0;
   long long last_total = 0;
   rc = PDFS_OK;
-  HIPCHK(hipEventRecord(c->ev0, c->st));
+  HIPGO(hipEventRecord(c->ev0, c->st));
   for (int p = 0; p < maxp && nactive > 0; p++) {
     long long npts = 0;
     for (int f = 0; f < nf; f++)
       if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
     float t_prev = (float)ms;
     if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph && !tuning, &kms))) break;
-    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPGO(hipEventRecord(c->ev1, c->st));
     unsigned err = 0;
     if ((rc = read_changed(c, delta, &err))) break;
     float t = 0.f;
@@ -563,8 +575,8 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
                           p + 1 >= (o->certify_from > 0 ? o->certify_from : 1);
     float tj0 = 0.f;
     if (judge_now) {
-      HIPCHK(hipEventRecord(c->ev1, c->st));
-      HIPCHK(hipEventSynchronize(c->ev1));
+      HIPGO(hipEventRecord(c->ev1, c->st));
+      HIPGO(hipEventSynchronize(c->ev1));
       hipEventElapsedTime(&tj0, c->ev0, c->ev1);
     }
     for (int f = 0; f < nf; f++) {
@@ -592,8 +604,8 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     }
     if (rc) break;
     if (judge_now) {
-      HIPCHK(hipEventRecord(c->ev1, c->st));
-      HIPCHK(hipEventSynchronize(c->ev1));
+      HIPGO(hipEventRecord(c->ev1, c->st));
+      HIPGO(hipEventSynchronize(c->ev1));
       float tj1 = 0.f;
       hipEventElapsedTime(&tj1, c->ev0, c->ev1);
       s.ms_judge += tj1 - tj0;
@@ -608,7 +620,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   if (rc == PDFS_OK && o->judge && inits) {
     float tj_before = (float)s.ms_judge;
     int judged = 0;
-    HIPCHK(hipEventRecord(c->ev0, c->st));
+    HIPGO(hipEventRecord(c->ev0, c->st));
     for (int f = 0; f < nf && rc == PDFS_OK; f++) {
       if (!inits[f] || certified[f]) continue;
       long long fl = 0;
@@ -617,8 +629,8 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
       s.judge_flagged += fl;
       judged = 1;
     }
-    HIPCHK(hipEventRecord(c->ev1, c->st));
-    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPGO(hipEventRecord(c->ev1, c->st));
+    HIPGO(hipEventSynchronize(c->ev1));
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
     s.ms_judge = tj_before + t;
@@ -628,6 +640,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   if (st) *st = s;
   if (rc == PDFS_OK && o->fixed_passes <= 0 && nactive > 0)
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, s.passes);
+out:
   free(delta);
   free(active);
   free(certified);

@@ -232,18 +232,24 @@ __device__ __forceinline__ uint64_t ld_edge(const uint64_t *p) {
 }
 
 // Bounded wait for an edge entry of the current epoch (2 s of the 100 MHz
-// constant clock). On timeout the error flag is raised and the stale value
-// used, so every wave still exits.
+// constant clock). SLEEP is the pause between polls in units of 64 clocks;
+// the shared error flag is read on every 16th poll only, so a waiting lane
+// puts little traffic on the memory side. On timeout the error flag is
+// raised and the stale value used, so every wave still exits.
+template <int SLEEP>
 __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsigned *err) {
-  uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
   uint64_t e = ld_edge(p);
+  unsigned n = 0;
   while ((uint32_t)(e >> 32) != epoch) {
-    __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_s_sleep(SLEEP);
     e = ld_edge(p);
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
-      atomicOr(err, 1u);
-      break;
+    if ((++n & 15u) == 0u) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+        atomicOr(err, 1u);
+        break;
+      }
     }
   }
   return __uint_as_float((uint32_t)e);
@@ -384,7 +390,7 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
 
     // interior plane loads / stores
     const uint32_t lxy = dom ? xyb(vx, vy) : 0x80000000u;
-    uint32_t ld = zb(2) + lxy;                  // next plane to load: 2
+    uint32_t ld = zb(3) + lxy;                  // next plane to load: 3
     uint32_t st = dom ? zb(1) + lxy : 0xFFFFFFF0u;
     const uint32_t st_inc = dom ? dZ : 0u;
     // halo loaders: lanes [0, TY) east column (row tid), [TY, TY+TX) north row;
@@ -408,56 +414,53 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       hval = hx <= X && hy <= Y;
       if (hval) hxy = xyb(hx, hy);
     }
-    uint32_t hl = zb(2) + hxy;
+    uint32_t hl = zb(3) + hxy;
     const int hmask = hmul ? -1 : 0;  // slot offset applies to halo lanes only
-    // west / south edge entries (upwind columns), byte offsets from the
-    // allocation base; entry of step l is (vz - 1) with vz = l + 1 - skew
+    // Edge entries, byte offsets from the allocation base. A column's
+    // east-face block holds one row of TY entries per column step: the lane
+    // (TX-1, ty) result of step l' is entry l' * TY + ty (north face alike,
+    // TX entries per step). Lane (0, ty) of the downstream column needs at
+    // its step l the value of plane l + 1 - ty, which its upwind column
+    // computed at step l + TX - 1 for every ty: a step's entries are
+    // written together and read together (one or two cache lines per wave).
     const bool useW = tx == 0, useS = ty == 0;
     const bool hasW = ca > 0, hasS = cb > 0;
     const bool waitW = useW && hasW, waitS = useS && hasS;
     constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
     const uint32_t exb = PADB + fd.ex_off * 8u, eyb = PADB + fd.ey_off * 8u;
-    uint32_t wo = (useW && hasW)
-                      ? exb + (uint32_t)((((ca - 1) * nTy + cb) * Z - skew) * TY + ty) * 8u
-                      : 0u;
-    uint32_t so = (useS && hasS)
-                      ? eyb + (uint32_t)(((ca * nTy + cb - 1) * Z - skew) * TX + tx) * 8u
-                      : 0u;
-    const uint32_t wo_inc = (useW && hasW) ? TY * 8u : 0u, so_inc = (useS && hasS) ? TX * 8u : 0u;
-    // own east / north edge entries (published for the downstream columns)
+    const int NS = Z + TX + TY - 1;  // steps (entry rows) per column
+    // Lanes that consume no entry load from an out-of-range offset: the
+    // range check drops the request, so they put no traffic on the memory
+    // side (a shared in-range dummy address would be one hot line that
+    // every wave of every column reads each step).
+    uint32_t wo = waitW ? exb + (uint32_t)((((ca - 1) * nTy + cb) * NS + TX - 1) * TY + ty) * 8u
+                        : 0xFFFFFFF0u;
+    uint32_t so = waitS ? eyb + (uint32_t)(((ca * nTy + cb - 1) * NS + TY - 1) * TX + tx) * 8u
+                        : 0xFFFFFFF0u;
+    const uint32_t wo_inc = waitW ? TY * 8u : 0u, so_inc = waitS ? TX * 8u : 0u;
+    // own east / north entries (published for the downstream columns)
     const bool pubE = tx == TX - 1, pubN = ty == TY - 1;
-    uint32_t pe = exb + (uint32_t)(((ca * nTy + cb) * Z - skew) * TY + ty) * 8u;
-    uint32_t pn = eyb + (uint32_t)(((ca * nTy + cb) * Z - skew) * TX + tx) * 8u;
+    uint32_t pe = exb + (uint32_t)((ca * nTy + cb) * NS * TY + ty) * 8u;
+    uint32_t pn = eyb + (uint32_t)((ca * nTy + cb) * NS * TX + tx) * 8u;
 
-    // prologue: plane 0 (ghost) and plane 1 into the ring
+    // prologue: planes 0 (ghost), 1 and 2 into the ring
     ring[lo] = INF;
     lds[hcell] = INF;
     {
       const float v1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + lxy, 0, 0));
       const float h1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + hxy, 0, 0));
+      const float v2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + lxy, 0, 0));
+      const float h2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + hxy, 0, 0));
+      const bool live2 = Z >= 2;
       ring[PS + lo] = dom ? v1 : INF;
       lds[hcell + hmul] = hval ? h1 : INF;
+      ring[2 * PS + lo] = (dom && live2) ? v2 : INF;
+      lds[hcell + 2 * hmul] = (hval && live2) ? h2 : INF;
     }
-    // Start gate: wait until the upwind columns have published this column's
-    // entries of step GATE. Edge entries are prefetched UE steps ahead; a
-    // column that starts at least GATE > UE steps behind its producers finds
-    // them published in the steady state instead of catching up and paying a
-    // memory round trip on every step. (Upwind columns never wait on this one,
-    // so the wait ends; the spin is bounded like every edge wait.)
-    {
-      const int nl = Z + TX + TY - 2;  // last step
-      const int gl = nl < GATE ? nl : GATE;
-      const int gz = gl + 1 - skew;
-      const bool gact = dom && (unsigned)(gz - 1) < (unsigned)Z;
-      if (waitW && gact) {
-        const uint64_t *p = a.edge_x + (wo + (uint32_t)gl * wo_inc) / 8u;
-        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge(p, epoch, a.err);
-      }
-      if (waitS && gact) {
-        const uint64_t *p = a.edge_y + (so + (uint32_t)gl * so_inc) / 8u;
-        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge(p, epoch, a.err);
-      }
-    }
+    // plane queue (planes 3 .. U+2): the column's own planes and its east /
+    // north halos hold the previous ordering's final values, so they are
+    // requested before the start gate and the first steps after it do not
+    // wait for HBM
     float qP[U], qH[U];
     u32x2 qW[UE], qS[UE];
 #pragma unroll
@@ -467,6 +470,27 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       ld += dZ;
       hl += dZ;
     }
+    // Start gate: wait until the upwind columns have published this column's
+    // entries of step GATE (those of thread (0, 0); the producers publish the
+    // entries every lane needs at the gate step in one hyperplane step).
+    // Edge entries are prefetched UE steps ahead; a column that starts at
+    // least GATE > UE steps behind its producers finds them published in the
+    // steady state instead of catching up and paying a memory round trip on
+    // every step. One lane polls, with a pause between polls, so the columns
+    // waiting here put little traffic on the memory side. (Upwind columns
+    // never wait on this one, so the wait ends; it is bounded like every
+    // edge wait.)
+    if (tid == 0) {
+      const int gl = Z - 1 < GATE ? Z - 1 : GATE;  // entry of thread (0, 0) at step gl
+      if (hasW) {
+        const uint64_t *p = a.edge_x + (wo + (uint32_t)gl * wo_inc) / 8u;
+        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge<8>(p, epoch, a.err);
+      }
+      if (hasS) {
+        const uint64_t *p = a.edge_y + (so + (uint32_t)gl * so_inc) / 8u;
+        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge<8>(p, epoch, a.err);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < UE; j++) {
       qW[j] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
@@ -475,107 +499,141 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       so += so_inc;
     }
     __syncthreads();
-    // slot byte offset of this thread's current plane vz = 1 - skew
-    constexpr uint32_t PS4 = PS * 4, RPS4 = R * PS4;
-    uint32_t sv = (uint32_t)(((1 - skew) % R + R) % R) * PS4;
-    float c = ring[sv / 4 + lo];
+    // Ring addressing: plane vz of a cell lives in slot vz mod R. so_c is the
+    // float offset of slot (vz mod R) for this thread's current plane vz =
+    // l + 1 - skew; the step wraps it with an unsigned min.
+    constexpr uint32_t RPS = (uint32_t)R * PS;
+    auto wrap_next = [](uint32_t s) -> uint32_t {
+      const uint32_t n = s + PS;
+      return __builtin_elementwise_min(n, n - RPS);
+    };
+    uint32_t so_c = (uint32_t)(((1 - skew) % R + R) % R) * PS;
+    // registers of step 0: own plane vz (c) and vz + 1 (zp), the east and
+    // north cells' plane vz (xp, yp); zm is the own result of the step before
+    float c, zp, xp, yp;
+    {
+      const float *pc = ring + so_c + lo;
+      c = pc[0];
+      xp = pc[1];
+      yp = pc[PW];
+      zp = ring[wrap_next(so_c) + lo];
+    }
     float zm = INF;
     unsigned nchg = 0;
     const int NL = Z + TX + TY - 2;  // steps 0..NL; the last one only stores
-    int slot_w = 2, slot_s = (R - (TX + TY - 2) % R) % R;
-
-    // U steps; the first block is peeled so that in the steady-state loop
-    // every queued load is U steps old on entry as on the back edge (the
-    // compiler's wait counts are then exact instead of draining each step)
-    auto block = [&](const int l0, const bool guard) __attribute__((always_inline)) {
-#pragma unroll
-      for (int j = 0; j < U; j++) {
-        const int l = l0 + j;
-        if (!guard || l <= NL) {
-          // (1) plane l+2 (+ halos) into the ring, then refill the queues
-          {
-            const bool live = l + 2 <= Z;
-            ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
-            lds[hcell + ((slot_w * PS) & hmask)] = (hval && live) ? qH[j] : INF;
-          }
-          qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
-          qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
-          ld += dZ;
-          hl += dZ;
-          // the old values above, east and north of this thread's point
-          // (plane vz + 1 of its own cell, plane vz of the x+1 / y+1 cells)
-          // entered the ring in earlier steps and are updated only after this
-          // step: read them before the barrier, off the critical path
-          uint32_t svp = sv + PS4;
-          svp = svp >= RPS4 ? svp - RPS4 : svp;
-          const float *pc = ring + sv / 4 + lo;
-          const float zp = ring[svp / 4 + lo];
-          const float xp = pc[1], yp = pc[PW];
-          __syncthreads();
-          // (2) this thread's hyperplane point; west / south are the values
-          // the neighbouring threads wrote in the previous step
-          float xm = pc[-1], ym = pc[-PW];
-          const int vz = l + 1 - skew;
-          const bool act = dom && (unsigned)(vz - 1) < (unsigned)Z;
-          // edge entries of this step (issued U steps ago; wo/so point U ahead);
-          // a stale entry (its upwind column is behind) is waited for on a
-          // rare path shared by both directions
-          {
-            const int je = j % UE;
-            float w = __uint_as_float(qW[je].x), v = __uint_as_float(qS[je].x);
-            const bool sw = waitW && act && qW[je].y != epoch;
-            const bool ss = waitS && act && qS[je].y != epoch;
-            if (__builtin_expect(sw || ss, 0)) {
-              if (sw) w = wait_edge(a.edge_x + (wo - UE * wo_inc) / 8u, epoch, a.err);
-              if (ss) v = wait_edge(a.edge_y + (so - UE * so_inc) / 8u, epoch, a.err);
-            }
-            xm = useW ? (hasW ? w : INF) : xm;
-            ym = useS ? (hasS ? v : INF) : ym;
-          }
-          const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
-          nchg += (act && u != c) ? 1u : 0u;
-          // result into the ring (inactive lanes: own dummy cell) and the
-          // published edges (other lanes: out of range, dropped)
-          ring[act ? sv / 4 + lo : G::DUMMY - G::LEAD + tid] = u;
-          {
-            const u32x2 e = {__float_as_uint(u), epoch};
-            __builtin_amdgcn_raw_buffer_store_b64(e, rx, (act && pubE) ? pe : 0xFFFFFFF0u, 0,
-                                                  PDFSK_SC1);
-            __builtin_amdgcn_raw_buffer_store_b64(e, ry, (act && pubN) ? pn : 0xFFFFFFF0u, 0,
-                                                  PDFSK_SC1);
-          }
-          pe += TY * 8u;
-          pn += TX * 8u;
-          // (3) store the finished plane zs = l - (TX+TY-2) (complete since the
-          // barrier; its ring slot is refilled only at step l + 2); before the
-          // first finished plane the store goes out of range (dropped), so
-          // every step issues the same memory operations
-          {
-            const bool sto = l >= TX + TY - 1;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ring[slot_s * PS + lo]), fr,
-                                                  sto ? st : 0xFFFFFFF0u, 0, 0);
-            st += sto ? st_inc : 0u;
-          }
-          // refill this step's edge queue entries with step l + UE
-          qW[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
-          qS[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
-          wo += wo_inc;
-          so += so_inc;
-          zm = act ? u : INF;
-          c = zp;
-          sv = svp;
-          slot_w = slot_w == R - 1 ? 0 : slot_w + 1;
-          slot_s = slot_s == R - 1 ? 0 : slot_s + 1;
-        }
-      }
+    // uniform slots: plane l+3 is written at step l, plane l-(TX+TY-2) is
+    // stored at step l
+    int slot_w = 3 % R, slot_s = (R - (TX + TY - 2) % R) % R;
+    // stores that do not happen go to an out-of-range offset; the empty asm
+    // keeps the compiler from turning the select into a branch around the
+    // store, so every step issues the same memory operations
+    auto opq = [](uint32_t v) -> uint32_t {
+      asm volatile("" : "+v"(v));
+      return v;
     };
-    // peeled first block, unguarded full blocks, guarded tail: the steady
-    // state issues identical memory operations every step
+
+    // One step l. Between the barrier that ended step l-1 and the one that
+    // ends step l: (a) read the west / south neighbours' results of step l-1
+    // and the finished plane l-(TX+TY-2); (b) write plane l+3 into the ring
+    // and refill the plane queue; (d) read the old values step l+1 needs
+    // (none of them is written in this step by another thread); (c) update
+    // this thread's hyperplane point, publish it to the ring and the edge
+    // entries, store the finished plane. `steady`: every lane of the field
+    // is active, plane l+3 is inside the field and the stores have begun.
+    auto step = [&](const int l, const int j, const bool steady) __attribute__((always_inline)) {
+      // (a)
+      float *const pc = ring + so_c + lo;
+      float xm = pc[-1], ym = pc[-PW];
+      const float sval = ring[slot_s * PS + lo];
+      // (b)
+      {
+        const bool live = steady || l + 3 <= Z;
+        ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
+        lds[hcell + ((slot_w * PS) & hmask)] = (hval && live) ? qH[j] : INF;
+      }
+      qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
+      qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+      ld += dZ;
+      hl += dZ;
+      // (d)
+      const uint32_t so_n = wrap_next(so_c);
+      const float *pn1 = ring + so_n + lo;
+      const float xp1 = pn1[1], yp1 = pn1[PW];
+      const float zp1 = ring[wrap_next(so_n) + lo];
+      // (c)
+      const int vz = l + 1 - skew;
+      const bool act = steady ? dom : (dom && (unsigned)(vz - 1) < (unsigned)Z);
+      // edge entries of this step (issued UE steps ago; wo/so point UE
+      // ahead); a stale entry (its upwind column is behind) is waited for on
+      // a rare path shared by both directions
+      {
+        const int je = j % UE;
+        float w = __uint_as_float(qW[je].x), v = __uint_as_float(qS[je].x);
+        const bool sw = waitW && act && qW[je].y != epoch;
+        const bool ss = waitS && act && qS[je].y != epoch;
+        if (__builtin_expect(sw || ss, 0)) {
+          if (sw) w = wait_edge<1>(a.edge_x + (wo - UE * wo_inc) / 8u, epoch, a.err);
+          if (ss) v = wait_edge<1>(a.edge_y + (so - UE * so_inc) / 8u, epoch, a.err);
+        }
+        xm = useW ? (hasW ? w : INF) : xm;
+        ym = useS ? (hasS ? v : INF) : ym;
+      }
+      const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
+      nchg += (act && u != c) ? 1u : 0u;
+      // lanes outside the field keep their cells at +inf (the ghost values
+      // their neighbours read); inactive lanes inside it write a value no
+      // active lane reads before the slot is refilled
+      pc[0] = dom ? u : INF;
+      {
+        const u32x2 e = {__float_as_uint(u), epoch};
+        __builtin_amdgcn_raw_buffer_store_b64(e, rx, opq((act && pubE) ? pe : 0xFFFFFFF0u), 0,
+                                              PDFSK_SC1);
+        __builtin_amdgcn_raw_buffer_store_b64(e, ry, opq((act && pubN) ? pn : 0xFFFFFFF0u), 0,
+                                              PDFSK_SC1);
+      }
+      pe += TY * 8u;
+      pn += TX * 8u;
+      {
+        const bool sto = steady || l >= TX + TY - 1;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sval), fr,
+                                              steady ? st : opq(sto ? st : 0xFFFFFFF0u), 0, 0);
+        st += sto ? st_inc : 0u;
+      }
+      // refill this step's edge queue entries with step l + UE
+      qW[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
+      qS[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
+      wo += wo_inc;
+      so += so_inc;
+      zm = (steady || act) ? u : INF;
+      c = zp;
+      zp = zp1;
+      xp = xp1;
+      yp = yp1;
+      so_c = so_n;
+      slot_w = slot_w == R - 1 ? 0 : slot_w + 1;
+      slot_s = slot_s == R - 1 ? 0 : slot_s + 1;
+      __syncthreads();
+    };
+    // U steps per block: queue entry j serves step l0 + j. The first block is
+    // peeled so that in the loop every queued load is U steps old on entry as
+    // on the back edge (the compiler's wait counts are then exact instead of
+    // draining each step).
+    auto block = [&](const int l0, const bool guard, const bool steady)
+                     __attribute__((always_inline)) {
+#pragma unroll
+      for (int j = 0; j < U; j++)
+        if (!guard || l0 + j <= NL) step(l0 + j, j, steady);
+    };
     static_assert(TX + TY - 2 >= U - 1, "first block must be full (NL >= U - 1)");
-    block(0, false);
+    block(0, false, false);
     int l0 = U;
-    for (; l0 + U - 1 <= NL; l0 += U) block(l0, false);
-    if (l0 <= NL) block(l0, true);
+    for (; l0 + U - 1 <= NL; l0 += U) {
+      if (l0 >= TX + TY - 1 && l0 + U + 2 <= Z)
+        block(l0, false, true);
+      else
+        block(l0, false, false);
+    }
+    if (l0 <= NL) block(l0, true, false);
     if (nchg) atomicAdd(&s_changed, nchg);
     __syncthreads();
     if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);

@@ -310,8 +310,9 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
     fd->ey_off = (uint32_t)ey;
     fd->pad = 0;
     size_t nc = (size_t)fd->nTx * fd->nTy;
-    ex += nc * (size_t)z * TY;
-    ey += nc * (size_t)z * TX;
+    /* one row of face entries per column step (z + TX + TY - 1 steps) */
+    ex += nc * (size_t)(z + TX + TY - 1) * TY;
+    ey += nc * (size_t)(z + TX + TY - 1) * TX;
     ncols += (int)nc;
   }
   if ((ex + 2 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31) || (ey + 2 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31))

@@ -53,8 +53,9 @@ typedef struct {
   unsigned *state;       /* [8] */
   unsigned *changed;     /* [nfields] changed-point counters (accumulate) */
   unsigned *err;         /* bounded-spin timeout flag */
-  uint64_t *edge_x, *edge_y;        /* allocation bases (index -PDFSK_EDGE_PAD) */
-  size_t edge_x_bytes, edge_y_bytes;
+  uint64_t *edge_x, *edge_y;        /* face parts (index -PDFSK_EDGE_PAD) of one allocation
+                                       that starts at edge_x; edge_y follows the x part */
+  size_t edge_x_bytes, edge_y_bytes;  /* whole allocation from edge_x; y part alone */
 } pdfsk_batch;
 
 /* Variants of the column pipeline (autotuner chooses one): tile shape and

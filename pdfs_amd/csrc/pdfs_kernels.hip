@@ -256,30 +256,47 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
 }
 
 // ---------------------------------------------------------------------------
+// Edge queue depth and start-gate slack, in steps. Measured at 512^3 with the
+// 32x16 tile: depth 2 with slack 0..2 gives the shortest ordering (1.08 ms;
+// depth 4: 1.11-1.15 ms, depth 8: 1.25 ms); depth 1 stalls every step on the
+// in-order memory counter (the edge load then waits for plane loads issued a
+// step before it).
 #ifndef PDFSK_UE
-#define PDFSK_UE 4
+#define PDFSK_UE 2
 #endif
 #ifndef PDFSK_GATE_EXTRA
-#define PDFSK_GATE_EXTRA 4
+#define PDFSK_GATE_EXTRA 2
 #endif
 
 // Column-pipeline sweep, FP32, one ordering.
 //
 // Virtual coordinates make every ordering (+,+,+): virtual v in [1, dim]
 // maps to physical flip ? dim - v : v - 1. Workgroup = one TX x TY column
-// tile; thread (tx, ty) updates virtual z = l + 1 - tx - ty at step l, so a
-// step sweeps one i+j+k hyperplane of the tile. The LDS ring holds R =
-// TX+TY+2 z-planes of the tile plus its east column and north row halos
-// (old values, loaded with the plane); plane l+2 enters the ring at step l
-// and plane l-(TX+TY-2) leaves it (coalesced store) at step l. West and
-// south halos are the upwind columns' edge values, read from epoch-tagged
-// 64-bit entries U steps ahead. All field accesses are buffer loads/stores
-// with a range-checked descriptor: lanes outside the field or outside
-// [1, Z] read 0 / drop their store, so the step has no lane branches.
+// tile plus one edge wave; compute thread (tx, ty) updates virtual z =
+// l + 1 - tx - ty at step l, so a step sweeps one i+j+k hyperplane of the
+// tile. The LDS ring holds R = TX+TY+2 z-planes of the tile with a halo on
+// all four sides: east column / north row hold old values (loaded with the
+// plane by compute lanes), west column / south row hold the upwind columns'
+// results of the same hyperplane, which the edge wave brings in. Plane l+3
+// enters the ring at step l and plane l-(TX+TY-2) leaves it (coalesced
+// store) at step l.
+//
+// The edge wave moves every value that crosses a column boundary: during
+// step l it loads the epoch-tagged entries the tile's west / south lanes
+// need at step l+1 (waiting for an entry its upwind column has not
+// published yet) and writes them into the halo cells, and it publishes the
+// east / north results of step l-1 from the ring. Compute waves therefore
+// issue only plane loads and stores, and the device-coherent edge traffic
+// is one load and one store instruction per step with its own in-order
+// memory counter. All field and edge accesses are buffer loads/stores with
+// a range-checked descriptor: lanes outside the field, outside [1, Z] or
+// without an entry use an out-of-range offset (the load returns 0 and puts
+// no request on the memory side, the store is dropped), so steps have no
+// lane branches.
 // ---------------------------------------------------------------------------
 // Field table entry (one per field of a batch; a single solve is a batch
 // of one). Dense FP32 field [Z][Y][X]; edge blocks start at ex_off / ey_off
-// (entries from the edge allocations' index 0, padding included).
+// (entries from the face parts' index 0, padding included).
 struct FieldDesc {
   float *f;
   int X, Y, Z;
@@ -299,8 +316,8 @@ struct ColArgs {
   const int *cols;  // ticket -> (field, a, b)
   unsigned *state;
   int fx, fy, fz;   // the ordering's reflections (same for every field)
-  uint64_t *edge_x, *edge_y;  // allocation bases (entry 0 = index -PDFSK_EDGE_PAD)
-  uint32_t edge_x_bytes, edge_y_bytes;  // allocation sizes incl. paddings
+  uint64_t *edge_x, *edge_y;  // face parts (entry 0 = index -PDFSK_EDGE_PAD) of one allocation
+  uint32_t edge_x_bytes, edge_y_bytes;  // whole allocation from edge_x; y part alone
   unsigned *changed;  // per field
   unsigned *err;
   pdfs_sp32 sp;
@@ -308,31 +325,32 @@ struct ColArgs {
 
 template <int TX, int TY>
 struct ColGeom {
-  static constexpr int NT = TX * TY;
-  static constexpr int R = TX + TY + 2;     // ring depth (z-planes)
-  static constexpr int PW = TX + 1;         // row pitch: x = 0..TX-1, east halo at TX
-  static constexpr int PA = PW * (TY + 1);  // with the north halo row at y = TY
+  static constexpr int NT = TX * TY;         // compute threads
+  static constexpr int NB = NT + 64;         // plus the edge wave
+  static constexpr int R = TX + TY + 2;      // ring depth (z-planes)
+  static constexpr int PW = TX + 2;          // row pitch: x = -1 .. TX
+  static constexpr int PA = PW * (TY + 2);   // rows y = -1 .. TY
   static constexpr int PS = PA + ((2 - PA % 32) + 32) % 32;  // plane stride = 2 mod 32 banks
-  static constexpr int LEAD = PW + 1;       // guard so tx-1 / ty-1 of slot 0 stay in range
-  static constexpr int DUMMY = LEAD + R * PS;
-  static constexpr int LDS_FLOATS = DUMMY + NT;
+  static constexpr int DUMMY = R * PS;       // one dummy cell per thread
+  static constexpr int LDS_FLOATS = DUMMY + NB;
+  static constexpr int cell(int x, int y) { return (y + 1) * PW + x + 1; }
 };
 
 template <int TX, int TY, int U, bool ISO>
-__global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
-  // queue depths: interior/halo plane loads U steps ahead (HBM latency),
-  // edge entries UE steps ahead; GATE = start lag beyond the inherent skew
+__global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
+  // queue depths: plane loads U steps ahead (HBM latency), edge entries UE
+  // steps ahead; GATE = start lag beyond the inherent skew
   constexpr int UE = PDFSK_UE, GATE = PDFSK_UE + PDFSK_GATE_EXTRA;
-  static_assert(U % UE == 0, "edge queue depth must divide the plane queue depth");
   using G = ColGeom<TX, TY>;
   constexpr int NT = G::NT, R = G::R, PW = G::PW, PS = G::PS;
+  static_assert(NT % 64 == 0 && TX + TY <= 64, "compute waves and edge-wave lanes");
   __shared__ float lds[G::LDS_FLOATS];
   __shared__ int s_col;
   __shared__ unsigned s_changed;
-  float *const ring = lds + G::LEAD;
+  float *const ring = lds;
 
   const int tid = threadIdx.x;
-  const int tx = tid % TX, ty = tid / TX;
+  const bool edge_wave = tid >= NT;  // wave-uniform: NT is a multiple of 64
   const float INF = __builtin_inff();
   const unsigned ncols = __hip_atomic_load(a.state + ST_NCOLS, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
@@ -340,10 +358,17 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
                                            __HIP_MEMORY_SCOPE_AGENT);
   const unsigned tbase = __hip_atomic_load(a.state + ST_BASE, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
-  const __amdgpu_buffer_rsrc_t rx =
+  // one descriptor over both face parts (edge_y follows edge_x)
+  const __amdgpu_buffer_rsrc_t re =
       __builtin_amdgcn_make_buffer_rsrc(a.edge_x, (short)0, (int)a.edge_x_bytes, 0x00020000);
-  const __amdgpu_buffer_rsrc_t ry =
-      __builtin_amdgcn_make_buffer_rsrc(a.edge_y, (short)0, (int)a.edge_y_bytes, 0x00020000);
+  const uint32_t ybase = (uint32_t)((const char *)a.edge_y - (const char *)a.edge_x);
+  // stores that do not happen go to an out-of-range offset; the empty asm
+  // keeps the compiler from turning the select into a branch around the
+  // store, so every step issues the same memory operations
+  auto opq = [](uint32_t v) -> uint32_t {
+    asm volatile("" : "+v"(v));
+    return v;
+  };
 
   for (;;) {
     if (tid == 0) {
@@ -373,268 +398,285 @@ __global__ __launch_bounds__(TX *TY) void k_sweep_col(ColArgs a) {
       fd.ey_off = __builtin_amdgcn_readfirstlane(fd.ey_off);
     }
     const int X = fd.X, Y = fd.Y, Z = fd.Z, nTy = fd.nTy;
-    const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
-    const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
-    const __amdgpu_buffer_rsrc_t fr =
-        __builtin_amdgcn_make_buffer_rsrc(fd.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
-    // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
-    auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
-    auto xyb = [&](int vx, int vy) -> uint32_t {
-      const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
-      return (uint32_t)(py * X + px) * 4u;
-    };
-    const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
-    const bool dom = vx <= X && vy <= Y;
-    const int skew = tx + ty;
-    const int lo = ty * PW + tx;  // ring cell of this thread
-
-    // interior plane loads / stores
-    const uint32_t lxy = dom ? xyb(vx, vy) : 0x80000000u;
-    uint32_t ld = zb(3) + lxy;                  // next plane to load: 3
-    uint32_t st = dom ? zb(1) + lxy : 0xFFFFFFF0u;
-    const uint32_t st_inc = dom ? dZ : 0u;
-    // halo loaders: lanes [0, TY) east column (row tid), [TY, TY+TX) north row;
-    // the LDS cell of slot s is lds[hcell + s * hmul] (other lanes: a fixed
-    // dummy cell, hmul = 0)
-    int hcell = G::DUMMY + tid;
-    int hmul = 0;
-    bool hval = false;
-    uint32_t hxy = 0x80000000u;
-    if (tid < TY) {
-      const int hx = ca * TX + TX + 1, hy = cb * TY + tid + 1;
-      hcell = G::LEAD + tid * PW + TX;
-      hmul = PS;
-      hval = hx <= X && hy <= Y;
-      if (hval) hxy = xyb(hx, hy);
-    } else if (tid < TY + TX) {
-      const int q = tid - TY;
-      const int hx = ca * TX + q + 1, hy = cb * TY + TY + 1;
-      hcell = G::LEAD + TY * PW + q;
-      hmul = PS;
-      hval = hx <= X && hy <= Y;
-      if (hval) hxy = xyb(hx, hy);
-    }
-    uint32_t hl = zb(3) + hxy;
-    const int hmask = hmul ? -1 : 0;  // slot offset applies to halo lanes only
-    // Edge entries, byte offsets from the allocation base. A column's
-    // east-face block holds one row of TY entries per column step: the lane
-    // (TX-1, ty) result of step l' is entry l' * TY + ty (north face alike,
-    // TX entries per step). Lane (0, ty) of the downstream column needs at
-    // its step l the value of plane l + 1 - ty, which its upwind column
-    // computed at step l + TX - 1 for every ty: a step's entries are
-    // written together and read together (one or two cache lines per wave).
-    const bool useW = tx == 0, useS = ty == 0;
-    const bool hasW = ca > 0, hasS = cb > 0;
-    const bool waitW = useW && hasW, waitS = useS && hasS;
-    constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
-    const uint32_t exb = PADB + fd.ex_off * 8u, eyb = PADB + fd.ey_off * 8u;
-    const int NS = Z + TX + TY - 1;  // steps (entry rows) per column
-    // Lanes that consume no entry load from an out-of-range offset: the
-    // range check drops the request, so they put no traffic on the memory
-    // side (a shared in-range dummy address would be one hot line that
-    // every wave of every column reads each step).
-    uint32_t wo = waitW ? exb + (uint32_t)((((ca - 1) * nTy + cb) * NS + TX - 1) * TY + ty) * 8u
-                        : 0xFFFFFFF0u;
-    uint32_t so = waitS ? eyb + (uint32_t)(((ca * nTy + cb - 1) * NS + TY - 1) * TX + tx) * 8u
-                        : 0xFFFFFFF0u;
-    const uint32_t wo_inc = waitW ? TY * 8u : 0u, so_inc = waitS ? TX * 8u : 0u;
-    // own east / north entries (published for the downstream columns)
-    const bool pubE = tx == TX - 1, pubN = ty == TY - 1;
-    uint32_t pe = exb + (uint32_t)((ca * nTy + cb) * NS * TY + ty) * 8u;
-    uint32_t pn = eyb + (uint32_t)((ca * nTy + cb) * NS * TX + tx) * 8u;
-
-    // prologue: planes 0 (ghost), 1 and 2 into the ring
-    ring[lo] = INF;
-    lds[hcell] = INF;
-    {
-      const float v1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + lxy, 0, 0));
-      const float h1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + hxy, 0, 0));
-      const float v2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + lxy, 0, 0));
-      const float h2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + hxy, 0, 0));
-      const bool live2 = Z >= 2;
-      ring[PS + lo] = dom ? v1 : INF;
-      lds[hcell + hmul] = hval ? h1 : INF;
-      ring[2 * PS + lo] = (dom && live2) ? v2 : INF;
-      lds[hcell + 2 * hmul] = (hval && live2) ? h2 : INF;
-    }
-    // plane queue (planes 3 .. U+2): the column's own planes and its east /
-    // north halos hold the previous ordering's final values, so they are
-    // requested before the start gate and the first steps after it do not
-    // wait for HBM
-    float qP[U], qH[U];
-    u32x2 qW[UE], qS[UE];
-#pragma unroll
-    for (int j = 0; j < U; j++) {
-      qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
-      qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
-      ld += dZ;
-      hl += dZ;
-    }
-    // Start gate: wait until the upwind columns have published this column's
-    // entries of step GATE (those of thread (0, 0); the producers publish the
-    // entries every lane needs at the gate step in one hyperplane step).
-    // Edge entries are prefetched UE steps ahead; a column that starts at
-    // least GATE > UE steps behind its producers finds them published in the
-    // steady state instead of catching up and paying a memory round trip on
-    // every step. One lane polls, with a pause between polls, so the columns
-    // waiting here put little traffic on the memory side. (Upwind columns
-    // never wait on this one, so the wait ends; it is bounded like every
-    // edge wait.)
-    if (tid == 0) {
-      const int gl = Z - 1 < GATE ? Z - 1 : GATE;  // entry of thread (0, 0) at step gl
-      if (hasW) {
-        const uint64_t *p = a.edge_x + (wo + (uint32_t)gl * wo_inc) / 8u;
-        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge<8>(p, epoch, a.err);
-      }
-      if (hasS) {
-        const uint64_t *p = a.edge_y + (so + (uint32_t)gl * so_inc) / 8u;
-        if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge<8>(p, epoch, a.err);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < UE; j++) {
-      qW[j] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
-      qS[j] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
-      wo += wo_inc;
-      so += so_inc;
-    }
-    __syncthreads();
-    // Ring addressing: plane vz of a cell lives in slot vz mod R. so_c is the
-    // float offset of slot (vz mod R) for this thread's current plane vz =
-    // l + 1 - skew; the step wraps it with an unsigned min.
-    constexpr uint32_t RPS = (uint32_t)R * PS;
-    auto wrap_next = [](uint32_t s) -> uint32_t {
-      const uint32_t n = s + PS;
-      return __builtin_elementwise_min(n, n - RPS);
-    };
-    uint32_t so_c = (uint32_t)(((1 - skew) % R + R) % R) * PS;
-    // registers of step 0: own plane vz (c) and vz + 1 (zp), the east and
-    // north cells' plane vz (xp, yp); zm is the own result of the step before
-    float c, zp, xp, yp;
-    {
-      const float *pc = ring + so_c + lo;
-      c = pc[0];
-      xp = pc[1];
-      yp = pc[PW];
-      zp = ring[wrap_next(so_c) + lo];
-    }
-    float zm = INF;
-    unsigned nchg = 0;
     const int NL = Z + TX + TY - 2;  // steps 0..NL; the last one only stores
-    // uniform slots: plane l+3 is written at step l, plane l-(TX+TY-2) is
-    // stored at step l
-    int slot_w = 3 % R, slot_s = (R - (TX + TY - 2) % R) % R;
-    // stores that do not happen go to an out-of-range offset; the empty asm
-    // keeps the compiler from turning the select into a branch around the
-    // store, so every step issues the same memory operations
-    auto opq = [](uint32_t v) -> uint32_t {
-      asm volatile("" : "+v"(v));
-      return v;
-    };
+    const int NS = Z + TX + TY - 1;  // steps (entry rows) per column
+    // Edge entries, byte offsets from edge_x. A column's east-face block
+    // holds one row of TY entries per column step: the lane (TX-1, r) result
+    // of step l' is entry l' * TY + r (north face alike, TX entries per
+    // step). Lane (0, r) of the downstream column needs at its step s the
+    // value of plane s + 1 - r, which its upwind column computed at step
+    // s + TX - 1 for every r: a step's entries are written together and read
+    // together.
+    constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
+    const uint32_t exb = PADB + fd.ex_off * 8u, eyb = ybase + PADB + fd.ey_off * 8u;
+    const int self = ca * nTy + cb;
 
-    // One step l. Between the barrier that ended step l-1 and the one that
-    // ends step l: (a) read the west / south neighbours' results of step l-1
-    // and the finished plane l-(TX+TY-2); (b) write plane l+3 into the ring
-    // and refill the plane queue; (d) read the old values step l+1 needs
-    // (none of them is written in this step by another thread); (c) update
-    // this thread's hyperplane point, publish it to the ring and the edge
-    // entries, store the finished plane. `steady`: every lane of the field
-    // is active, plane l+3 is inside the field and the stores have begun.
-    auto step = [&](const int l, const int j, const bool steady) __attribute__((always_inline)) {
-      // (a)
-      float *const pc = ring + so_c + lo;
-      float xm = pc[-1], ym = pc[-PW];
-      const float sval = ring[slot_s * PS + lo];
-      // (b)
-      {
-        const bool live = steady || l + 3 <= Z;
-        ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
-        lds[hcell + ((slot_w * PS) & hmask)] = (hval && live) ? qH[j] : INF;
+    if (edge_wave) {
+      // ---- edge wave: lane e < TX serves column c = e of the south / north
+      // faces, lane TX + r row r of the west / east faces; other lanes idle
+      const int e = tid - NT;
+      const bool isS = e < TX, isW = !isS && e < TX + TY;
+      const int k = isS ? e : (isW ? e - TX : 0);  // the served lane's skew (c or r)
+      const bool up = isS ? cb > 0 : (isW && ca > 0);  // an upwind column exists
+      // entry of step s: coff + s * cinc
+      uint32_t coff = 0xFFFFFFF0u, cinc = 0u;
+      if (isS && cb > 0) {
+        coff = eyb + (uint32_t)(((ca * nTy + cb - 1) * NS + TY - 1) * TX + k) * 8u;
+        cinc = TX * 8u;
+      } else if (isW && ca > 0) {
+        coff = exb + (uint32_t)((((ca - 1) * nTy + cb) * NS + TX - 1) * TY + k) * 8u;
+        cinc = TY * 8u;
       }
-      qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
-      qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
-      ld += dZ;
-      hl += dZ;
-      // (d)
-      const uint32_t so_n = wrap_next(so_c);
-      const float *pn1 = ring + so_n + lo;
-      const float xp1 = pn1[1], yp1 = pn1[PW];
-      const float zp1 = ring[wrap_next(so_n) + lo];
-      // (c)
-      const int vz = l + 1 - skew;
-      const bool act = steady ? dom : (dom && (unsigned)(vz - 1) < (unsigned)Z);
-      // edge entries of this step (issued UE steps ago; wo/so point UE
-      // ahead); a stale entry (its upwind column is behind) is waited for on
-      // a rare path shared by both directions
+      // halo cell written (plane of step s: s + 1 - k) and result cell
+      // published (plane of producer step l': l' + 1 - pk)
+      const int hcl = isS ? G::cell(k, -1) : (isW ? G::cell(-1, k) : G::DUMMY + tid);
+      const int pcl = isS ? G::cell(k, TY - 1) : (isW ? G::cell(TX - 1, k) : G::DUMMY + tid);
+      const int pk = isS ? k + TY - 1 : TX - 1 + k;
+      const uint32_t pslot = (isS || isW) ? PS : 0u;  // slot stride (idle lanes: fixed dummy)
+      uint32_t poff = isS ? eyb + (uint32_t)(self * NS * TX + k) * 8u
+                          : (isW ? exb + (uint32_t)(self * NS * TY + k) * 8u : 0xFFFFFFF0u);
+      const uint32_t pinc = isS ? TX * 8u : (isW ? TY * 8u : 0u);
+      // slot (in floats) of plane p: (p mod R) * PS
+      auto slot_of = [&](int p) -> uint32_t { return (uint32_t)(((p % R) + R) % R) * pslot; };
+      uint32_t hs = slot_of(1 - k);   // halo slot of step 0
+      uint32_t ps = slot_of(1 - pk);  // result slot of producer step 0
+      auto next = [&](uint32_t s) -> uint32_t {
+        const uint32_t n = s + pslot;
+        return __builtin_elementwise_min(n, n - (uint32_t)R * PS);
+      };
+      // Start gate: wait until the upwind columns have published the entries
+      // of step GATE. A column that starts at least GATE > UE steps behind
+      // its producers finds the entries published in the steady state
+      // instead of catching up and paying a memory round trip on every step.
+      // The pause between polls keeps waiting columns off the memory side.
+      // (Upwind columns never wait on this one, so the wait ends; it is
+      // bounded like every edge wait.)
       {
-        const int je = j % UE;
-        float w = __uint_as_float(qW[je].x), v = __uint_as_float(qS[je].x);
-        const bool sw = waitW && act && qW[je].y != epoch;
-        const bool ss = waitS && act && qS[je].y != epoch;
-        if (__builtin_expect(sw || ss, 0)) {
-          if (sw) w = wait_edge<1>(a.edge_x + (wo - UE * wo_inc) / 8u, epoch, a.err);
-          if (ss) v = wait_edge<1>(a.edge_y + (so - UE * so_inc) / 8u, epoch, a.err);
+        const int gl = Z - 1 < GATE ? Z - 1 : GATE;
+        if (up && (unsigned)(gl - k) < (unsigned)Z) {
+          const uint64_t *p = a.edge_x + (coff + (uint32_t)gl * cinc) / 8u;
+          if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge<8>(p, epoch, a.err);
         }
-        xm = useW ? (hasW ? w : INF) : xm;
-        ym = useS ? (hasS ? v : INF) : ym;
       }
-      const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
-      nchg += (act && u != c) ? 1u : 0u;
-      // lanes outside the field keep their cells at +inf (the ghost values
-      // their neighbours read); inactive lanes inside it write a value no
-      // active lane reads before the slot is refilled
-      pc[0] = dom ? u : INF;
-      {
-        const u32x2 e = {__float_as_uint(u), epoch};
-        __builtin_amdgcn_raw_buffer_store_b64(e, rx, opq((act && pubE) ? pe : 0xFFFFFFF0u), 0,
-                                              PDFSK_SC1);
-        __builtin_amdgcn_raw_buffer_store_b64(e, ry, opq((act && pubN) ? pn : 0xFFFFFFF0u), 0,
-                                              PDFSK_SC1);
-      }
-      pe += TY * 8u;
-      pn += TX * 8u;
-      {
-        const bool sto = steady || l >= TX + TY - 1;
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sval), fr,
-                                              steady ? st : opq(sto ? st : 0xFFFFFFF0u), 0, 0);
-        st += sto ? st_inc : 0u;
-      }
-      // refill this step's edge queue entries with step l + UE
-      qW[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(rx, wo, 0, PDFSK_SC1);
-      qS[j % UE] = __builtin_amdgcn_raw_buffer_load_b64(ry, so, 0, PDFSK_SC1);
-      wo += wo_inc;
-      so += so_inc;
-      zm = (steady || act) ? u : INF;
-      c = zp;
-      zp = zp1;
-      xp = xp1;
-      yp = yp1;
-      so_c = so_n;
-      slot_w = slot_w == R - 1 ? 0 : slot_w + 1;
-      slot_s = slot_s == R - 1 ? 0 : slot_s + 1;
-      __syncthreads();
-    };
-    // U steps per block: queue entry j serves step l0 + j. The first block is
-    // peeled so that in the loop every queued load is U steps old on entry as
-    // on the back edge (the compiler's wait counts are then exact instead of
-    // draining each step).
-    auto block = [&](const int l0, const bool guard, const bool steady)
-                     __attribute__((always_inline)) {
+      // consume the entry of step s (loaded UE steps earlier) into the halo
+      // cell; a stale entry (its upwind column is behind) is waited for
+      auto consume = [&](const int s, const u32x2 q) __attribute__((always_inline)) {
+        float v = __uint_as_float(q.x);
+        if (__builtin_expect(up && (unsigned)(s - k) < (unsigned)Z && q.y != epoch, 0))
+          v = wait_edge<1>(a.edge_x + (coff + (uint32_t)s * cinc) / 8u, epoch, a.err);
+        lds[hcl + hs] = up ? v : INF;
+        hs = next(hs);
+      };
+      // publish the result of producer step lp (complete since the barrier
+      // that ended step lp)
+      auto publish = [&](const int lp) __attribute__((always_inline)) {
+        const float v = lds[pcl + ps];
+        ps = next(ps);
+        const u32x2 q = {__float_as_uint(v), epoch};
+        const bool live = (isS || isW) && (unsigned)(lp - pk) < (unsigned)Z;
+        __builtin_amdgcn_raw_buffer_store_b64(q, re, opq(live ? poff : 0xFFFFFFF0u), 0, PDFSK_SC1);
+        poff += pinc;
+      };
+      u32x2 q[UE];
 #pragma unroll
-      for (int j = 0; j < U; j++)
-        if (!guard || l0 + j <= NL) step(l0 + j, j, steady);
-    };
-    static_assert(TX + TY - 2 >= U - 1, "first block must be full (NL >= U - 1)");
-    block(0, false, false);
-    int l0 = U;
-    for (; l0 + U - 1 <= NL; l0 += U) {
-      if (l0 >= TX + TY - 1 && l0 + U + 2 <= Z)
-        block(l0, false, true);
-      else
-        block(l0, false, false);
+      for (int j = 0; j < UE; j++)
+        q[j] = __builtin_amdgcn_raw_buffer_load_b64(re, coff + (uint32_t)j * cinc, 0, PDFSK_SC1);
+      consume(0, q[0]);
+      q[0] = __builtin_amdgcn_raw_buffer_load_b64(re, coff + (uint32_t)UE * cinc, 0, PDFSK_SC1);
+      __syncthreads();
+      // step l: halo cells of step l + 1 (queue entry (l + 1) % UE, then
+      // refilled with step l + 1 + UE) and the results of step l - 1
+      auto estep = [&](const int l, const int j) __attribute__((always_inline)) {
+        consume(l + 1, q[j]);
+        q[j] = __builtin_amdgcn_raw_buffer_load_b64(re, coff + (uint32_t)(l + 1 + UE) * cinc, 0,
+                                                   PDFSK_SC1);
+        if (l >= 1) publish(l - 1);
+        __syncthreads();
+      };
+      // blocks of UE steps, so the queue index is a constant
+      int l = 0;
+      for (; l + UE - 1 <= NL; l += UE) {
+#pragma unroll
+        for (int j = 0; j < UE; j++) estep(l + j, (j + 1) % UE);
+      }
+#pragma unroll
+      for (int j = 0; j < UE; j++)
+        if (l + j <= NL) estep(l + j, (j + 1) % UE);
+      publish(NL);
+    } else {
+      // ---- compute waves
+      const int tx = tid % TX, ty = tid / TX;
+      const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
+      const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
+      const __amdgpu_buffer_rsrc_t fr = __builtin_amdgcn_make_buffer_rsrc(
+          fd.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
+      // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
+      auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
+      auto xyb = [&](int vx, int vy) -> uint32_t {
+        const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
+        return (uint32_t)(py * X + px) * 4u;
+      };
+      const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
+      const bool dom = vx <= X && vy <= Y;
+      const int skew = tx + ty;
+      const int lo = G::cell(tx, ty);  // ring cell of this thread
+
+      // interior plane loads / stores
+      const uint32_t lxy = dom ? xyb(vx, vy) : 0x80000000u;
+      uint32_t ld = zb(3) + lxy;  // next plane to load: 3
+      uint32_t st = dom ? zb(1) + lxy : 0xFFFFFFF0u;
+      const uint32_t st_inc = dom ? dZ : 0u;
+      // halo loaders: lanes [0, TY) east column (row tid), [TY, TY+TX) north
+      // row; the LDS cell of slot s is lds[hcell + s * hmul] (other lanes: a
+      // fixed dummy cell, hmul = 0)
+      int hcell = G::DUMMY + tid;
+      int hmul = 0;
+      bool hval = false;
+      uint32_t hxy = 0x80000000u;
+      if (tid < TY) {
+        const int hx = ca * TX + TX + 1, hy = cb * TY + tid + 1;
+        hcell = G::cell(TX, tid);
+        hmul = PS;
+        hval = hx <= X && hy <= Y;
+        if (hval) hxy = xyb(hx, hy);
+      } else if (tid < TY + TX) {
+        const int q = tid - TY;
+        const int hx = ca * TX + q + 1, hy = cb * TY + TY + 1;
+        hcell = G::cell(q, TY);
+        hmul = PS;
+        hval = hx <= X && hy <= Y;
+        if (hval) hxy = xyb(hx, hy);
+      }
+      uint32_t hl = zb(3) + hxy;
+      const int hmask = hmul ? -1 : 0;  // slot offset applies to halo lanes only
+
+      // prologue: planes 0 (ghost), 1 and 2 into the ring, then the plane
+      // queue (planes 3 .. U+2)
+      ring[lo] = INF;
+      lds[hcell] = INF;
+      {
+        const float v1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + lxy, 0, 0));
+        const float h1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + hxy, 0, 0));
+        const float v2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + lxy, 0, 0));
+        const float h2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + hxy, 0, 0));
+        const bool live2 = Z >= 2;
+        ring[PS + lo] = dom ? v1 : INF;
+        lds[hcell + hmul] = hval ? h1 : INF;
+        ring[2 * PS + lo] = (dom && live2) ? v2 : INF;
+        lds[hcell + 2 * hmul] = (hval && live2) ? h2 : INF;
+      }
+      float qP[U], qH[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
+        qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+        ld += dZ;
+        hl += dZ;
+      }
+      __syncthreads();
+      // Ring addressing: plane vz of a cell lives in slot vz mod R. so_c is
+      // the float offset of slot (vz mod R) for this thread's current plane
+      // vz = l + 1 - skew; the step wraps it with an unsigned min.
+      constexpr uint32_t RPS = (uint32_t)R * PS;
+      auto wrap_next = [](uint32_t s) -> uint32_t {
+        const uint32_t n = s + PS;
+        return __builtin_elementwise_min(n, n - RPS);
+      };
+      uint32_t so_c = (uint32_t)(((1 - skew) % R + R) % R) * PS;
+      // registers of step 0: own plane vz (c) and vz + 1 (zp), the east and
+      // north cells' plane vz (xp, yp); zm is the own result of the step
+      // before
+      float c, zp, xp, yp;
+      {
+        const float *pc = ring + so_c + lo;
+        c = pc[0];
+        xp = pc[1];
+        yp = pc[PW];
+        zp = ring[wrap_next(so_c) + lo];
+      }
+      float zm = INF;
+      unsigned nchg = 0;
+      // uniform slots: plane l+3 is written at step l, plane l-(TX+TY-2) is
+      // stored at step l
+      int slot_w = 3 % R, slot_s = (R - (TX + TY - 2) % R) % R;
+
+      // One step l. Between the barrier that ended step l-1 and the one
+      // that ends step l: (a) read the west / south neighbours' results of
+      // step l-1 (the edge wave's halo values at the tile's west / south
+      // border) and the finished plane l-(TX+TY-2); (b) write plane l+3 into
+      // the ring and refill the plane queue; (d) read the old values step
+      // l+1 needs (none of them is written in this step by another thread);
+      // (c) update this thread's hyperplane point, publish it to the ring,
+      // store the finished plane. `steady`: every lane of the field is
+      // active, plane l+3 is inside the field and the stores have begun.
+      auto step = [&](const int l, const int j, const bool steady)
+                      __attribute__((always_inline)) {
+        // (a)
+        float *const pc = ring + so_c + lo;
+        const float xm = pc[-1], ym = pc[-PW];
+        const float sval = ring[slot_s * PS + lo];
+        // (b)
+        {
+          const bool live = steady || l + 3 <= Z;
+          ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
+          lds[hcell + ((slot_w * PS) & hmask)] = (hval && live) ? qH[j] : INF;
+        }
+        qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
+        qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+        ld += dZ;
+        hl += dZ;
+        // (d)
+        const uint32_t so_n = wrap_next(so_c);
+        const float *pn1 = ring + so_n + lo;
+        const float xp1 = pn1[1], yp1 = pn1[PW];
+        const float zp1 = ring[wrap_next(so_n) + lo];
+        // (c)
+        const int vz = l + 1 - skew;
+        const bool act = steady ? dom : (dom && (unsigned)(vz - 1) < (unsigned)Z);
+        const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
+        nchg += (act && u != c) ? 1u : 0u;
+        // lanes outside the field keep their cells at +inf (the ghost
+        // values their neighbours read); inactive lanes inside it write a
+        // value no active lane reads before the slot is refilled
+        pc[0] = dom ? u : INF;
+        {
+          const bool sto = steady || l >= TX + TY - 1;
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sval), fr,
+                                                steady ? st : opq(sto ? st : 0xFFFFFFF0u), 0, 0);
+          st += sto ? st_inc : 0u;
+        }
+        zm = (steady || act) ? u : INF;
+        c = zp;
+        zp = zp1;
+        xp = xp1;
+        yp = yp1;
+        so_c = so_n;
+        slot_w = slot_w == R - 1 ? 0 : slot_w + 1;
+        slot_s = slot_s == R - 1 ? 0 : slot_s + 1;
+        __syncthreads();
+      };
+      // U steps per block: queue entry j serves step l0 + j. The first block
+      // is peeled so that in the loop every queued load is U steps old on
+      // entry as on the back edge (the compiler's wait counts are then exact
+      // instead of draining each step).
+      auto block = [&](const int l0, const bool guard, const bool steady)
+                       __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < U; j++)
+          if (!guard || l0 + j <= NL) step(l0 + j, j, steady);
+      };
+      static_assert(TX + TY - 2 >= U - 1, "first block must be full (NL >= U - 1)");
+      block(0, false, false);
+      int l0 = U;
+      for (; l0 + U - 1 <= NL; l0 += U) {
+        if (l0 >= TX + TY - 1 && l0 + U + 2 <= Z)
+          block(l0, false, true);
+        else
+          block(l0, false, false);
+      }
+      if (l0 <= NL) block(l0, true, false);
+      if (nchg) atomicAdd(&s_changed, nchg);
     }
-    if (l0 <= NL) block(l0, true, false);
-    if (nchg) atomicAdd(&s_changed, nchg);
     __syncthreads();
     if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);
   }
@@ -923,7 +965,7 @@ template <int TX, int TY, bool ISO>
 static int max_resident(int per_cu) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, PDFSK_U, ISO>,
-                                                   TX * TY, 0) != hipSuccess)
+                                                   TX * TY + 64, 0) != hipSuccess)
     return 0;
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -948,9 +990,11 @@ extern "C" int pdfsk_col_max_resident(int variant, int iso) {
 template <int TX, int TY>
 static void launch_col(const ColArgs &a, int grid, hipStream_t st, int iso) {
   if (iso)
-    hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, true>), dim3(grid), dim3(TX * TY), 0, st, a);
+    hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, true>), dim3(grid), dim3(TX * TY + 64), 0, st,
+                       a);
   else
-    hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, false>), dim3(grid), dim3(TX * TY), 0, st, a);
+    hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, false>), dim3(grid), dim3(TX * TY + 64), 0,
+                       st, a);
 }
 
 extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp32 *sp,

@@ -147,8 +147,7 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   if (!c) return;
   hipStreamSynchronize(c->st);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->bt.edge_x) hipFree(c->bt.edge_x);
-  if (c->bt.edge_y) hipFree(c->bt.edge_y);
+  if (c->bt.edge_x) hipFree(c->bt.edge_x); /* edge_y lies in the same allocation */
   hipFree(c->bt.fields);
   hipFree(c->bt.cols);
   hipFree(c->bt.changed);
@@ -315,25 +314,24 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
     ey += nc * (size_t)(z + TX + TY - 1) * TX;
     ncols += (int)nc;
   }
-  if ((ex + 2 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31) || (ey + 2 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31))
+  if ((ex + ey + 4 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31))
     return fail(PDFS_EINVAL, "batch too large for one sweep (edge buffers exceed 2 GiB)");
-  if (ex > c->cap_ex) {
+  if (ex > c->cap_ex || ey > c->cap_ey) {
+    /* one allocation: the x-face part, then the y-face part, each with a
+     * padding of PDFSK_EDGE_PAD entries on both sides; the kernel's single
+     * buffer descriptor covers both, so one instruction moves a column's
+     * west and south entries */
     if (c->bt.edge_x) hipFree(c->bt.edge_x);
-    c->bt.edge_x = NULL;
-    size_t n = ex + 2 * PDFSK_EDGE_PAD;
-    HIPCHK(hipMalloc((void **)&c->bt.edge_x, n * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->bt.edge_x, 0, n * sizeof(uint64_t), c->st));
-    c->bt.edge_x_bytes = n * sizeof(uint64_t);
-    c->cap_ex = ex;
-  }
-  if (ey > c->cap_ey) {
-    if (c->bt.edge_y) hipFree(c->bt.edge_y);
-    c->bt.edge_y = NULL;
-    size_t n = ey + 2 * PDFSK_EDGE_PAD;
-    HIPCHK(hipMalloc((void **)&c->bt.edge_y, n * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->bt.edge_y, 0, n * sizeof(uint64_t), c->st));
-    c->bt.edge_y_bytes = n * sizeof(uint64_t);
-    c->cap_ey = ey;
+    c->bt.edge_x = c->bt.edge_y = NULL;
+    const size_t cx = ex > c->cap_ex ? ex : c->cap_ex, cy = ey > c->cap_ey ? ey : c->cap_ey;
+    const size_t nx = cx + 2 * PDFSK_EDGE_PAD, ny = cy + 2 * PDFSK_EDGE_PAD;
+    HIPCHK(hipMalloc((void **)&c->bt.edge_x, (nx + ny) * sizeof(uint64_t)));
+    HIPCHK(hipMemsetAsync(c->bt.edge_x, 0, (nx + ny) * sizeof(uint64_t), c->st));
+    c->bt.edge_y = c->bt.edge_x + nx;
+    c->bt.edge_x_bytes = (nx + ny) * sizeof(uint64_t);
+    c->bt.edge_y_bytes = ny * sizeof(uint64_t);
+    c->cap_ex = cx;
+    c->cap_ey = cy;
   }
   int cap_f = c->cap_f;
   if (grow_dev((void **)&c->bt.fields, &c->cap_f, nf, sizeof(pdfsk_field)) ||

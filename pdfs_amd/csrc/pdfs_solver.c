@@ -227,10 +227,25 @@ static void tune_store(pdfs_ctx *c, long long key, int tile)
 
 /* ---- batched column sweep: plans ---------------------------------------
  * A plan is a list of dense FP32 fields swept by the same launches. Column
- * tickets run anti-diagonal by anti-diagonal (a + b ascending) across all
- * active fields, so every column's upwind neighbours hold earlier tickets
- * of the same field; the persistent grid therefore never waits on a column
- * that no running workgroup has taken. */
+ * tickets run in the order the columns become ready: column (a, b) of a
+ * field can start about a * (TX + E) + b * (TY + E) steps after the field's
+ * first column (the hyperplane skew across a tile plus E steps of edge
+ * latency per boundary), so tickets are sorted by that key across all
+ * active fields. Both upwind neighbours of a column have smaller keys, so
+ * every column's producers hold earlier tickets and the persistent grid
+ * never waits on a column that no running workgroup has taken. */
+#define PDFS_TICKET_EDGE_STEPS 6
+typedef struct {
+  int key, f, a, b;
+} col_key;
+static int col_key_cmp(const void *x, const void *y)
+{
+  const col_key *p = (const col_key *)x, *q = (const col_key *)y;
+  if (p->key != q->key) return p->key < q->key ? -1 : 1;
+  if (p->f != q->f) return p->f < q->f ? -1 : 1;
+  return p->a < q->a ? -1 : (p->a > q->a);
+}
+
 static int grow_dev(void **p, int *cap, int need, size_t elem)
 {
   if (need <= *cap) return 0;
@@ -245,25 +260,32 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
 {
   int TX = 0, TY = 0;
   pdfsk_tile_dims(c->tile, &TX, &TY);
-  int maxd = 0;
-  for (int f = 0; f < c->nf; f++) {
-    int d = c->h_fields[f].nTx + c->h_fields[f].nTy - 2;
-    if (d > maxd) maxd = d;
-  }
+  const int wx = TX + PDFS_TICKET_EDGE_STEPS, wy = TY + PDFS_TICKET_EDGE_STEPS;
+  int n_all = 0;
+  for (int f = 0; f < c->nf; f++)
+    if (!active || active[f]) n_all += c->h_fields[f].nTx * c->h_fields[f].nTy;
+  col_key *keys = (col_key *)malloc((size_t)(n_all > 0 ? n_all : 1) * sizeof(col_key));
+  if (!keys) return fail(PDFS_ENOMEM, "plan: host alloc");
   int t = 0;
-  for (int d = 0; d <= maxd; d++)
-    for (int f = 0; f < c->nf; f++) {
-      if (active && !active[f]) continue;
-      const pdfsk_field *fd = &c->h_fields[f];
-      for (int a = 0; a < fd->nTx; a++) {
-        int b = d - a;
-        if (b < 0 || b >= fd->nTy) continue;
-        c->h_cols[3 * t] = f;
-        c->h_cols[3 * t + 1] = a;
-        c->h_cols[3 * t + 2] = b;
+  for (int f = 0; f < c->nf; f++) {
+    if (active && !active[f]) continue;
+    const pdfsk_field *fd = &c->h_fields[f];
+    for (int a = 0; a < fd->nTx; a++)
+      for (int b = 0; b < fd->nTy; b++) {
+        keys[t].key = a * wx + b * wy;
+        keys[t].f = f;
+        keys[t].a = a;
+        keys[t].b = b;
         t++;
       }
-    }
+  }
+  qsort(keys, (size_t)t, sizeof(col_key), col_key_cmp);
+  for (int i = 0; i < t; i++) {
+    c->h_cols[3 * i] = keys[i].f;
+    c->h_cols[3 * i + 1] = keys[i].a;
+    c->h_cols[3 * i + 2] = keys[i].b;
+  }
+  free(keys);
   unsigned n = (unsigned)t;
   HIPCHK(hipMemcpyAsync(c->bt.cols, c->h_cols, (size_t)(3 * (t > 0 ? t : 1)) * sizeof(int),
                         hipMemcpyHostToDevice, c->st));

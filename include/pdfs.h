@@ -199,6 +199,11 @@ int pdfs_net_train_step_dev(pdfs_net *n, const uint16_t *mask, const float *targ
 /* w = min(init, a * d + b * hmax), d the upsampled prediction */
 int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int y, int z,
                             double hmax);
+/* Warm starts of nf uneven fields (dims[3f..3f+2] = x, y, z) in one launch
+ * per network layer; per field bit-identical to pdfs_warm_start_f32_dev.
+ * No counterpart in the reference (the learned warm start is new). */
+int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits,
+                                  float *const *ws, const int *dims, double hmax);
 /* warm start + certified FP32 solve; on a judge failure the field is reset to
  * its init and solved plainly (stats.fallbacks = 1) */
 int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float *init, int x,

@@ -34,7 +34,8 @@ EXPORTED = [
     "pdfs_net_load", "pdfs_net_save", "pdfs_net_num_params", "pdfs_net_get", "pdfs_net_set",
     "pdfs_net_get_calibration", "pdfs_net_set_calibration", "pdfs_net_grid",
     "pdfs_net_features_dev", "pdfs_net_target_dev", "pdfs_net_forward_dev",
-    "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_solve_warm_f32_dev",
+    "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_warm_start_batch_f32_dev",
+    "pdfs_solve_warm_f32_dev",
     "pdfs_solve_batch_warm_f32_dev",
 ]
 WEIGHTS_PATH = os.path.join(HERE, "weights", "warm_unet.bin")
@@ -162,6 +163,8 @@ def lib():
         L.pdfs_net_train_step_dev.restype = I
         L.pdfs_warm_start_f32_dev.argtypes = [vp, vp, vp, I, I, I, D]
         L.pdfs_warm_start_f32_dev.restype = I
+        L.pdfs_warm_start_batch_f32_dev.argtypes = [vp, I, vp, vp, vp, D]
+        L.pdfs_warm_start_batch_f32_dev.restype = I
         L.pdfs_solve_warm_f32_dev.argtypes = [vp, vp, vp, vp, I, I, I, D, D, D, C.POINTER(Opts),
                                               C.POINTER(Stats)]
         L.pdfs_solve_warm_f32_dev.restype = I
@@ -509,4 +512,20 @@ class Net:
         Z, Y, X = init.shape
         torch.cuda.synchronize()
         _check(lib().pdfs_warm_start_f32_dev(self.n, init.data_ptr(), w.data_ptr(), X, Y, Z, hmax))
+        torch.cuda.synchronize()
+
+    def warm_start_batch(self, inits, ws, hmax=1.0):
+        """Warm starts of uneven fields in one launch per layer (ws[i] gets
+        the warm start of inits[i]; dense [Z][Y][X] FP32 CUDA tensors)."""
+        import torch
+        nf = len(inits)
+        assert nf == len(ws) and nf > 0
+        ip = (C.c_void_p * nf)(*[t.data_ptr() for t in inits])
+        wp = (C.c_void_p * nf)(*[t.data_ptr() for t in ws])
+        dims = (C.c_int * (3 * nf))()
+        for i, t in enumerate(inits):
+            assert t.dtype == torch.float32 and t.is_contiguous() and ws[i].shape == t.shape
+            dims[3 * i], dims[3 * i + 1], dims[3 * i + 2] = t.shape[2], t.shape[1], t.shape[0]
+        torch.cuda.synchronize()
+        _check(lib().pdfs_warm_start_batch_f32_dev(self.n, nf, ip, wp, dims, hmax))
         torch.cuda.synchronize()

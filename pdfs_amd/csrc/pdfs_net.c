@@ -60,6 +60,9 @@ struct pdfs_net {
   /* warm-start scratch */
   uint16_t *mask;
   long long cap_mask;
+  /* uneven-batch tables: 4 levels of nf + 1 segments, then nf + 1 fields */
+  void *rag, *h_rag;
+  size_t cap_rag;
 };
 
 #define HCK(call)                                                                    \
@@ -248,6 +251,8 @@ void pdfs_net_destroy(pdfs_net *n)
   hipFree(n->WT);
   hipFree(n->loss);
   hipFree(n->mask);
+  hipFree(n->rag);
+  free(n->h_rag);
   if (n->h_loss) hipHostFree(n->h_loss);
   free(n);
 }
@@ -402,6 +407,45 @@ static int forward(pdfs_net *n, const uint16_t *x, int nb, int Dz, int Dy, int D
   UPCAT(A[A_D1], 32, A[A_S0], 16, A[A_C0], 0);
   CV(L_D0, A[A_C0], A[A_D0], 0);
   if ((rc = conv(n, L_HEAD, A[A_D0], NULL, n->out, nb, 0, Dz, Dy, Dx, 0))) return rc;
+  return 0;
+}
+
+/* forward over an uneven batch: sg[k] (device) are the level-k segment
+ * tables, nblk[k] the convolution blocks and npos[k] the packed positions of
+ * level k; the same layer sequence as forward() */
+static int forward_rag(pdfs_net *n, const uint16_t *x, int nf, const pdfsnn_seg *const *sg,
+                       const int *nblk, const long *npos)
+{
+  uint16_t **A = n->act;
+#define CVR(l, in, out, lvl)                                                                  \
+  NCK(pdfsnn_conv3_rag(in, CIN[l], n->WB + n->woff[l], n->P + n->boff[l], COUT[l], out, NULL, \
+                       sg[lvl], nf, nblk[lvl], 1, n->st))
+#define POOLR(in, out, lvl, C) \
+  NCK(pdfsnn_pool2_rag(in, out, sg[lvl], sg[(lvl) + 1], nf, npos[(lvl) + 1], C, n->st))
+#define UPCATR(src, Cs, skip, Ck, dst, lvl) \
+  NCK(pdfsnn_up2_concat_rag(src, Cs, skip, Ck, dst, sg[lvl], sg[(lvl) + 1], nf, npos[lvl], n->st))
+  CVR(L_E0A, x, A[A_A0], 0);
+  CVR(L_E0B, A[A_A0], A[A_S0], 0);
+  POOLR(A[A_S0], A[A_P0], 0, 16);
+  CVR(L_E1A, A[A_P0], A[A_A1], 1);
+  CVR(L_E1B, A[A_A1], A[A_S1], 1);
+  POOLR(A[A_S1], A[A_P1], 1, 32);
+  CVR(L_E2A, A[A_P1], A[A_A2], 2);
+  CVR(L_E2B, A[A_A2], A[A_S2], 2);
+  POOLR(A[A_S2], A[A_P2], 2, 64);
+  CVR(L_E3A, A[A_P2], A[A_A3], 3);
+  CVR(L_E3B, A[A_A3], A[A_B3], 3);
+  UPCATR(A[A_B3], 64, A[A_S2], 64, A[A_C2], 2);
+  CVR(L_D2, A[A_C2], A[A_D2], 2);
+  UPCATR(A[A_D2], 64, A[A_S1], 32, A[A_C1], 1);
+  CVR(L_D1, A[A_C1], A[A_D1], 1);
+  UPCATR(A[A_D1], 32, A[A_S0], 16, A[A_C0], 0);
+  CVR(L_D0, A[A_C0], A[A_D0], 0);
+  NCK(pdfsnn_conv3_rag(A[A_D0], CIN[L_HEAD], n->WB + n->woff[L_HEAD], n->P + n->boff[L_HEAD],
+                       COUT[L_HEAD], NULL, n->out, sg[0], nf, nblk[0], 0, n->st));
+#undef CVR
+#undef POOLR
+#undef UPCATR
   return 0;
 }
 
@@ -566,6 +610,95 @@ int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int
   return PDFS_OK;
 }
 
+/* Warm starts of an uneven batch in one launch per layer: the fields'
+ * coarse grids are packed level by level (segment tables), so every
+ * convolution of the batch fills the GPU at once. Per field the result is
+ * bit-identical to pdfs_warm_start_f32_dev. */
+int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits,
+                                  float *const *ws, const int *dims, double hmax)
+{
+  if (!n || nf < 1 || !inits || !ws || !dims)
+    return pdfsi_fail(PDFS_EINVAL, "pdfs_warm_start_batch_f32_dev: bad arguments");
+  const size_t segb = 4 * (size_t)(nf + 1) * sizeof(pdfsnn_seg);
+  const size_t need = segb + (size_t)(nf + 1) * sizeof(pdfsnn_fld);
+  if (need > n->cap_rag) {
+    HCK(hipStreamSynchronize(n->st));
+    hipFree(n->rag);
+    free(n->h_rag);
+    n->rag = NULL;
+    n->h_rag = malloc(need);
+    if (!n->h_rag) return pdfsi_fail(PDFS_ENOMEM, "pdfs_warm_start_batch_f32_dev: out of memory");
+    HCK(hipMalloc(&n->rag, need));
+    n->cap_rag = need;
+  } else {
+    /* the host tables may still feed an earlier copy */
+    HCK(hipStreamSynchronize(n->st));
+  }
+  pdfsnn_seg *hs = (pdfsnn_seg *)n->h_rag;
+  pdfsnn_fld *hf = (pdfsnn_fld *)((char *)n->h_rag + segb);
+  long long pos[4] = {0, 0, 0, 0}, blk[4] = {0, 0, 0, 0}, fine = 0;
+  for (int f = 0; f < nf; f++) {
+    const int x = dims[3 * f], y = dims[3 * f + 1], z = dims[3 * f + 2];
+    if (!inits[f] || !ws[f] || x < 1 || y < 1 || z < 1)
+      return pdfsi_fail(PDFS_EINVAL, "pdfs_warm_start_batch_f32_dev: field %d: bad pointer or dims", f);
+    const int Dz = coarse(z), Dy = coarse(y), Dx = coarse(x);
+    for (int k = 0; k < 4; k++) {
+      const int dz = Dz >> k, dy = Dy >> k, dx = Dx >> k;
+      pdfsnn_seg *g = &hs[k * (nf + 1) + f];
+      g->pos = (int)pos[k];
+      g->Dz = dz;
+      g->Dy = dy;
+      g->Dx = dx;
+      g->blk = (int)blk[k];
+      pos[k] += (long long)dz * dy * dx;
+      blk[k] += (long long)((dz + 3) / 4) * ((dy + 7) / 8) * ((dx + 7) / 8);
+    }
+    hf[f].init = inits[f];
+    hf[f].w = ws[f];
+    hf[f].X = x;
+    hf[f].Y = y;
+    hf[f].Z = z;
+    hf[f].pad = 0;
+    hf[f].fine = fine;
+    fine += (long long)x * y * z;
+  }
+  if (pos[0] >= (1LL << 31) / 16 || blk[0] >= (1LL << 31))
+    return pdfsi_fail(PDFS_EINVAL, "pdfs_warm_start_batch_f32_dev: batch too large for one launch");
+  for (int k = 0; k < 4; k++) {
+    pdfsnn_seg *g = &hs[k * (nf + 1) + nf];
+    g->pos = (int)pos[k];
+    g->Dz = g->Dy = g->Dx = 0;
+    g->blk = (int)blk[k];
+  }
+  memset(&hf[nf], 0, sizeof hf[nf]);
+  hf[nf].fine = fine;
+  const long long V0 = pos[0];
+  int rc;
+  if ((rc = ensure_ws(n, V0, 0))) return rc;
+  if (V0 * 16 > n->cap_mask) {
+    HCK(hipStreamSynchronize(n->st));
+    hipFree(n->mask);
+    n->mask = NULL;
+    HCK(hipMalloc((void **)&n->mask, (size_t)V0 * 16 * sizeof(uint16_t)));
+    n->cap_mask = V0 * 16;
+  }
+  HCK(hipMemcpyAsync(n->rag, n->h_rag, need, hipMemcpyHostToDevice, n->st));
+  const pdfsnn_seg *dsg[4];
+  int nblk[4];
+  long npos[4];
+  for (int k = 0; k < 4; k++) {
+    dsg[k] = (const pdfsnn_seg *)n->rag + k * (nf + 1);
+    nblk[k] = (int)blk[k];
+    npos[k] = (long)pos[k];
+  }
+  const pdfsnn_fld *dfl = (const pdfsnn_fld *)((const char *)n->rag + segb);
+  NCK(pdfsnn_mask_down4_rag(dfl, dsg[0], nf, npos[0], n->mask, 1e6f, n->st));
+  if ((rc = forward_rag(n, n->mask, nf, dsg, nblk, npos))) return rc;
+  NCK(pdfsnn_warm_up4_rag(n->out, 16, dsg[0], dfl, nf, fine, (float)(hmax / n->scale), n->a,
+                          (float)(n->b * hmax), n->st));
+  return PDFS_OK;
+}
+
 /* Warm-started certified solve: field = warm start, FP32 passes until the
  * FP64 judge passes; if the judge still flags points (the warm start was
  * below the solution somewhere) or the pass cap is hit, the field is reset
@@ -647,33 +780,13 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   double hmax = dx > dy ? dx : dy;
   if (dz > hmax) hmax = dz;
   hipEvent_t e0, e1;
-  /* size the network workspace and mask once, for the largest coarse grid */
-  long long vmax = 0;
-  for (int f = 0; f < nf; f++) {
-    const long long v = (long long)coarse(dims[3 * f]) * coarse(dims[3 * f + 1]) *
-                        coarse(dims[3 * f + 2]);
-    if (v > vmax) vmax = v;
-  }
-  int rc = ensure_ws(n, vmax, 0);
-  if (rc) return rc;
-  if (vmax * 16 > n->cap_mask) {
-    HCK(hipStreamSynchronize(n->st));
-    hipFree(n->mask);
-    n->mask = NULL;
-    HCK(hipMalloc((void **)&n->mask, (size_t)vmax * 16 * sizeof(uint16_t)));
-    n->cap_mask = vmax * 16;
-  }
   HCK(hipEventCreate(&e0));
-  HCK(hipEventCreate(&e1));
-  HCK(hipEventRecord(e0, s));
-  for (int f = 0; f < nf && !rc; f++)
-    rc = pdfs_warm_start_f32_dev(n, inits[f], fields[f], dims[3 * f], dims[3 * f + 1],
-                                 dims[3 * f + 2], hmax);
-  if (rc) return rc;
-  HCK(hipEventRecord(e1, s));
-  HCK(hipEventSynchronize(e1));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return pdfsi_fail(PDFS_EHIP, "pdfs_solve_batch_warm_f32_dev: hipEventCreate failed");
+  }
+  int rc = 0;
   float wms = 0.f;
-  hipEventElapsedTime(&wms, e0, e1);
   pdfs_opts ow = oo;
   ow.certify = 1;
   ow.judge = 0;
@@ -687,6 +800,12 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
     rc = pdfsi_fail(PDFS_ENOMEM, "pdfs_solve_batch_warm_f32_dev: out of memory");
     goto out;
   }
+  /* all warm starts in one launch per network layer */
+  HGO(hipEventRecord(e0, s));
+  if ((rc = pdfs_warm_start_batch_f32_dev(n, nf, inits, fields, dims, hmax))) goto out;
+  HGO(hipEventRecord(e1, s));
+  HGO(hipEventSynchronize(e1));
+  hipEventElapsedTime(&wms, e0, e1);
   rc = pdfs_solve_batch_f32_dev(c, nf, fields, inits, dims, dx, dy, dz, &ow, &sw, passes, NULL);
   if (rc && rc != PDFS_ENOTCONV) goto out;
   int notconv = rc == PDFS_ENOTCONV;

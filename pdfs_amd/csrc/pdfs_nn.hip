@@ -44,25 +44,21 @@ static inline unsigned grid1(long n, int bs) {
 // each optional. With flip = 1 the taps are mirrored (dz,dy,dx) ->
 // (2-dz, 2-dy, 2-dx): with transposed weights this is the input gradient.
 // ---------------------------------------------------------------------------
+constexpr int CT_HX = 10, CT_HY = 10, CT_HZ = 6, CT_HP = CT_HX * CT_HY * CT_HZ;
+
+// One output tile (bx, by, bz) of sample b; tile = the block's LDS staging
+// area ([z][y][x][channel quad], CT_HP * 4 entries).
 template <int NT>
-__global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t *__restrict__ in, int Cin,
-                                                    const uint16_t *__restrict__ W,
-                                                    const float *__restrict__ bias, int Cout,
-                                                    uint16_t *__restrict__ out,
-                                                    float *__restrict__ out_f32, int n, int Dz,
-                                                    int Dy, int Dx, int relu, int flip) {
-  constexpr int HX = 10, HY = 10, HZ = 6, HP = HX * HY * HZ;
-  __shared__ bf16x4 tile[HP * 4];  // [z][y][x][channel quad]
+__device__ __forceinline__ void conv3_tile_body(const uint16_t *__restrict__ in, int Cin,
+                                                const uint16_t *__restrict__ W,
+                                                const float *__restrict__ bias, int Cout,
+                                                uint16_t *__restrict__ out,
+                                                float *__restrict__ out_f32, long b, int Dz,
+                                                int Dy, int Dx, int bx, int by, int bz, int relu,
+                                                int flip, bf16x4 *tile) {
+  constexpr int HX = CT_HX, HY = CT_HY, HP = CT_HP;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
-  const int ntx = (Dx + 7) >> 3, nty = (Dy + 7) >> 3, ntz = (Dz + 3) >> 2;
-  int bid = blockIdx.x;
-  const int bx = bid % ntx;
-  bid /= ntx;
-  const int by = bid % nty;
-  bid /= nty;
-  const int bz = bid % ntz;
-  const long b = bid / ntz;
   const int x0 = bx * 8, y0 = by * 8, z0 = bz * 4;
   f32x4 acc[4][NT];
 #pragma unroll
@@ -132,6 +128,69 @@ __global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t *__restrict__
       }
     }
   }
+}
+
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t *__restrict__ in, int Cin,
+                                                    const uint16_t *__restrict__ W,
+                                                    const float *__restrict__ bias, int Cout,
+                                                    uint16_t *__restrict__ out,
+                                                    float *__restrict__ out_f32, int n, int Dz,
+                                                    int Dy, int Dx, int relu, int flip) {
+  __shared__ bf16x4 tile[CT_HP * 4];
+  const int ntx = (Dx + 7) >> 3, nty = (Dy + 7) >> 3, ntz = (Dz + 3) >> 2;
+  int bid = blockIdx.x;
+  const int bx = bid % ntx;
+  bid /= ntx;
+  const int by = bid % nty;
+  bid /= nty;
+  const int bz = bid % ntz;
+  const long b = bid / ntz;
+  conv3_tile_body<NT>(in, Cin, W, bias, Cout, out, out_f32, b, Dz, Dy, Dx, bx, by, bz, relu, flip,
+                      tile);
+}
+
+// Uneven batch: segment f owns blocks [seg[f].blk, seg[f+1].blk) and the
+// packed positions from seg[f].pos (seg[nf] is the sentinel).
+__device__ __forceinline__ int seg_by_blk(const pdfsnn_seg *seg, int nf, int v) {
+  int lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg[mid].blk <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int seg_by_pos(const pdfsnn_seg *seg, int nf, int v) {
+  int lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg[mid].pos <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3_tile_rag(const uint16_t *__restrict__ in, int Cin,
+                                                        const uint16_t *__restrict__ W,
+                                                        const float *__restrict__ bias, int Cout,
+                                                        uint16_t *__restrict__ out,
+                                                        float *__restrict__ out_f32,
+                                                        const pdfsnn_seg *__restrict__ seg,
+                                                        int nf, int relu) {
+  __shared__ bf16x4 tile[CT_HP * 4];
+  const int f = seg_by_blk(seg, nf, (int)blockIdx.x);
+  const pdfsnn_seg sg = seg[f];
+  const int ntx = (sg.Dx + 7) >> 3, nty = (sg.Dy + 7) >> 3;
+  int bid = (int)blockIdx.x - sg.blk;
+  const int bx = bid % ntx;
+  bid /= ntx;
+  const int by = bid % nty;
+  const int bz = bid / nty;
+  const long p0 = sg.pos;
+  conv3_tile_body<NT>(in + p0 * Cin, Cin, W, bias, Cout, out ? out + p0 * Cout : nullptr,
+                      out_f32 ? out_f32 + p0 * Cout : nullptr, 0, sg.Dz, sg.Dy, sg.Dx, bx, by, bz,
+                      relu, 0, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -483,6 +542,122 @@ __global__ void k_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, flo
 }
 
 // ---------------------------------------------------------------------------
+// Uneven batches: the same per-field arithmetic as the kernels above, with
+// each element's field found from the segment tables.
+// ---------------------------------------------------------------------------
+__global__ void k_pool2_rag(const uint16_t *in, uint16_t *out, const pdfsnn_seg *si,
+                            const pdfsnn_seg *so, int nf, int C) {
+  const long N = (long)so[nf].pos * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const int p = (int)(i / C);
+    const int f = seg_by_pos(so, nf, p);
+    const pdfsnn_seg a = si[f], o = so[f];
+    int q = p - o.pos;
+    const int x = q % o.Dx;
+    q /= o.Dx;
+    const int y = q % o.Dy, z = q / o.Dy;
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+      const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+      s += bf2f(in[((long)a.pos + ((long)zz * a.Dy + yy) * a.Dx + xx) * C + c]);
+    }
+    out[i] = f2bf(0.125f * s);
+  }
+}
+
+__global__ void k_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
+                                 uint16_t *dst, const pdfsnn_seg *sf, const pdfsnn_seg *sc,
+                                 int nf) {
+  const int Cd = Cs + Ck;
+  const long N = (long)sf[nf].pos * Cd;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Cd);
+    const long p = i / Cd;
+    if (c < Cs) {
+      const int f = seg_by_pos(sf, nf, (int)p);
+      const pdfsnn_seg a = sf[f];
+      int q = (int)p - a.pos;
+      const int x = q % a.Dx;
+      q /= a.Dx;
+      const int y = q % a.Dy, z = q / a.Dy;
+      dst[i] = src[((long)sc[f].pos +
+                    ((long)(z / 2) * (a.Dy / 2) + y / 2) * (a.Dx / 2) + x / 2) * Cs + c];
+    } else {
+      dst[i] = skip[p * Ck + (c - Cs)];
+    }
+  }
+}
+
+__global__ void k_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *s0, int nf,
+                                 uint16_t *m, float far) {
+  const long N = s0[nf].pos;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long)gridDim.x * blockDim.x) {
+    const int f = seg_by_pos(s0, nf, (int)i);
+    const pdfsnn_seg a = s0[f];
+    const pdfsnn_fld d = fld[f];
+    int q = (int)i - a.pos;
+    const int x = q % a.Dx;
+    q /= a.Dx;
+    const int y = q % a.Dy, z = q / a.Dy;
+    int known = 0, cnt = 0;
+    for (int k = 0; k < 64; k++) {
+      const int zz = 4 * z + (k >> 4), yy = 4 * y + ((k >> 2) & 3), xx = 4 * x + (k & 3);
+      if (zz < d.Z && yy < d.Y && xx < d.X) {
+        cnt++;
+        known += d.init[((long)zz * d.Y + yy) * d.X + xx] < far;
+      }
+    }
+    for (int c = 0; c < 16; c++) m[i * 16 + c] = c == 0 ? f2bf(cnt ? (float)known / 64.f : 0.f)
+                                                         : (uint16_t)0;
+  }
+}
+
+__device__ __forceinline__ int fld_by_fine(const pdfsnn_fld *fld, int nf, long long v) {
+  int lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (fld[mid].fine <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *s0,
+                               const pdfsnn_fld *fld, int nf, float inv_scale, float a, float b) {
+#pragma clang fp contract(off)
+  const long long N = fld[nf].fine;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int f = fld_by_fine(fld, nf, i);
+    const pdfsnn_fld d = fld[f];
+    const pdfsnn_seg sg = s0[f];
+    const int Dz = sg.Dz, Dy = sg.Dy, Dx = sg.Dx;
+    const float *pr = pred + (long)sg.pos * C;
+    const long li = (long)(i - d.fine);
+    const int x = (int)(li % d.X);
+    const long q = li / d.X;
+    const int y = (int)(q % d.Y), z = (int)(q / d.Y);
+    const float fx = 0.25f * x, fy = 0.25f * y, fz = 0.25f * z;
+    const int x0 = min((int)fx, Dx - 1), y0 = min((int)fy, Dy - 1), z0 = min((int)fz, Dz - 1);
+    const int x1 = min(x0 + 1, Dx - 1), y1 = min(y0 + 1, Dy - 1), z1 = min(z0 + 1, Dz - 1);
+    const float tx = fx - x0, ty = fy - y0, tz = fz - z0;
+    auto P = [&](int zz, int yy, int xx) { return pr[(((long)zz * Dy + yy) * Dx + xx) * C]; };
+    const float c00 = P(z0, y0, x0) * (1 - tx) + P(z0, y0, x1) * tx;
+    const float c01 = P(z0, y1, x0) * (1 - tx) + P(z0, y1, x1) * tx;
+    const float c10 = P(z1, y0, x0) * (1 - tx) + P(z1, y0, x1) * tx;
+    const float c11 = P(z1, y1, x0) * (1 - tx) + P(z1, y1, x1) * tx;
+    const float c0 = c00 * (1 - ty) + c01 * ty, c1 = c10 * (1 - ty) + c11 * ty;
+    const float dd = (c0 * (1 - tz) + c1 * tz) * inv_scale;
+    const float up = a * (dd > 0.f ? dd : 0.f) + b;
+    const float f0 = d.init[li];
+    d.w[li] = f0 < up ? f0 : up;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI launchers
 // ---------------------------------------------------------------------------
 extern "C" int pdfsnn_conv3(const uint16_t *in, int Cin, const uint16_t *W, const float *bias,
@@ -612,5 +787,59 @@ extern "C" int pdfsnn_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx,
                                int Z, hipStream_t st) {
   hipLaunchKernelGGL(k_warm_up4, dim3(grid1((long)X * Y * Z, 256)), dim3(256), 0, st, pred, C,
                      Dz, Dy, Dx, inv_scale, a, b, init, w, X, Y, Z);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_conv3_rag(const uint16_t *in, int Cin, const uint16_t *W, const float *bias,
+                                int Cout, uint16_t *out, float *out_f32, const pdfsnn_seg *seg,
+                                int nf, int nblocks, int relu, hipStream_t st) {
+  if (Cin % 16 || Cout % 16 || Cout > 128 || Cin > 128 || Cin < 16 || Cout < 16) return -1;
+  if (nf < 1 || nblocks < 1) return -1;
+  const int nt = Cout / 16;
+  const dim3 grid((unsigned)nblocks), blk(256);
+  if (nt == 1)
+    hipLaunchKernelGGL(k_conv3_tile_rag<1>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
+                       out_f32, seg, nf, relu);
+  else if (nt == 2)
+    hipLaunchKernelGGL(k_conv3_tile_rag<2>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
+                       out_f32, seg, nf, relu);
+  else if (nt <= 4)
+    hipLaunchKernelGGL(k_conv3_tile_rag<4>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
+                       out_f32, seg, nf, relu);
+  else
+    hipLaunchKernelGGL(k_conv3_tile_rag<8>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
+                       out_f32, seg, nf, relu);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_pool2_rag(const uint16_t *in, uint16_t *out, const pdfsnn_seg *seg_in,
+                                const pdfsnn_seg *seg_out, int nf, long npos_out, int C,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_pool2_rag, dim3(grid1(npos_out * C, 256)), dim3(256), 0, st, in, out,
+                     seg_in, seg_out, nf, C);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
+                                     uint16_t *dst, const pdfsnn_seg *seg_fine,
+                                     const pdfsnn_seg *seg_coarse, int nf, long npos_fine,
+                                     hipStream_t st) {
+  hipLaunchKernelGGL(k_up2_concat_rag, dim3(grid1(npos_fine * (Cs + Ck), 256)), dim3(256), 0, st,
+                     src, Cs, skip, Ck, dst, seg_fine, seg_coarse, nf);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *seg0, int nf,
+                                     long npos0, uint16_t *m, float far, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_down4_rag, dim3(grid1(npos0, 256)), dim3(256), 0, st, fld, seg0, nf,
+                     m, far);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsnn_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *seg0,
+                                   const pdfsnn_fld *fld, int nf, long long nfine,
+                                   float inv_scale, float a, float b, hipStream_t st) {
+  hipLaunchKernelGGL(k_warm_up4_rag, dim3(grid1((long)nfine, 256)), dim3(256), 0, st, pred, C,
+                     seg0, fld, nf, inv_scale, a, b);
   return hip_rc(hipGetLastError());
 }

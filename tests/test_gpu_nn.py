@@ -139,6 +139,27 @@ def test_features_and_warm_start_match_numpy(net, dev):
     assert np.array_equal(w.cpu().numpy().view(np.uint32), want_w.view(np.uint32))
 
 
+def test_warm_start_batch_matches_per_field(net, dev):
+    """The uneven-batch warm start (all fields in one launch per layer) gives
+    every field exactly the per-field warm start: fields of different coarse
+    grids, including a single point and fields thinner than one 4-cell block."""
+    net.set_params(_random_params(53))
+    net.set_calibration(1.0 / 32, 1.25, 2.0)
+    shapes = [(45, 38, 29), (1, 1, 1), (70, 17, 33), (64, 8, 9), (3, 100, 5), (96, 96, 40)]
+    inits, ws_one = [], []
+    for i, (X, Y, Z) in enumerate(shapes):
+        a = O.gen_spheres(X, Y, Z, seed=300 + i, n=3, r=(1.0, 4.0))
+        init = torch.from_numpy(np.ascontiguousarray(O.interior(a)).astype(np.float32)).to(dev)
+        w = torch.empty_like(init)
+        net.warm_start(init, w, hmax=1.25)
+        inits.append(init)
+        ws_one.append(w)
+    ws_batch = [torch.full_like(t, -1.0) for t in inits]
+    net.warm_start_batch(inits, ws_batch, hmax=1.25)
+    for i, (a, b) in enumerate(zip(ws_one, ws_batch)):
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32)), f"field {i} {shapes[i]}"
+
+
 def _spheres(seed, X=64, Y=56, Z=48):
     return O.gen_spheres(X, Y, Z, seed=seed, n=5, r=(3.0, 8.0))
 

@@ -411,19 +411,19 @@ static int forward(pdfs_net *n, const uint16_t *x, int nb, int Dz, int Dy, int D
 }
 
 /* forward over an uneven batch: sg[k] (device) are the level-k segment
- * tables, nblk[k] the convolution blocks and npos[k] the packed positions of
- * level k; the same layer sequence as forward() */
+ * tables, nblk[k] / neblk[k] the convolution / element-wise blocks of level
+ * k; the same layer sequence as forward() */
 static int forward_rag(pdfs_net *n, const uint16_t *x, int nf, const pdfsnn_seg *const *sg,
-                       const int *nblk, const long *npos)
+                       const int *nblk, const int *neblk)
 {
   uint16_t **A = n->act;
 #define CVR(l, in, out, lvl)                                                                  \
   NCK(pdfsnn_conv3_rag(in, CIN[l], n->WB + n->woff[l], n->P + n->boff[l], COUT[l], out, NULL, \
                        sg[lvl], nf, nblk[lvl], 1, n->st))
 #define POOLR(in, out, lvl, C) \
-  NCK(pdfsnn_pool2_rag(in, out, sg[lvl], sg[(lvl) + 1], nf, npos[(lvl) + 1], C, n->st))
+  NCK(pdfsnn_pool2_rag(in, out, sg[lvl], sg[(lvl) + 1], nf, neblk[(lvl) + 1], C, n->st))
 #define UPCATR(src, Cs, skip, Ck, dst, lvl) \
-  NCK(pdfsnn_up2_concat_rag(src, Cs, skip, Ck, dst, sg[lvl], sg[(lvl) + 1], nf, npos[lvl], n->st))
+  NCK(pdfsnn_up2_concat_rag(src, Cs, skip, Ck, dst, sg[lvl], sg[(lvl) + 1], nf, neblk[lvl], n->st))
   CVR(L_E0A, x, A[A_A0], 0);
   CVR(L_E0B, A[A_A0], A[A_S0], 0);
   POOLR(A[A_S0], A[A_P0], 0, 16);
@@ -636,7 +636,7 @@ int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits
   }
   pdfsnn_seg *hs = (pdfsnn_seg *)n->h_rag;
   pdfsnn_fld *hf = (pdfsnn_fld *)((char *)n->h_rag + segb);
-  long long pos[4] = {0, 0, 0, 0}, blk[4] = {0, 0, 0, 0}, fine = 0;
+  long long pos[4] = {0, 0, 0, 0}, blk[4] = {0, 0, 0, 0}, eblk[4] = {0, 0, 0, 0}, fblk = 0;
   for (int f = 0; f < nf; f++) {
     const int x = dims[3 * f], y = dims[3 * f + 1], z = dims[3 * f + 2];
     if (!inits[f] || !ws[f] || x < 1 || y < 1 || z < 1)
@@ -650,28 +650,30 @@ int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits
       g->Dy = dy;
       g->Dx = dx;
       g->blk = (int)blk[k];
+      g->eblk = (int)eblk[k];
       pos[k] += (long long)dz * dy * dx;
       blk[k] += (long long)((dz + 3) / 4) * ((dy + 7) / 8) * ((dx + 7) / 8);
+      eblk[k] += ((long long)dz * dy * dx + RAG_EP - 1) / RAG_EP;
     }
     hf[f].init = inits[f];
     hf[f].w = ws[f];
     hf[f].X = x;
     hf[f].Y = y;
     hf[f].Z = z;
-    hf[f].pad = 0;
-    hf[f].fine = fine;
-    fine += (long long)x * y * z;
+    hf[f].blk = (int)fblk;
+    fblk += ((long long)x * y * z + RAG_EF - 1) / RAG_EF;
   }
-  if (pos[0] >= (1LL << 31) / 16 || blk[0] >= (1LL << 31))
+  if (pos[0] >= (1LL << 31) / 16 || blk[0] >= (1LL << 31) || fblk >= (1LL << 31))
     return pdfsi_fail(PDFS_EINVAL, "pdfs_warm_start_batch_f32_dev: batch too large for one launch");
   for (int k = 0; k < 4; k++) {
     pdfsnn_seg *g = &hs[k * (nf + 1) + nf];
     g->pos = (int)pos[k];
     g->Dz = g->Dy = g->Dx = 0;
     g->blk = (int)blk[k];
+    g->eblk = (int)eblk[k];
   }
   memset(&hf[nf], 0, sizeof hf[nf]);
-  hf[nf].fine = fine;
+  hf[nf].blk = (int)fblk;
   const long long V0 = pos[0];
   int rc;
   if ((rc = ensure_ws(n, V0, 0))) return rc;
@@ -684,17 +686,16 @@ int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits
   }
   HCK(hipMemcpyAsync(n->rag, n->h_rag, need, hipMemcpyHostToDevice, n->st));
   const pdfsnn_seg *dsg[4];
-  int nblk[4];
-  long npos[4];
+  int nblk[4], neblk[4];
   for (int k = 0; k < 4; k++) {
     dsg[k] = (const pdfsnn_seg *)n->rag + k * (nf + 1);
     nblk[k] = (int)blk[k];
-    npos[k] = (long)pos[k];
+    neblk[k] = (int)eblk[k];
   }
   const pdfsnn_fld *dfl = (const pdfsnn_fld *)((const char *)n->rag + segb);
-  NCK(pdfsnn_mask_down4_rag(dfl, dsg[0], nf, npos[0], n->mask, 1e6f, n->st));
-  if ((rc = forward_rag(n, n->mask, nf, dsg, nblk, npos))) return rc;
-  NCK(pdfsnn_warm_up4_rag(n->out, 16, dsg[0], dfl, nf, fine, (float)(hmax / n->scale), n->a,
+  NCK(pdfsnn_mask_down4_rag(dfl, dsg[0], nf, neblk[0], n->mask, 1e6f, n->st));
+  if ((rc = forward_rag(n, n->mask, nf, dsg, nblk, neblk))) return rc;
+  NCK(pdfsnn_warm_up4_rag(n->out, 16, dsg[0], dfl, nf, (int)fblk, (float)(hmax / n->scale), n->a,
                           (float)(n->b * hmax), n->st));
   return PDFS_OK;
 }

@@ -58,34 +58,35 @@ int pdfsnn_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_
 /* Uneven batches. The tensors of a batch's fields are packed one after the
  * other: at a network level, field f's tensor starts at position seg[f].pos
  * of the level's packed tensor and has dims (Dz, Dy, Dx); seg[f].blk is its
- * first block in a convolution launch. seg[nf] is a sentinel holding the
- * totals (positions, blocks). The kernels compute, per field, exactly what
- * their single-field forms compute. */
+ * first block in a convolution launch and seg[f].eblk its first block in an
+ * element-wise launch (RAG_EP positions per block). seg[nf] is a sentinel
+ * holding the totals. The kernels compute, per field, exactly what their
+ * single-field forms compute. */
+#define RAG_EP 64
+#define RAG_EF 4096
 typedef struct {
-  int pos, Dz, Dy, Dx, blk;
+  int pos, Dz, Dy, Dx, blk, eblk;
 } pdfsnn_seg;
-/* a field's dense ends of the warm start: init (read), w (written), dims
- * and its first point in the batch's concatenated points (fld[nf].fine is
- * the total) */
+/* a field's dense ends of the warm start: init (read), w (written), dims,
+ * its first block in the dense warm-start launch (RAG_EF points per block;
+ * fld[nf].blk is the total) */
 typedef struct {
   const float *init;
   float *w;
-  int X, Y, Z, pad;
-  long long fine;
+  int X, Y, Z, blk;
 } pdfsnn_fld;
 int pdfsnn_conv3_rag(const uint16_t *in, int Cin, const uint16_t *W, const float *bias, int Cout,
                      uint16_t *out, float *out_f32, const pdfsnn_seg *seg, int nf, int nblocks,
                      int relu, hipStream_t st);
 int pdfsnn_pool2_rag(const uint16_t *in, uint16_t *out, const pdfsnn_seg *seg_in,
-                     const pdfsnn_seg *seg_out, int nf, long npos_out, int C, hipStream_t st);
+                     const pdfsnn_seg *seg_out, int nf, int eblocks_out, int C, hipStream_t st);
 int pdfsnn_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
                           uint16_t *dst, const pdfsnn_seg *seg_fine, const pdfsnn_seg *seg_coarse,
-                          int nf, long npos_fine, hipStream_t st);
-int pdfsnn_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *seg0, int nf, long npos0,
+                          int nf, int eblocks_fine, hipStream_t st);
+int pdfsnn_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *seg0, int nf, int eblocks0,
                           uint16_t *m, float far, hipStream_t st);
 int pdfsnn_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *seg0, const pdfsnn_fld *fld,
-                        int nf, long long nfine, float inv_scale, float a, float b,
-                        hipStream_t st);
+                        int nf, int fblocks, float inv_scale, float a, float b, hipStream_t st);
 
 #ifdef __cplusplus
 }

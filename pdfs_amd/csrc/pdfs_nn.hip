@@ -161,14 +161,6 @@ __device__ __forceinline__ int seg_by_blk(const pdfsnn_seg *seg, int nf, int v) 
   }
   return lo;
 }
-__device__ __forceinline__ int seg_by_pos(const pdfsnn_seg *seg, int nf, int v) {
-  int lo = 0, hi = nf;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (seg[mid].pos <= v) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv3_tile_rag(const uint16_t *__restrict__ in, int Cin,
@@ -545,101 +537,127 @@ __global__ void k_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, flo
 // Uneven batches: the same per-field arithmetic as the kernels above, with
 // each element's field found from the segment tables.
 // ---------------------------------------------------------------------------
-__global__ void k_pool2_rag(const uint16_t *in, uint16_t *out, const pdfsnn_seg *si,
-                            const pdfsnn_seg *so, int nf, int C) {
-  const long N = (long)so[nf].pos * C;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const int p = (int)(i / C);
-    const int f = seg_by_pos(so, nf, p);
-    const pdfsnn_seg a = si[f], o = so[f];
-    int q = p - o.pos;
+// Element-wise kernels of an uneven batch: segment f owns the blocks
+// [seg[f].eblk, seg[f+1].eblk), RAG_EP positions of the level each (the
+// warm start's dense end: fld[f].blk, RAG_EF points each), so a block finds
+// its field once and works on one field only.
+__device__ __forceinline__ int seg_by_eblk(const pdfsnn_seg *seg, int nf, int v) {
+  int lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg[mid].eblk <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_pool2_rag(const uint16_t *in, uint16_t *out,
+                                                   const pdfsnn_seg *si, const pdfsnn_seg *so,
+                                                   int nf, int C) {
+  const int f = seg_by_eblk(so, nf, (int)blockIdx.x);
+  const pdfsnn_seg a = si[f], o = so[f];
+  const int p0 = ((int)blockIdx.x - o.eblk) * RAG_EP;
+  const int np = min(RAG_EP, o.Dz * o.Dy * o.Dx - p0);
+  for (int e = threadIdx.x; e < np * C; e += 256) {
+    const int c = e % C;
+    int q = p0 + e / C;
     const int x = q % o.Dx;
     q /= o.Dx;
     const int y = q % o.Dy, z = q / o.Dy;
     float s = 0.f;
     for (int k = 0; k < 8; k++) {
       const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
-      s += bf2f(in[((long)a.pos + ((long)zz * a.Dy + yy) * a.Dx + xx) * C + c]);
+      s += bf2f(in[((long)a.pos + (zz * a.Dy + yy) * a.Dx + xx) * C + c]);
     }
-    out[i] = f2bf(0.125f * s);
+    out[((long)o.pos + p0) * C + e] = f2bf(0.125f * s);
   }
 }
 
-__global__ void k_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
-                                 uint16_t *dst, const pdfsnn_seg *sf, const pdfsnn_seg *sc,
-                                 int nf) {
+__global__ __launch_bounds__(256) void k_up2_concat_rag(const uint16_t *src, int Cs,
+                                                        const uint16_t *skip, int Ck,
+                                                        uint16_t *dst, const pdfsnn_seg *sf,
+                                                        const pdfsnn_seg *sc, int nf) {
   const int Cd = Cs + Ck;
-  const long N = (long)sf[nf].pos * Cd;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % Cd);
-    const long p = i / Cd;
+  const int f = seg_by_eblk(sf, nf, (int)blockIdx.x);
+  const pdfsnn_seg a = sf[f];
+  const long cpos = sc[f].pos;
+  const int p0 = ((int)blockIdx.x - a.eblk) * RAG_EP;
+  const int np = min(RAG_EP, a.Dz * a.Dy * a.Dx - p0);
+  const long g0 = (long)a.pos + p0;  // packed position of the block's first point
+  for (int e = threadIdx.x; e < np * Cd; e += 256) {
+    const int c = e % Cd;
+    const int pl = e / Cd;
     if (c < Cs) {
-      const int f = seg_by_pos(sf, nf, (int)p);
-      const pdfsnn_seg a = sf[f];
-      int q = (int)p - a.pos;
+      int q = p0 + pl;
       const int x = q % a.Dx;
       q /= a.Dx;
       const int y = q % a.Dy, z = q / a.Dy;
-      dst[i] = src[((long)sc[f].pos +
-                    ((long)(z / 2) * (a.Dy / 2) + y / 2) * (a.Dx / 2) + x / 2) * Cs + c];
+      dst[g0 * Cd + e] = src[(cpos + ((z / 2) * (a.Dy / 2) + y / 2) * (a.Dx / 2) + x / 2) * Cs + c];
     } else {
-      dst[i] = skip[p * Ck + (c - Cs)];
+      dst[g0 * Cd + e] = skip[(g0 + pl) * Ck + (c - Cs)];
     }
   }
 }
 
-__global__ void k_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *s0, int nf,
-                                 uint16_t *m, float far) {
-  const long N = s0[nf].pos;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (long)gridDim.x * blockDim.x) {
-    const int f = seg_by_pos(s0, nf, (int)i);
-    const pdfsnn_seg a = s0[f];
-    const pdfsnn_fld d = fld[f];
-    int q = (int)i - a.pos;
+__global__ __launch_bounds__(256) void k_mask_down4_rag(const pdfsnn_fld *fld,
+                                                        const pdfsnn_seg *s0, int nf, uint16_t *m,
+                                                        float far) {
+  const int f = seg_by_eblk(s0, nf, (int)blockIdx.x);
+  const pdfsnn_seg a = s0[f];
+  const pdfsnn_fld d = fld[f];
+  const int p0 = ((int)blockIdx.x - a.eblk) * RAG_EP;
+  const int np = min(RAG_EP, a.Dz * a.Dy * a.Dx - p0);
+  // 4 threads per coarse position, 16 cells each
+  for (int e = threadIdx.x; e < np * 4; e += 256) {
+    const int part = e & 3;
+    int q = p0 + (e >> 2);
     const int x = q % a.Dx;
     q /= a.Dx;
     const int y = q % a.Dy, z = q / a.Dy;
     int known = 0, cnt = 0;
-    for (int k = 0; k < 64; k++) {
+    for (int k = 16 * part; k < 16 * part + 16; k++) {
       const int zz = 4 * z + (k >> 4), yy = 4 * y + ((k >> 2) & 3), xx = 4 * x + (k & 3);
       if (zz < d.Z && yy < d.Y && xx < d.X) {
         cnt++;
         known += d.init[((long)zz * d.Y + yy) * d.X + xx] < far;
       }
     }
-    for (int c = 0; c < 16; c++) m[i * 16 + c] = c == 0 ? f2bf(cnt ? (float)known / 64.f : 0.f)
-                                                         : (uint16_t)0;
+    // the four parts of a position are consecutive lanes of one wave
+    known += __shfl_xor(known, 1);
+    known += __shfl_xor(known, 2);
+    cnt += __shfl_xor(cnt, 1);
+    cnt += __shfl_xor(cnt, 2);
+    const long i = (long)a.pos + p0 + (e >> 2);
+    uint16_t *mi = m + i * 16;
+    for (int c = 4 * part; c < 4 * part + 4; c++)
+      mi[c] = c == 0 ? f2bf(cnt ? (float)known / 64.f : 0.f) : (uint16_t)0;
   }
 }
 
-__device__ __forceinline__ int fld_by_fine(const pdfsnn_fld *fld, int nf, long long v) {
+__device__ __forceinline__ int fld_by_blk(const pdfsnn_fld *fld, int nf, int v) {
   int lo = 0, hi = nf;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (fld[mid].fine <= v) lo = mid; else hi = mid;
+    if (fld[mid].blk <= v) lo = mid; else hi = mid;
   }
   return lo;
 }
 
-__global__ void k_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *s0,
-                               const pdfsnn_fld *fld, int nf, float inv_scale, float a, float b) {
+__global__ __launch_bounds__(256) void k_warm_up4_rag(const float *pred, int C,
+                                                      const pdfsnn_seg *s0, const pdfsnn_fld *fld,
+                                                      int nf, float inv_scale, float a, float b) {
 #pragma clang fp contract(off)
-  const long long N = fld[nf].fine;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int f = fld_by_fine(fld, nf, i);
-    const pdfsnn_fld d = fld[f];
-    const pdfsnn_seg sg = s0[f];
-    const int Dz = sg.Dz, Dy = sg.Dy, Dx = sg.Dx;
-    const float *pr = pred + (long)sg.pos * C;
-    const long li = (long)(i - d.fine);
-    const int x = (int)(li % d.X);
-    const long q = li / d.X;
-    const int y = (int)(q % d.Y), z = (int)(q / d.Y);
+  const int f = fld_by_blk(fld, nf, (int)blockIdx.x);
+  const pdfsnn_fld d = fld[f];
+  const pdfsnn_seg sg = s0[f];
+  const int Dz = sg.Dz, Dy = sg.Dy, Dx = sg.Dx;
+  const float *pr = pred + (long)sg.pos * C;
+  const int i0 = ((int)blockIdx.x - d.blk) * RAG_EF;
+  const int n = min(RAG_EF, d.X * d.Y * d.Z - i0);
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int li = i0 + e;
+    const int x = li % d.X;
+    const int q = li / d.X;
+    const int y = q % d.Y, z = q / d.Y;
     const float fx = 0.25f * x, fy = 0.25f * y, fz = 0.25f * z;
     const int x0 = min((int)fx, Dx - 1), y0 = min((int)fy, Dy - 1), z0 = min((int)fz, Dz - 1);
     const int x1 = min(x0 + 1, Dx - 1), y1 = min(y0 + 1, Dy - 1), z1 = min(z0 + 1, Dz - 1);
@@ -813,33 +831,37 @@ extern "C" int pdfsnn_conv3_rag(const uint16_t *in, int Cin, const uint16_t *W, 
 }
 
 extern "C" int pdfsnn_pool2_rag(const uint16_t *in, uint16_t *out, const pdfsnn_seg *seg_in,
-                                const pdfsnn_seg *seg_out, int nf, long npos_out, int C,
+                                const pdfsnn_seg *seg_out, int nf, int eblocks_out, int C,
                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_pool2_rag, dim3(grid1(npos_out * C, 256)), dim3(256), 0, st, in, out,
-                     seg_in, seg_out, nf, C);
+  if (nf < 1 || eblocks_out < 1) return -1;
+  hipLaunchKernelGGL(k_pool2_rag, dim3((unsigned)eblocks_out), dim3(256), 0, st, in, out, seg_in,
+                     seg_out, nf, C);
   return hip_rc(hipGetLastError());
 }
 
 extern "C" int pdfsnn_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
                                      uint16_t *dst, const pdfsnn_seg *seg_fine,
-                                     const pdfsnn_seg *seg_coarse, int nf, long npos_fine,
+                                     const pdfsnn_seg *seg_coarse, int nf, int eblocks_fine,
                                      hipStream_t st) {
-  hipLaunchKernelGGL(k_up2_concat_rag, dim3(grid1(npos_fine * (Cs + Ck), 256)), dim3(256), 0, st,
-                     src, Cs, skip, Ck, dst, seg_fine, seg_coarse, nf);
+  if (nf < 1 || eblocks_fine < 1) return -1;
+  hipLaunchKernelGGL(k_up2_concat_rag, dim3((unsigned)eblocks_fine), dim3(256), 0, st, src, Cs,
+                     skip, Ck, dst, seg_fine, seg_coarse, nf);
   return hip_rc(hipGetLastError());
 }
 
 extern "C" int pdfsnn_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *seg0, int nf,
-                                     long npos0, uint16_t *m, float far, hipStream_t st) {
-  hipLaunchKernelGGL(k_mask_down4_rag, dim3(grid1(npos0, 256)), dim3(256), 0, st, fld, seg0, nf,
+                                     int eblocks0, uint16_t *m, float far, hipStream_t st) {
+  if (nf < 1 || eblocks0 < 1) return -1;
+  hipLaunchKernelGGL(k_mask_down4_rag, dim3((unsigned)eblocks0), dim3(256), 0, st, fld, seg0, nf,
                      m, far);
   return hip_rc(hipGetLastError());
 }
 
 extern "C" int pdfsnn_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *seg0,
-                                   const pdfsnn_fld *fld, int nf, long long nfine,
-                                   float inv_scale, float a, float b, hipStream_t st) {
-  hipLaunchKernelGGL(k_warm_up4_rag, dim3(grid1((long)nfine, 256)), dim3(256), 0, st, pred, C,
-                     seg0, fld, nf, inv_scale, a, b);
+                                   const pdfsnn_fld *fld, int nf, int fblocks, float inv_scale,
+                                   float a, float b, hipStream_t st) {
+  if (nf < 1 || fblocks < 1) return -1;
+  hipLaunchKernelGGL(k_warm_up4_rag, dim3((unsigned)fblocks), dim3(256), 0, st, pred, C, seg0,
+                     fld, nf, inv_scale, a, b);
   return hip_rc(hipGetLastError());
 }

@@ -205,6 +205,33 @@ __device__ __forceinline__ float god32(float c, float xm, float xp, float ym, fl
 
 // FP64 difference form (judge). Spacing constants in double, same op order
 // as the oracle's double instantiation.
+// Branch-free isotropic FP64 solve for the residual judge: the operations of
+// god_diff_iso<double> with the 1-D / 2-D / 3-D choice made by selects, so
+// the value is the same; lanes of a wave no longer diverge over the branches.
+// Inputs are non-negative or +inf (no NaN): the minima / sort equal the '<'
+// / '>' selects, and a negative 2-D radicand (NaN root) implies u1 <= m1, so
+// the NaN candidate is never selected.
+__device__ __forceinline__ double god_iso64_sel(double cur, double mx, double my, double mz,
+                                                double h, double two_h2, double three_h2,
+                                                double third) {
+#pragma clang fp contract(off)
+  // non-negative, non-NaN operands: min / max equal the '<' / '>' selects
+  const double lo = __builtin_fmin(mx, my), hi = __builtin_fmax(mx, my);
+  const double m0 = __builtin_fmin(lo, mz), mid = __builtin_fmax(lo, mz);
+  const double m1 = __builtin_fmin(hi, mid), m2 = __builtin_fmax(hi, mid);
+  const double u1 = m0 + h;
+  const double a = m1 - m0;
+  const double r = __builtin_sqrt(two_h2 - a * a);
+  const double u2 = m0 + 0.5 * (a + r);
+  const double b = m2 - m0;
+  double D = three_h2 - 2.0 * ((a * a + b * b) - a * b);
+  D = D > 0.0 ? D : 0.0;
+  const double u3 = m0 + ((a + b) + __builtin_sqrt(D)) * third;
+  const double t = u2 > m2 ? u3 : u2;
+  const double u = u1 > m1 ? t : u1;
+  return __builtin_fmin(cur, u);
+}
+
 __device__ __forceinline__ double god_d64(double cur, double mx, double my, double mz,
                                           double dx, double dy, double dz) {
 #pragma clang fp contract(off)
@@ -734,47 +761,70 @@ __global__ __launch_bounds__(256) void k_level_ref64(double *d, int X, int Y, in
 
 // ---------------------------------------------------------------------------
 // FP64 residual judge of a dense FP32 field (new). tol = |e| 2^-22 + hmin 2^-21.
-// One thread per point, 64 consecutive points per wave -> two flag words.
+// A workgroup owns a 64 x 4 (x, y) tile of columns over JUDGE_Z planes; each
+// thread walks its column in z with the values of planes z-1, z, z+1 in
+// registers, so a point costs one new load of its own column plus its four
+// in-plane neighbours (cache hits) and its init, and no index division.
+// Tiles are dealt XCD-contiguously: workgroups go to the 8 XCDs round robin
+// (workgroup b runs on XCD b % 8), so workgroup b takes tile
+// (b % 8) * per + b / 8 and each XCD sweeps one contiguous part of the field,
+// whose neighbour rows then sit in its own L2. A point that fails sets its
+// flag bit (atomic OR, flags zeroed by the caller); the count is summed per
+// wave.
 // ---------------------------------------------------------------------------
+#define JUDGE_Z 32
+template <bool ISO>
 __global__ __launch_bounds__(256) void k_judge_f32(const float *__restrict__ v,
                                                    const float *__restrict__ init, int X,
                                                    int Y, int Z, double dx, double dy,
                                                    double dz, uint32_t *flags,
                                                    unsigned long long *count) {
 #pragma clang fp contract(off)
+  const int ntx = (X + 63) >> 6, nty = (Y + 3) >> 2, ntz = (Z + JUDGE_Z - 1) / JUDGE_Z;
+  const uint32_t ntiles = (uint32_t)ntx * (uint32_t)nty * (uint32_t)ntz;
+  const uint32_t per = gridDim.x / 8u;
+  uint32_t t = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+  if (t >= ntiles) return;  // the grid is a multiple of 8 workgroups
+  const int bx = (int)(t % (uint32_t)ntx);
+  t /= (uint32_t)ntx;
+  const int by = (int)(t % (uint32_t)nty);
+  const int bz = (int)(t / (uint32_t)nty);
+  const int x = bx * 64 + (int)(threadIdx.x & 63), y = by * 4 + (int)(threadIdx.x >> 6);
+  const int z0 = bz * JUDGE_Z, z1 = min(z0 + JUDGE_Z, Z);
+  const bool in = x < X && y < Y;
+  const double INF = __builtin_inf();
   // FP32 fields hold fewer than 2^29 points: 32-bit index arithmetic
-  const uint32_t N = (uint32_t)X * (uint32_t)Y * (uint32_t)Z;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  bool bad = false;
-  if (p < N) {
-    const uint32_t XY = (uint32_t)X * (uint32_t)Y;
-    const uint32_t q = p / (uint32_t)X;
-    const int x = (int)(p - q * (uint32_t)X);
-    const uint32_t zq = q / (uint32_t)Y;
-    const int y = (int)(q - zq * (uint32_t)Y);
-    const int z = (int)zq;
-    const float c = v[p];
-    const double xm = x > 0 ? (double)v[p - 1] : __builtin_inf();
-    const double xp = x < X - 1 ? (double)v[p + 1] : __builtin_inf();
-    const double ym = y > 0 ? (double)v[p - X] : __builtin_inf();
-    const double yp = y < Y - 1 ? (double)v[p + X] : __builtin_inf();
-    const double zm = z > 0 ? (double)v[p - XY] : __builtin_inf();
-    const double zp = z < Z - 1 ? (double)v[p + XY] : __builtin_inf();
-    const double e = god_d64((double)init[p], xm < xp ? xm : xp, ym < yp ? ym : yp,
-                             zm < zp ? zm : zp, dx, dy, dz);
-    const double val = (double)c;
-    const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
-    if (__builtin_isinf(e) || __builtin_isinf(val)) bad = !(e == val);
-    else bad = !(__builtin_fabs(val - e) <= __builtin_fabs(e) * 0x1.0p-22 + hmin * 0x1.0p-21);
+  const uint32_t XY = (uint32_t)X * (uint32_t)Y;
+  uint32_t p = ((uint32_t)z0 * (uint32_t)Y + (uint32_t)y) * (uint32_t)X + (uint32_t)x;
+  const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
+  double zm = (in && z0 > 0) ? (double)v[p - XY] : INF;
+  double cur = in ? (double)v[p] : INF;
+  unsigned long long nbad = 0;
+  for (int z = z0; z < z1; z++, p += XY) {
+    bool bad = false;
+    if (in) {
+      // neighbours outside the field are +inf; per axis the smaller of the
+      // two (FP32 values are exact in FP64, so the minimum commutes with the
+      // conversion)
+      const float FINF = __builtin_inff();
+      const double zp = (double)(z < Z - 1 ? v[p + XY] : FINF);
+      const float xm = x > 0 ? v[p - 1] : FINF, xp = x < X - 1 ? v[p + 1] : FINF;
+      const float ym = y > 0 ? v[p - X] : FINF, yp = y < Y - 1 ? v[p + X] : FINF;
+      const double mx = (double)__builtin_fminf(xm, xp), my = (double)__builtin_fminf(ym, yp);
+      const double mz = __builtin_fmin(zm, zp);
+      const double e =
+          ISO ? god_iso64_sel((double)init[p], mx, my, mz, dx, 2.0 * (dx * dx), 3.0 * (dx * dx),
+                              1.0 / 3.0)
+              : god_d64((double)init[p], mx, my, mz, dx, dy, dz);
+      if (__builtin_isinf(e) || __builtin_isinf(cur)) bad = !(e == cur);
+      else bad = !(__builtin_fabs(cur - e) <= __builtin_fabs(e) * 0x1.0p-22 + hmin * 0x1.0p-21);
+      zm = cur;
+      cur = zp;
+    }
+    if (bad) atomicOr(flags + (p >> 5), 1u << (p & 31u));
+    nbad += (unsigned long long)__popcll(__ballot(bad));
   }
-  const unsigned long long m = __ballot(bad);
-  const int lane = threadIdx.x & 63;
-  const uint32_t w0 = (p - (uint32_t)lane) >> 5;
-  if (lane == 0) {
-    if ((w0 << 5) < N) flags[w0] |= (uint32_t)m;
-    if (((w0 + 1) << 5) < N) flags[w0 + 1] |= (uint32_t)(m >> 32);
-    if (m) atomicAdd(count, (unsigned long long)__popcll(m));
-  }
+  if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(count, nbad);
 }
 
 // ---------------------------------------------------------------------------
@@ -1067,11 +1117,15 @@ extern "C" int pdfsk_judge_f32(const float *v, const float *init, int X, int Y, 
                                double dx, double dy, double dz, uint32_t *flags,
                                unsigned long long *count, hipStream_t st) {
   const long N = (long)X * Y * Z;
-  if (N >= (1L << 31)) return -1;
-  const int bs = 256;
-  const long g = (N + bs - 1) / bs;
-  hipLaunchKernelGGL(k_judge_f32, dim3((unsigned)g), dim3(bs), 0, st, v, init, X, Y, Z, dx, dy,
-                     dz, flags, count);
+  if (N >= (1L << 31) || X < 1 || Y < 1 || Z < 1) return -1;
+  const long tiles = (long)((X + 63) / 64) * ((Y + 3) / 4) * ((Z + JUDGE_Z - 1) / JUDGE_Z);
+  const long g = (tiles + 7) / 8 * 8;  // a multiple of the 8 XCDs
+  if (dx == dy && dy == dz)
+    hipLaunchKernelGGL(k_judge_f32<true>, dim3((unsigned)g), dim3(256), 0, st, v, init, X, Y, Z, dx,
+                       dy, dz, flags, count);
+  else
+    hipLaunchKernelGGL(k_judge_f32<false>, dim3((unsigned)g), dim3(256), 0, st, v, init, X, Y, Z,
+                       dx, dy, dz, flags, count);
   return hip_rc(hipGetLastError());
 }
 

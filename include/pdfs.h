@@ -83,6 +83,12 @@ typedef struct {
 } pdfs_opts;
 
 typedef struct {
+  /* FP32 path: a field converges on a pass that changes nothing; the
+   * orderings of that pass that follow its 8th consecutive unchanged
+   * ordering (counting back into the pass before) are skipped on the device.
+   * orderings counts the orderings swept (for a single field without the
+   * skipped ones; for a batch, the launches); point_updates excludes skipped
+   * orderings. */
   int passes, orderings;
   long long point_updates;   /* interior points x orderings executed */
   long long last_changed;    /* points changed in the last pass */

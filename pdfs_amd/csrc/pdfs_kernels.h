@@ -52,6 +52,9 @@ typedef struct {
   int *cols;             /* [ncols][3] ticket -> (field, a, b), upwind-first */
   unsigned *state;       /* [8] */
   unsigned *changed;     /* [nfields] changed-point counters (accumulate) */
+  unsigned *streak;      /* [nfields][2]: counter value after the last ordering, and the
+                            number of consecutive orderings that changed nothing */
+  int nfields;
   unsigned *err;         /* bounded-spin timeout flag */
   uint64_t *edge_x, *edge_y;        /* face parts (index -PDFSK_EDGE_PAD) of one allocation
                                        that starts at edge_x; edge_y follows the x part */

@@ -346,9 +346,15 @@ struct ColArgs {
   uint64_t *edge_x, *edge_y;  // face parts (entry 0 = index -PDFSK_EDGE_PAD) of one allocation
   uint32_t edge_x_bytes, edge_y_bytes;  // whole allocation from edge_x; y part alone
   unsigned *changed;  // per field
+  unsigned *streak;   // per field: [2f] counter after the last ordering, [2f+1] unchanged run
+  int nfields;
   unsigned *err;
   pdfs_sp32 sp;
 };
+
+// A field whose last 8 orderings (one of each sweep direction) changed
+// nothing is at the fixed point: later orderings skip its columns.
+#define PDFSK_CONVERGED_RUN 8u
 
 template <int TX, int TY>
 struct ColGeom {
@@ -409,6 +415,17 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     const int col = __builtin_amdgcn_readfirstlane(s_col);
     if ((unsigned)col >= ncols) break;
     const int fi = __builtin_amdgcn_readfirstlane(a.cols[3 * col]);
+    {
+      // skip the columns of a converged field (its whole field skips, so no
+      // column waits on one of them); the barrier keeps thread 0 from
+      // taking the next ticket before every wave has read this one
+      const unsigned run = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+          a.streak + 2 * fi + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (run >= PDFSK_CONVERGED_RUN) {
+        __syncthreads();
+        continue;
+      }
+    }
     const int ca = __builtin_amdgcn_readfirstlane(a.cols[3 * col + 1]);
     const int cb = __builtin_amdgcn_readfirstlane(a.cols[3 * col + 2]);
     FieldDesc fd = a.fields[fi];
@@ -707,11 +724,30 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     __syncthreads();
     if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);
   }
-  // the last workgroup to finish advances the launch state for the next launch
-  if (tid == 0 && atomicAdd(a.state + ST_DONE, 1u) == gridDim.x - 1) {
-    a.state[ST_EPOCH] = epoch + 1u;
-    a.state[ST_BASE] = tbase + ncols + gridDim.x;
-    a.state[ST_DONE] = 0u;
+  // The last workgroup to finish advances the launch state for the next
+  // launch and each field's run of orderings without change. (Release: this
+  // workgroup's counter updates before its done count; the last workgroup
+  // then sees every workgroup's.)
+  __shared__ int s_last;
+  if (tid == 0) {
+    __threadfence();
+    s_last = atomicAdd(a.state + ST_DONE, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    __threadfence();
+    for (int f = tid; f < a.nfields; f += blockDim.x) {
+      const unsigned cur =
+          __hip_atomic_load(a.changed + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned last = a.streak[2 * f], run = a.streak[2 * f + 1];
+      a.streak[2 * f] = cur;
+      a.streak[2 * f + 1] = cur == last ? (run < 0xFFFFu ? run + 1u : run) : 0u;
+    }
+    if (tid == 0) {
+      a.state[ST_EPOCH] = epoch + 1u;
+      a.state[ST_BASE] = tbase + ncols + gridDim.x;
+      a.state[ST_DONE] = 0u;
+    }
   }
 }
 
@@ -1068,6 +1104,8 @@ extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp
   a.edge_x_bytes = (uint32_t)b->edge_x_bytes;
   a.edge_y_bytes = (uint32_t)b->edge_y_bytes;
   a.changed = b->changed;
+  a.streak = b->streak;
+  a.nfields = b->nfields;
   a.err = b->err;
   a.sp = *sp;
   int tx = 16, ty = 16;

@@ -24,6 +24,7 @@ struct pdfs_ctx {
   int *h_cols;
   int nf, ncols, tile, cap_f, cap_cols;
   unsigned *h_changed, *prev_changed; /* pinned readback, last values */
+  unsigned *h_streak;              /* per field: counter, run of unchanged orderings */
   unsigned *h_err;
   hipGraphExec_t gexec;            /* captured pass (8 launches) */
   long long tune_keys[32];         /* tile autotuner cache: size class -> tile */
@@ -151,6 +152,8 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->bt.fields);
   hipFree(c->bt.cols);
   hipFree(c->bt.changed);
+  hipFree(c->bt.streak);
+  free(c->h_streak);
   hipFree(c->bt.state);
   hipFree(c->d_u64);
   hipFree(c->flags);
@@ -303,11 +306,13 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
   if (nf > c->nf || !c->h_fields) {
     free(c->h_fields);
     free(c->prev_changed);
+    free(c->h_streak);
     c->h_fields = (pdfsk_field *)calloc((size_t)nf, sizeof(pdfsk_field));
     c->prev_changed = (unsigned *)calloc((size_t)nf, sizeof(unsigned));
+    c->h_streak = (unsigned *)calloc((size_t)2 * nf, sizeof(unsigned));
     hipHostFree(c->h_changed);
     c->h_changed = NULL;
-    if (!c->h_fields || !c->prev_changed ||
+    if (!c->h_fields || !c->prev_changed || !c->h_streak ||
         hipHostMalloc((void **)&c->h_changed, (size_t)nf * sizeof(unsigned), 0) != hipSuccess)
       return fail(PDFS_ENOMEM, "plan: host alloc");
   }
@@ -355,9 +360,10 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
     c->cap_ex = cx;
     c->cap_ey = cy;
   }
-  int cap_f = c->cap_f;
+  int cap_f = c->cap_f, cap_s = 2 * c->cap_f;
   if (grow_dev((void **)&c->bt.fields, &c->cap_f, nf, sizeof(pdfsk_field)) ||
       grow_dev((void **)&c->bt.changed, &cap_f, nf, sizeof(unsigned)) ||
+      grow_dev((void **)&c->bt.streak, &cap_s, 2 * nf, sizeof(unsigned)) ||
       grow_dev((void **)&c->bt.cols, &c->cap_cols, 3 * ncols, sizeof(int)))
     return fail(PDFS_ENOMEM, "plan: device alloc");
   free(c->h_cols);
@@ -368,6 +374,8 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
   HIPCHK(hipMemcpyAsync(c->bt.fields, c->h_fields, (size_t)nf * sizeof(pdfsk_field),
                         hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemsetAsync(c->bt.changed, 0, (size_t)nf * sizeof(unsigned), c->st));
+  HIPCHK(hipMemsetAsync(c->bt.streak, 0, (size_t)2 * nf * sizeof(unsigned), c->st));
+  c->bt.nfields = nf;
   memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);
@@ -436,6 +444,8 @@ static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
   HIPCHK(hipMemcpyAsync(c->h_changed, c->bt.changed, (size_t)c->nf * sizeof(unsigned),
                         hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(c->h_err, c->bt.err, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_streak, c->bt.streak, (size_t)2 * c->nf * sizeof(unsigned),
+                        hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   for (int f = 0; f < c->nf; f++) {
     delta[f] = (long long)(unsigned)(c->h_changed[f] - c->prev_changed[f]);
@@ -605,6 +615,15 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
       last_total += delta[f];
       if (field_passes) field_passes[f] = p + 1;
       int done = o->fixed_passes <= 0 && delta[f] == 0;
+      if (delta[f] == 0) {
+        /* a pass without change: the device skipped the orderings of it
+         * that followed the field's 8th consecutive unchanged one */
+        const unsigned run = c->h_streak[2 * f + 1];
+        long long skipped = run > 8u ? (long long)(run - 8u) : 0;
+        if (skipped > 8) skipped = 8;
+        s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+        if (nf == 1) s.orderings -= (int)skipped;
+      }
       if (!done && judge_now && inits[f]) {
         long long fl = -1;
         if ((rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],

@@ -89,11 +89,21 @@ int pdfsk_sweep_ref64(double *d, int X, int Y, int Z, int sn, double dx,
                       double dy, double dz, unsigned long long *maxdiff,
                       hipStream_t st);
 
-/* FP64 residual judge of a dense FP32 field against its init. Sets bits of
- * flags ([ceil(N/32)] words, must be zeroed) and adds to *count. */
-int pdfsk_judge_f32(const float *v, const float *init, int X, int Y, int Z,
-                    double dx, double dy, double dz, uint32_t *flags,
-                    unsigned long long *count, hipStream_t st);
+/* FP64 residual judge of dense FP32 fields against their inits, all fields
+ * of a batch in one launch. Field f owns judge tiles [tile0, next tile0);
+ * entry nf is a sentinel whose tile0 is the total; flags (one bit per point,
+ * [ceil(N/32)] words zeroed by the caller) optional per field. */
+typedef struct {
+  const float *v, *init;
+  uint32_t *flags;
+  int X, Y, Z, tile0;
+} pdfsk_judge_field;
+/* judge tiles of one field (the tile0 stride) */
+unsigned pdfsk_judge_tiles(int X, int Y, int Z);
+/* counts[f] += points of field f failing the FP64 residual (counts zeroed by
+ * the caller); tab on the device, ntiles = tab[nf].tile0 */
+int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long ntiles, double dx, double dy,
+                          double dz, unsigned long long *counts, hipStream_t st);
 
 /* Layout conversions: padded FP64 <-> dense FP32 and dense copies. */
 int pdfsk_pack_f64(const double *padded, float *dense, int X, int Y, int Z,

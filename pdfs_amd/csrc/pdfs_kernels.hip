@@ -809,18 +809,35 @@ __global__ __launch_bounds__(256) void k_level_ref64(double *d, int X, int Y, in
 // wave.
 // ---------------------------------------------------------------------------
 #define JUDGE_Z 32
+// tiles of a field (64 x 4 columns x JUDGE_Z planes)
+__host__ __device__ inline uint32_t judge_tiles(int X, int Y, int Z) {
+  return (uint32_t)((X + 63) / 64) * (uint32_t)((Y + 3) / 4) * (uint32_t)((Z + JUDGE_Z - 1) / JUDGE_Z);
+}
+
+// Fields of a batch: field f owns tiles [tab[f].tile0, tab[f+1].tile0)
+// (tab[nf].tile0 = total); flags optional per field.
 template <bool ISO>
-__global__ __launch_bounds__(256) void k_judge_f32(const float *__restrict__ v,
-                                                   const float *__restrict__ init, int X,
-                                                   int Y, int Z, double dx, double dy,
-                                                   double dz, uint32_t *flags,
-                                                   unsigned long long *count) {
+__global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__restrict__ tab,
+                                                   int nf, double dx, double dy, double dz,
+                                                   unsigned long long *counts) {
 #pragma clang fp contract(off)
-  const int ntx = (X + 63) >> 6, nty = (Y + 3) >> 2, ntz = (Z + JUDGE_Z - 1) / JUDGE_Z;
-  const uint32_t ntiles = (uint32_t)ntx * (uint32_t)nty * (uint32_t)ntz;
   const uint32_t per = gridDim.x / 8u;
   uint32_t t = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
-  if (t >= ntiles) return;  // the grid is a multiple of 8 workgroups
+  if (t >= (uint32_t)tab[nf].tile0) return;  // the grid is a multiple of 8 workgroups
+  int f = 0;
+  {
+    int hi = nf;
+    while (hi - f > 1) {
+      const int mid = (f + hi) >> 1;
+      if ((uint32_t)tab[mid].tile0 <= t) f = mid; else hi = mid;
+    }
+  }
+  const pdfsk_judge_field fd = tab[f];
+  const float *__restrict__ v = fd.v;
+  const float *__restrict__ init = fd.init;
+  const int X = fd.X, Y = fd.Y, Z = fd.Z;
+  t -= (uint32_t)fd.tile0;
+  const int ntx = (X + 63) >> 6, nty = (Y + 3) >> 2;
   const int bx = (int)(t % (uint32_t)ntx);
   t /= (uint32_t)ntx;
   const int by = (int)(t % (uint32_t)nty);
@@ -857,10 +874,10 @@ __global__ __launch_bounds__(256) void k_judge_f32(const float *__restrict__ v,
       zm = cur;
       cur = zp;
     }
-    if (bad) atomicOr(flags + (p >> 5), 1u << (p & 31u));
+    if (bad && fd.flags) atomicOr(fd.flags + (p >> 5), 1u << (p & 31u));
     nbad += (unsigned long long)__popcll(__ballot(bad));
   }
-  if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(count, nbad);
+  if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(counts + f, nbad);
 }
 
 // ---------------------------------------------------------------------------
@@ -1151,19 +1168,19 @@ extern "C" int pdfsk_sweep_ref64(double *d, int X, int Y, int Z, int sn, double 
   return hip_rc(hipGetLastError());
 }
 
-extern "C" int pdfsk_judge_f32(const float *v, const float *init, int X, int Y, int Z,
-                               double dx, double dy, double dz, uint32_t *flags,
-                               unsigned long long *count, hipStream_t st) {
-  const long N = (long)X * Y * Z;
-  if (N >= (1L << 31) || X < 1 || Y < 1 || Z < 1) return -1;
-  const long tiles = (long)((X + 63) / 64) * ((Y + 3) / 4) * ((Z + JUDGE_Z - 1) / JUDGE_Z);
-  const long g = (tiles + 7) / 8 * 8;  // a multiple of the 8 XCDs
+extern "C" unsigned pdfsk_judge_tiles(int X, int Y, int Z) { return judge_tiles(X, Y, Z); }
+
+extern "C" int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long ntiles, double dx,
+                                     double dy, double dz, unsigned long long *counts,
+                                     hipStream_t st) {
+  if (nf < 1 || ntiles < 1 || ntiles >= (1L << 31) - 8) return -1;
+  const long g = (ntiles + 7) / 8 * 8;  // a multiple of the 8 XCDs
   if (dx == dy && dy == dz)
-    hipLaunchKernelGGL(k_judge_f32<true>, dim3((unsigned)g), dim3(256), 0, st, v, init, X, Y, Z, dx,
-                       dy, dz, flags, count);
+    hipLaunchKernelGGL(k_judge_f32<true>, dim3((unsigned)g), dim3(256), 0, st, tab, nf, dx, dy, dz,
+                       counts);
   else
-    hipLaunchKernelGGL(k_judge_f32<false>, dim3((unsigned)g), dim3(256), 0, st, v, init, X, Y, Z,
-                       dx, dy, dz, flags, count);
+    hipLaunchKernelGGL(k_judge_f32<false>, dim3((unsigned)g), dim3(256), 0, st, tab, nf, dx, dy,
+                       dz, counts);
   return hip_rc(hipGetLastError());
 }
 

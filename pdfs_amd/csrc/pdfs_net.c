@@ -815,16 +815,22 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
    * catches fields that stopped at the pass cap) */
   HGO(hipEventRecord(e0, s));
   int nbad = 0;
-  for (int f = 0; f < nf; f++) {
-    long long fl = 0;
-    if ((rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
-                                 dims[3 * f + 2], dx, dy, dz, NULL, &fl)))
+  {
+    long long *fl = (long long *)calloc((size_t)nf, sizeof(long long));
+    if (!fl) {
+      rc = pdfsi_fail(PDFS_ENOMEM, "pdfs_solve_batch_warm_f32_dev: out of memory");
       goto out;
-    sw.judge_runs++;
-    if (fl > 0 || notconv) {
-      bad[f] = 1;
-      nbad++;
     }
+    rc = pdfsi_judge_batch(c, nf, (const float *const *)fields, inits, dims, NULL, dx, dy, dz, fl);
+    for (int f = 0; f < nf && rc == PDFS_OK; f++) {
+      if (fl[f] > 0 || notconv) {
+        bad[f] = 1;
+        nbad++;
+      }
+    }
+    free(fl);
+    if (rc) goto out;
+    sw.judge_runs += nf;
   }
   HGO(hipEventRecord(e1, s));
   HGO(hipEventSynchronize(e1));

@@ -31,10 +31,12 @@ struct pdfs_ctx {
   int tune_tiles[32], ntune;
   int g_tile, g_iso, g_grid, g_sig[8];
   /* FP64 path / judge */
-  unsigned long long *d_u64; /* [0] maxdiff bits, [1] judge count */
+  unsigned long long *d_u64; /* [0] maxdiff bits */
   unsigned long long *h_pin64;
-  uint32_t *flags;
-  size_t cap_flags;
+  /* batched judge: field table (host staging, device) and per-field counts */
+  pdfsk_judge_field *jt_host, *jt_dev;
+  unsigned long long *jc_dev, *jc_host;
+  int cap_jt;
   double *d_params;
   hipEvent_t ev0, ev1;
   hipEvent_t kev[16];
@@ -156,7 +158,10 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   free(c->h_streak);
   hipFree(c->bt.state);
   hipFree(c->d_u64);
-  hipFree(c->flags);
+  hipFree(c->jt_dev);
+  hipFree(c->jc_dev);
+  free(c->jt_host);
+  hipHostFree(c->jc_host);
   hipFree(c->d_params);
   hipHostFree(c->h_changed);
   hipHostFree(c->h_err);
@@ -479,29 +484,71 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
   return PDFS_OK;
 }
 
+/* FP64 residual judge of n fields in one launch; flagged[i] = failing points
+ * of field i. flags[i] (optional) receive one bit per point. */
+int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *const *init,
+                      const int *dims, uint32_t *const *flags, double dx, double dy, double dz,
+                      long long *flagged)
+{
+  if (!c || n < 1 || !v || !init || !dims) return fail(PDFS_EINVAL, "judge: bad arguments");
+  if (n + 1 > c->cap_jt) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    hipFree(c->jt_dev);
+    hipFree(c->jc_dev);
+    free(c->jt_host);
+    hipHostFree(c->jc_host);
+    c->jt_dev = NULL;
+    c->jc_dev = NULL;
+    c->jc_host = NULL;
+    c->cap_jt = 0;
+    c->jt_host = (pdfsk_judge_field *)malloc((size_t)(n + 1) * sizeof(pdfsk_judge_field));
+    if (!c->jt_host) return fail(PDFS_ENOMEM, "judge: host alloc");
+    HIPCHK(hipMalloc((void **)&c->jt_dev, (size_t)(n + 1) * sizeof(pdfsk_judge_field)));
+    HIPCHK(hipMalloc((void **)&c->jc_dev, (size_t)(n + 1) * sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc((void **)&c->jc_host, (size_t)(n + 1) * sizeof(unsigned long long), 0));
+    c->cap_jt = n + 1;
+  }
+  long long tiles = 0;
+  for (int i = 0; i < n; i++) {
+    const int x = dims[3 * i], y = dims[3 * i + 1], z = dims[3 * i + 2];
+    if (!v[i] || !init[i] || x < 1 || y < 1 || z < 1)
+      return fail(PDFS_EINVAL, "judge: field %d: bad pointer or dims", i);
+    pdfsk_judge_field *t = &c->jt_host[i];
+    t->v = v[i];
+    t->init = init[i];
+    t->flags = flags ? flags[i] : NULL;
+    t->X = x;
+    t->Y = y;
+    t->Z = z;
+    t->tile0 = (int)tiles;
+    tiles += pdfsk_judge_tiles(x, y, z);
+    if (tiles >= (1LL << 31) - 8) return fail(PDFS_EINVAL, "judge: batch too large");
+    if (t->flags)
+      KCHK(pdfsk_fill_u32(t->flags, 0u, ((size_t)x * y * z + 31) / 32, c->st));
+  }
+  memset(&c->jt_host[n], 0, sizeof c->jt_host[n]);
+  c->jt_host[n].tile0 = (int)tiles;
+  HIPCHK(hipMemcpyAsync(c->jt_dev, c->jt_host, (size_t)(n + 1) * sizeof(pdfsk_judge_field),
+                        hipMemcpyHostToDevice, c->st));
+  KCHK(pdfsk_fill_u32((uint32_t *)c->jc_dev, 0u, (size_t)2 * n, c->st));
+  KCHK(pdfsk_judge_batch_f32(c->jt_dev, n, (long)tiles, dx, dy, dz, c->jc_dev, c->st));
+  HIPCHK(hipMemcpyAsync(c->jc_host, c->jc_dev, (size_t)n * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, c->st));
+  /* also keeps the host table from being rewritten while its copy is pending */
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (int i = 0; i < n; i++) flagged[i] = (long long)c->jc_host[i];
+  return PDFS_OK;
+}
+
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags, long long *flagged)
 {
   if (!c || !field || !init) return fail(PDFS_EINVAL, "pdfs_judge_f32_dev: bad arguments");
-  size_t n = (size_t)x * y * z, words = (n + 31) / 32;
-  uint32_t *fl = flags;
-  if (!fl) {
-    if (words > c->cap_flags) {
-      hipFree(c->flags);
-      c->flags = NULL;
-      HIPCHK(hipMalloc((void **)&c->flags, words * sizeof(uint32_t)));
-      c->cap_flags = words;
-    }
-    fl = c->flags;
-  }
-  KCHK(pdfsk_fill_u32(fl, 0u, words, c->st));
-  KCHK(pdfsk_fill_u32((uint32_t *)(c->d_u64 + 1), 0u, 2, c->st));
-  KCHK(pdfsk_judge_f32(field, init, x, y, z, dx, dy, dz, fl, c->d_u64 + 1, c->st));
-  HIPCHK(hipMemcpyAsync(c->h_pin64, c->d_u64 + 1, sizeof(unsigned long long),
-                        hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  if (flagged) *flagged = (long long)c->h_pin64[0];
-  return PDFS_OK;
+  const int dims[3] = {x, y, z};
+  long long fl = 0;
+  int rc = pdfsi_judge_batch(c, 1, &field, &init, dims, flags ? &flags : NULL, dx, dy, dz, &fl);
+  if (rc == PDFS_OK && flagged) *flagged = fl;
+  return rc;
 }
 
 int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
@@ -544,10 +591,23 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   int *active = (int *)malloc((size_t)nf * sizeof(int));
   /* fields whose passes ended on a passing judge: the final judge skips them */
   char *certified = (char *)calloc((size_t)nf, 1);
-  if (!delta || !active || !certified) {
+  /* fields judged together: indices, pointers, dims, counts, outcome */
+  int *jidx = (int *)malloc((size_t)nf * sizeof(int));
+  int *jdone = (int *)malloc((size_t)nf * sizeof(int));
+  int *jd = (int *)malloc((size_t)3 * nf * sizeof(int));
+  const float **jv = (const float **)malloc((size_t)nf * sizeof(float *));
+  const float **ji = (const float **)malloc((size_t)nf * sizeof(float *));
+  long long *jfl = (long long *)malloc((size_t)nf * sizeof(long long));
+  if (!delta || !active || !certified || !jidx || !jdone || !jd || !jv || !ji || !jfl) {
     free(delta);
     free(active);
     free(certified);
+    free(jidx);
+    free(jdone);
+    free(jd);
+    free(jv);
+    free(ji);
+    free(jfl);
     return fail(PDFS_ENOMEM, "batch: host alloc");
   }
   for (int f = 0; f < nf; f++) {
@@ -610,11 +670,13 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
       HIPGO(hipEventSynchronize(c->ev1));
       hipEventElapsedTime(&tj0, c->ev0, c->ev1);
     }
+    int nj = 0;
     for (int f = 0; f < nf; f++) {
+      jdone[f] = 0;
       if (!active[f]) continue;
       last_total += delta[f];
       if (field_passes) field_passes[f] = p + 1;
-      int done = o->fixed_passes <= 0 && delta[f] == 0;
+      jdone[f] = o->fixed_passes <= 0 && delta[f] == 0;
       if (delta[f] == 0) {
         /* a pass without change: the device skipped the orderings of it
          * that followed the field's 8th consecutive unchanged one */
@@ -624,16 +686,26 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
         s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
         if (nf == 1) s.orderings -= (int)skipped;
       }
-      if (!done && judge_now && inits[f]) {
-        long long fl = -1;
-        if ((rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
-                                     dims[3 * f + 2], dx, dy, dz, NULL, &fl)))
-          break;
-        s.judge_runs++;
-        done = fl == 0;
-        certified[f] = (char)done;
+      if (!jdone[f] && judge_now && inits[f]) {
+        jidx[nj] = f;
+        jv[nj] = fields[f];
+        ji[nj] = inits[f];
+        memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
+        nj++;
       }
-      if (done) {
+    }
+    /* all fields to certify in one judge launch */
+    if (nj > 0) {
+      if ((rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl))) break;
+      s.judge_runs += nj;
+      for (int k = 0; k < nj; k++) {
+        jdone[jidx[k]] = jfl[k] == 0;
+        certified[jidx[k]] = (char)(jfl[k] == 0);
+      }
+    }
+    for (int f = 0; f < nf; f++) {
+      if (!active[f]) continue;
+      if (jdone[f]) {
         active[f] = 0;
         nactive--;
         changed_set = 1;
@@ -661,13 +733,17 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     float tj_before = (float)s.ms_judge;
     int judged = 0;
     HIPGO(hipEventRecord(c->ev0, c->st));
-    for (int f = 0; f < nf && rc == PDFS_OK; f++) {
+    int nj = 0;
+    for (int f = 0; f < nf; f++) {
       if (!inits[f] || certified[f]) continue;
-      long long fl = 0;
-      rc = pdfs_judge_f32_dev(c, fields[f], inits[f], dims[3 * f], dims[3 * f + 1],
-                              dims[3 * f + 2], dx, dy, dz, NULL, &fl);
-      s.judge_flagged += fl;
-      judged = 1;
+      jv[nj] = fields[f];
+      ji[nj] = inits[f];
+      memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
+      nj++;
+    }
+    if (nj > 0 && (rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl)) == PDFS_OK) {
+      for (int k = 0; k < nj; k++) s.judge_flagged += jfl[k];
+      judged = nj;
     }
     HIPGO(hipEventRecord(c->ev1, c->st));
     HIPGO(hipEventSynchronize(c->ev1));
@@ -684,6 +760,12 @@ out:
   free(delta);
   free(active);
   free(certified);
+  free(jidx);
+  free(jdone);
+  free(jd);
+  free(jv);
+  free(ji);
+  free(jfl);
   return rc;
 }
 

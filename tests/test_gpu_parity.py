@@ -307,6 +307,37 @@ def test_f32_no_sources_field_unchanged(ctx, dev):
     assert torch.equal(t, init)
 
 
+def test_converged_field_skips_rest_of_last_pass(ctx, dev):
+    """A field whose last change falls early in a pass is at the fixed point
+    after 8 unchanged orderings: the device skips the rest of the pass that
+    confirms it. The result equals the one swept ordering by ordering, the
+    pass count is the plain count, and the stats count only swept orderings
+    (point_updates = orderings x points)."""
+    X, Y, Z = 72, 40, 56
+    init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+    ctx.generate(pdfs_amd.GEN_TERRAIN, 2024, X, Y, Z, 1.0, dense=init)
+    # reference: single orderings until a whole pass changes nothing
+    ref = init.clone()
+    runs, run, passes = [], 0, 0
+    while True:
+        passes += 1
+        chg = [ctx.sweep_f32(ref, sn) for sn in range(1, 9)]
+        for ch in chg:
+            run = run + 1 if ch == 0 else 0
+            runs.append(run)
+        if not any(chg):
+            break
+        assert passes < 20
+    # the ordering at which the run of unchanged orderings first reached 8
+    first8 = next(i for i, r in enumerate(runs) if r >= 8) + 1
+    f = init.clone()
+    st = ctx.solve_f32(f, init, judge=0)
+    assert torch.equal(f.view(torch.int32), ref.view(torch.int32))
+    assert st["passes"] == passes
+    assert st["orderings"] == first8
+    assert st["point_updates"] == st["orderings"] * X * Y * Z
+
+
 def test_batch_with_single_point_field(ctx, dev):
     fields = [O.gen_point(1, 1, 1, seed=1), O.gen_spheres(40, 31, 22, seed=5, n=3, r=(2.0, 5.0)),
               O.gen_point(2, 1, 3, seed=2), O.gen_terrain(17, 19, 15, seed=6)]

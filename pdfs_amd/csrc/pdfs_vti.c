@@ -21,6 +21,7 @@
 
 #include <errno.h>
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -56,6 +57,116 @@ static const char *fgets511(const char *p, const char *end)
     if (*p++ == '\n') break;
   }
   return p;
+}
+
+static int is_ws(char c)
+{
+  return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f';
+}
+
+/* Serial form of the reference's fscanf("%d ") / fscanf("%lf ") loop: count
+ * numbers from p, each followed by its whitespace; NULL on a bad number. */
+static const char *parse_serial(const char *p, const char *end, size_t count, int as_int,
+                                int *iv, double *dv)
+{
+  for (size_t t = 0; t < count; t++) {
+    char *e;
+    if (as_int) {
+      errno = 0;
+      long v = strtol(p, &e, 10);
+      if (e == p || errno) return NULL;
+      iv[t] = (int)v;
+    } else {
+      double v = strtod(p, &e);
+      if (e == p) return NULL;
+      dv[t] = v;
+    }
+    p = e;
+    while (p < end && is_ws(*p)) p++;
+  }
+  return p;
+}
+
+/* The same parse in parallel. The text from p is cut into chunks at
+ * whitespace; each chunk's tokens are counted, a prefix sum gives every
+ * token its index, and the chunks holding tokens < count parse them into
+ * place. A token that strtol / strtod does not consume whole (e.g. "1-2",
+ * which the serial loop reads as two numbers) sends the whole block to the
+ * serial parse, so the result is always the serial one. */
+static const char *parse_block(const char *p, const char *end, size_t count, int as_int, int *iv,
+                               double *dv)
+{
+  const size_t len = (size_t)(end - p);
+  const int nthr = omp_get_max_threads();
+  const int nch = len < ((size_t)1 << 20) || nthr < 2 ? 1 : 16 * nthr;
+  if (nch == 1 || count == 0) return parse_serial(p, end, count, as_int, iv, dv);
+  const char **cb = (const char **)malloc((size_t)(nch + 1) * sizeof(char *));
+  size_t *cnt = (size_t *)calloc((size_t)nch + 1, sizeof(size_t));
+  if (!cb || !cnt) {
+    free(cb);
+    free(cnt);
+    return parse_serial(p, end, count, as_int, iv, dv);
+  }
+  /* chunk starts: at a token start (after whitespace), or the end */
+  cb[0] = p;
+  for (int c = 1; c <= nch; c++) {
+    const char *q = p + len / nch * (size_t)c;
+    if (c == nch) q = end;
+    while (q < end && !is_ws(*q)) q++;
+    while (q < end && is_ws(*q)) q++;
+    cb[c] = q < cb[c - 1] ? cb[c - 1] : q;
+  }
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nch; c++) {
+    size_t n = 0;
+    int in = 0;
+    for (const char *q = cb[c]; q < cb[c + 1]; q++) {
+      const int w = is_ws(*q);
+      if (!w && !in) n++;
+      in = !w;
+    }
+    cnt[c + 1] = n;
+  }
+  for (int c = 0; c < nch; c++) cnt[c + 1] += cnt[c];
+  if (cnt[nch] < count) {  /* not enough tokens: let the serial parse report it */
+    free(cb);
+    free(cnt);
+    return parse_serial(p, end, count, as_int, iv, dv);
+  }
+  int ok = 1;
+  const char *after = NULL;
+#pragma omp parallel for schedule(dynamic, 1) reduction(& : ok)
+  for (int c = 0; c < nch; c++) {
+    if (cnt[c] >= count) continue;
+    size_t t = cnt[c];
+    const char *q = cb[c];
+    while (q < cb[c + 1] && is_ws(*q)) q++;
+    while (q < cb[c + 1] && t < count && ok) {
+      char *e;
+      if (as_int) {
+        errno = 0;
+        long v = strtol(q, &e, 10);
+        if (e == q || errno) ok = 0;
+        else iv[t] = (int)v;
+      } else {
+        double v = strtod(q, &e);
+        if (e == q) ok = 0;
+        else dv[t] = v;
+      }
+      if (!ok) break;
+      if (e < end && !is_ws(*e)) {  /* token not consumed whole */
+        ok = 0;
+        break;
+      }
+      q = e;
+      while (q < end && is_ws(*q)) q++;
+      if (++t == count) after = q;  /* one chunk holds the last token */
+    }
+  }
+  free(cb);
+  free(cnt);
+  if (!ok || !after) return parse_serial(p, end, count, as_int, iv, dv);
+  return after;
 }
 
 int pdfs_vti_read(const char *path, pdfs_vti *out)
@@ -107,46 +218,49 @@ int pdfs_vti_read(const char *path, pdfs_vti *out)
     free(buf);
     return -3;
   }
-  /* vti_get_data (vti_parser.c:112-180) */
+  /* vti_get_data (vti_parser.c:112-180): the interior values in order,
+   * parsed in parallel, then placed into the padded arrays */
   const char *p = buf;
   for (int k = 0; k < 6; k++) p = fgets511(p, end);
-  int gx = X + 2, gy = Y + 2, gz = Z + 2;
-  size_t t = 0;
+  const int gx = X + 2, gy = Y + 2, gz = Z + 2;
+  const size_t ni = (size_t)X * Y * Z;
+  int *iv = (int *)malloc(ni * sizeof(int));
+  double *dv = (double *)malloc(ni * sizeof(double));
+  if (!iv || !dv) {
+    free(iv);
+    free(dv);
+    free(loc);
+    free(dist);
+    free(buf);
+    return -3;
+  }
+  p = parse_block(p, end, ni, 1, iv, NULL);
+  if (p) {
+    for (int k = 0; k < 2; k++) p = fgets511(p, end);
+    p = parse_block(p, end, ni, 0, NULL, dv);
+  }
+  if (!p) {
+    free(iv);
+    free(dv);
+    goto bad;
+  }
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < gz; i++)
-    for (int j = 0; j < gy; j++)
+    for (int j = 0; j < gy; j++) {
+      size_t t = ((size_t)i * gy + j) * gx;
       for (int k = 0; k < gx; k++, t++) {
         if (k == 0 || k == gx - 1 || j == 0 || j == gy - 1 || i == 0 || i == gz - 1) {
           loc[t] = -1;
-        } else {
-          char *e;
-          errno = 0;
-          long v = strtol(p, &e, 10);
-          if (e == p || errno) goto bad;
-          loc[t] = (int)v;
-          p = e;
-          while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\v' ||
-                             *p == '\f'))
-            p++;
-        }
-      }
-  for (int k = 0; k < 2; k++) p = fgets511(p, end);
-  t = 0;
-  for (int i = 0; i < gz; i++)
-    for (int j = 0; j < gy; j++)
-      for (int k = 0; k < gx; k++, t++) {
-        if (k == 0 || k == gx - 1 || j == 0 || j == gy - 1 || i == 0 || i == gz - 1) {
           dist[t] = INFINITY;
         } else {
-          char *e;
-          double v = strtod(p, &e);
-          if (e == p) goto bad;
-          dist[t] = v;
-          p = e;
-          while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\v' ||
-                             *p == '\f'))
-            p++;
+          const size_t q = ((size_t)(i - 1) * Y + (j - 1)) * X + (k - 1);
+          loc[t] = iv[q];
+          dist[t] = dv[q];
         }
       }
+    }
+  free(iv);
+  free(dv);
   free(buf);
   out->x = X;
   out->y = Y;

@@ -89,3 +89,55 @@ def test_writer_output_is_not_rereadable_like_the_reference(tmp_path):
     p = str(tmp_path / "o.vti")
     _lib().pdfs_vti_write(p.encode(), a.ctypes.data, 3, 3, 3, 1.0, 1.0, 1.0)
     assert read(p)[0] == -2
+
+
+def _check_read(v, loc, dist):
+    L, D = v["loc"], v["dist"]
+    assert np.array_equal(L[1:-1, 1:-1, 1:-1], loc)
+    assert np.array_equal(D[1:-1, 1:-1, 1:-1], dist)
+    ghost = np.ones(L.shape, bool)
+    ghost[1:-1, 1:-1, 1:-1] = False
+    assert np.all(L[ghost] == -1) and np.all(np.isinf(D[ghost]))
+
+
+def test_reader_parallel_path_large_file(tmp_path):
+    """A file of several MiB is parsed in parallel chunks; the values land
+    where the serial parse puts them."""
+    loc, dist = random_cli_input(96, 64, 40, 11)
+    p = str(tmp_path / "big.vti")
+    write_input_vti(p, loc, dist, h=(0.5, 0.5, 0.5))
+    assert os.path.getsize(p) > 2 << 20
+    rc, v = read(p)
+    assert rc == 0 and v["dims"] == (96, 64, 40)
+    _check_read(v, loc, dist)
+
+
+def test_reader_parallel_split_token_reads_like_serial(tmp_path):
+    """'1.25-0.5' is two numbers to the serial strtod loop (as to the
+    reference's fscanf); a chunked parse that sees one token must give the
+    same values."""
+    loc, dist = random_cli_input(96, 64, 40, 12)
+    t = 70001
+    dist.reshape(-1)[t] = 1.25
+    dist.reshape(-1)[t + 1] = -0.5
+    p = str(tmp_path / "split.vti")
+    write_input_vti(p, loc, dist)
+    txt = open(p).read()
+    assert txt.count("1.25 -0.5") == 1
+    open(p, "w").write(txt.replace("1.25 -0.5", "1.25-0.5"))
+    rc, v = read(p)
+    assert rc == 0
+    _check_read(v, loc, dist)
+
+
+def test_reader_parallel_errors(tmp_path):
+    loc, dist = random_cli_input(96, 64, 40, 13)
+    p = str(tmp_path / "trunc.vti")
+    write_input_vti(p, loc, dist)
+    txt = open(p).read()
+    open(p, "w").write(txt[: int(len(txt) * 0.9)])
+    assert read(p)[0] == -4
+    t = txt.index("distance")
+    k = txt.index(" ", t + 200000)
+    open(p, "w").write(txt[:k] + " abc" + txt[k:])
+    assert read(p)[0] == -4

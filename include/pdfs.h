@@ -83,10 +83,11 @@ typedef struct {
 } pdfs_opts;
 
 typedef struct {
-  /* FP32 path: a field converges on a pass that changes nothing; the
-   * orderings of that pass that follow its 8th consecutive unchanged
-   * ordering (counting back into the pass before) are skipped on the device.
-   * orderings counts the orderings swept (for a single field without the
+  /* FP32 path: a field converges on a pass that changes nothing. Once one
+   * ordering changes nothing the field is at the fixed point, and the
+   * device skips its remaining orderings (the rest of that pass and the
+   * whole next one, which changes nothing); passes and the result are
+   * those of sweeping every ordering. orderings counts the orderings swept (for a single field without the
    * skipped ones; for a batch, the launches); point_updates excludes skipped
    * orderings. */
   int passes, orderings;

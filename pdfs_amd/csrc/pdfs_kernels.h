@@ -61,6 +61,12 @@ typedef struct {
   size_t edge_x_bytes, edge_y_bytes;  /* whole allocation from edge_x; y part alone */
 } pdfsk_batch;
 
+/* A field whose last ordering changed nothing is at the fixed point: every
+ * point met u <= G(neighbours) when it was visited, and as nothing changed
+ * those neighbours are still the current values, so no ordering can change
+ * it again. Later orderings skip its columns (streak[2f+1] >= this run). */
+#define PDFSK_CONVERGED_RUN 1u
+
 /* Variants of the column pipeline (autotuner chooses one): tile shape and
  * workgroups per CU (max = as many as fit). */
 enum {

@@ -352,9 +352,6 @@ struct ColArgs {
   pdfs_sp32 sp;
 };
 
-// A field whose last 8 orderings (one of each sweep direction) changed
-// nothing is at the fixed point: later orderings skip its columns.
-#define PDFSK_CONVERGED_RUN 8u
 
 template <int TX, int TY>
 struct ColGeom {

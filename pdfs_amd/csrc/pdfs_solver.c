@@ -677,11 +677,11 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
       last_total += delta[f];
       if (field_passes) field_passes[f] = p + 1;
       jdone[f] = o->fixed_passes <= 0 && delta[f] == 0;
-      if (delta[f] == 0) {
-        /* a pass without change: the device skipped the orderings of it
-         * that followed the field's 8th consecutive unchanged one */
+      {
+        /* the device skipped the orderings of this pass that followed the
+         * field's PDFSK_CONVERGED_RUN-th consecutive unchanged one */
         const unsigned run = c->h_streak[2 * f + 1];
-        long long skipped = run > 8u ? (long long)(run - 8u) : 0;
+        long long skipped = run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
         if (skipped > 8) skipped = 8;
         s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
         if (nf == 1) s.orderings -= (int)skipped;

@@ -323,3 +323,27 @@ def test_generators_independent_recomputation():
     p = O.gen_point(X, Y, Z, seed)
     assert p[i, j, k] == 0.0 and np.count_nonzero(O.interior(p) == 0) == 1
     assert np.all(np.isinf(p[0])) and np.all(np.isinf(p[:, :, -1]))
+
+
+def test_one_unchanged_ordering_is_the_fixed_point():
+    """The device skips a field's orderings after its first ordering that
+    changes nothing. Pinned here on the FP32 oracle sweep: after such an
+    ordering, two more full passes (every direction) change no bit, on
+    terrain, sphere, point-source and anisotropic fields."""
+    cases = [(O.gen_terrain(20, 17, 13, seed=3), (1.0, 1.0, 1.0)),
+             (O.gen_spheres(19, 14, 16, seed=4, n=4, r=(1.5, 4.0)), (1.0, 1.0, 1.0)),
+             (O.gen_point(15, 18, 12, seed=5), (1.0, 1.0, 1.0)),
+             (O.gen_point(16, 11, 14, seed=6), (0.5, 1.25, 0.75))]
+    for a0, h in cases:
+        a = a0.astype(np.float32)
+        for k in range(200):
+            prev = a.copy()
+            O.sweep_f32(a, k % 8 + 1, h)
+            if np.array_equal(prev.view(np.int32), a.view(np.int32)):
+                break
+        else:
+            raise AssertionError("no unchanged ordering")
+        fixed = a.copy()
+        for k2 in range(k + 1, k + 17):
+            O.sweep_f32(a, k2 % 8 + 1, h)
+        assert np.array_equal(fixed.view(np.int32), a.view(np.int32))

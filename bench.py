@@ -9,12 +9,15 @@ orderings until a pass changes no FP32 value) + the FP64 residual judge.
 
 Also reported (in "extra"): the certified stop (passes until the FP64 judge
 passes) and the learned warm start on the same field and on a fresh
-256x256x128 terrain field; the 512-field uneven batch solved in shared
-persistent launches (plain, certified, warm-started) against each field in
-its own launches; FP32 vs FP64 error; the reference CPU core's rate.
+256x256x128 terrain field; a batch of 512^3 sphere fields (this field and
+the next seeds) in shared persistent launches, plain, certified and
+warm-started, against the single-field plain rate; the 512-field uneven batch
+solved in shared persistent launches (plain, certified, warm-started) against
+each field in its own launches; FP32 vs FP64 error; the reference CPU core's
+rate.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
-       [--no-oracle] [--no-cpu] [--batch N] [--no-warm]
+       [--no-oracle] [--no-cpu] [--batch N] [--batch512 K] [--no-warm]
 Prints one JSON line.
 """
 import argparse
@@ -45,6 +48,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--cpu-orderings", type=int, default=3)
     ap.add_argument("--batch", type=int, default=512, help="fields of the c4 batch leg (0: skip)")
+    ap.add_argument("--batch512", type=int, default=8,
+                    help="fields of the same-size batch leg (0: skip)")
     ap.add_argument("--no-warm", action="store_true", help="skip the certified / warm-start legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run plain 1 GiB copies (for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -141,6 +146,8 @@ def main():
             f"rms {extra['f32_vs_f64_rms_err']:.3e}")
         del padded, inner, err
 
+    if args.batch512 > 0:
+        extra["batch_512"] = same_size_batch_leg(ctx, args, init, net, fields_per_s)
     if args.batch > 0:
         extra["batch"] = batch_leg(ctx, args.batch, net)
     if net is not None:
@@ -261,6 +268,65 @@ def solve_modes(ctx, net, init, field, tile, plain_ms):
                        "fallbacks": sw["fallbacks"], "judge_flagged": int(sw["judge_flagged"]),
                        "speedup_vs_plain": round(plain_ms / wms, 3)}
     log(f"[bench] modes {tuple(init.shape)}: {out}")
+    return out
+
+
+def same_size_batch_leg(ctx, args, init, net, single_fields_per_s):
+    """K fields of the value's workload (seeds seed, seed + 1, ...) solved
+    together: each ordering is one persistent launch over all K fields
+    (graph-replayed passes, tile tuned online and cached), plain, certified
+    and warm-started. Fields/s against the single-field plain rate; the sweep
+    kernel's algorithmic bandwidth over the plain batch's executed
+    point-updates."""
+    import torch
+
+    import pdfs_amd
+    K = args.batch512
+    Z, Y, X = init.shape
+    inits = [init]
+    for i in range(1, K):
+        t = torch.empty_like(init)
+        ctx.generate(pdfs_amd.GEN_SPHERES, args.seed + i, X, Y, Z, 1.0, dense=t)
+        inits.append(t)
+    fields = [torch.empty_like(t) for t in inits]
+    log(f"[bench] {K} x {X}x{Y}x{Z} batch")
+
+    def reset():
+        for f, t in zip(fields, inits):
+            f.copy_(t)
+        torch.cuda.synchronize()
+
+    def timed(fn):
+        reset()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    auto = pdfs_amd.TILE_AUTO
+    timed(lambda: ctx.solve_batch_f32(fields, None, tile=auto, graph=1, judge=0))  # tune
+    tp, (sp, pp, _) = timed(lambda: ctx.solve_batch_f32(fields, None, tile=auto, graph=1, judge=0))
+    _, (sk, _, _) = timed(lambda: ctx.solve_batch_f32(fields, None, tile=auto, profile=1, judge=0))
+    gbs = 8.0 * sk["point_updates"] / (sk["ms_sweep_kernels"] * 1e-3) / 1e9
+    out = {"fields": K, "grid": [X, Y, Z], "tile": sp["tile"],
+           "plain": {"seconds": round(tp, 4), "fields_per_s": round(K / tp, 2),
+                     "passes_max": max(pp),
+                     "speedup_vs_single_plain": round(K / tp / single_fields_per_s, 3),
+                     "sweep_gbs": round(gbs, 1), "sweep_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}}
+    tc, (sc, pc, _) = timed(lambda: ctx.solve_batch_f32(fields, inits, tile=auto, graph=1, certify=1))
+    out["certified"] = {"seconds": round(tc, 4), "fields_per_s": round(K / tc, 2),
+                        "passes_max": max(pc), "judge_flagged": int(sc["judge_flagged"]),
+                        "speedup_vs_single_plain": round(K / tc / single_fields_per_s, 3)}
+    if net is not None:
+        tw, (sw, pw, fb) = timed(lambda: ctx.solve_batch_warm_f32(net, fields, inits, tile=auto,
+                                                                 graph=1))
+        out["warm"] = {"seconds": round(tw, 4), "fields_per_s": round(K / tw, 2),
+                       "passes_max": max(pw), "fallbacks": int(sum(fb)),
+                       "ms_network": round(sw["ms_warm"], 3), "judge_flagged": int(sw["judge_flagged"]),
+                       "speedup_vs_single_plain": round(K / tw / single_fields_per_s, 3)}
+    log(f"[bench] {K} x 512^3 batch: {out}")
+    del inits, fields
+    torch.cuda.empty_cache()
     return out
 
 

@@ -192,6 +192,10 @@ int pdfs_net_set(pdfs_net *n, const float *host);
  * inflation of the warm start w = min(init, a * d + b * hmax) */
 void pdfs_net_get_calibration(const pdfs_net *n, float *scale, float *a, float *b);
 void pdfs_net_set_calibration(pdfs_net *n, float scale, float a, float b);
+/* cutoff (in h): where the raw prediction d reaches cut * hmax, beyond the
+ * distances the network resolves, the warm start keeps the init (0: none) */
+float pdfs_net_get_cut(const pdfs_net *n);
+void pdfs_net_set_cut(pdfs_net *n, float cut);
 void pdfs_net_grid(int x, int y, int z, int *Dz, int *Dy, int *Dx);
 /* mask (BF16 [Dz][Dy][Dx][16]) of a dense init; target (FP32 [Dz][Dy][Dx]) of
  * a converged field */

@@ -32,7 +32,8 @@ EXPORTED = [
     "pdfs_unpack_dev", "pdfs_solve_batch_f32_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
     "pdfs_vti_write", "pdfs_vti_free", "pdfs_ctx_stream", "pdfs_net_create", "pdfs_net_destroy",
     "pdfs_net_load", "pdfs_net_save", "pdfs_net_num_params", "pdfs_net_get", "pdfs_net_set",
-    "pdfs_net_get_calibration", "pdfs_net_set_calibration", "pdfs_net_grid",
+    "pdfs_net_get_calibration", "pdfs_net_set_calibration", "pdfs_net_get_cut", "pdfs_net_set_cut",
+    "pdfs_net_grid",
     "pdfs_net_features_dev", "pdfs_net_target_dev", "pdfs_net_forward_dev",
     "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_warm_start_batch_f32_dev",
     "pdfs_solve_warm_f32_dev",
@@ -151,6 +152,10 @@ def lib():
         L.pdfs_net_get_calibration.restype = None
         L.pdfs_net_set_calibration.argtypes = [vp, F, F, F]
         L.pdfs_net_set_calibration.restype = None
+        L.pdfs_net_get_cut.argtypes = [vp]
+        L.pdfs_net_get_cut.restype = F
+        L.pdfs_net_set_cut.argtypes = [vp, F]
+        L.pdfs_net_set_cut.restype = None
         L.pdfs_net_grid.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
         L.pdfs_net_grid.restype = None
         L.pdfs_net_features_dev.argtypes = [vp, vp, I, I, I, vp]
@@ -474,6 +479,15 @@ class Net:
 
     def set_calibration(self, scale, a, b):
         lib().pdfs_net_set_calibration(self.n, scale, a, b)
+
+    @property
+    def cut(self):
+        """Cutoff in h: the warm start keeps the init where the raw
+        prediction reaches cut * hmax (0: none)."""
+        return lib().pdfs_net_get_cut(self.n)
+
+    def set_cut(self, cut):
+        lib().pdfs_net_set_cut(self.n, cut)
 
     def features(self, init, mask):
         import torch

@@ -49,6 +49,7 @@ struct pdfs_net {
   uint16_t *WB, *WT;    /* BF16 weights [27][Co][Ci] and transposes [27][Ci][Co] */
   int step;
   float scale, a, b;    /* target scale; upper-bound inflation */
+  float cut;            /* predictions at or beyond cut (in h) keep the init; 0: none */
   /* workspace for (nb, D0) */
   long long cap_v0;
   int train_ws;
@@ -198,6 +199,7 @@ pdfs_net *pdfs_net_create(pdfs_ctx *c, unsigned long long seed)
   n->scale = 1.f / 32.f;
   n->a = 1.f;
   n->b = 0.f;
+  n->cut = 0.f;
   size_t bytes = (size_t)off * sizeof(float);
   float *h = (float *)malloc(bytes);
   if (!h || hipMalloc((void **)&n->P, bytes) != hipSuccess ||
@@ -294,8 +296,24 @@ void pdfs_net_set_calibration(pdfs_net *n, float scale, float a, float b)
   n->b = b;
 }
 
+float pdfs_net_get_cut(const pdfs_net *n)
+{
+  return n ? n->cut : 0.f;
+}
+
+void pdfs_net_set_cut(pdfs_net *n, float cut)
+{
+  if (n) n->cut = cut > 0.f ? cut : 0.f;
+}
+
+/* the cutoff in the units of the prediction d (h * hmax); none: +inf */
+static float cut_units(const pdfs_net *n, double hmax)
+{
+  return n->cut > 0.f ? (float)(n->cut * hmax) : INFINITY;
+}
+
 /* File: "PDFSNET1", int32 NL, NL x (int32 cin, int32 cout), float32 scale,
- * a, b, 0, then per layer BF16 weights [27][cout][cin] and FP32 bias[cout].
+ * a, b, cut (0: none), then per layer BF16 weights [27][cout][cin] and FP32 bias[cout].
  * Little-endian. */
 int pdfs_net_save(pdfs_net *n, const char *path)
 {
@@ -319,7 +337,7 @@ int pdfs_net_save(pdfs_net *n, const char *path)
     int d[2] = {CIN[l], COUT[l]};
     ok &= fwrite(d, 4, 2, f) == 2;
   }
-  float cal[4] = {n->scale, n->a, n->b, 0.f};
+  float cal[4] = {n->scale, n->a, n->b, n->cut};
   ok &= fwrite(cal, 4, 4, f) == 4;
   for (int l = 0; l < NL && ok; l++) {
     long long nw = 27LL * CIN[l] * COUT[l];
@@ -366,6 +384,7 @@ int pdfs_net_load(pdfs_net *n, const char *path)
     n->scale = cal[0];
     n->a = cal[1];
     n->b = cal[2];
+    n->cut = cal[3] > 0.f ? cal[3] : 0.f;
   }
   return rc;
 }
@@ -606,7 +625,7 @@ int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int
   NCK(pdfsnn_mask_down4(init, x, y, z, n->mask, Dz, Dy, Dx, 1e6f, n->st));
   if ((rc = forward(n, n->mask, 1, Dz, Dy, Dx))) return rc;
   NCK(pdfsnn_warm_up4(n->out, 16, Dz, Dy, Dx, (float)(hmax / n->scale), n->a,
-                      (float)(n->b * hmax), init, w, x, y, z, n->st));
+                      (float)(n->b * hmax), cut_units(n, hmax), init, w, x, y, z, n->st));
   return PDFS_OK;
 }
 
@@ -696,7 +715,7 @@ int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits
   NCK(pdfsnn_mask_down4_rag(dfl, dsg[0], nf, neblk[0], n->mask, 1e6f, n->st));
   if ((rc = forward_rag(n, n->mask, nf, dsg, nblk, neblk))) return rc;
   NCK(pdfsnn_warm_up4_rag(n->out, 16, dsg[0], dfl, nf, (int)fblk, (float)(hmax / n->scale), n->a,
-                          (float)(n->b * hmax), n->st));
+                          (float)(n->b * hmax), cut_units(n, hmax), n->st));
   return PDFS_OK;
 }
 

@@ -51,9 +51,11 @@ int pdfsnn_mask_down4(const float *f, int X, int Y, int Z, uint16_t *m, int Dz, 
 /* t[z][y][x] = scale * u(4z, 4y, 4x) (clamped to the field) */
 int pdfsnn_target_down4(const float *u, int X, int Y, int Z, float *t, int Dz, int Dy, int Dx,
                         float scale, hipStream_t st);
-/* w = min(init, a * max(trilinear(pred) * inv_scale, 0) + b) on the dense field */
+/* d = trilinear(pred) * inv_scale; w = min(init, a * max(d, 0) + b) on the
+ * dense field, and w = init where d >= cut */
 int pdfsnn_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_scale, float a,
-                    float b, const float *init, float *w, int X, int Y, int Z, hipStream_t st);
+                    float b, float cut, const float *init, float *w, int X, int Y, int Z,
+                    hipStream_t st);
 
 /* Uneven batches. The tensors of a batch's fields are packed one after the
  * other: at a network level, field f's tensor starts at position seg[f].pos
@@ -86,7 +88,8 @@ int pdfsnn_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t *skip, int
 int pdfsnn_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *seg0, int nf, int eblocks0,
                           uint16_t *m, float far, hipStream_t st);
 int pdfsnn_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *seg0, const pdfsnn_fld *fld,
-                        int nf, int fblocks, float inv_scale, float a, float b, hipStream_t st);
+                        int nf, int fblocks, float inv_scale, float a, float b, float cut,
+                        hipStream_t st);
 
 #ifdef __cplusplus
 }

@@ -508,7 +508,8 @@ __global__ void k_target_down4(const float *u, int X, int Y, int Z, float *t, in
 // samples at 4i, FP32 channel 0 of a C-channel tensor) back to the dense
 // field, inflated to an upper bound w = a * d + b, capped by the init.
 __global__ void k_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_scale,
-                           float a, float b, const float *init, float *w, int X, int Y, int Z) {
+                           float a, float b, float cut, const float *init, float *w, int X, int Y,
+                           int Z) {
 #pragma clang fp contract(off)
   const long N = (long)X * Y * Z;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -527,8 +528,9 @@ __global__ void k_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, flo
     const float c11 = P(z1, y1, x0) * (1 - tx) + P(z1, y1, x1) * tx;
     const float c0 = c00 * (1 - ty) + c01 * ty, c1 = c10 * (1 - ty) + c11 * ty;
     const float d = (c0 * (1 - tz) + c1 * tz) * inv_scale;
-    const float up = a * (d > 0.f ? d : 0.f) + b;
     const float f0 = init[i];
+    // beyond the cutoff the prediction has not seen a source: keep the init
+    const float up = d >= cut ? f0 : a * (d > 0.f ? d : 0.f) + b;
     w[i] = f0 < up ? f0 : up;
   }
 }
@@ -644,7 +646,8 @@ __device__ __forceinline__ int fld_by_blk(const pdfsnn_fld *fld, int nf, int v) 
 
 __global__ __launch_bounds__(256) void k_warm_up4_rag(const float *pred, int C,
                                                       const pdfsnn_seg *s0, const pdfsnn_fld *fld,
-                                                      int nf, float inv_scale, float a, float b) {
+                                                      int nf, float inv_scale, float a, float b,
+                                                      float cut) {
 #pragma clang fp contract(off)
   const int f = fld_by_blk(fld, nf, (int)blockIdx.x);
   const pdfsnn_fld d = fld[f];
@@ -669,8 +672,8 @@ __global__ __launch_bounds__(256) void k_warm_up4_rag(const float *pred, int C,
     const float c11 = P(z1, y1, x0) * (1 - tx) + P(z1, y1, x1) * tx;
     const float c0 = c00 * (1 - ty) + c01 * ty, c1 = c10 * (1 - ty) + c11 * ty;
     const float dd = (c0 * (1 - tz) + c1 * tz) * inv_scale;
-    const float up = a * (dd > 0.f ? dd : 0.f) + b;
     const float f0 = d.init[li];
+    const float up = dd >= cut ? f0 : a * (dd > 0.f ? dd : 0.f) + b;
     d.w[li] = f0 < up ? f0 : up;
   }
 }
@@ -801,10 +804,10 @@ extern "C" int pdfsnn_target_down4(const float *u, int X, int Y, int Z, float *t
 }
 
 extern "C" int pdfsnn_warm_up4(const float *pred, int C, int Dz, int Dy, int Dx, float inv_scale,
-                               float a, float b, const float *init, float *w, int X, int Y,
-                               int Z, hipStream_t st) {
+                               float a, float b, float cut, const float *init, float *w, int X,
+                               int Y, int Z, hipStream_t st) {
   hipLaunchKernelGGL(k_warm_up4, dim3(grid1((long)X * Y * Z, 256)), dim3(256), 0, st, pred, C,
-                     Dz, Dy, Dx, inv_scale, a, b, init, w, X, Y, Z);
+                     Dz, Dy, Dx, inv_scale, a, b, cut, init, w, X, Y, Z);
   return hip_rc(hipGetLastError());
 }
 
@@ -859,9 +862,9 @@ extern "C" int pdfsnn_mask_down4_rag(const pdfsnn_fld *fld, const pdfsnn_seg *se
 
 extern "C" int pdfsnn_warm_up4_rag(const float *pred, int C, const pdfsnn_seg *seg0,
                                    const pdfsnn_fld *fld, int nf, int fblocks, float inv_scale,
-                                   float a, float b, hipStream_t st) {
+                                   float a, float b, float cut, hipStream_t st) {
   if (nf < 1 || fblocks < 1) return -1;
   hipLaunchKernelGGL(k_warm_up4_rag, dim3((unsigned)fblocks), dim3(256), 0, st, pred, C, seg0,
-                     fld, nf, inv_scale, a, b);
+                     fld, nf, inv_scale, a, b, cut);
   return hip_rc(hipGetLastError());
 }

@@ -219,6 +219,9 @@ static int default_tile(int nf, int x, int y)
   return PDFSK_TILE_16x16;
 }
 
+/* the online tuner gives each candidate one of orderings 2..7 */
+_Static_assert(PDFSK_TILE_COUNT <= 6, "tile candidates must fit orderings 2..7 of a pass");
+
 static int tune_lookup(pdfs_ctx *c, long long key)
 {
   for (int i = 0; i < c->ntune; i++)
@@ -303,11 +306,33 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
   return PDFS_OK;
 }
 
-static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile)
+/* Face entries of a batch's edge allocation for a tile (one row per column
+ * step, z + TX + TY - 1 steps); 0 if the batch does not fit the kernel's
+ * 32-bit buffer offsets (2 GiB with padding). */
+static int plan_fits(int nf, const int *dims, int tile)
+{
+  int TX, TY;
+  pdfsk_tile_dims(tile, &TX, &TY);
+  size_t ex = 0, ey = 0;
+  for (int f = 0; f < nf; f++) {
+    const int x = dims[3 * f], y = dims[3 * f + 1], z = dims[3 * f + 2];
+    const size_t nc = (size_t)((x + TX - 1) / TX) * (size_t)((y + TY - 1) / TY);
+    ex += nc * (size_t)(z + TX + TY - 1) * TY;
+    ey += nc * (size_t)(z + TX + TY - 1) * TX;
+  }
+  return (ex + ey + 4 * PDFSK_EDGE_PAD) * 8 < (1ull << 31);
+}
+
+/* keep: re-tile the same batch mid-solve, keeping the per-field change
+ * counters and unchanged-ordering runs (they do not depend on the tile) */
+static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile,
+                         int keep)
 {
   if (nf < 1) return fail(PDFS_EINVAL, "empty batch");
   int TX, TY;
   pdfsk_tile_dims(tile, &TX, &TY);
+  if (!plan_fits(nf, dims, tile))
+    return fail(PDFS_EINVAL, "batch too large for one sweep (edge buffers exceed 2 GiB)");
   if (nf > c->nf || !c->h_fields) {
     free(c->h_fields);
     free(c->prev_changed);
@@ -346,8 +371,6 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
     ey += nc * (size_t)(z + TX + TY - 1) * TX;
     ncols += (int)nc;
   }
-  if ((ex + ey + 4 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31))
-    return fail(PDFS_EINVAL, "batch too large for one sweep (edge buffers exceed 2 GiB)");
   if (ex > c->cap_ex || ey > c->cap_ey) {
     /* one allocation: the x-face part, then the y-face part, each with a
      * padding of PDFSK_EDGE_PAD entries on both sides; the kernel's single
@@ -378,15 +401,22 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
   c->tile = tile;
   HIPCHK(hipMemcpyAsync(c->bt.fields, c->h_fields, (size_t)nf * sizeof(pdfsk_field),
                         hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMemsetAsync(c->bt.changed, 0, (size_t)nf * sizeof(unsigned), c->st));
-  HIPCHK(hipMemsetAsync(c->bt.streak, 0, (size_t)2 * nf * sizeof(unsigned), c->st));
+  if (!keep) {
+    HIPCHK(hipMemsetAsync(c->bt.changed, 0, (size_t)nf * sizeof(unsigned), c->st));
+    HIPCHK(hipMemsetAsync(c->bt.streak, 0, (size_t)2 * nf * sizeof(unsigned), c->st));
+    memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
+  }
   c->bt.nfields = nf;
-  memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);
     c->gexec = NULL;
   }
   return plan_upload_cols(c, NULL);
+}
+
+static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile)
+{
+  return plan_build_ex(c, nf, fields, dims, tile, 0);
 }
 
 static int launch_grid(pdfs_ctx *c, int iso)
@@ -565,20 +595,23 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   }
   pdfs_stats s;
   memset(&s, 0, sizeof s);
-  /* tile: explicit, default, or tuned online (PDFS_TILE_AUTO): the first
-   * passes use one candidate each (results do not depend on the tile), the
-   * fastest is kept for the rest and cached for the batch's size class */
+  /* tile: explicit, default, or tuned online (PDFS_TILE_AUTO): inside the
+   * first pass, after one untimed ordering, each candidate sweeps one
+   * ordering (results do not depend on the tile, and every ordering of the
+   * first pass sweeps the whole batch); the fastest sweeps the rest and is
+   * cached for the batch's size class */
   long long key = tune_key(nf, dims);
-  int tuning = 0, cand = 0;
-  float cand_ms[PDFSK_TILE_COUNT];
+  int tuning = 0;
   int tile;
   if (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) {
     tile = o->tile;
   } else if (o->tile == PDFS_TILE_AUTO) {
     tile = tune_lookup(c, key);
     if (tile < 0) {
+      /* the first candidate whose edge buffers fit */
       tuning = 1;
-      tile = 0;
+      for (tile = 0; tile < PDFSK_TILE_COUNT - 1 && !plan_fits(nf, dims, tile); tile++) {
+      }
     }
   } else {
     tile = default_tile(nf, dims[0], dims[1]);
@@ -625,32 +658,41 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     long long npts = 0;
     for (int f = 0; f < nf; f++)
       if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
-    float t_prev = (float)ms;
-    if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph && !tuning, &kms))) break;
+    if (tuning) {
+      /* ordering 1 untimed, candidate v on ordering v + 2, the best on 8 */
+      double cand_ms[PDFSK_TILE_COUNT], tk = 0.0;
+      if ((rc = run_orderings(c, &sp, 1, 1, o->profile, 0, &kms))) break;
+      for (int v = 0; v < PDFSK_TILE_COUNT && !rc; v++) {
+        cand_ms[v] = 1e30;
+        if (!plan_fits(nf, dims, v)) {
+          /* its ordering still runs, on the current tile */
+          rc = run_orderings(c, &sp, v + 2, v + 2, o->profile, 0, &kms);
+          continue;
+        }
+        if (c->tile != v && (rc = plan_build_ex(c, nf, fields, dims, v, 1))) break;
+        tk = 0.0;
+        rc = run_orderings(c, &sp, v + 2, v + 2, 1, 0, &tk);
+        cand_ms[v] = tk;
+        if (o->profile) kms += tk;
+      }
+      if (rc) break;
+      int best = 0;
+      for (int v = 1; v < PDFSK_TILE_COUNT; v++)
+        if (cand_ms[v] < cand_ms[best]) best = v;
+      tuning = 0;
+      tune_store(c, key, best);
+      tile = best;
+      if (c->tile != best && (rc = plan_build_ex(c, nf, fields, dims, best, 1))) break;
+      if ((rc = run_orderings(c, &sp, PDFSK_TILE_COUNT + 2, 8, o->profile, 0, &kms))) break;
+    } else if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph, &kms))) {
+      break;
+    }
     HIPGO(hipEventRecord(c->ev1, c->st));
     unsigned err = 0;
     if ((rc = read_changed(c, delta, &err))) break;
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
     ms = t;
-    if (tuning) {
-      /* only full passes (every field active) are comparable */
-      cand_ms[cand] = nactive == nf ? t - t_prev : 1e30f;
-      if (++cand == PDFSK_TILE_COUNT || nactive < nf) {
-        int best = 0;
-        for (int v = 1; v < cand; v++)
-          if (cand_ms[v] < cand_ms[best]) best = v;
-        tuning = 0;
-        if (nactive == nf) tune_store(c, key, best);
-        tile = best;
-      } else {
-        tile = cand;
-      }
-      if (tile != c->tile) {
-        /* carry the convergence bookkeeping across the re-plan */
-        if ((rc = plan_build(c, nf, fields, dims, tile))) break;
-      }
-    }
     s.passes++;
     s.orderings += 8;
     s.point_updates += 8 * npts;

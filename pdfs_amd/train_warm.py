@@ -10,10 +10,14 @@ the solution at the coarse points scaled by 1/32.
 Training: Adam, batch 8, cosine learning-rate decay, random axis flips (the
 mapping from sources to distances commutes with them).
 
-Calibration: on held-out fields, for each inflation a in a small set, b(a)
-= max over unknown cells of (u - a d) + margin; the (a, b) with the least
-mean slack a d + b - u is kept, so that w = min(init, a d + b) lies above the
-solution on every held-out cell.
+Calibration: the network resolves distances up to about its receptive
+field; beyond a cutoff its prediction levels off below the solution, so
+cells whose raw prediction d reaches the cutoff keep their init. On held-out
+fields, for each cutoff and each inflation a in a small set, b = max over the
+unknown cells inside the cutoff of (u - a d) + margin, so that
+w = min(init, a d + b) lies above the solution on every held-out cell; the
+(cutoff, a, b) leaving the least work to the sweeps is kept (a cell inside
+counts its slack a d + b - u, a cell beyond its distance u).
 
 Calibration fields also include larger grids (256x256x128 terrain, 512^3
 spheres by default), whose distances exceed the 128^3 training fields'.
@@ -43,6 +47,9 @@ def draw_spec(rng, i, S):
 
 
 CAL_A = (1.0, 1.1, 1.2, 1.35, 1.5, 1.75, 2.0)
+# cutoff candidates in h (0: none): beyond the distances the network resolves
+# (its receptive field spans about 150 cells) the warm start keeps the init
+CAL_CUTS = (48.0, 64.0, 80.0, 96.0, 112.0, 128.0, 160.0, 0.0)
 
 
 def make_set(ctx, net, n, seed, S=128, chunk=125, dev="cuda:0"):
@@ -104,15 +111,20 @@ def calibrate(ctx, net, n, seed, S=128, dev="cuda:0", margin=1.0, large=LARGE_CA
     def add(init):
         u = init.clone()
         ctx.solve_f32(u, None, judge=0)
-        d = raw_prediction(ctx, net, init)
+        d = raw_prediction(ctx, net, init).clamp(min=0)
         unk = init >= 1.0e6
         # a subsample of the unknown cells keeps the host memory small
         idx = torch.nonzero(unk.flatten()).flatten()
         if idx.numel() > 4_000_000:
             idx = idx[:: idx.numel() // 4_000_000 + 1]
         uu, dd = u.flatten()[idx].double(), d.flatten()[idx].double()
-        # the worst cell over all unknowns is kept exactly
-        worst = {a: float((u[unk] - a * d[unk].clamp(min=0)).max()) for a in CAL_A}
+        # the worst cell inside each cutoff over all unknowns is kept exactly
+        worst = {}
+        for cut in CAL_CUTS:
+            m = unk & (d < cut) if cut > 0 else unk
+            um, dm = u[m], d[m]
+            for a in CAL_A:
+                worst[(a, cut)] = float((um - a * dm).max()) if um.numel() else -1e30
         stats.append((uu, dd, worst))
 
     for i in range(n):
@@ -125,14 +137,26 @@ def calibrate(ctx, net, n, seed, S=128, dev="cuda:0", margin=1.0, large=LARGE_CA
         ctx.generate_ex(kind, seed * 7919 + j, X, Y, Z, 1.0, nobj, p, q, dense=init)
         add(init)
         del init
+    # per cutoff, the (a, b) that bound every unknown cell inside it with the
+    # least slack; across cutoffs, the least work left to the sweeps: a cell
+    # inside counts its slack, a cell beyond (kept at its init) its distance
     cands = []
-    for a in CAL_A:
-        worst = max(w[a] for _, _, w in stats)
-        b = max(0.0, worst + margin)
-        slack = float(np.mean([float((a * d.clamp(min=0) + b - u).mean()) for u, d, _ in stats]))
-        cands.append({"a": a, "b": b, "mean_slack": slack})
-        log(f"[calib] a={a:.2f} b={b:.2f} mean slack {slack:.2f} h")
-    best = min(cands, key=lambda c: c["mean_slack"])
+    for cut in CAL_CUTS:
+        lim = cut if cut > 0 else float("inf")
+        best_a = None
+        for a in CAL_A:
+            worst = max(w[(a, cut)] for _, _, w in stats)
+            b = max(0.0, worst + margin)
+            cost = float(np.mean([float(torch.where(d < lim, a * d + b - u, u).mean())
+                                  for u, d, _ in stats]))
+            beyond = float(np.mean([float((d >= lim).double().mean()) for _, d, _ in stats]))
+            c = {"cut": cut, "a": a, "b": b, "cost": cost, "beyond": beyond}
+            if best_a is None or cost < best_a["cost"]:
+                best_a = c
+        cands.append(best_a)
+        log(f"[calib] cut={cut:.0f} a={best_a['a']:.2f} b={best_a['b']:.2f} "
+            f"cost {best_a['cost']:.2f} h, {best_a['beyond']:.3f} of the cells beyond")
+    best = min(cands, key=lambda c: c["cost"])
     return best, cands
 
 
@@ -163,10 +187,12 @@ def main():
         scale = net.calibration[0]
         best, cands = calibrate(ctx, net, args.val, args.seed + 104729)
         net.set_calibration(scale, best["a"], best["b"])
+        net.set_cut(best["cut"])
         net.save(args.out)
         jp = os.path.splitext(args.recalibrate)[0] + ".json"
         summary = json.load(open(jp)) if os.path.exists(jp) else {}
-        summary["calibration"] = {"scale": scale, "a": best["a"], "b": best["b"], "candidates": cands,
+        summary["calibration"] = {"scale": scale, "a": best["a"], "b": best["b"], "cut": best["cut"],
+                                  "candidates": cands,
                                   "large_fields": [list(map(str, c)) for c in LARGE_CALIB]}
         with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
             json.dump(summary, f, indent=1)
@@ -211,6 +237,7 @@ def main():
         f"p99 {err.quantile(0.99).item():.3f} max {err.max().item():.3f}")
     best, cands = calibrate(ctx, net, args.val, args.seed + 104729)
     net.set_calibration(scale, best["a"], best["b"])
+    net.set_cut(best["cut"])
     net.save(args.out)
     summary = {"parameters": net.num_params, "fields": args.fields, "field_size": 128,
                "steps": args.steps, "batch": args.batch, "lr": args.lr, "seed": args.seed,
@@ -221,7 +248,7 @@ def main():
                                          "p99": err.quantile(0.99).item(),
                                          "max": err.max().item()},
                "calibration": {"scale": scale, "a": best["a"], "b": best["b"],
-                               "candidates": cands}}
+                               "cut": best["cut"], "candidates": cands}}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(summary, f, indent=1)
     log(json.dumps(summary))

@@ -307,12 +307,15 @@ def test_f32_no_sources_field_unchanged(ctx, dev):
     assert torch.equal(t, init)
 
 
-def test_converged_field_skips_rest_of_last_pass(ctx, dev):
+@pytest.mark.parametrize("tile", [-1, "auto"])
+def test_converged_field_skips_rest_of_last_pass(ctx, dev, tile):
     """A field is at the fixed point after its first ordering that changes
     nothing: the device skips the rest of that pass and the pass that
     confirms it. The result equals the one swept ordering by ordering, the
     pass count is the plain count, and the stats count only swept orderings
-    (point_updates = orderings x points)."""
+    (point_updates = orderings x points). With the online tile tuner the
+    first pass sweeps its orderings with six different tiles."""
+    tile = pdfs_amd.TILE_AUTO if tile == "auto" else tile
     X, Y, Z = 72, 40, 56
     init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
     ctx.generate(pdfs_amd.GEN_TERRAIN, 2024, X, Y, Z, 1.0, dense=init)
@@ -332,7 +335,7 @@ def test_converged_field_skips_rest_of_last_pass(ctx, dev):
     first = next(i for i, r in enumerate(runs) if r >= 1) + 1
     assert all(r >= 1 for r in runs[first - 1:])
     f = init.clone()
-    st = ctx.solve_f32(f, init, judge=0)
+    st = ctx.solve_f32(f, init, judge=0, tile=tile)
     assert torch.equal(f.view(torch.int32), ref.view(torch.int32))
     assert st["passes"] == passes
     assert st["orderings"] == first
@@ -351,6 +354,31 @@ def test_batch_with_single_point_field(ctx, dev):
     ctx.solve_batch_f32(ts, [t.clone() for t in ts])
     for t, w in zip(ts, wants):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
+
+
+def test_tuner_skips_tiles_whose_edges_do_not_fit(ctx, dev):
+    """Eleven 512^3 fields: the 16x8 tiles would need more than 2 GiB of edge
+    entries, so the online tuner leaves them out (their orderings still run
+    on the current tile) and the batch solves; results equal per-field
+    solves bit for bit."""
+    n, K = 512, 11
+    inits = []
+    for i in range(K):
+        t = torch.empty((n, n, n), dtype=torch.float32, device=dev)
+        ctx.generate(pdfs_amd.GEN_SPHERES, 900 + i, n, n, n, 1.0, dense=t)
+        inits.append(t)
+    fields = [t.clone() for t in inits]
+    st, passes, _ = ctx.solve_batch_f32(fields, None, tile=pdfs_amd.TILE_AUTO, judge=0)
+    assert st["tile"] not in (pdfs_amd.TILE_16x8, pdfs_amd.TILE_16x8_2CU)
+    for i in (0, K // 2, K - 1):
+        f = inits[i].clone()
+        sp = ctx.solve_f32(f, None, judge=0, tile=pdfs_amd.TILE_32x16)
+        assert sp["passes"] == passes[i]
+        assert torch.equal(f.view(torch.int32), fields[i].view(torch.int32))
+    with pytest.raises(pdfs_amd.PdfsError):
+        ctx.solve_batch_f32([t.clone() for t in inits], None, tile=pdfs_amd.TILE_16x8, judge=0)
+    del inits, fields
+    torch.cuda.empty_cache()
 
 
 def test_largest_f32_field_properties(ctx, dev):

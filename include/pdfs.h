@@ -74,7 +74,7 @@ typedef struct {
   int judge;       /* FP32: run the FP64 residual judge after convergence */
   int tile;        /* column variant: 0 16x16, 1 32x8, 2 16x8, 3 32x16, 4 16x16 one
                       workgroup per CU, 5 16x8 two per CU; -1 default; PDFS_TILE_AUTO tuned
-                      online on the first passes and cached per size class */
+                      online on orderings of the first pass and cached per size class */
   int quiet;       /* no stdout */
   int profile;     /* time every sweep launch with events on the solver stream */
   int graph;       /* replay each pass (8 persistent launches) from a captured hipGraph */

@@ -74,7 +74,12 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
 /* counterpart). Same sort and branch structure as the reference, but the   */
 /* roots are written as u = m0 + t(m1 - m0, m2 - m0, spacing), so FP32 keeps */
 /* the O(h) radicand at any distance magnitude. Isotropic spacing uses      */
-/* constant multiplies; anisotropic spacing the general weighted form.      */
+/* constant multiplies and one square root: with a = m1 - m0, b = m2 - m0,  */
+/* the 2-D root m0 + (a + sqrt(2h^2 - a^2)) / 2 exceeds m2 exactly when     */
+/* 2h^2 - a^2 > (2b - a)^2 (both sides of r > 2b - a are >= 0), so that    */
+/* test picks the 2-D or 3-D radicand before the root is taken, and        */
+/* u = m0 + (P + sqrt(R)) K with (P, K) = (a, 1/2) or (a + b, 1/3).        */
+/* Anisotropic spacing uses the general weighted form.                      */
 /* FP32: a 3-D radicand at or below 2^-96 is flushed to 0 (its root is below */
 /* 2^-48 and cannot move any value >= 2^-24), so the GPU square root never  */
 /* sees a denormal or tiny input. FP64 clamps at 0.                         */
@@ -97,14 +102,17 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
     T u = m0 + h;                                                               \
     if (u > m1) {                                                               \
       T a = m1 - m0;                                                            \
-      T r = SQRT(two_h2 - a * a);                                               \
-      u = m0 + (T)0.5 * (a + r);                                                \
-      if (u > m2) {                                                             \
-        T b = m2 - m0;                                                          \
+      T b = m2 - m0;                                                            \
+      T r2 = two_h2 - a * a;                                                    \
+      T e = b + (b - a);                                                        \
+      T R = r2, P = a, K = (T)0.5;                                              \
+      if (r2 > e * e) {                                                         \
         T D = three_h2 - (T)2 * ((a * a + b * b) - a * b);                      \
-        D = D > (T)FLUSH ? D : (T)0;                                            \
-        u = m0 + ((a + b) + SQRT(D)) * ((T)1 / (T)3);                           \
+        R = D > (T)FLUSH ? D : (T)0;                                            \
+        P = a + b;                                                              \
+        K = (T)1 / (T)3;                                                        \
       }                                                                         \
+      u = m0 + (P + SQRT(R)) * K;                                               \
     }                                                                           \
     return cur < u ? cur : u;                                                   \
   }                                                                             \

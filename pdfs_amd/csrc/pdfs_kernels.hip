@@ -80,7 +80,12 @@ template <> __device__ __forceinline__ float radicand_floor<float>() { return 0x
 template <> __device__ __forceinline__ double radicand_floor<double>() { return 0.0; }
 
 // Shift-invariant difference form u = m0 + t(m1 - m0, m2 - m0, spacing).
-// Isotropic spacing: sorting network + constant multiplies, no division.
+// Isotropic spacing: sorting network + constant multiplies, no division, one
+// square root. With a = m1 - m0 and b = m2 - m0 (b >= a >= 0), the 2-D root
+// m0 + (a + sqrt(2h^2 - a^2)) / 2 exceeds m2 exactly when
+// 2h^2 - a^2 > (2b - a)^2, so the 2-D / 3-D choice is made on the radicands
+// and u = m0 + (P + sqrt(R)) K with (R, P, K) = (2h^2 - a^2, a, 1/2) or
+// (3-D radicand, a + b, 1/3).
 template <typename T>
 __device__ __forceinline__ T god_diff_iso(T cur, T mx, T my, T mz, T h, T two_h2, T three_h2,
                                           T third) {
@@ -94,14 +99,17 @@ __device__ __forceinline__ T god_diff_iso(T cur, T mx, T my, T mz, T h, T two_h2
   T u = m0 + h;
   if (u > m1) {
     T a = m1 - m0;
-    T r = sqrt_t(two_h2 - a * a);
-    u = m0 + (T)0.5 * (a + r);
-    if (u > m2) {
-      T b = m2 - m0;
+    T b = m2 - m0;
+    T r2 = two_h2 - a * a;
+    T e = b + (b - a);
+    T R = r2, P = a, K = (T)0.5;
+    if (r2 > e * e) {  // the 2-D root would exceed m2: 3-D
       T D = three_h2 - (T)2 * ((a * a + b * b) - a * b);
-      D = D > radicand_floor<T>() ? D : (T)0;
-      u = m0 + ((a + b) + sqrt_t(D)) * third;
+      R = D > radicand_floor<T>() ? D : (T)0;
+      P = a + b;
+      K = third;
     }
+    u = m0 + (P + sqrt_t(R)) * K;
   }
   return cur < u ? cur : u;
 }
@@ -127,7 +135,8 @@ __device__ __forceinline__ float sqrt_cr_nonneg(float x) {
 // Inputs are non-negative or +inf (no NaN, no -0): the per-axis minima and
 // the 3-value sort (min3 / med3 / max3) equal the '<' / '>' selects of the
 // oracle. A negative 2-D radicand (a^2 > 2h^2) implies a > h, so u1 <= m1
-// and the 2-D / 3-D candidates (NaN then) are not selected.
+// and the root candidate (NaN then) is not selected; with m2 = +inf the 3-D
+// test is false and its (NaN) radicand is not selected either.
 __device__ __forceinline__ float god_iso32(float c, float xm, float xp, float ym, float yp,
                                            float zm, float zp, float h, float two_h2,
                                            float three_h2, float third) {
@@ -140,14 +149,17 @@ __device__ __forceinline__ float god_iso32(float c, float xm, float xp, float ym
   const float m1 = __builtin_amdgcn_fmed3f(mx, my, mz);
   const float u1 = m0 + h;
   const float a = m1 - m0;
-  const float aa = a * a;
-  const float r = sqrt_cr_nonneg(two_h2 - aa);
-  const float u2 = m0 + 0.5f * (a + r);
   const float b = m2 - m0;
+  const float aa = a * a;
+  const float r2 = two_h2 - aa;
+  const float e = b + (b - a);
   float D = three_h2 - 2.0f * ((aa + b * b) - a * b);
   D = D > 0x1p-96f ? D : 0.0f;
-  const float u3 = m0 + ((a + b) + sqrt_cr_nonneg(D)) * third;
-  const float t = u2 > m2 ? u3 : u2;
+  const bool three = r2 > e * e;
+  const float R = three ? D : r2;
+  const float P = three ? a + b : a;
+  const float K = three ? third : 0.5f;
+  const float t = m0 + (P + sqrt_cr_nonneg(R)) * K;
   const float u = u1 > m1 ? t : u1;
   return c < u ? c : u;
 }
@@ -210,7 +222,7 @@ __device__ __forceinline__ float god32(float c, float xm, float xp, float ym, fl
 // the value is the same; lanes of a wave no longer diverge over the branches.
 // Inputs are non-negative or +inf (no NaN): the minima / sort equal the '<'
 // / '>' selects, and a negative 2-D radicand (NaN root) implies u1 <= m1, so
-// the NaN candidate is never selected.
+// the NaN candidate is never selected; with m2 = +inf the 3-D test is false.
 __device__ __forceinline__ double god_iso64_sel(double cur, double mx, double my, double mz,
                                                 double h, double two_h2, double three_h2,
                                                 double third) {
@@ -221,13 +233,16 @@ __device__ __forceinline__ double god_iso64_sel(double cur, double mx, double my
   const double m1 = __builtin_fmin(hi, mid), m2 = __builtin_fmax(hi, mid);
   const double u1 = m0 + h;
   const double a = m1 - m0;
-  const double r = __builtin_sqrt(two_h2 - a * a);
-  const double u2 = m0 + 0.5 * (a + r);
   const double b = m2 - m0;
+  const double r2 = two_h2 - a * a;
+  const double e = b + (b - a);
   double D = three_h2 - 2.0 * ((a * a + b * b) - a * b);
   D = D > 0.0 ? D : 0.0;
-  const double u3 = m0 + ((a + b) + __builtin_sqrt(D)) * third;
-  const double t = u2 > m2 ? u3 : u2;
+  const bool three = r2 > e * e;
+  const double R = three ? D : r2;
+  const double P = three ? a + b : a;
+  const double K = three ? third : 0.5;
+  const double t = m0 + (P + __builtin_sqrt(R)) * K;
   const double u = u1 > m1 ? t : u1;
   return __builtin_fmin(cur, u);
 }

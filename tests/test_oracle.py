@@ -126,6 +126,45 @@ def test_godunov_general_roots_vs_numpy(fn):
         assert abs(got - want) <= tol, (m, d, got, want)
 
 
+@pytest.mark.parametrize("fn", ["d64", "f32"])
+def test_godunov_iso_2d_3d_boundary(fn):
+    """The isotropic difference form picks the 3-D root when
+    2h^2 - a^2 > (2b - a)^2 (a = m1 - m0, b = m2 - m0), i.e. when the 2-D
+    root exceeds m2, and takes one square root. Checked where the choice is
+    closest: m2 just below, at and just above the 2-D root, against the
+    independent root finder; a wrong test (e.g. 2b + a, or b instead of
+    2b - a) picks the other root there and fails."""
+    g = {"d64": O.lib().orc_godunov_d64, "f32": O.lib().orc_godunov_f32}[fn]
+    rng = np.random.default_rng(17)
+    for it in range(2000):
+        h = float(rng.choice([1.0, 0.5, 1.5]))
+        m0 = float(rng.uniform(0, 200))
+        a = float(rng.uniform(0, 0.99 * h))
+        u2 = m0 + 0.5 * (a + math.sqrt(2 * h * h - a * a))
+        for rel in (-1e-3, -1e-6, 0.0, 1e-6, 1e-3, 0.05):
+            m2 = u2 + rel * h
+            if m2 < m0 + a:
+                continue
+            m = [m0, m0 + a, m2]
+            rng.shuffle(m)
+            want = godunov_brute(m[0], m[1], m[2], h, h, h)
+            got = g(1e9, m[0], m[1], m[2], h, h, h)
+            tol = (1e-9 if fn == "d64" else 4e-5) * max(1.0, m0)
+            assert abs(got - want) <= tol, (m, h, got, want)
+    # away from the boundary the roots differ by far more than the
+    # tolerance: for a = 0.6 h the 2-D root is m0 + 0.94 h; with
+    # m2 = m0 + 1.4 h it is the answer, with m2 = m0 + 0.75 h the 3-D root
+    # (lower, and above m2) is
+    for h in (1.0, 0.5):
+        m0, a = 10.0, 0.6 * h
+        u2 = m0 + 0.5 * (a + math.sqrt(2 * h * h - a * a))
+        got2 = g(1e9, m0, m0 + a, m0 + 1.4 * h, h, h, h)
+        assert abs(got2 - u2) <= 1e-5
+        got3 = g(1e9, m0, m0 + a, m0 + 0.75 * h, h, h, h)
+        assert abs(got3 - godunov_brute(m0, m0 + a, m0 + 0.75 * h, h, h, h)) <= 1e-5
+        assert m0 + 0.75 * h < got3 < u2 - 1e-3 * h
+
+
 def test_godunov_invariants():
     L = O.lib()
     rng = np.random.default_rng(9)

@@ -142,6 +142,13 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
                        double dx, double dy, double dz, uint32_t *flags,
                        long long *flagged);
 
+/* Self-test (no counterpart in the reference): the FP32 sweep's square root
+ * (hardware estimate plus a one-ulp correction) against the correctly
+ * rounded root (FP64 root rounded to FP32) on every float in [2^-96, +inf),
+ * the radicands the sweep can take; *mismatches = 0 when they agree
+ * everywhere. */
+int pdfs_selftest_sqrt_f32(pdfs_ctx *c, unsigned long long *mismatches);
+
 /* Synthetic initial fields; dense FP32 and/or padded FP64 outputs (NULL to
  * skip). kind: 0 point source, 1 terrain, 2 spheres. */
 enum { PDFS_GEN_POINT = 0, PDFS_GEN_TERRAIN = 1, PDFS_GEN_SPHERES = 2 };

@@ -27,7 +27,8 @@ TILES = (TILE_16x16, TILE_32x8, TILE_16x8, TILE_32x16, TILE_16x16_1CU, TILE_16x8
 EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
     "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
-    "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_generate_dev", "pdfs_generate_ex_dev",
+    "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_selftest_sqrt_f32", "pdfs_generate_dev",
+    "pdfs_generate_ex_dev",
     "pdfs_pack_dev",
     "pdfs_unpack_dev", "pdfs_solve_batch_f32_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
     "pdfs_vti_write", "pdfs_vti_free", "pdfs_ctx_stream", "pdfs_net_create", "pdfs_net_destroy",
@@ -116,6 +117,8 @@ def lib():
         L.pdfs_solve_ref64_dev.restype = I
         L.pdfs_judge_f32_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, vp, C.POINTER(LL)]
         L.pdfs_judge_f32_dev.restype = I
+        L.pdfs_selftest_sqrt_f32.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+        L.pdfs_selftest_sqrt_f32.restype = I
         L.pdfs_generate_dev.argtypes = [vp, I, C.c_uint64, I, I, I, D, vp, vp]
         L.pdfs_generate_dev.restype = I
         L.pdfs_generate_ex_dev.argtypes = [vp, I, C.c_uint64, I, I, I, D, I, D, D, D, D, vp, vp]
@@ -336,6 +339,13 @@ class Context:
         _check(lib().pdfs_solve_ref64_dev(self.c, self._ptr(padded), X, Y, Z, *h, passes, tol,
                                           C.byref(st)))
         return st.as_dict()
+
+    def selftest_sqrt_f32(self):
+        """Floats in [2^-96, inf) where the sweep's FP32 square root differs
+        from the correctly rounded one (0 expected)."""
+        n = C.c_ulonglong(0)
+        _check(lib().pdfs_selftest_sqrt_f32(self.c, C.byref(n)))
+        return n.value
 
     def judge_f32(self, field, init, h=(1.0, 1.0, 1.0), flags=None):
         import torch

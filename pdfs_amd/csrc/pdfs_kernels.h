@@ -117,6 +117,10 @@ int pdfsk_pack_f64(const double *padded, float *dense, int X, int Y, int Z,
 int pdfsk_unpack_f64(const float *dense, double *padded, int X, int Y, int Z,
                      hipStream_t st);
 int pdfsk_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+/* count += floats with bit patterns in [lo, hi) where the sweep's FP32
+ * square root differs from the correctly rounded one (the FP64 root rounded
+ * to FP32); count zeroed by the caller */
+int pdfsk_check_sqrt_cr(uint32_t lo, uint32_t hi, unsigned long long *count, hipStream_t st);
 
 /* Synthetic initial fields, dense FP32 and/or padded FP64 (either may be
  * NULL). params: see pdfs_gen.c for the packing. */

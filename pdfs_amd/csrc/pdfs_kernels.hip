@@ -1210,6 +1210,29 @@ extern "C" int pdfsk_unpack_f64(const float *dense, double *padded, int X, int Y
   return hip_rc(hipGetLastError());
 }
 
+// Every float bit pattern in [lo, hi): count where the sweep's square root
+// (sqrt_cr_nonneg) differs from the FP64 square root rounded to FP32. That
+// is the correctly rounded FP32 root (double carries more than 2 * 24 + 2
+// bits, so rounding the double root again is innocuous for sqrt), computed
+// by a different method (the FP64 square root sequence), so the check does
+// not share the FP32 expansion's arithmetic.
+__global__ __launch_bounds__(256) void k_check_sqrt_cr(uint32_t lo, uint32_t n,
+                                                       unsigned long long *count) {
+  unsigned long long bad = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float x = __uint_as_float(lo + i);
+    bad += __float_as_uint(sqrt_cr_nonneg(x)) != __float_as_uint((float)__builtin_sqrt((double)x));
+  }
+  if (bad) atomicAdd(count, bad);
+}
+
+extern "C" int pdfsk_check_sqrt_cr(uint32_t lo, uint32_t hi, unsigned long long *count,
+                                   hipStream_t st) {
+  if (hi <= lo) return -1;
+  hipLaunchKernelGGL(k_check_sqrt_cr, dim3(256 * 64), dim3(256), 0, st, lo, hi - lo, count);
+  return hip_rc(hipGetLastError());
+}
+
 extern "C" int pdfsk_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((long)n, 256)), dim3(256), 0, st, p, v, n);
   return hip_rc(hipGetLastError());

@@ -570,6 +570,21 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
   return PDFS_OK;
 }
 
+int pdfs_selftest_sqrt_f32(pdfs_ctx *c, unsigned long long *mismatches)
+{
+  if (!c || !mismatches) return fail(PDFS_EINVAL, "pdfs_selftest_sqrt_f32: bad arguments");
+  unsigned long long *d = NULL;
+  HIPCHK(hipMalloc((void **)&d, sizeof *d));
+  int rc = PDFS_OK;
+  if (hipMemsetAsync(d, 0, sizeof *d, c->st) != hipSuccess ||
+      pdfsk_check_sqrt_cr(0x0F800000u, 0x7F800000u, d, c->st) ||
+      hipMemcpyAsync(mismatches, d, sizeof *d, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    rc = fail(PDFS_EHIP, "pdfs_selftest_sqrt_f32: launch failed");
+  hipFree(d);
+  return rc;
+}
+
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags, long long *flagged)
 {

@@ -307,6 +307,16 @@ def test_f32_no_sources_field_unchanged(ctx, dev):
     assert torch.equal(t, init)
 
 
+def test_sweep_sqrt_is_correctly_rounded_everywhere(ctx):
+    """The FP32 sweep's square root (v_sqrt_f32, which alone misses the
+    correctly rounded result on about 15% of inputs, plus a one-ulp
+    correction from two FMA residuals) equals the correctly rounded root (the
+    FP64 root rounded to FP32) on every one of the 1,879,048,192 floats in
+    [2^-96, +inf), so the sweep's roots are the CPU oracle's IEEE sqrtf
+    roots bit for bit."""
+    assert ctx.selftest_sqrt_f32() == 0
+
+
 @pytest.mark.parametrize("tile", [-1, "auto"])
 def test_converged_field_skips_rest_of_last_pass(ctx, dev, tile):
     """A field is at the fixed point after its first ordering that changes

@@ -79,7 +79,8 @@ typedef struct {
   int profile;     /* time every sweep launch with events on the solver stream */
   int graph;       /* replay each pass (8 persistent launches) from a captured hipGraph */
   int certify;     /* FP32: also stop a field once the FP64 residual judge passes on it */
-  int certify_from;/* first pass after which the judge is run (default 1) */
+  int certify_from;/* first pass after which the judge is run (default 1); from then on it
+                      also runs after every half pass (orderings 4 and 8) */
 } pdfs_opts;
 
 typedef struct {

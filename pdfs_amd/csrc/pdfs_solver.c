@@ -26,10 +26,10 @@ struct pdfs_ctx {
   unsigned *h_changed, *prev_changed; /* pinned readback, last values */
   unsigned *h_streak;              /* per field: counter, run of unchanged orderings */
   unsigned *h_err;
-  hipGraphExec_t gexec;            /* captured pass (8 launches) */
+  hipGraphExec_t gexec[3];         /* captured pass (orderings 1-8), halves 1-4 and 5-8 */
   long long tune_keys[32];         /* tile autotuner cache: size class -> tile */
   int tune_tiles[32], ntune;
-  int g_tile, g_iso, g_grid, g_sig[8];
+  int g_tile[3], g_iso[3], g_grid[3];
   /* FP64 path / judge */
   unsigned long long *d_u64; /* [0] maxdiff bits */
   unsigned long long *h_pin64;
@@ -149,7 +149,8 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
 {
   if (!c) return;
   hipStreamSynchronize(c->st);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  for (int g = 0; g < 3; g++)
+    if (c->gexec[g]) hipGraphExecDestroy(c->gexec[g]);
   if (c->bt.edge_x) hipFree(c->bt.edge_x); /* edge_y lies in the same allocation */
   hipFree(c->bt.fields);
   hipFree(c->bt.cols);
@@ -407,10 +408,11 @@ static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *d
     memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
   }
   c->bt.nfields = nf;
-  if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
-    c->gexec = NULL;
-  }
+  for (int g = 0; g < 3; g++)
+    if (c->gexec[g]) {
+      hipGraphExecDestroy(c->gexec[g]);
+      c->gexec[g] = NULL;
+    }
   return plan_upload_cols(c, NULL);
 }
 
@@ -426,35 +428,37 @@ static int launch_grid(pdfs_ctx *c, int iso)
   return grid > c->ncols ? (c->ncols > 0 ? c->ncols : 1) : grid;
 }
 
-/* One pass (orderings sn0..sn1) of the current plan, optionally profiled
- * per launch or replayed from a captured graph. */
+/* Orderings sn0..sn1 of the current plan, optionally profiled per launch
+ * or replayed from a captured graph (a whole pass, or one of its halves). */
 static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int profile,
                          int graph, double *kernel_ms)
 {
   int grid = launch_grid(c, sp->iso);
   if (grid <= 0) return fail(PDFS_EHIP, "column kernel cannot be resident");
-  if (graph && sn0 == 1 && sn1 == 8 && !profile) {
-    if (!c->gexec || c->g_tile != c->tile || c->g_iso != sp->iso || c->g_grid != grid) {
-      if (c->gexec) {
-        hipGraphExecDestroy(c->gexec);
-        c->gexec = NULL;
+  const int gi = sn0 == 1 && sn1 == 8 ? 0 : sn0 == 1 && sn1 == 4 ? 1 : sn0 == 5 && sn1 == 8 ? 2 : -1;
+  if (graph && gi >= 0 && !profile) {
+    if (!c->gexec[gi] || c->g_tile[gi] != c->tile || c->g_iso[gi] != sp->iso ||
+        c->g_grid[gi] != grid) {
+      if (c->gexec[gi]) {
+        hipGraphExecDestroy(c->gexec[gi]);
+        c->gexec[gi] = NULL;
       }
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-      for (int sn = 1; sn <= 8; sn++)
+      for (int sn = sn0; sn <= sn1; sn++)
         if (pdfsk_sweep_batch_f32(&c->bt, sn, sp, c->tile, grid, c->st)) {
           hipStreamEndCapture(c->st, &g);
           return fail(PDFS_EHIP, "launch during capture failed");
         }
       HIPCHK(hipStreamEndCapture(c->st, &g));
-      hipError_t e = hipGraphInstantiate(&c->gexec, g, NULL, NULL, 0);
+      hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, NULL, NULL, 0);
       hipGraphDestroy(g);
       if (e != hipSuccess) return fail(PDFS_EHIP, "graph instantiate: %s", hipGetErrorString(e));
-      c->g_tile = c->tile;
-      c->g_iso = sp->iso;
-      c->g_grid = grid;
+      c->g_tile[gi] = c->tile;
+      c->g_iso[gi] = sp->iso;
+      c->g_grid[gi] = grid;
     }
-    HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    HIPCHK(hipGraphLaunch(c->gexec[gi], c->st));
     return PDFS_OK;
   }
   for (int sn = sn0; sn <= sn1; sn++) {
@@ -636,6 +640,10 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   int rc;
   if ((rc = plan_build(c, nf, fields, dims, tile))) return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
+  /* per-field change count of the current pass (summed over its segments),
+   * and the points the field's last judge flagged */
+  long long *pdel = (long long *)calloc((size_t)nf, sizeof(long long));
+  long long *lastfl = (long long *)malloc((size_t)nf * sizeof(long long));
   int *active = (int *)malloc((size_t)nf * sizeof(int));
   /* fields whose passes ended on a passing judge: the final judge skips them */
   char *certified = (char *)calloc((size_t)nf, 1);
@@ -646,8 +654,11 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   const float **jv = (const float **)malloc((size_t)nf * sizeof(float *));
   const float **ji = (const float **)malloc((size_t)nf * sizeof(float *));
   long long *jfl = (long long *)malloc((size_t)nf * sizeof(long long));
-  if (!delta || !active || !certified || !jidx || !jdone || !jd || !jv || !ji || !jfl) {
+  if (!delta || !pdel || !lastfl || !active || !certified || !jidx || !jdone || !jd || !jv ||
+      !ji || !jfl) {
     free(delta);
+    free(pdel);
+    free(lastfl);
     free(active);
     free(certified);
     free(jidx);
@@ -660,6 +671,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   }
   for (int f = 0; f < nf; f++) {
     active[f] = 1;
+    lastfl[f] = -1;  /* not judged yet */
     if (field_passes) field_passes[f] = 0;
     if (field_ms) field_ms[f] = 0.0;
   }
@@ -669,118 +681,139 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   long long last_total = 0;
   rc = PDFS_OK;
   HIPGO(hipEventRecord(c->ev0, c->st));
+  /* A certified solve judges each field at the end of every pass from
+   * certify_from on, and also after the first half of a pass once its last
+   * judge flagged at most 1/1024 of its points: such a field often passes a
+   * few orderings before the pass ends. The plain stop stays a whole pass
+   * without change. */
+  const int certifying = o->certify && inits && o->fixed_passes <= 0;
+  const int cfrom = o->certify_from > 0 ? o->certify_from : 1;
   for (int p = 0; p < maxp && nactive > 0; p++) {
-    long long npts = 0;
-    for (int f = 0; f < nf; f++)
-      if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
-    if (tuning) {
-      /* ordering 1 untimed, candidate v on ordering v + 2, the best on 8 */
-      double cand_ms[PDFSK_TILE_COUNT], tk = 0.0;
-      if ((rc = run_orderings(c, &sp, 1, 1, o->profile, 0, &kms))) break;
-      for (int v = 0; v < PDFSK_TILE_COUNT && !rc; v++) {
-        cand_ms[v] = 1e30;
-        if (!plan_fits(nf, dims, v)) {
-          /* its ordering still runs, on the current tile */
-          rc = run_orderings(c, &sp, v + 2, v + 2, o->profile, 0, &kms);
-          continue;
+    int halves = 0;
+    for (int f = 0; f < nf && certifying && !tuning && p >= cfrom && !halves; f++)
+      halves = active[f] && inits[f] && lastfl[f] >= 0 &&
+               lastfl[f] * 1024 <= (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+    const int nseg = halves ? 2 : 1;
+    for (int f = 0; f < nf; f++) pdel[f] = 0;
+    for (int g = 0; g < nseg && nactive > 0 && rc == PDFS_OK; g++) {
+      const int sn0 = halves ? 1 + 4 * g : 1, sn1 = halves ? 4 + 4 * g : 8;
+      const int nord = sn1 - sn0 + 1, last_seg = g == nseg - 1;
+      long long npts = 0;
+      for (int f = 0; f < nf; f++)
+        if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+      if (tuning) {
+        /* ordering 1 untimed, candidate v on ordering v + 2, the best on 8 */
+        double cand_ms[PDFSK_TILE_COUNT], tk = 0.0;
+        if ((rc = run_orderings(c, &sp, 1, 1, o->profile, 0, &kms))) break;
+        for (int v = 0; v < PDFSK_TILE_COUNT && !rc; v++) {
+          cand_ms[v] = 1e30;
+          if (!plan_fits(nf, dims, v)) {
+            /* its ordering still runs, on the current tile */
+            rc = run_orderings(c, &sp, v + 2, v + 2, o->profile, 0, &kms);
+            continue;
+          }
+          if (c->tile != v && (rc = plan_build_ex(c, nf, fields, dims, v, 1))) break;
+          tk = 0.0;
+          rc = run_orderings(c, &sp, v + 2, v + 2, 1, 0, &tk);
+          cand_ms[v] = tk;
+          if (o->profile) kms += tk;
         }
-        if (c->tile != v && (rc = plan_build_ex(c, nf, fields, dims, v, 1))) break;
-        tk = 0.0;
-        rc = run_orderings(c, &sp, v + 2, v + 2, 1, 0, &tk);
-        cand_ms[v] = tk;
-        if (o->profile) kms += tk;
+        if (rc) break;
+        int best = 0;
+        for (int v = 1; v < PDFSK_TILE_COUNT; v++)
+          if (cand_ms[v] < cand_ms[best]) best = v;
+        tuning = 0;
+        tune_store(c, key, best);
+        tile = best;
+        if (c->tile != best && (rc = plan_build_ex(c, nf, fields, dims, best, 1))) break;
+        if ((rc = run_orderings(c, &sp, PDFSK_TILE_COUNT + 2, 8, o->profile, 0, &kms))) break;
+      } else if ((rc = run_orderings(c, &sp, sn0, sn1, o->profile, o->graph, &kms))) {
+        break;
       }
-      if (rc) break;
-      int best = 0;
-      for (int v = 1; v < PDFSK_TILE_COUNT; v++)
-        if (cand_ms[v] < cand_ms[best]) best = v;
-      tuning = 0;
-      tune_store(c, key, best);
-      tile = best;
-      if (c->tile != best && (rc = plan_build_ex(c, nf, fields, dims, best, 1))) break;
-      if ((rc = run_orderings(c, &sp, PDFSK_TILE_COUNT + 2, 8, o->profile, 0, &kms))) break;
-    } else if ((rc = run_orderings(c, &sp, 1, 8, o->profile, o->graph, &kms))) {
-      break;
-    }
-    HIPGO(hipEventRecord(c->ev1, c->st));
-    unsigned err = 0;
-    if ((rc = read_changed(c, delta, &err))) break;
-    float t = 0.f;
-    hipEventElapsedTime(&t, c->ev0, c->ev1);
-    ms = t;
-    s.passes++;
-    s.orderings += 8;
-    s.point_updates += 8 * npts;
-    if (err) {
-      rc = fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
-      break;
-    }
-    last_total = 0;
-    int changed_set = 0;
-    /* certified stop: the FP64 residual judge passes on a field from pass
-     * certify_from on (needs its init); plain stop: a pass changed nothing */
-    const int judge_now = o->certify && inits && o->fixed_passes <= 0 &&
-                          p + 1 >= (o->certify_from > 0 ? o->certify_from : 1);
-    float tj0 = 0.f;
-    if (judge_now) {
       HIPGO(hipEventRecord(c->ev1, c->st));
-      HIPGO(hipEventSynchronize(c->ev1));
-      hipEventElapsedTime(&tj0, c->ev0, c->ev1);
-    }
-    int nj = 0;
-    for (int f = 0; f < nf; f++) {
-      jdone[f] = 0;
-      if (!active[f]) continue;
-      last_total += delta[f];
-      if (field_passes) field_passes[f] = p + 1;
-      jdone[f] = o->fixed_passes <= 0 && delta[f] == 0;
-      {
-        /* the device skipped the orderings of this pass that followed the
-         * field's PDFSK_CONVERGED_RUN-th consecutive unchanged one */
-        const unsigned run = c->h_streak[2 * f + 1];
-        long long skipped = run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
-        if (skipped > 8) skipped = 8;
-        s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
-        if (nf == 1) s.orderings -= (int)skipped;
+      unsigned err = 0;
+      if ((rc = read_changed(c, delta, &err))) break;
+      float t = 0.f;
+      hipEventElapsedTime(&t, c->ev0, c->ev1);
+      ms = t;
+      if (g == 0) s.passes++;
+      s.orderings += nord;
+      s.point_updates += nord * npts;
+      if (err) {
+        rc = fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+        break;
       }
-      if (!jdone[f] && judge_now && inits[f]) {
-        jidx[nj] = f;
-        jv[nj] = fields[f];
-        ji[nj] = inits[f];
-        memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
-        nj++;
+      last_total = 0;
+      int changed_set = 0;
+      /* certified stop: the FP64 residual judge passes on a field (needs its
+       * init); plain stop: a pass changed nothing */
+      const int judge_now = certifying && (last_seg ? p + 1 >= cfrom : 1);
+      float tj0 = 0.f;
+      if (judge_now) {
+        HIPGO(hipEventRecord(c->ev1, c->st));
+        HIPGO(hipEventSynchronize(c->ev1));
+        hipEventElapsedTime(&tj0, c->ev0, c->ev1);
       }
-    }
-    /* all fields to certify in one judge launch */
-    if (nj > 0) {
-      if ((rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl))) break;
-      s.judge_runs += nj;
-      for (int k = 0; k < nj; k++) {
-        jdone[jidx[k]] = jfl[k] == 0;
-        certified[jidx[k]] = (char)(jfl[k] == 0);
+      int nj = 0;
+      for (int f = 0; f < nf; f++) {
+        jdone[f] = 0;
+        if (!active[f]) continue;
+        pdel[f] += delta[f];
+        last_total += pdel[f];
+        if (field_passes) field_passes[f] = p + 1;
+        jdone[f] = last_seg && o->fixed_passes <= 0 && pdel[f] == 0;
+        {
+          /* the device skipped the orderings of this segment that followed
+           * the field's PDFSK_CONVERGED_RUN-th consecutive unchanged one */
+          const unsigned run = c->h_streak[2 * f + 1];
+          long long skipped =
+              run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
+          if (skipped > nord) skipped = nord;
+          s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+          if (nf == 1) s.orderings -= (int)skipped;
+        }
+        const int near = last_seg || (lastfl[f] >= 0 && lastfl[f] * 1024 <=
+                                                   (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2]);
+        if (!jdone[f] && judge_now && inits[f] && near) {
+          jidx[nj] = f;
+          jv[nj] = fields[f];
+          ji[nj] = inits[f];
+          memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
+          nj++;
+        }
       }
-    }
-    for (int f = 0; f < nf; f++) {
-      if (!active[f]) continue;
-      if (jdone[f]) {
-        active[f] = 0;
-        nactive--;
-        changed_set = 1;
-        if (field_ms) field_ms[f] = ms;
-      } else if (field_ms) {
-        field_ms[f] = ms;
+      /* all fields to certify in one judge launch */
+      if (nj > 0) {
+        if ((rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl))) break;
+        s.judge_runs += nj;
+        for (int k = 0; k < nj; k++) {
+          jdone[jidx[k]] = jfl[k] == 0;
+          certified[jidx[k]] = (char)(jfl[k] == 0);
+          lastfl[jidx[k]] = jfl[k];
+        }
       }
+      for (int f = 0; f < nf; f++) {
+        if (!active[f]) continue;
+        if (jdone[f]) {
+          active[f] = 0;
+          nactive--;
+          changed_set = 1;
+          if (field_ms) field_ms[f] = ms;
+        } else if (field_ms) {
+          field_ms[f] = ms;
+        }
+      }
+      if (judge_now) {
+        HIPGO(hipEventRecord(c->ev1, c->st));
+        HIPGO(hipEventSynchronize(c->ev1));
+        float tj1 = 0.f;
+        hipEventElapsedTime(&tj1, c->ev0, c->ev1);
+        s.ms_judge += tj1 - tj0;
+        ms = tj1;
+      }
+      if (changed_set && nactive > 0 && (rc = plan_upload_cols(c, active))) break;
     }
     if (rc) break;
-    if (judge_now) {
-      HIPGO(hipEventRecord(c->ev1, c->st));
-      HIPGO(hipEventSynchronize(c->ev1));
-      float tj1 = 0.f;
-      hipEventElapsedTime(&tj1, c->ev0, c->ev1);
-      s.ms_judge += tj1 - tj0;
-      ms = tj1;
-    }
-    if (changed_set && nactive > 0 && (rc = plan_upload_cols(c, active))) break;
   }
   s.tile = c->tile;
   s.last_changed = last_total;
@@ -815,6 +848,8 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, s.passes);
 out:
   free(delta);
+  free(pdel);
+  free(lastfl);
   free(active);
   free(certified);
   free(jidx);

@@ -307,6 +307,43 @@ def test_f32_no_sources_field_unchanged(ctx, dev):
     assert torch.equal(t, init)
 
 
+@pytest.mark.parametrize("gen", ["spheres512", "terrain"])
+def test_certified_stop_at_half_passes(ctx, dev, gen):
+    """The certified solve judges at the end of every pass and, once a
+    field's last judge flagged at most 1/1024 of its points, also after the
+    first half of a pass; it stops at the first judge that passes: same bits
+    and ordering count as sweeping ordering by ordering and judging at those
+    points. The 512^3 sphere field stops after the first half of its third
+    pass (ordering 20)."""
+    X, Y, Z = (512, 512, 512) if gen == "spheres512" else (160, 128, 96)
+    init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+    if gen == "spheres512":
+        ctx.generate(pdfs_amd.GEN_SPHERES, 2024, X, Y, Z, 1.0, dense=init)
+    else:
+        ctx.generate_ex(pdfs_amd.GEN_TERRAIN, 4243, X, Y, Z, 1.0, 8, (5.0, 40.0), (10.0, 60.0),
+                        dense=init)
+    ref = init.clone()
+    stop, last, mid = None, None, 0
+    for k in range(1, 97):
+        ctx.sweep_f32(ref, (k - 1) % 8 + 1)
+        if k % 8 == 0 or (k % 8 == 4 and k > 8 and last * 1024 <= X * Y * Z):
+            mid += k % 8 == 4
+            last = ctx.judge_f32(ref, init)
+            if last == 0:
+                stop = k
+                break
+    assert stop is not None
+    f = init.clone()
+    st = ctx.solve_f32(f, init, certify=1)
+    print(f"{gen}: certified at ordering {stop} ({mid} mid-pass judges), passes {st['passes']}")
+    assert st["orderings"] == stop and st["passes"] == (stop + 7) // 8
+    assert torch.equal(f.view(torch.int32), ref.view(torch.int32))
+    if gen == "spheres512":
+        assert stop % 8 == 4 and mid >= 1
+        del f, ref, init
+        torch.cuda.empty_cache()
+
+
 def test_sweep_sqrt_is_correctly_rounded_everywhere(ctx):
     """The FP32 sweep's square root (v_sqrt_f32, which alone misses the
     correctly rounded result on about 15% of inputs, plus a one-ulp

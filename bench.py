@@ -146,10 +146,14 @@ def main():
             f"rms {extra['f32_vs_f64_rms_err']:.3e}")
         del padded, inner, err
 
+    if not args.no_oracle:
+        extra["parity"] = parity_leg(ctx, args)
     if args.batch512 > 0:
         extra["batch_512"] = same_size_batch_leg(ctx, args, init, net, fields_per_s)
     if args.batch > 0:
         extra["batch"] = batch_leg(ctx, args.batch, net)
+        if not args.no_cpu:
+            extra["batch"]["cpu_reference"] = cpu_batch_baseline(ctx, args.batch, extra["batch"])
     if net is not None:
         net.close()
 
@@ -268,6 +272,59 @@ def solve_modes(ctx, net, init, field, tile, plain_ms):
                        "fallbacks": sw["fallbacks"], "judge_flagged": int(sw["judge_flagged"]),
                        "speedup_vs_plain": round(plain_ms / wms, 3)}
     log(f"[bench] modes {tuple(init.shape)}: {out}")
+    return out
+
+
+def parity_leg(ctx, args):
+    """GPU against the reference on the small configurations: c0 (64^3
+    point source, seed 12345, FP64 to max |delta| <= 1e-12) on the GPU's
+    reference-arithmetic path and on the reference core (oracle/_ref, when
+    present): passes, time, SHA-256 of the padded field, bitwise equality;
+    and the converged FP32 field's max / RMS error against the converged FP64
+    field on c0 and on the c1 terrain (256x256x128, 8 hills)."""
+    import hashlib
+
+    import numpy as np
+    import torch
+
+    import pdfs_amd
+    out = {}
+
+    def f32_err(kind, seed, X, Y, Z, nobj=0, p=(0.0, 0.0), q=(0.0, 0.0)):
+        pad = torch.empty((Z + 2, Y + 2, X + 2), dtype=torch.float64, device="cuda:0")
+        dense = torch.empty((Z, Y, X), dtype=torch.float32, device="cuda:0")
+        ctx.generate_ex(kind, seed, X, Y, Z, 1.0, nobj, p, q, dense=dense, padded=pad)
+        t = time.perf_counter()
+        r64 = ctx.solve_ref64(pad, passes=0, tol=1e-12)
+        t64 = time.perf_counter() - t
+        s32 = ctx.solve_f32(dense, None, judge=0)
+        err = (dense.double() - pad[1:-1, 1:-1, 1:-1]).abs()
+        return pad, r64, t64, s32, float(err.max()), float(err.pow(2).mean().sqrt())
+
+    X = Y = Z = 64
+    pad, r64, t64, s32, emax, erms = f32_err(pdfs_amd.GEN_POINT, 12345, X, Y, Z)
+    host = pad.cpu().numpy()
+    c0 = {"grid": [X, Y, Z], "seed": 12345, "gpu_f64_passes": r64["passes"],
+          "gpu_f64_seconds": round(t64, 4), "sha256": hashlib.sha256(host.tobytes()).hexdigest(),
+          "f32_passes": s32["passes"], "f32_vs_f64_max_err": emax, "f32_vs_f64_rms_err": erms}
+    so = os.path.join(ROOT, "oracle", "_ref", "libpdfs_ref.so")
+    if os.path.exists(so):
+        ref = torch.empty_like(pad)
+        ctx.generate(pdfs_amd.GEN_POINT, 12345, X, Y, Z, 1.0, padded=ref)
+        a = ref.cpu().numpy()
+        passes, sec = _ref_solve_worker((so, X, Y, Z, a))
+        c0.update({"cpu_reference_passes": passes, "cpu_reference_seconds": round(sec, 4),
+                   "cpu_reference_sha256": hashlib.sha256(a.tobytes()).hexdigest(),
+                   "bitwise_equal": bool(np.array_equal(a.view(np.uint64), host.view(np.uint64)))})
+    out["c0"] = c0
+    X, Y, Z = 256, 256, 128
+    _, r64, t64, s32, emax, erms = f32_err(pdfs_amd.GEN_TERRAIN, 777, X, Y, Z, 8, (5.0, 40.0),
+                                           (10.0, 60.0))
+    out["c1"] = {"grid": [X, Y, Z], "seed": 777, "gpu_f64_passes": r64["passes"],
+                 "gpu_f64_seconds": round(t64, 3), "f32_passes": s32["passes"],
+                 "f32_vs_f64_max_err": emax, "f32_vs_f64_rms_err": erms}
+    log(f"[bench] parity: {out}")
+    torch.cuda.empty_cache()
     return out
 
 
@@ -398,6 +455,62 @@ def batch_leg(ctx, nfields, net=None):
     log(f"[bench] c4 batched {tb * 1e3:.1f} ms ({out['fields_per_s']} fields/s, p50 "
         f"{out['p50_ms']} p99 {out['p99_ms']} ms) vs per-field {ts * 1e3:.1f} ms; equal={same}")
     return out
+
+
+def _ref_solve_worker(job):
+    """One field of the c4 CPU sample in a worker process: the reference
+    core (oracle/_ref) to a pass with max |delta| <= 1e-12."""
+    import ctypes as C
+
+    import numpy as np
+    so, X, Y, Z, dist = job
+    R = C.CDLL(so)
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    R.ref_solve.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                            C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    R.ref_solve.restype = C.c_int
+    sec, md = C.c_double(0), C.c_double(0)
+    passes = R.ref_solve(dist, X, Y, Z, 1.0, 1.0, 1.0, 1e-12, 100, C.byref(sec), C.byref(md))
+    return passes, sec.value
+
+
+def cpu_batch_baseline(ctx, nfields, gpu, sample=48):
+    """The reference FSM core (oracle/_ref) on a sample of the c4 batch: the
+    first `sample` fields, each converged (max |delta| <= 1e-12) in its own
+    worker process, one process per host core (at most 16, the box's CPU
+    share for one GPU). fields/s = sample / wall time of the pool."""
+    import multiprocessing as mp
+
+    import torch
+
+    import pdfs_amd
+    from pdfs_amd.workloads import c4_specs
+
+    so = os.path.join(ROOT, "oracle", "_ref", "libpdfs_ref.so")
+    if not os.path.exists(so):
+        return None
+    specs = c4_specs(seed=4, nfields=nfields)[:sample]
+    jobs = []
+    for kind, (X, Y, Z), fseed, nobj, p, q in specs:
+        t = torch.empty((Z + 2, Y + 2, X + 2), dtype=torch.float64, device="cuda:0")
+        ctx.generate_ex(kind, fseed, X, Y, Z, 1.0, nobj, p, q, padded=t)
+        jobs.append((so, X, Y, Z, t.cpu().numpy()))
+        del t
+    procs = max(1, min(16, os.cpu_count() or 1, len(jobs)))
+    log(f"[bench] CPU reference core on {len(jobs)} c4 fields, {procs} processes ...")
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_ref_solve_worker, jobs, chunksize=1)
+    wall = time.perf_counter() - t0
+    fps = len(jobs) / wall
+    log(f"[bench] CPU c4 sample: {wall:.1f} s -> {fps:.2f} fields/s")
+    return {"value": round(fps, 3), "unit": "fields/s", "cores": procs, "kind": "reference",
+            "sample": f"first {len(jobs)} of the {nfields} c4 fields, each converged to max |delta| "
+                      f"<= 1e-12 by the reference runFSM (mpi_openacc/src/phi3D_fsm.c, gcc -O3) "
+                      f"in one of {procs} worker processes",
+            "wall_seconds": round(wall, 2), "sweep_seconds": round(sum(r[1] for r in res), 2),
+            "passes_max": max(r[0] for r in res),
+            "gpu_batched_vs_cpu": round(gpu["fields_per_s"] / fps, 1)}
 
 
 def cpu_baseline(ctx, args, ref_passes, X, Y, Z):

@@ -42,6 +42,61 @@ EXPORTED = [
 ]
 WEIGHTS_PATH = os.path.join(HERE, "weights", "warm_unet.bin")
 CLI_PATH = os.path.join(HERE, "bin", "PDFS")
+NC_LIB_PATH = os.path.join(HERE, "lib", "libpdfs_nc.so")
+NC_EXPORTED = ["pdfs_nc_read", "pdfs_nc_write", "pdfs_nc_free"]
+
+
+class NcField(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double), ("flags", C.POINTER(C.c_int)),
+                ("distance", C.POINTER(C.c_double))]
+
+
+_NC = None
+
+
+def nc_lib():
+    """lib/libpdfs_nc.so: NetCDF-4 field files over HDF5 (host only)."""
+    global _NC
+    if _NC is None:
+        if not os.path.exists(NC_LIB_PATH):
+            raise PdfsError(PDFS_EINVAL, f"{NC_LIB_PATH} is missing (built where HDF5 is present)")
+        L = C.CDLL(NC_LIB_PATH)
+        L.pdfs_nc_read.argtypes = [C.c_char_p, C.POINTER(NcField)]
+        L.pdfs_nc_read.restype = C.c_int
+        L.pdfs_nc_free.argtypes = [C.POINTER(NcField)]
+        L.pdfs_nc_free.restype = None
+        L.pdfs_nc_write.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                    C.c_double, C.c_double, C.c_double]
+        L.pdfs_nc_write.restype = C.c_int
+        _NC = L
+    return _NC
+
+
+def nc_write(path, flags, distance, h=(1.0, 1.0, 1.0)):
+    """Write Flags (int32) and Distance (float64) [z][y][x] as a NetCDF-4
+    file of the MPI CLI's layout; returns the library's code (0 = ok)."""
+    import numpy as np
+    fl = np.ascontiguousarray(flags, dtype=np.int32)
+    di = np.ascontiguousarray(distance, dtype=np.float64)
+    assert fl.shape == di.shape and fl.ndim == 3
+    Z, Y, X = fl.shape
+    return nc_lib().pdfs_nc_write(str(path).encode(), fl.ctypes.data, di.ctypes.data, X, Y, Z, *h)
+
+
+def nc_read(path):
+    """(code, dict(flags, distance [z][y][x], h=(dx, dy, dz)) or None)."""
+    import numpy as np
+    f = NcField()
+    rc = nc_lib().pdfs_nc_read(str(path).encode(), C.byref(f))
+    if rc:
+        return rc, None
+    n = f.x * f.y * f.z
+    out = {"flags": np.ctypeslib.as_array(f.flags, (n,)).copy().reshape(f.z, f.y, f.x),
+           "distance": np.ctypeslib.as_array(f.distance, (n,)).copy().reshape(f.z, f.y, f.x),
+           "h": (f.dx, f.dy, f.dz)}
+    nc_lib().pdfs_nc_free(C.byref(f))
+    return 0, out
 
 
 class Phi(C.Structure):

@@ -1,5 +1,6 @@
-"""CPU checks of the C-ABI library: it loads, and exports every function that
-include/*.h declares. No compute calls (no GPU here)."""
+"""CPU checks of the C-ABI libraries: they load, and export every function
+that include/*.h declares (include/pdfs_nc.h: lib/libpdfs_nc.so; the other
+headers: lib/libpdfs.so). No compute calls (no GPU here)."""
 import ctypes
 import os
 import re
@@ -10,10 +11,13 @@ import pdfs_amd
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+NC_HEADER = "pdfs_nc.h"
+
+
+def declared_functions(nc=False):
     names = set()
     for fn in os.listdir(os.path.join(ROOT, "include")):
-        if not fn.endswith(".h"):
+        if not fn.endswith(".h") or (fn == NC_HEADER) != nc:
             continue
         src = open(os.path.join(ROOT, "include", fn)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
@@ -33,6 +37,18 @@ def test_library_loads_and_exports_declared_symbols():
     assert not missing, missing
     for n in names:
         assert hasattr(L, n)
+
+
+def test_netcdf_library_exports_declared_symbols():
+    if not os.path.exists(pdfs_amd.NC_LIB_PATH):
+        import pytest
+        pytest.skip("lib/libpdfs_nc.so not built (no HDF5)")
+    names = declared_functions(nc=True)
+    assert names == sorted(pdfs_amd.NC_EXPORTED)
+    out = subprocess.run(["nm", "-D", "--defined-only", pdfs_amd.NC_LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(names) <= exported
 
 
 def test_phi_struct_layouts_match_reference():

@@ -138,6 +138,15 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
  * max |delta| <= tol), in place. */
 int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx,
                          double dy, double dz, int passes, double tol, pdfs_stats *st);
+/* Spatially varying speed (no counterpart in the reference, which declares
+ * Phi.F and SPEED, phi3D.h:37, 54, but never uses them): the Eikonal
+ * |grad u| = 1/F with the padded slowness array s = 1/F (device, same
+ * layout as padded; interior values finite and > 0). Each point's update is
+ * the reference arithmetic's local solve with spacings dx s, dy s, dz s, on
+ * the reference hyperplane schedule; passes as pdfs_solve_ref64_dev. */
+int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
+                           int z, double dx, double dy, double dz, int passes, double tol,
+                           pdfs_stats *st);
 /* FP64 residual judge; flags: [ceil(x*y*z/32)] words or NULL. */
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags,

@@ -223,6 +223,42 @@ void orc_sweep_ref64(double *d, int X, int Y, int Z, double dx, double dy,
   }
 }
 
+/* Spatially varying speed (new; the reference declares Phi.F / SPEED,
+ * phi3D.h:37, 54, and never uses them): |grad u| = s with slowness s = 1/F
+ * per point. The point's update is the reference's local solve with its
+ * spacings scaled by its slowness (dx s, dy s, dz s), since
+ * sum ((u - m_i) / d_i)^2 = s^2 is sum ((u - m_i) / (d_i s))^2 = 1; the
+ * hyperplane schedule is orc_sweep_ref64's. s is the padded array of the
+ * field (only interior points are read). */
+void orc_sweep_ref64_speed(double *d, const double *s, int X, int Y, int Z, double dx,
+                           double dy, double dz, int sn)
+{
+  orc_order o = make_order(X, Y, Z, sn);
+  size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
+  int first = 3, last = X + Y + Z;
+  int start = o.incr ? first : last;
+  int end = o.incr ? last + 1 : first - 1;
+  for (int level = start; level != end; level += o.incr ? 1 : -1) {
+    int xs = ORC_MAX(1, level - (Y + Z));
+    int ys = ORC_MAX(1, level - (X + Z));
+    int xe = ORC_MIN(X, level - (first - 1));
+    int ye = ORC_MIN(Y, level - (first - 1));
+    for (int x = xs; x <= xe; x++) {
+      int k = abs(x - o.xo);
+      for (int y = ys; y <= ye; y++) {
+        int z = level - (x + y);
+        if (z > 0 && z <= Z) {
+          int j = abs(y - o.yo);
+          int i = abs(z - o.zo);
+          size_t id = (size_t)i * extXY + (size_t)j * extX + (size_t)k;
+          const double sp = s[id];
+          d[id] = ref_point(d, id, extX, extXY, dx * sp, dy * sp, dz * sp);
+        }
+      }
+    }
+  }
+}
+
 /* Physical traversal direction of ordering sn per axis: +1 ascending. */
 static void order_dirs(int sn, int *sx, int *sy, int *sz)
 {
@@ -314,6 +350,26 @@ int orc_solve_ref64(double *d, int X, int Y, int Z, double dx, double dy,
   while (passes < max_passes) {
     memcpy(prev, d, n * sizeof(double));
     for (int sn = 1; sn <= 8; sn++) orc_sweep_ref64(d, X, Y, Z, dx, dy, dz, sn);
+    passes++;
+    md = max_abs_diff64(prev, d, n);
+    if (md <= tol) break;
+  }
+  free(prev);
+  if (last_maxdiff) *last_maxdiff = md;
+  return passes;
+}
+
+/* Passes of orc_sweep_ref64_speed until max |delta| <= tol. */
+int orc_solve_ref64_speed(double *d, const double *s, int X, int Y, int Z, double dx,
+                          double dy, double dz, double tol, int max_passes, double *last_maxdiff)
+{
+  size_t n = (size_t)(X + 2) * (Y + 2) * (Z + 2);
+  double *prev = (double *)malloc(n * sizeof(double));
+  int passes = 0;
+  double md = 0.0;
+  while (passes < max_passes) {
+    memcpy(prev, d, n * sizeof(double));
+    for (int sn = 1; sn <= 8; sn++) orc_sweep_ref64_speed(d, s, X, Y, Z, dx, dy, dz, sn);
     passes++;
     md = max_abs_diff64(prev, d, n);
     if (md <= tol) break;

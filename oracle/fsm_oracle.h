@@ -60,6 +60,12 @@ void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
 /* Passes of sn = 1..8 until converged (or max_passes). Return passes run. */
 int orc_solve_ref64(double *d, int X, int Y, int Z, double dx, double dy,
                     double dz, double tol, int max_passes, double *last_maxdiff);
+/* Spatially varying speed: slowness s (padded, = 1/F) scales the point's
+ * spacings; the same schedule and local solve as orc_sweep_ref64. */
+void orc_sweep_ref64_speed(double *d, const double *s, int X, int Y, int Z, double dx,
+                           double dy, double dz, int sn);
+int orc_solve_ref64_speed(double *d, const double *s, int X, int Y, int Z, double dx,
+                          double dy, double dz, double tol, int max_passes, double *last_maxdiff);
 int orc_solve_f32(float *d, int X, int Y, int Z, double dx, double dy,
                   double dz, int max_passes, long *last_changed);
 int orc_solve_d64(double *d, int X, int Y, int Z, double dx, double dy,

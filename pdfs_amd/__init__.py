@@ -27,7 +27,8 @@ TILES = (TILE_16x16, TILE_32x8, TILE_16x8, TILE_32x16, TILE_16x16_1CU, TILE_16x8
 EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
     "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
-    "pdfs_solve_ref64_dev", "pdfs_judge_f32_dev", "pdfs_selftest_sqrt_f32", "pdfs_generate_dev",
+    "pdfs_solve_ref64_dev", "pdfs_solve_speed64_dev", "pdfs_judge_f32_dev",
+    "pdfs_selftest_sqrt_f32", "pdfs_generate_dev",
     "pdfs_generate_ex_dev",
     "pdfs_pack_dev",
     "pdfs_unpack_dev", "pdfs_solve_batch_f32_dev", "pdfs_calc_dist_field", "pdfs_prepost", "pdfs_vti_read",
@@ -170,6 +171,8 @@ def lib():
         L.pdfs_sweep_f32_dev.restype = I
         L.pdfs_solve_ref64_dev.argtypes = [vp, vp, I, I, I, D, D, D, I, D, C.POINTER(Stats)]
         L.pdfs_solve_ref64_dev.restype = I
+        L.pdfs_solve_speed64_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, I, D, C.POINTER(Stats)]
+        L.pdfs_solve_speed64_dev.restype = I
         L.pdfs_judge_f32_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, vp, C.POINTER(LL)]
         L.pdfs_judge_f32_dev.restype = I
         L.pdfs_selftest_sqrt_f32.argtypes = [vp, C.POINTER(C.c_ulonglong)]
@@ -393,6 +396,20 @@ class Context:
         torch.cuda.synchronize()
         _check(lib().pdfs_solve_ref64_dev(self.c, self._ptr(padded), X, Y, Z, *h, passes, tol,
                                           C.byref(st)))
+        return st.as_dict()
+
+    def solve_speed64(self, padded, slowness, passes=0, tol=1e-12, h=(1.0, 1.0, 1.0)):
+        """Eikonal |grad u| = slowness (= 1/F per point) on a padded FP64
+        field, reference arithmetic and schedule; both float64 cuda tensors
+        of the padded shape."""
+        import torch
+        assert padded.dtype == torch.float64 and padded.dim() == 3
+        assert slowness.dtype == torch.float64 and slowness.shape == padded.shape
+        Z, Y, X = (s - 2 for s in padded.shape)
+        st = Stats()
+        torch.cuda.synchronize()
+        _check(lib().pdfs_solve_speed64_dev(self.c, self._ptr(padded), self._ptr(slowness), X, Y,
+                                            Z, *h, passes, tol, C.byref(st)))
         return st.as_dict()
 
     def selftest_sqrt_f32(self):

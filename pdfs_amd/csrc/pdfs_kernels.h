@@ -90,10 +90,11 @@ int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp32 *sp, int
 
 /* One ordering sn of the reference hyperplane schedule, FP64, on the padded
  * array, one launch per level (reference arithmetic, bitwise parity).
- * If maxdiff != NULL, atomically maxes |old - new| (as uint64 bits). */
-int pdfsk_sweep_ref64(double *d, int X, int Y, int Z, int sn, double dx,
-                      double dy, double dz, unsigned long long *maxdiff,
-                      hipStream_t st);
+ * If maxdiff != NULL, atomically maxes |old - new| (as uint64 bits).
+ * slow (padded slowness 1/F, or NULL for unit speed) scales each point's
+ * spacings. */
+int pdfsk_sweep_ref64(double *d, const double *slow, int X, int Y, int Z, int sn, double dx,
+                      double dy, double dz, unsigned long long *maxdiff, hipStream_t st);
 
 /* FP64 residual judge of dense FP32 fields against their inits, all fields
  * of a batch in one launch. Field f owns judge tiles [tile0, next tile0);

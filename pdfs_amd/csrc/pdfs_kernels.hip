@@ -767,7 +767,11 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
 // Reference hyperplane schedule, FP64, one launch per level
 // (single_gpu_cuda/src/phi3D_fsm.cu:161-185 geometry, :199-233 body).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_level_ref64(double *d, int X, int Y, int Z,
+// With `slow` (padded, slowness s = 1/F per point) the point's spacings are
+// scaled by its slowness: the Eikonal |grad u| = s (a speed field; the
+// reference declares Phi.F and never uses it).
+__global__ __launch_bounds__(256) void k_level_ref64(double *d, const double *__restrict__ slow,
+                                                     int X, int Y, int Z,
                                                      int level, int xs, int ys, int xe,
                                                      int ye, int xo, int yo, int zo,
                                                      double dx, double dy, double dz,
@@ -788,7 +792,9 @@ __global__ __launch_bounds__(256) void k_level_ref64(double *d, int X, int Y, in
       const double mx = l < r ? l : r;
       const double my = dn < up ? dn : up;
       const double mz = fr < bk ? fr : bk;
-      const double u = god_ref64(c, mx, my, mz, dx, dy, dz);
+      const double sp = slow ? slow[id] : 1.0;
+      const double u = slow ? god_ref64(c, mx, my, mz, dx * sp, dy * sp, dz * sp)
+                            : god_ref64(c, mx, my, mz, dx, dy, dz);
       d[id] = u;
       if (u != c) {
         double df = __builtin_fabs(c - u);
@@ -1150,8 +1156,8 @@ extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp
   return hip_rc(hipGetLastError());
 }
 
-extern "C" int pdfsk_sweep_ref64(double *d, int X, int Y, int Z, int sn, double dx,
-                                 double dy, double dz, unsigned long long *maxdiff,
+extern "C" int pdfsk_sweep_ref64(double *d, const double *slow, int X, int Y, int Z, int sn,
+                                 double dx, double dy, double dz, unsigned long long *maxdiff,
                                  hipStream_t st) {
   if (sn < 1 || sn > 8) return -1;
   // mpi_openacc/src/phi3D_fsm.c:195-221 ordering table, :92-98 level bounds.
@@ -1174,8 +1180,8 @@ extern "C" int pdfsk_sweep_ref64(double *d, int X, int Y, int Z, int sn, double 
     if (xr <= 0 || yr <= 0) continue;
     dim3 bs(64, 4);
     dim3 gs((xr + 63) / 64, (yr + 3) / 4);
-    hipLaunchKernelGGL(k_level_ref64, gs, bs, 0, st, d, X, Y, Z, level, xs, ys, xe, ye, xo,
-                       yo, zo, dx, dy, dz, maxdiff);
+    hipLaunchKernelGGL(k_level_ref64, gs, bs, 0, st, d, slow, X, Y, Z, level, xs, ys, xe, ye,
+                       xo, yo, zo, dx, dy, dz, maxdiff);
   }
   return hip_rc(hipGetLastError());
 }

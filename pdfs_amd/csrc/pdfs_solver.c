@@ -870,11 +870,9 @@ int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int 
   return pdfs_solve_batch_f32_dev(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL);
 }
 
-int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx, double dy,
-                         double dz, int passes, double tol, pdfs_stats *st)
+static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, int y, int z,
+                       double dx, double dy, double dz, int passes, double tol, pdfs_stats *st)
 {
-  if (!c || !padded || x < 1 || y < 1 || z < 1)
-    return fail(PDFS_EINVAL, "pdfs_solve_ref64_dev: bad arguments");
   pdfs_stats s;
   memset(&s, 0, sizeof s);
   int maxp = passes > 0 ? passes : 1000;
@@ -884,7 +882,7 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
     HIPCHK(hipMemsetAsync(c->d_u64, 0, sizeof(unsigned long long), c->st));
     HIPCHK(hipEventRecord(c->ev0, c->st));
     for (int sn = 1; sn <= 8; sn++)
-      KCHK(pdfsk_sweep_ref64(padded, x, y, z, sn, dx, dy, dz, c->d_u64, c->st));
+      KCHK(pdfsk_sweep_ref64(padded, slow, x, y, z, sn, dx, dy, dz, c->d_u64, c->st));
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipMemcpyAsync(c->h_pin64, c->d_u64, sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, c->st));
@@ -905,6 +903,23 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
   s.ms_total = ms;
   if (st) *st = s;
   return PDFS_OK;
+}
+
+int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx, double dy,
+                         double dz, int passes, double tol, pdfs_stats *st)
+{
+  if (!c || !padded || x < 1 || y < 1 || z < 1)
+    return fail(PDFS_EINVAL, "pdfs_solve_ref64_dev: bad arguments");
+  return solve_ref64(c, padded, NULL, x, y, z, dx, dy, dz, passes, tol, st);
+}
+
+int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
+                           int z, double dx, double dy, double dz, int passes, double tol,
+                           pdfs_stats *st)
+{
+  if (!c || !padded || !slowness || x < 1 || y < 1 || z < 1)
+    return fail(PDFS_EINVAL, "pdfs_solve_speed64_dev: bad arguments");
+  return solve_ref64(c, padded, slowness, x, y, z, dx, dy, dz, passes, tol, st);
 }
 
 /* ---- host padded FP64 entry points ---- */
@@ -1009,7 +1024,7 @@ void runFSM(PhiACC *pf, int sn)
   double *d = NULL;
   if (hipMalloc((void **)&d, np * sizeof(double)) != hipSuccess ||
       hipMemcpy(d, pf->distance, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      pdfsk_sweep_ref64(d, pf->x, pf->y, pf->z, sn, pf->dx, pf->dy, pf->dz, NULL, c->st) != 0 ||
+      pdfsk_sweep_ref64(d, NULL, pf->x, pf->y, pf->z, sn, pf->dx, pf->dy, pf->dz, NULL, c->st) != 0 ||
       hipStreamSynchronize(c->st) != hipSuccess ||
       hipMemcpy(pf->distance, d, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
     printf("\nHIP failure in runFSM\n");

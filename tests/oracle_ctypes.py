@@ -43,6 +43,11 @@ def lib():
             getattr(L, nm).restype = None
         L.orc_solve_ref64.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _D, _I, C.POINTER(_D)]
         L.orc_solve_ref64.restype = _I
+        L.orc_sweep_ref64_speed.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D, _I]
+        L.orc_sweep_ref64_speed.restype = None
+        L.orc_solve_ref64_speed.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D, _D, _I,
+                                            C.POINTER(_D)]
+        L.orc_solve_ref64_speed.restype = _I
         L.orc_solve_d64.argtypes = [_dp, _I, _I, _I, _D, _D, _D, _D, _I]
         L.orc_solve_d64.restype = _I
         L.orc_solve_f32.argtypes = [_fp, _I, _I, _I, _D, _D, _D, _I, C.POINTER(C.c_long)]
@@ -107,6 +112,17 @@ def sweep_lex64(a, sn, h=(1.0, 1.0, 1.0)):
 def sweep_f32(a, sn, h=(1.0, 1.0, 1.0)):
     lib().orc_sweep_f32(a, *dims(a), *h, sn)
     return a
+
+
+def sweep_ref64_speed(a, s, sn, h=(1.0, 1.0, 1.0)):
+    lib().orc_sweep_ref64_speed(a, s, *dims(a), *h, sn)
+    return a
+
+
+def solve_ref64_speed(a, s, h=(1.0, 1.0, 1.0), tol=1e-12, max_passes=100):
+    md = _D()
+    p = lib().orc_solve_ref64_speed(a, s, *dims(a), *h, tol, max_passes, C.byref(md))
+    return p, md.value
 
 
 def solve_ref64(a, h=(1.0, 1.0, 1.0), tol=1e-12, max_passes=100):

@@ -450,3 +450,54 @@ def test_largest_f32_field_properties(ctx, dev):
     big = torch.empty((2, 1 << 14, 1 << 14), dtype=torch.float32, device=dev)  # 2^29 points
     with pytest.raises(pdfs_amd.PdfsError):
         ctx.solve_f32(big, None, judge=0)
+
+
+@pytest.mark.parametrize("shape,h", [((17, 13, 11), (1.0, 1.0, 1.0)),
+                                     ((24, 9, 14), (0.5, 0.75, 1.25)),
+                                     ((1, 6, 5), (1.0, 1.0, 1.0))])
+def test_speed64_bitwise_against_oracle(ctx, dev, shape, h):
+    """Spatially varying speed: the GPU's FP64 hyperplane sweeps with a
+    slowness field equal the oracle's, one ordering at a time and converged
+    (bit for bit); unit slowness is the unit-speed solve."""
+    X, Y, Z = shape
+    rng = np.random.default_rng(X * 100 + Y)
+    a = O.gen_point(X, Y, Z, seed=X + Y)
+    s = rng.uniform(0.3, 3.0, size=a.shape)
+    want = a.copy()
+    O.solve_ref64_speed(want, s, h, tol=0.0, max_passes=60)
+    t = torch.from_numpy(a.copy()).to(dev)
+    st = ctx.solve_speed64(t, torch.from_numpy(s).to(dev), passes=0, tol=0.0, h=h)
+    assert np.array_equal(t.cpu().numpy().view(np.uint64), want.view(np.uint64)), st
+    one = torch.from_numpy(a.copy()).to(dev)
+    ctx.solve_speed64(one, torch.ones_like(one), passes=2, h=h)
+    ref = torch.from_numpy(a.copy()).to(dev)
+    ctx.solve_ref64(ref, passes=2, h=h)
+    assert torch.equal(one.view(torch.int64), ref.view(torch.int64))
+    # a single pass from the same init, ordering by ordering on the CPU
+    b = a.copy()
+    for sn in range(1, 9):
+        O.sweep_ref64_speed(b, s, sn, h)
+    t = torch.from_numpy(a.copy()).to(dev)
+    ctx.solve_speed64(t, torch.from_numpy(s).to(dev), passes=1, h=h)
+    assert np.array_equal(t.cpu().numpy().view(np.uint64), b.view(np.uint64))
+
+
+def test_speed64_layered_medium(ctx, dev):
+    """Source plane z = 1 in a medium layered in z: u(z) = u(z-1) + dz s(z)
+    exactly, on the GPU."""
+    X, Y, Z, dz = 40, 33, 64, 0.5
+    a = O.padded(X, Y, Z)
+    a[1:-1, 1:-1, 1:-1] = 1e6
+    a[1, 1:-1, 1:-1] = 0.0
+    layers = np.random.default_rng(11).uniform(0.25, 4.0, size=Z)
+    s = np.ones_like(a)
+    for k in range(Z):
+        s[k + 1] = layers[k]
+    t = torch.from_numpy(a).to(dev)
+    ctx.solve_speed64(t, torch.from_numpy(s).to(dev), passes=0, tol=0.0, h=(1.0, 1.0, dz))
+    u = t.cpu().numpy()
+    want = 0.0
+    for k in range(Z):
+        if k:
+            want = want + dz * layers[k]
+        assert np.all(u[k + 1, 1:-1, 1:-1] == want), k

@@ -386,3 +386,55 @@ def test_one_unchanged_ordering_is_the_fixed_point():
         for k2 in range(k + 1, k + 17):
             O.sweep_f32(a, k2 % 8 + 1, h)
         assert np.array_equal(fixed.view(np.int32), a.view(np.int32))
+
+
+def test_speed_sweep_unit_and_scaled_slowness():
+    """Slowness 1 is the unit-speed sweep bit for bit; slowness 2 is the
+    unit-speed sweep with doubled spacings bit for bit (dx * 2 is exact)."""
+    rng = np.random.default_rng(8)
+    for X, Y, Z, h in ((9, 7, 6, (1.0, 1.0, 1.0)), (8, 10, 5, (0.5, 0.75, 1.25))):
+        a = O.gen_spheres(X, Y, Z, seed=int(rng.integers(100)), n=3, r=(1.0, 2.5))
+        for sn in range(1, 9):
+            b, c = a.copy(), a.copy()
+            O.sweep_ref64_speed(b, np.ones_like(a), sn, h)
+            O.sweep_ref64(c, sn, h)
+            assert np.array_equal(b.view(np.uint64), c.view(np.uint64))
+            b, c = a.copy(), a.copy()
+            O.sweep_ref64_speed(b, np.full_like(a, 2.0), sn, h)
+            O.sweep_ref64(c, sn, tuple(2 * v for v in h))
+            assert np.array_equal(b.view(np.uint64), c.view(np.uint64))
+
+
+def test_speed_layered_medium_closed_form():
+    """A source plane z = 1 in a medium layered in z: the travel time is the
+    sum of slowness x dz over the layers crossed, u(z) = u(z-1) + dz s(z),
+    exactly (every update is the 1-D one)."""
+    X, Y, Z, dz = 6, 5, 12, 0.5
+    a = O.padded(X, Y, Z)
+    a[1:-1, 1:-1, 1:-1] = 1e6
+    a[1, 1:-1, 1:-1] = 0.0
+    s = np.ones_like(a)
+    layers = np.random.default_rng(3).uniform(0.25, 4.0, size=Z)
+    for k in range(Z):
+        s[k + 1] = layers[k]
+    O.solve_ref64_speed(a, s, (1.0, 1.0, dz), tol=0.0)
+    want = [0.0]
+    for k in range(1, Z):
+        want.append(want[-1] + dz * layers[k])
+    for k in range(Z):
+        assert np.all(a[k + 1, 1:-1, 1:-1] == want[k]), k
+
+
+def test_speed_monotone_in_slowness():
+    """Comparison principle: a pointwise larger slowness gives a pointwise
+    larger (or equal) converged travel time."""
+    rng = np.random.default_rng(4)
+    a = O.gen_point(14, 11, 9, seed=9)
+    s1 = rng.uniform(0.5, 2.0, size=a.shape)
+    s2 = s1 * rng.uniform(1.0, 1.5, size=a.shape)
+    u1, u2 = a.copy(), a.copy()
+    O.solve_ref64_speed(u1, s1, tol=0.0)
+    O.solve_ref64_speed(u2, s2, tol=0.0)
+    inner = (slice(1, -1),) * 3
+    assert np.all(u1[inner] <= u2[inner])
+    assert np.any(u1[inner] < u2[inner])

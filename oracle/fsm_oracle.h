@@ -17,8 +17,13 @@
  *   orc_sweep_lex        bitwise equal to orc_sweep_ref64 (schedule
  *                        independence), brute force on tiny grids.
  *   orc_godunov_f32/_d64 closed forms (1-D m+h, 2-D m+h/sqrt2, 3-D m+h/sqrt3),
- *                        agreement with orc_godunov_ref to fp tolerance,
- *                        shift invariance, monotonicity.
+ *                        general and 2-D/3-D-boundary roots against an
+ *                        independent root finder, shift invariance,
+ *                        monotonicity.
+ *   orc_sweep_ref64_speed / orc_solve_ref64_speed
+ *                        slowness 1 = orc_sweep_ref64 and slowness 2 = the
+ *                        doubled-spacing sweep, bitwise; a layered medium's
+ *                        exact travel times; monotonicity in the slowness.
  *   orc_solve_*          point-source axis exactness (n*h), fixed-point
  *                        residual 0, agreement with the compiled reference.
  *   orc_update_distance, orc_set_negative_inside, orc_adjust_boundary

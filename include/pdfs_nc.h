@@ -26,7 +26,8 @@ typedef struct {
 /* Read Flags, Distance and the x / y / z lengths and deltas, as ncGetDim
  * and ncGetInt / ncGetDouble do (ncwrap.c:121-141, 283-373). Returns 0, or
  * -1 cannot open, -2 a variable, dimension or attribute is missing or has
- * another shape, -3 out of memory. Arrays are malloc'd; pdfs_nc_free. */
+ * another shape (or an axis of 2^30 or more, or over 2^36 points), -3 out
+ * of memory. Arrays are malloc'd; pdfs_nc_free. */
 int pdfs_nc_read(const char *path, pdfs_nc_field *out);
 void pdfs_nc_free(pdfs_nc_field *f);
 

@@ -21,7 +21,8 @@ typedef struct {
   double *distance;   /* padded, ghost = +inf */
 } pdfs_vti;
 
-/* 0 on success; -1 open, -2 header, -3 memory, -4 data. */
+/* 0 on success; -1 open, -2 header (missing, or extents of 2^30 or more per
+ * axis or over 2^36 padded points), -3 memory, -4 data. */
 int pdfs_vti_read(const char *path, pdfs_vti *out);
 void pdfs_vti_free(pdfs_vti *v);
 /* Writes the padded array of padded dims gx, gy, gz; 0 on success. */

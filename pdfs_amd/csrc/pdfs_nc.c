@@ -81,6 +81,9 @@ int pdfs_nc_read(const char *path, pdfs_nc_field *out)
   double dlt[3];
   int rc = 0;
   for (int i = 0; i < 3 && !rc; i++) rc = read_axis(f, AXES[i], &len[i], &dlt[i]);
+  /* each axis < 2^30 (read_axis); at most 2^36 points in all, so neither
+   * this size nor the padded one overflows */
+  if (!rc && (double)len[0] * (double)len[1] * (double)len[2] > (double)(1ull << 36)) rc = -2;
   if (!rc) {
     const size_t n = (size_t)len[0] * len[1] * len[2];
     const hsize_t shape[3] = {(hsize_t)len[2], (hsize_t)len[1], (hsize_t)len[0]};

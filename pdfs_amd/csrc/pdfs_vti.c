@@ -203,9 +203,16 @@ int pdfs_vti_read(const char *path, pdfs_vti *out)
     free(buf);
     return -2;
   }
-  /* phi3D.c:57-62 */
+  /* phi3D.c:57-62; extents beyond 2^30 per axis or 2^36 padded points are
+   * malformed (they could not be held, and their products would overflow) */
+  for (int k = 0; k < 3; k++)
+    if (!(d[k] >= 0.0 && d[k] < (double)(1 << 30))) {
+      free(buf);
+      return -2;
+    }
   int X = (int)d[0] + 1, Y = (int)d[1] + 1, Z = (int)d[2] + 1;
-  if (X < 1 || Y < 1 || Z < 1) {
+  if (X < 1 || Y < 1 || Z < 1 ||
+      (double)(X + 2) * (double)(Y + 2) * (double)(Z + 2) > (double)(1ull << 36)) {
     free(buf);
     return -2;
   }

@@ -141,3 +141,18 @@ def test_reader_parallel_errors(tmp_path):
     k = txt.index(" ", t + 200000)
     open(p, "w").write(txt[:k] + " abc" + txt[k:])
     assert read(p)[0] == -4
+
+
+def test_reader_rejects_extents_that_cannot_be_held(tmp_path):
+    """Header extents of 2^30 or more per axis, or whose padded product
+    passes 2^36, are a malformed header (-2), not an allocation of a
+    wrapped-around size."""
+    loc, dist = random_cli_input(4, 3, 2, 2)
+    p = str(tmp_path / "ok.vti")
+    write_input_vti(p, loc, dist)
+    txt = open(p).read()
+    for ext in ("0 1073741824 0 2 0 1", "0 3000000 0 3000000 0 3000000", "0 -5 0 2 0 1",
+                "0 nan 0 2 0 1"):
+        q = str(tmp_path / "bad.vti")
+        open(q, "w").write(txt.replace("WholeExtent=\"0 3 0 2 0 1\"", f"WholeExtent=\"{ext}\"", 1))
+        assert read(q)[0] == -2, ext

@@ -501,3 +501,30 @@ def test_speed64_layered_medium(ctx, dev):
         if k:
             want = want + dz * layers[k]
         assert np.all(u[k + 1, 1:-1, 1:-1] == want), k
+
+
+def test_certified_batch_retires_fields_mid_pass(ctx, dev):
+    """A certified batch in which one field (512^3 spheres) passes the judge
+    after the first half of its third pass while the others pass at pass
+    ends: the batch retires it mid-pass and re-plans the rest, and every
+    field equals its own certified single-field solve bit for bit."""
+    inits = []
+    t = torch.empty((512, 512, 512), dtype=torch.float32, device=dev)
+    ctx.generate(pdfs_amd.GEN_SPHERES, 2024, 512, 512, 512, 1.0, dense=t)
+    inits.append(t)
+    for i, (X, Y, Z) in enumerate(((160, 128, 96), (96, 80, 112), (33, 17, 25))):
+        u = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
+        ctx.generate_ex(pdfs_amd.GEN_TERRAIN if i % 2 == 0 else pdfs_amd.GEN_SPHERES, 4300 + i,
+                        X, Y, Z, 1.0, 8, (5.0, 40.0) if i % 2 == 0 else (2.0, 9.0),
+                        (10.0, 60.0) if i % 2 == 0 else (0.0, 0.0), dense=u)
+        inits.append(u)
+    fields = [x.clone() for x in inits]
+    st, passes, _ = ctx.solve_batch_f32(fields, inits, certify=1, graph=1)
+    assert st["judge_flagged"] == 0
+    for i, x in enumerate(inits):
+        f = x.clone()
+        s1 = ctx.solve_f32(f, x, certify=1)
+        assert passes[i] == s1["passes"], (i, passes, s1)
+        assert torch.equal(f.view(torch.int32), fields[i].view(torch.int32)), i
+    del inits, fields, t
+    torch.cuda.empty_cache()

@@ -218,19 +218,15 @@ __device__ __forceinline__ float god32(float c, float xm, float xp, float ym, fl
 // FP64 difference form (judge). Spacing constants in double, same op order
 // as the oracle's double instantiation.
 // Branch-free isotropic FP64 solve for the residual judge: the operations of
-// god_diff_iso<double> with the 1-D / 2-D / 3-D choice made by selects, so
-// the value is the same; lanes of a wave no longer diverge over the branches.
-// Inputs are non-negative or +inf (no NaN): the minima / sort equal the '<'
-// / '>' selects, and a negative 2-D radicand (NaN root) implies u1 <= m1, so
-// the NaN candidate is never selected; with m2 = +inf the 3-D test is false.
-__device__ __forceinline__ double god_iso64_sel(double cur, double mx, double my, double mz,
-                                                double h, double two_h2, double three_h2,
-                                                double third) {
+// god_diff_iso<double> after its sort, with the 1-D / 2-D / 3-D choice made
+// by selects, so the value is the same; lanes of a wave do not diverge over
+// the branches. m0 <= m1 <= m2 are the sorted per-axis minima, non-negative
+// or +inf: a negative 2-D radicand (NaN root) implies u1 <= m1, so the NaN
+// candidate is never selected; with m2 = +inf the 3-D test is false.
+__device__ __forceinline__ double god_iso64_sorted(double cur, double m0, double m1, double m2,
+                                                   double h, double two_h2, double three_h2,
+                                                   double third) {
 #pragma clang fp contract(off)
-  // non-negative, non-NaN operands: min / max equal the '<' / '>' selects
-  const double lo = __builtin_fmin(mx, my), hi = __builtin_fmax(mx, my);
-  const double m0 = __builtin_fmin(lo, mz), mid = __builtin_fmax(lo, mz);
-  const double m1 = __builtin_fmin(hi, mid), m2 = __builtin_fmax(hi, mid);
   const double u1 = m0 + h;
   const double a = m1 - m0;
   const double b = m2 - m0;
@@ -244,7 +240,7 @@ __device__ __forceinline__ double god_iso64_sel(double cur, double mx, double my
   const double K = three ? third : 0.5;
   const double t = m0 + (P + __builtin_sqrt(R)) * K;
   const double u = u1 > m1 ? t : u1;
-  return __builtin_fmin(cur, u);
+  return cur < u ? cur : u;
 }
 
 __device__ __forceinline__ double god_d64(double cur, double mx, double my, double mz,
@@ -827,9 +823,97 @@ __global__ __launch_bounds__(256) void k_level_ref64(double *d, const double *__
 // wave.
 // ---------------------------------------------------------------------------
 #define JUDGE_Z 32
+
+// Element i (< 2^29) of a device array, loaded as global memory with a
+// 32-bit byte offset from a wave-uniform base. Pointers read from a table in
+// memory are generic to the compiler, which would otherwise emit flat loads
+// (counted against both the vector-memory and the LDS counters) with 64-bit
+// per-lane address arithmetic.
+__device__ __forceinline__ float ld_global(const float *base, uint32_t i) {
+  typedef const __attribute__((address_space(1))) char gchar;
+  typedef const __attribute__((address_space(1))) float gfloat;
+  return *(gfloat *)((gchar *)base + (i << 2));
+}
 // tiles of a field (64 x 4 columns x JUDGE_Z planes)
 __host__ __device__ inline uint32_t judge_tiles(int X, int Y, int Z) {
   return (uint32_t)((X + 63) / 64) * (uint32_t)((Y + 3) / 4) * (uint32_t)((Z + JUDGE_Z - 1) / JUDGE_Z);
+}
+
+// Per-lane constants of a judge column.
+struct JudgeLane {
+  uint32_t XY, col;                // plane size; (x, y) offset of the column
+  uint32_t oxm, oxp, oym, oyp;     // in-plane neighbour offsets (0 outside)
+  int Z;
+  bool in, hxm, hxp, hym, hyp;     // judged lane; neighbour inside the field
+  double htol;                     // hmin 2^-21
+};
+// The operands of one point: its value, its six neighbours' and its init.
+struct JudgeOps {
+  float zm, cur, zp, xm, xp, ym, yp, ini;
+};
+
+// Operands of plane zl (clamped to the field: the planes past its end are
+// loaded but never judged). Out-of-field neighbours load the point itself
+// and are replaced by +inf in judge_plane.
+__device__ __forceinline__ JudgeOps judge_load(const float *v, const float *init,
+                                               const JudgeLane &L, int zl) {
+  const int zc = zl < L.Z ? zl : L.Z - 1;
+  const uint32_t p = (uint32_t)zc * L.XY + L.col;
+  JudgeOps o;
+  o.zm = ld_global(v, zc > 0 ? p - L.XY : p);
+  o.cur = ld_global(v, p);
+  o.zp = ld_global(v, zc + 1 < L.Z ? p + L.XY : p);
+  o.xm = ld_global(v, p - L.oxm);
+  o.xp = ld_global(v, p + L.oxp);
+  o.ym = ld_global(v, p - L.oym);
+  o.yp = ld_global(v, p + L.oyp);
+  o.ini = ld_global(init, p);
+  return o;
+}
+
+// Judge the point of plane z (< Z) whose operands are o: sets its flag bit
+// when it fails and returns the number of failing points of the wave.
+template <bool ISO>
+__device__ __forceinline__ unsigned judge_plane(const JudgeOps &o, const JudgeLane &L, int z,
+                                                double dx, double dy, double dz,
+                                                uint32_t *flags) {
+#pragma clang fp contract(off)
+  const float FINF = __builtin_inff();
+  const float zm = z > 0 ? o.zm : FINF, zp = z + 1 < L.Z ? o.zp : FINF;
+  const float xm = L.hxm ? o.xm : FINF, xp = L.hxp ? o.xp : FINF;
+  const float ym = L.hym ? o.ym : FINF, yp = L.hyp ? o.yp : FINF;
+  // Per axis the smaller neighbour, and for the isotropic solve the sort,
+  // in FP32: FP32 values are exact in FP64, so both commute with the
+  // conversion (non-NaN fields; a NaN operand makes the point fail).
+  const float mx = __builtin_elementwise_minimum(xm, xp);
+  const float my = __builtin_elementwise_minimum(ym, yp);
+  const float mz = __builtin_elementwise_minimum(zm, zp);
+  double e;
+  if (ISO) {
+    const float m0 = __builtin_elementwise_minimum(__builtin_elementwise_minimum(mx, my), mz);
+    const float m2 = __builtin_elementwise_maximum(__builtin_elementwise_maximum(mx, my), mz);
+    const float m1 = __builtin_amdgcn_fmed3f(mx, my, mz);
+    e = god_iso64_sorted((double)o.ini, (double)m0, (double)m1, (double)m2, dx,
+                         2.0 * (dx * dx), 3.0 * (dx * dx), 1.0 / 3.0);
+  } else {
+    e = god_d64((double)o.ini, (double)mx, (double)my, (double)mz, dx, dy, dz);
+  }
+  // |v - e| <= |e| 2^-22 + hmin 2^-21, and equal infinities pass: with |e|
+  // capped at DBL_MAX an infinite e against a finite v (or the reverse)
+  // fails, as does a NaN.
+  const double dv = (double)o.cur;
+  const double ae = __builtin_fabs(e) < 0x1.fffffffffffffp+1023 ? __builtin_fabs(e)
+                                                                : 0x1.fffffffffffffp+1023;
+  const bool bad = L.in && !(dv == e) && !(__builtin_fabs(dv - e) <= ae * 0x1.0p-22 + L.htol);
+  if (bad && flags) {
+    const uint32_t p = (uint32_t)z * L.XY + L.col;
+    // a global (not flat) atomic: a flat one in the loop would make the
+    // compiler wait for every load in flight, prefetches included
+    typedef __attribute__((address_space(1))) uint32_t gu32;
+    __hip_atomic_fetch_or((gu32 *)(flags + (p >> 5)), 1u << (p & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return (unsigned)__popcll(__ballot(bad));
 }
 
 // Fields of a batch: field f owns tiles [tab[f].tile0, tab[f+1].tile0)
@@ -862,39 +946,39 @@ __global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__re
   const int bz = (int)(t / (uint32_t)nty);
   const int x = bx * 64 + (int)(threadIdx.x & 63), y = by * 4 + (int)(threadIdx.x >> 6);
   const int z0 = bz * JUDGE_Z, z1 = min(z0 + JUDGE_Z, Z);
-  const bool in = x < X && y < Y;
-  const double INF = __builtin_inf();
+  // Lanes past the field's x / y end load the clamped column and judge
+  // nothing, so every load is in bounds and none needs a branch.
+  const int xc = min(x, X - 1), yc = min(y, Y - 1);
+  JudgeLane L;
+  L.in = x < X && y < Y;
   // FP32 fields hold fewer than 2^29 points: 32-bit index arithmetic
-  const uint32_t XY = (uint32_t)X * (uint32_t)Y;
-  uint32_t p = ((uint32_t)z0 * (uint32_t)Y + (uint32_t)y) * (uint32_t)X + (uint32_t)x;
+  L.XY = (uint32_t)X * (uint32_t)Y;
+  L.Z = Z;
+  L.col = (uint32_t)yc * (uint32_t)X + (uint32_t)xc;
+  // in-plane neighbours outside the field are +inf: their offset is 0 (the
+  // point itself is loaded) and the value is replaced
+  L.hxm = xc > 0;
+  L.hxp = xc < X - 1;
+  L.hym = yc > 0;
+  L.hyp = yc < Y - 1;
+  L.oxm = L.hxm ? 1u : 0u;
+  L.oxp = L.hxp ? 1u : 0u;
+  L.oym = L.hym ? (uint32_t)X : 0u;
+  L.oyp = L.hyp ? (uint32_t)X : 0u;
   const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
-  double zm = (in && z0 > 0) ? (double)v[p - XY] : INF;
-  double cur = in ? (double)v[p] : INF;
+  L.htol = hmin * 0x1.0p-21;
   unsigned long long nbad = 0;
-  for (int z = z0; z < z1; z++, p += XY) {
-    bool bad = false;
-    if (in) {
-      // neighbours outside the field are +inf; per axis the smaller of the
-      // two (FP32 values are exact in FP64, so the minimum commutes with the
-      // conversion)
-      const float FINF = __builtin_inff();
-      const double zp = (double)(z < Z - 1 ? v[p + XY] : FINF);
-      const float xm = x > 0 ? v[p - 1] : FINF, xp = x < X - 1 ? v[p + 1] : FINF;
-      const float ym = y > 0 ? v[p - X] : FINF, yp = y < Y - 1 ? v[p + X] : FINF;
-      const double mx = (double)__builtin_fminf(xm, xp), my = (double)__builtin_fminf(ym, yp);
-      const double mz = __builtin_fmin(zm, zp);
-      const double e =
-          ISO ? god_iso64_sel((double)init[p], mx, my, mz, dx, 2.0 * (dx * dx), 3.0 * (dx * dx),
-                              1.0 / 3.0)
-              : god_d64((double)init[p], mx, my, mz, dx, dy, dz);
-      if (__builtin_isinf(e) || __builtin_isinf(cur)) bad = !(e == cur);
-      else bad = !(__builtin_fabs(cur - e) <= __builtin_fabs(e) * 0x1.0p-22 + hmin * 0x1.0p-21);
-      zm = cur;
-      cur = zp;
-    }
-    if (bad && fd.flags) atomicOr(fd.flags + (p >> 5), 1u << (p & 31u));
-    nbad += (unsigned long long)__popcll(__ballot(bad));
+  // Two operand sets, alternately loaded one plane ahead of the plane being
+  // judged: the loads of plane z + 1 are in flight while plane z is solved.
+  JudgeOps A = judge_load(v, init, L, z0), B;
+  int z = z0;
+  for (; z + 1 < z1; z += 2) {
+    B = judge_load(v, init, L, z + 1);
+    nbad += judge_plane<ISO>(A, L, z, dx, dy, dz, fd.flags);
+    A = judge_load(v, init, L, z + 2);
+    nbad += judge_plane<ISO>(B, L, z + 1, dx, dy, dz, fd.flags);
   }
+  if (z < z1) nbad += judge_plane<ISO>(A, L, z, dx, dy, dz, fd.flags);
   if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(counts + f, nbad);
 }
 

@@ -113,26 +113,45 @@ def test_f32_converged_bitwise_and_vs_f64(ctx, dev, gen):
     assert st["judge_flagged"] == 0
 
 
-def test_judge_matches_oracle_and_flags_planted_defect(ctx, dev):
-    X, Y, Z = 40, 36, 28
+# Shapes cover the judge's tiling (64 x 4 columns x 32 planes, planes judged
+# in pairs): x and y past a tile's end, one z-tile of an even plane count,
+# three z-tiles whose last has an odd count, and anisotropic spacing.
+@pytest.mark.parametrize("X,Y,Z,h", [(40, 36, 28, (1.0, 1.0, 1.0)),
+                                     (70, 37, 75, (1.0, 1.0, 1.0)),
+                                     (33, 18, 45, (1.0, 0.7, 1.3))])
+def test_judge_matches_oracle_and_flags_planted_defect(ctx, dev, X, Y, Z, h):
     a = O.gen_spheres(X, Y, Z, seed=31, n=6, r=(2.0, 6.0))
     init32 = a.astype(np.float32)
     v = init32.copy()
-    O.solve_f32(v)
-    assert O.judge_f32(v, init32) == 0
-    # plant a too-low region: every point in a box lowered by 0.5 h
+    O.solve_f32(v, h=h)
+    assert O.judge_f32(v, init32, h=h) == 0
+    # plant too-low regions: every point in a box lowered by 0.5 h, a
+    # box across the first and last planes of z-tiles, and single points
+    # on the field's faces; one point raised to +inf
     bad = v.copy()
     bad[10:14, 10:15, 12:20] -= 0.5
-    n_orc, fl = O.judge_f32(bad, init32, want_flags=True)
+    bad[min(31, Z - 2):min(35, Z), 3:6, 20:24] -= 0.5
+    bad[1, 2, 3] -= 0.5
+    bad[Z, Y, X] -= 0.5
+    bad[Z // 2, 1, X] -= 0.5
+    bad[5, Y - 3, 7] = np.inf
+    n_orc, fl = O.judge_f32(bad, init32, h=h, want_flags=True)
     assert n_orc > 0
     tv = torch.from_numpy(np.ascontiguousarray(O.interior(bad))).to(dev)
     ti = torch.from_numpy(np.ascontiguousarray(O.interior(init32))).to(dev)
     words = (X * Y * Z + 31) // 32
     flags = torch.zeros(words, dtype=torch.int32, device=dev)
-    n_gpu = ctx.judge_f32(tv, ti, flags=flags)
+    n_gpu = ctx.judge_f32(tv, ti, h=h, flags=flags)
     assert n_gpu == n_orc
     bits = np.unpackbits(flags.cpu().numpy().view(np.uint8), bitorder="little")[: X * Y * Z]
     assert np.array_equal(bits.reshape(Z, Y, X), O.interior(fl))
+    # the converged field itself passes on the GPU as well
+    tv.copy_(torch.from_numpy(np.ascontiguousarray(O.interior(v))))
+    assert ctx.judge_f32(tv, ti, h=h) == 0
+    # a field with no source: +inf values, +inf solutions, nothing fails
+    tv.fill_(float("inf"))
+    ti.fill_(float("inf"))
+    assert ctx.judge_f32(tv, ti, h=h) == 0
 
 
 @pytest.mark.parametrize("kind", [pdfs_amd.GEN_POINT, pdfs_amd.GEN_TERRAIN, pdfs_amd.GEN_SPHERES])

@@ -99,13 +99,16 @@ def main():
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step passes={st['passes']}")
 
     # roofline of the dominant kernel (column sweep): per-launch HIP events on
-    # the solver's stream, one profiled solve. Every ordering of every pass is
-    # a launch; once the field is at its fixed point the remaining launches
-    # skip its columns, so the algorithmic bytes per launch are the executed
-    # point-updates' 8 B (FP32 read + write) averaged over the same launches
-    # as the time (and as rocprofv3's per-dispatch average).
+    # the solver's stream, one profiled solve. Every ordering of a pass that
+    # runs is a launch. The field reaches its fixed point at its last swept
+    # ordering (orderings counts the swept ones); the rest of that pass's
+    # launches skip its columns, and the plain stop's pass after it is
+    # counted but not launched: 8 * ceil(orderings / 8) launches. The
+    # algorithmic bytes per launch are the executed point-updates' 8 B (FP32
+    # read + write) averaged over the same launches as the time (and as
+    # rocprofv3's per-dispatch average).
     pst = step(profile=1)
-    launches = 8 * pst["passes"]
+    launches = 8 * ((pst["orderings"] + 7) // 8)
     avg_launch_ms = pst["ms_sweep_kernels"] / launches
     algo_bytes = 8.0 * pst["point_updates"] / launches
     achieved = algo_bytes / (avg_launch_ms * 1e-3) / 1e9

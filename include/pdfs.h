@@ -85,10 +85,11 @@ typedef struct {
 
 typedef struct {
   /* FP32 path: a field converges on a pass that changes nothing. Once one
-   * ordering changes nothing the field is at the fixed point, and the
-   * device skips its remaining orderings (the rest of that pass and the
-   * whole next one, which changes nothing); passes and the result are
-   * those of sweeping every ordering. orderings counts the orderings swept (for a single field without the
+   * ordering changes nothing the field is at the fixed point: the device
+   * skips its remaining orderings in that pass, and the next pass, which
+   * would change nothing, is counted but not launched; passes,
+   * last_changed and the result are those of sweeping every ordering.
+   * orderings counts the orderings swept (for a single field without the
    * skipped ones; for a batch, the launches); point_updates excludes skipped
    * orderings. */
   int passes, orderings;

@@ -679,6 +679,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
   int nactive = nf;
   double ms = 0.0, kms = 0.0;
   long long last_total = 0;
+  int counted_passes = 0; /* passes counted through a fixed point, not run */
   rc = PDFS_OK;
   HIPGO(hipEventRecord(c->ev0, c->st));
   /* A certified solve judges each field at the end of every pass from
@@ -771,6 +772,17 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
           if (skipped > nord) skipped = nord;
           s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
           if (nf == 1) s.orderings -= (int)skipped;
+          /* A pass that ends in an unchanged ordering left the field at its
+           * fixed point, so the next pass would change nothing: the plain
+           * stop's pass is counted (passes, and a last pass without change)
+           * but not launched. */
+          if (!jdone[f] && last_seg && o->fixed_passes <= 0 && run >= PDFSK_CONVERGED_RUN &&
+              p + 2 <= maxp) {
+            jdone[f] = 1;
+            last_total -= pdel[f];
+            if (field_passes) field_passes[f] = p + 2;
+            if (p + 2 > counted_passes) counted_passes = p + 2;
+          }
         }
         const int near = last_seg || (lastfl[f] >= 0 && lastfl[f] * 1024 <=
                                                    (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2]);
@@ -815,6 +827,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
     }
     if (rc) break;
   }
+  if (counted_passes > s.passes) s.passes = counted_passes;
   s.tile = c->tile;
   s.last_changed = last_total;
   s.ms_sweeps = ms - s.ms_judge;

@@ -376,15 +376,14 @@ def test_sweep_sqrt_is_correctly_rounded_everywhere(ctx):
 @pytest.mark.parametrize("tile", [-1, "auto"])
 def test_converged_field_skips_rest_of_last_pass(ctx, dev, tile):
     """A field is at the fixed point after its first ordering that changes
-    nothing: the device skips the rest of that pass and the pass that
-    confirms it. The result equals the one swept ordering by ordering, the
+    nothing: the device skips the rest of that pass, and the pass that
+    would confirm it is counted but not launched. The result equals the one swept ordering by ordering, the
     pass count is the plain count, and the stats count only swept orderings
     (point_updates = orderings x points). With the online tile tuner the
     first pass sweeps its orderings with six different tiles."""
     tile = pdfs_amd.TILE_AUTO if tile == "auto" else tile
     X, Y, Z = 72, 40, 56
-    init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
-    ctx.generate(pdfs_amd.GEN_TERRAIN, 2024, X, Y, Z, 1.0, dense=init)
+    init = _dense(O.gen_spheres(X, Y, Z, seed=2024, n=4, r=(2.0, 5.0)), dev)
     # reference: single orderings until a whole pass changes nothing
     ref = init.clone()
     runs, run, passes = [], 0, 0
@@ -400,17 +399,29 @@ def test_converged_field_skips_rest_of_last_pass(ctx, dev, tile):
     # the first ordering that changed nothing; every later one changes nothing
     first = next(i for i, r in enumerate(runs) if r >= 1) + 1
     assert all(r >= 1 for r in runs[first - 1:])
+    assert first > 8 and first % 8 != 0 and passes == (first + 7) // 8 + 1  # mid-pass fixed point
     f = init.clone()
     st = ctx.solve_f32(f, init, judge=0, tile=tile)
     assert torch.equal(f.view(torch.int32), ref.view(torch.int32))
     assert st["passes"] == passes
     assert st["orderings"] == first
     assert st["point_updates"] == st["orderings"] * X * Y * Z
+    assert st["last_changed"] == 0
+    # In a batch of two, orderings counts launches: every pass up to the one
+    # that reached the fixed point. The plain stop's pass after it, which
+    # changes nothing, is counted in passes but not launched.
+    gs = [init.clone(), init.clone()]
+    sb, pb, _ = ctx.solve_batch_f32(gs, None, judge=0, tile=tile)
+    assert pb == [passes, passes] and sb["passes"] == passes and sb["last_changed"] == 0
+    assert sb["orderings"] == 8 * ((first + 7) // 8)
+    for g in gs:
+        assert torch.equal(g.view(torch.int32), ref.view(torch.int32))
 
 
 def test_batch_with_single_point_field(ctx, dev):
     fields = [O.gen_point(1, 1, 1, seed=1), O.gen_spheres(40, 31, 22, seed=5, n=3, r=(2.0, 5.0)),
-              O.gen_point(2, 1, 3, seed=2), O.gen_terrain(17, 19, 15, seed=6)]
+              O.gen_point(2, 1, 3, seed=2),
+              O.gen_terrain(17, 19, 15, seed=6, amp=(1.0, 6.0), sig=(2.0, 9.0))]
     wants = []
     for a in fields:
         r = a.astype(np.float32)

@@ -148,7 +148,12 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
 int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
                            int z, double dx, double dy, double dz, int passes, double tol,
                            pdfs_stats *st);
-/* FP64 residual judge; flags: [ceil(x*y*z/32)] words or NULL. */
+/* FP64 residual judge: a point fails when |v - min(init, G64(v))| >
+ * |e| 2^-22 + hmin 2^-21 (e = min(init, G64(v)); equal infinities pass, an
+ * infinity against a finite value fails, and so does a point whose value or
+ * a neighbour's is NaN). flags: bit p of
+ * [ceil(x*y*z/32)] zeroed words is set for each failing point p, or NULL.
+ * *flagged = number of failing points. */
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags,
                        long long *flagged);

@@ -812,9 +812,10 @@ __global__ __launch_bounds__(256) void k_level_ref64(double *d, const double *__
 // ---------------------------------------------------------------------------
 // FP64 residual judge of a dense FP32 field (new). tol = |e| 2^-22 + hmin 2^-21.
 // A workgroup owns a 64 x 4 (x, y) tile of columns over JUDGE_Z planes; each
-// thread walks its column in z with the values of planes z-1, z, z+1 in
-// registers, so a point costs one new load of its own column plus its four
-// in-plane neighbours (cache hits) and its init, and no index division.
+// thread walks its column in z. A point's eight operands (its value, six
+// neighbours and init) are loaded while the plane before it is judged; of
+// them only the plane above and the init are new to the caches, and no
+// index division is needed.
 // Tiles are dealt XCD-contiguously: workgroups go to the 8 XCDs round robin
 // (workgroup b runs on XCD b % 8), so workgroup b takes tile
 // (b % 8) * per + b / 8 and each XCD sweeps one contiguous part of the field,

@@ -119,6 +119,10 @@ def main():
 
     extra = {"passes": st["passes"], "orderings": st["orderings"],
              "judge_flagged": int(st["judge_flagged"]), "ms_judge": round(pst["ms_judge"], 4),
+             # the FP64 judge reads the FP32 field and its init: 8 B per point
+             # (flag writes only for failing points), one run per plain solve
+             "judge_gbs": round(8.0 * N / (pst["ms_judge"] * 1e-3) / 1e9, 1)
+             if pst["judge_runs"] == 1 and pst["ms_judge"] > 0 else None,
              "sweep_pass_gbs": round(sweep_gbs_total, 1)}
     net = None
     if not args.no_warm:

@@ -335,6 +335,24 @@ def parity_leg(ctx, args):
     out["c1"] = {"grid": [X, Y, Z], "seed": 777, "gpu_f64_passes": r64["passes"],
                  "gpu_f64_seconds": round(t64, 3), "f32_passes": s32["passes"],
                  "f32_vs_f64_max_err": emax, "f32_vs_f64_rms_err": erms}
+    if os.path.exists(so) and not args.no_cpu:
+        # the reference core on one host core, two passes of the same init
+        # (the converged solve takes about a minute there), against two
+        # passes of the GPU's reference-arithmetic path: bitwise, and the CPU
+        # time per pass scaled to the passes the GPU's FP64 solve converged in
+        ref = torch.empty((Z + 2, Y + 2, X + 2), dtype=torch.float64, device="cuda:0")
+        ctx.generate_ex(pdfs_amd.GEN_TERRAIN, 777, X, Y, Z, 1.0, 8, (5.0, 40.0), (10.0, 60.0),
+                        padded=ref)
+        a = ref.cpu().numpy()
+        ctx.solve_ref64(ref, passes=2)
+        passes, sec = _ref_solve_worker((so, X, Y, Z, a, -1.0, 2))
+        out["c1"].update({"cpu_reference_passes_run": passes,
+                          "cpu_reference_seconds_per_pass": round(sec / passes, 3),
+                          "cpu_reference_converged_seconds_scaled": round(
+                              sec / passes * r64["passes"], 2),
+                          "bitwise_equal_after_2_passes": bool(np.array_equal(
+                              a.view(np.uint64), ref.cpu().numpy().view(np.uint64)))})
+        del ref, a
     log(f"[bench] parity: {out}")
     torch.cuda.empty_cache()
     return out
@@ -470,19 +488,23 @@ def batch_leg(ctx, nfields, net=None):
 
 
 def _ref_solve_worker(job):
-    """One field of the c4 CPU sample in a worker process: the reference
-    core (oracle/_ref) to a pass with max |delta| <= 1e-12."""
+    """One field on the reference core (oracle/_ref), in place: passes until
+    one has max |delta| <= tol (default 1e-12, at most 100 passes), or
+    exactly max_passes passes with tol < 0. Returns (passes, sweep seconds).
+    Runs in a worker process for the c4 CPU sample."""
     import ctypes as C
 
     import numpy as np
-    so, X, Y, Z, dist = job
+    so, X, Y, Z, dist = job[:5]
+    tol, max_passes = job[5:] if len(job) > 5 else (1e-12, 100)
     R = C.CDLL(so)
     dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
     R.ref_solve.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                             C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     R.ref_solve.restype = C.c_int
     sec, md = C.c_double(0), C.c_double(0)
-    passes = R.ref_solve(dist, X, Y, Z, 1.0, 1.0, 1.0, 1e-12, 100, C.byref(sec), C.byref(md))
+    passes = R.ref_solve(dist, X, Y, Z, 1.0, 1.0, 1.0, tol, max_passes, C.byref(sec),
+                         C.byref(md))
     return passes, sec.value
 
 

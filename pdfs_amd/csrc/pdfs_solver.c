@@ -55,6 +55,14 @@ static int fail(int code, const char *fmt, ...)
   return code;
 }
 
+/* spacings of the status-returning entry points: finite and positive (the
+ * comparisons are false for NaN) */
+static int bad_h(double dx, double dy, double dz)
+{
+  return !(dx > 0.0 && dx < HUGE_VAL) || !(dy > 0.0 && dy < HUGE_VAL) ||
+         !(dz > 0.0 && dz < HUGE_VAL);
+}
+
 int pdfsi_fail(int code, const char *fmt, ...)
 {
   va_list ap;
@@ -502,7 +510,7 @@ static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
 int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx, double dy,
                        double dz, int sn, int tile, long long *changed)
 {
-  if (!c || !field || x < 1 || y < 1 || z < 1 || sn < 1 || sn > 8)
+  if (!c || !field || x < 1 || y < 1 || z < 1 || sn < 1 || sn > 8 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_sweep_f32_dev: bad arguments");
   if (tile < 0 || tile >= PDFSK_TILE_COUNT) tile = default_tile(1, x, y);
   pdfs_sp32 sp;
@@ -592,7 +600,8 @@ int pdfs_selftest_sqrt_f32(pdfs_ctx *c, unsigned long long *mismatches)
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags, long long *flagged)
 {
-  if (!c || !field || !init) return fail(PDFS_EINVAL, "pdfs_judge_f32_dev: bad arguments");
+  if (!c || !field || !init || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_judge_f32_dev: bad arguments");
   const int dims[3] = {x, y, z};
   long long fl = 0;
   int rc = pdfsi_judge_batch(c, 1, &field, &init, dims, flags ? &flags : NULL, dx, dy, dz, &fl);
@@ -600,12 +609,27 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
   return rc;
 }
 
+/* The FP32 batch solve without the spacing check (run_fsm and the CLI take
+ * the caller's spacings as the reference does). */
+static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
+                     const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
+                     pdfs_stats *st, int *field_passes, double *field_ms);
+
 int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
                              const float *const *inits, const int *dims, double dx, double dy,
                              double dz, const pdfs_opts *o, pdfs_stats *st, int *field_passes,
                              double *field_ms)
 {
-  if (!c || nf < 1 || !fields || !dims || !(dx > 0) || !(dy > 0) || !(dz > 0))
+  if (!c || nf < 1 || !fields || !dims || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
+  return batch_f32(c, nf, fields, inits, dims, dx, dy, dz, o, st, field_passes, field_ms);
+}
+
+static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
+                     const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
+                     pdfs_stats *st, int *field_passes, double *field_ms)
+{
+  if (!c || nf < 1 || !fields || !dims)
     return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
   pdfs_opts dflt;
   if (!o) {
@@ -874,13 +898,22 @@ out:
   return rc;
 }
 
-int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
-                       double dx, double dy, double dz, const pdfs_opts *o, pdfs_stats *st)
+/* One FP32 field, without the spacing check. */
+static int single_f32(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
+                      double dx, double dy, double dz, const pdfs_opts *o, pdfs_stats *st)
 {
   if (!c || !field || x < 1 || y < 1 || z < 1)
     return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
   int dims[3] = {x, y, z};
-  return pdfs_solve_batch_f32_dev(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL);
+  return batch_f32(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL);
+}
+
+int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
+                       double dx, double dy, double dz, const pdfs_opts *o, pdfs_stats *st)
+{
+  if (!c || !field || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
+  return single_f32(c, field, init, x, y, z, dx, dy, dz, o, st);
 }
 
 static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, int y, int z,
@@ -921,7 +954,7 @@ static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, i
 int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx, double dy,
                          double dz, int passes, double tol, pdfs_stats *st)
 {
-  if (!c || !padded || x < 1 || y < 1 || z < 1)
+  if (!c || !padded || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_ref64_dev: bad arguments");
   return solve_ref64(c, padded, NULL, x, y, z, dx, dy, dz, passes, tol, st);
 }
@@ -930,7 +963,7 @@ int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, 
                            int z, double dx, double dy, double dz, int passes, double tol,
                            pdfs_stats *st)
 {
-  if (!c || !padded || !slowness || x < 1 || y < 1 || z < 1)
+  if (!c || !padded || !slowness || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_speed64_dev: bad arguments");
   return solve_ref64(c, padded, slowness, x, y, z, dx, dy, dz, passes, tol, st);
 }
@@ -944,10 +977,24 @@ static pdfs_ctx *default_ctx(void)
   return g_ctx;
 }
 
+/* pdfs_solve without the spacing check: run_fsm takes the spacings the
+ * caller's Phi holds, as the reference does */
+static int solve_host(double *distance, int x, int y, int z, double dx, double dy, double dz,
+                      const pdfs_opts *o, pdfs_stats *st);
+
 int pdfs_solve(double *distance, int x, int y, int z, double dx, double dy, double dz,
                const pdfs_opts *o, pdfs_stats *st)
 {
-  if (!distance || x < 1 || y < 1 || z < 1) return fail(PDFS_EINVAL, "pdfs_solve: bad arguments");
+  if (!distance || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_solve: bad arguments");
+  return solve_host(distance, x, y, z, dx, dy, dz, o, st);
+}
+
+static int solve_host(double *distance, int x, int y, int z, double dx, double dy, double dz,
+                      const pdfs_opts *o, pdfs_stats *st)
+{
+  if (!distance || x < 1 || y < 1 || z < 1)
+    return fail(PDFS_EINVAL, "pdfs_solve: bad arguments");
   pdfs_opts dflt;
   if (!o) {
     pdfs_default_opts(&dflt);
@@ -965,8 +1012,8 @@ int pdfs_solve(double *distance, int x, int y, int z, double dx, double dy, doub
     goto out;
   }
   if (o->precision == PDFS_F64_REF) {
-    rc = pdfs_solve_ref64_dev(c, d, x, y, z, dx, dy, dz,
-                              o->fixed_passes > 0 ? o->fixed_passes : 0, o->tol, st);
+    rc = solve_ref64(c, d, NULL, x, y, z, dx, dy, dz, o->fixed_passes > 0 ? o->fixed_passes : 0,
+                     o->tol, st);
   } else {
     if (hipMalloc((void **)&f, n * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&f0, n * sizeof(float)) != hipSuccess) {
@@ -978,7 +1025,7 @@ int pdfs_solve(double *distance, int x, int y, int z, double dx, double dy, doub
       rc = fail(PDFS_EHIP, "pdfs_solve: D2D copy");
       goto out;
     }
-    rc = pdfs_solve_f32_dev(c, f, f0, x, y, z, dx, dy, dz, o, st);
+    rc = single_f32(c, f, f0, x, y, z, dx, dy, dz, o, st);
     if (rc && rc != PDFS_ENOTCONV) goto out;
     int rc2 = pdfs_unpack_dev(c, f, d, x, y, z);
     if (rc2) {
@@ -1011,7 +1058,7 @@ void run_fsm(Phi *pf, int iterations)
   hipEventCreate(&a);
   hipEventCreate(&b);
   hipEventRecord(a, 0);
-  int rc = pdfs_solve(pf->distance, pf->x, pf->y, pf->z, pf->dx, pf->dy, pf->dz, &o, &s);
+  int rc = solve_host(pf->distance, pf->x, pf->y, pf->z, pf->dx, pf->dy, pf->dz, &o, &s);
   hipEventRecord(b, 0);
   hipEventSynchronize(b);
   if (rc != PDFS_OK) {
@@ -1158,7 +1205,7 @@ int pdfs_calc_dist_field(Phi *pf, int passes, int precision)
     goto out;
   }
   if (precision == PDFS_F64_REF) {
-    if ((rc = pdfs_solve_ref64_dev(c, d, x, y, z, pf->dx, pf->dy, pf->dz, passes, 1e-12, NULL)))
+    if ((rc = solve_ref64(c, d, NULL, x, y, z, pf->dx, pf->dy, pf->dz, passes, 1e-12, NULL)))
       goto out;
   } else {
     pdfs_opts o;
@@ -1170,7 +1217,7 @@ int pdfs_calc_dist_field(Phi *pf, int passes, int precision)
       goto out;
     }
     if ((rc = pdfs_pack_dev(c, d, f, x, y, z))) goto out;
-    rc = pdfs_solve_f32_dev(c, f, NULL, x, y, z, pf->dx, pf->dy, pf->dz, &o, NULL);
+    rc = single_f32(c, f, NULL, x, y, z, pf->dx, pf->dy, pf->dz, &o, NULL);
     if (rc && rc != PDFS_ENOTCONV) goto out;
     if ((rc = pdfs_unpack_dev(c, f, d, x, y, z))) goto out;
   }

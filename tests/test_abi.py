@@ -57,3 +57,47 @@ def test_phi_struct_layouts_match_reference():
     assert pdfs_amd.Phi.distance.offset == 48 and pdfs_amd.Phi.location.offset == 40
     assert ctypes.sizeof(pdfs_amd.PhiACC) == 56
     assert pdfs_amd.PhiACC.distance.offset == 40
+
+
+def test_entry_points_reject_bad_arguments_before_touching_the_gpu():
+    """Argument checks come first in every status-returning entry point:
+    PDFS_EINVAL (-1) and a message naming the call, with no context, no
+    device memory and no GPU. Spacings must be finite and positive (the
+    batch, single-field, FP64 and judge paths alike)."""
+    import math
+
+    import numpy as np
+    C = ctypes
+    L = pdfs_amd.lib()
+    st = pdfs_amd.Stats()
+    opts = pdfs_amd.Opts()
+    L.pdfs_default_opts(C.byref(opts))
+
+    def err():
+        return L.pdfs_last_error().decode()
+
+    a = np.zeros((6, 6, 6))
+    for h in ((0.0, 1.0, 1.0), (1.0, -2.0, 1.0), (1.0, 1.0, math.nan), (math.inf, 1.0, 1.0)):
+        assert L.pdfs_solve(a.ctypes.data, 4, 4, 4, *h, C.byref(opts), C.byref(st)) == -1, h
+        assert "pdfs_solve" in err()
+    assert L.pdfs_solve(None, 4, 4, 4, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st)) == -1
+    assert L.pdfs_solve(a.ctypes.data, 0, 4, 4, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st)) == -1
+    calls = [
+        ("pdfs_solve_f32_dev", lambda: L.pdfs_solve_f32_dev(None, None, None, 4, 4, 4, 1.0, 1.0,
+                                                            1.0, C.byref(opts), C.byref(st))),
+        ("pdfs_sweep_f32_dev", lambda: L.pdfs_sweep_f32_dev(None, None, 4, 4, 4, 1.0, 1.0, 1.0,
+                                                            1, -1, None)),
+        ("pdfs_solve_ref64_dev", lambda: L.pdfs_solve_ref64_dev(None, None, 4, 4, 4, 1.0, 1.0,
+                                                                1.0, 1, 0.0, C.byref(st))),
+        ("pdfs_solve_speed64_dev", lambda: L.pdfs_solve_speed64_dev(
+            None, None, None, 4, 4, 4, 1.0, 1.0, 1.0, 1, 0.0, C.byref(st))),
+        ("pdfs_judge_f32_dev", lambda: L.pdfs_judge_f32_dev(None, None, None, 4, 4, 4, 1.0, 1.0,
+                                                            1.0, None, None)),
+        ("pdfs_solve_batch_f32_dev", lambda: L.pdfs_solve_batch_f32_dev(
+            None, 1, None, None, None, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st), None, None)),
+        ("generate", lambda: L.pdfs_generate_dev(None, pdfs_amd.GEN_POINT, 1, 4, 4, 4, 1.0, None,
+                                                 None)),
+    ]
+    for name, call in calls:
+        assert call() == -1, name
+        assert name in err(), (name, err())

@@ -9,6 +9,9 @@
 /* set the thread's last-error message (pdfs_last_error) and return code */
 int pdfsi_fail(int code, const char *fmt, ...);
 
+/* nonzero unless every spacing is finite and positive */
+int pdfsi_bad_h(double dx, double dy, double dz);
+
 /* FP64 residual judge of n dense FP32 fields (dims[3i..3i+2] = x, y, z) in
  * one launch on the context's stream; flagged[i] = failing points of field
  * i; flags[i] (optional, may be NULL) receive one bit per point. */

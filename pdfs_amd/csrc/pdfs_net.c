@@ -609,7 +609,7 @@ int pdfs_net_target_dev(pdfs_net *n, const float *u, int x, int y, int z, float 
 int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int y, int z,
                             double hmax)
 {
-  if (!n || !init || !w || x < 1 || y < 1 || z < 1)
+  if (!n || !init || !w || x < 1 || y < 1 || z < 1 || pdfsi_bad_h(hmax, hmax, hmax))
     return pdfsi_fail(PDFS_EINVAL, "pdfs_warm_start_f32_dev: bad arguments");
   const int Dz = coarse(z), Dy = coarse(y), Dx = coarse(x);
   const long long V0 = (long long)Dz * Dy * Dx;
@@ -636,7 +636,7 @@ int pdfs_warm_start_f32_dev(pdfs_net *n, const float *init, float *w, int x, int
 int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits,
                                   float *const *ws, const int *dims, double hmax)
 {
-  if (!n || nf < 1 || !inits || !ws || !dims)
+  if (!n || nf < 1 || !inits || !ws || !dims || pdfsi_bad_h(hmax, hmax, hmax))
     return pdfsi_fail(PDFS_EINVAL, "pdfs_warm_start_batch_f32_dev: bad arguments");
   const size_t segb = 4 * (size_t)(nf + 1) * sizeof(pdfsnn_seg);
   const size_t need = segb + (size_t)(nf + 1) * sizeof(pdfsnn_fld);
@@ -727,7 +727,7 @@ int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float 
                             int y, int z, double dx, double dy, double dz, const pdfs_opts *o,
                             pdfs_stats *st)
 {
-  if (!c || !n || !field || !init || x < 1 || y < 1 || z < 1)
+  if (!c || !n || !field || !init || x < 1 || y < 1 || z < 1 || pdfsi_bad_h(dx, dy, dz))
     return pdfsi_fail(PDFS_EINVAL, "pdfs_solve_warm_f32_dev: bad arguments");
   pdfs_opts oo;
   if (o)
@@ -789,7 +789,7 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
                                   double dy, double dz, const pdfs_opts *o, pdfs_stats *st,
                                   int *field_passes, int *field_fallback)
 {
-  if (!c || !n || nf < 1 || !fields || !inits || !dims)
+  if (!c || !n || nf < 1 || !fields || !inits || !dims || pdfsi_bad_h(dx, dy, dz))
     return pdfsi_fail(PDFS_EINVAL, "pdfs_solve_batch_warm_f32_dev: bad arguments");
   pdfs_opts oo;
   if (o)

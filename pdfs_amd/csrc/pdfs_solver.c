@@ -63,6 +63,8 @@ static int bad_h(double dx, double dy, double dz)
          !(dz > 0.0 && dz < HUGE_VAL);
 }
 
+int pdfsi_bad_h(double dx, double dy, double dz) { return bad_h(dx, dy, dz); }
+
 int pdfsi_fail(int code, const char *fmt, ...)
 {
   va_list ap;

@@ -97,6 +97,10 @@ def test_entry_points_reject_bad_arguments_before_touching_the_gpu():
             None, 1, None, None, None, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st), None, None)),
         ("generate", lambda: L.pdfs_generate_dev(None, pdfs_amd.GEN_POINT, 1, 4, 4, 4, 1.0, None,
                                                  None)),
+        ("pdfs_warm_start_f32_dev", lambda: L.pdfs_warm_start_f32_dev(None, None, None, 4, 4, 4,
+                                                                      1.0)),
+        ("pdfs_solve_warm_f32_dev", lambda: L.pdfs_solve_warm_f32_dev(
+            None, None, None, None, 4, 4, 4, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st))),
     ]
     for name, call in calls:
         assert call() == -1, name

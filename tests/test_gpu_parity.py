@@ -558,3 +558,63 @@ def test_certified_batch_retires_fields_mid_pass(ctx, dev):
         assert torch.equal(f.view(torch.int32), fields[i].view(torch.int32)), i
     del inits, fields, t
     torch.cuda.empty_cache()
+
+
+def test_two_contexts_in_two_threads(dev):
+    """One context per host thread: two threads solve at the same time, each
+    on its own context and stream (a single field in one, a batch of two in
+    the other, twice each); every result equals the same solve run alone, bit
+    for bit, and both judges pass."""
+    import threading
+
+    shapes = [(64, 48, 40), (56, 72, 36), (40, 40, 64)]
+    inits = [_dense(O.gen_spheres(X, Y, Z, seed=600 + i, n=5, r=(2.0, 6.0)), dev)
+             for i, (X, Y, Z) in enumerate(shapes)]
+    alone = pdfs_amd.Context()
+    want = []
+    for t in inits:
+        f = t.clone()
+        alone.solve_f32(f, t)
+        want.append(f)
+    alone.close()
+    results, errors = {}, []
+
+    def single():
+        c = pdfs_amd.Context()
+        try:
+            for k in range(2):
+                f = inits[0].clone()
+                st = c.solve_f32(f, inits[0])
+                results[("single", k)] = (f, st["judge_flagged"])
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+        finally:
+            c.close()
+
+    def batch():
+        c = pdfs_amd.Context()
+        try:
+            for k in range(2):
+                fs = [inits[1].clone(), inits[2].clone()]
+                st, _, _ = c.solve_batch_f32(fs, [inits[1], inits[2]])
+                results[("batch", k)] = (fs, st["judge_flagged"])
+        except Exception as e:
+            errors.append(e)
+        finally:
+            c.close()
+
+    th = [threading.Thread(target=single), threading.Thread(target=batch)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errors, errors
+    torch.cuda.synchronize()
+    for k in range(2):
+        f, fl = results[("single", k)]
+        assert fl == 0 and torch.equal(f.view(torch.int32), want[0].view(torch.int32))
+        fs, fl = results[("batch", k)]
+        assert fl == 0
+        for g, w in zip(fs, want[1:]):
+            assert torch.equal(g.view(torch.int32), w.view(torch.int32))

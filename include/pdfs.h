@@ -46,10 +46,15 @@ typedef struct {
 
 /* Reference semantics: exactly `iterations` passes of orderings sn = 1..8 in
  * the reference's order, FP64, reference arithmetic, in place; ghost cells
- * never written. Extension: iterations <= 0 runs passes until no value
- * changes by more than 1e-12. Errors print a message and exit(EXIT_FAILURE)
- * as cudaCheckError does (single_gpu_cuda/src/phi3D_fsm.h:39-46). Prints
- * "Parallel FSM time: %f s." (phi3D_fsm.cu:104). */
+ * never written. Extension (a departure from the reference, which runs zero
+ * passes for iterations <= 0, phi3D_fsm.cu:143): iterations <= 0 runs passes
+ * until no value changes by more than 1e-12, at most 1000. Errors, and no
+ * convergence within 1000 passes, print a message and exit(EXIT_FAILURE) as
+ * cudaCheckError does (single_gpu_cuda/src/phi3D_fsm.h:39-46). Prints
+ * "Parallel FSM time: %f s." (phi3D_fsm.cu:104). The sweeps run on the FP64
+ * column pipeline when the ghost layer is all +inf (the reference's input)
+ * and the field has fewer than 2^28 points, else on one launch per
+ * hyperplane; both are bit-identical to the reference. */
 void run_fsm(Phi *pf, int iterations);
 /* run_fsm, runFSM, pdfs_solve and pdfs_calc_dist_field share one
  * process-wide context (created on first use, kept until exit): like the
@@ -81,6 +86,15 @@ typedef struct {
   int certify;     /* FP32: also stop a field once the FP64 residual judge passes on it */
   int certify_from;/* first pass after which the judge is run (default 1); from then on it
                       also runs after every half pass (orderings 4 and 8) */
+  int polish;      /* FP32 (converged solves of fields with an init; default 1): after the
+                      FP32 stop, FP64 passes of u = min(init, G64(u)) from the FP32 field
+                      until a pass changes no value by more than polish_tol * hmin, then
+                      the result is rounded to FP32. FP32 storage rounding accumulates
+                      along characteristics (at 512^3 up to ~3e-4 h); the polish brings
+                      the result to the FP64 solution's FP32 rounding. 0: the FP32
+                      fixed point itself. */
+  double polish_tol; /* default 1e-5 */
+  int tile64;      /* FP64 column variant (16-wide tiles: 0 16x16, 2 16x8, 4, 5); -1 default */
 } pdfs_opts;
 
 typedef struct {
@@ -103,6 +117,12 @@ typedef struct {
   int judge_runs;            /* judge launches (certification checks + final) */
   int fallbacks;             /* warm start: regions reset to the init and re-solved */
   double ms_warm;            /* warm start: network + upsample time */
+  int polish_passes;         /* FP64 polish passes (counted as the FP32 passes are) */
+  int polish_orderings;      /* FP64 polish orderings launched */
+  double polish_maxdiff;     /* largest change of the last polish pass */
+  double ms_polish;          /* polish time, conversions included */
+  int polish_skipped;        /* fields left unpolished: 2^28 points or more (FP64 column
+                                limit) */
 } pdfs_stats;
 
 void pdfs_default_opts(pdfs_opts *o);
@@ -136,7 +156,7 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nfields, float *const *fields,
 int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx, double dy,
                        double dz, int sn, int tile, long long *changed);
 /* Padded FP64 field, reference arithmetic, `passes` passes (<= 0: until
- * max |delta| <= tol), in place. */
+ * max |delta| <= tol, at most 1000 passes, else PDFS_ENOTCONV), in place. */
 int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, double dx,
                          double dy, double dz, int passes, double tol, pdfs_stats *st);
 /* Spatially varying speed (no counterpart in the reference, which declares
@@ -153,7 +173,8 @@ int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, 
  * infinity against a finite value fails, and so does a point whose value or
  * a neighbour's is NaN). flags: bit p of
  * [ceil(x*y*z/32)] zeroed words is set for each failing point p, or NULL.
- * *flagged = number of failing points. */
+ * *flagged = number of failing points. Fields of 2^29 points or more are
+ * rejected with PDFS_EINVAL (the FP32 field limit). */
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags,
                        long long *flagged);

@@ -323,6 +323,64 @@ void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
   LEX_SWEEP_BODY(double, orc_godunov_d64)
 }
 
+/* FP64 polish (new; no reference counterpart). A Gauss-Seidel step of the */
+/* fixed-point map u = min(init, G64(u)) in ordering sn's octant order, the  */
+/* local solve being the difference form (orc_godunov_d64) applied to the   */
+/* point's init instead of its current value: unlike the reference's        */
+/* min(u, G(u)) (mpi_openacc/src/phi3D_fsm.c:180) it can raise a value, so  */
+/* it converges to the FP64 solution from an FP32 solution that lies below  */
+/* it in places. Neighbour minima in the reference's order (lower index     */
+/* first, mpi_openacc/src/phi3D_fsm.c:138-140). Returns the largest         */
+/* |old - new| of any update (a NaN change counts as +inf).                 */
+/* ------------------------------------------------------------------------ */
+double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, double dx,
+                          double dy, double dz, int sn)
+{
+  int sx, sy, sz;
+  order_dirs(sn, &sx, &sy, &sz);
+  size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
+  double md = 0.0;
+  for (int a = 1; a <= Z; a++) {
+    int i = sz > 0 ? a : Z + 1 - a;
+    for (int b = 1; b <= Y; b++) {
+      int j = sy > 0 ? b : Y + 1 - b;
+      for (int c = 1; c <= X; c++) {
+        int k = sx > 0 ? c : X + 1 - c;
+        size_t id = idx3(X, Y, i, j, k);
+        double mx = ORC_MIN(d[id - 1], d[id + 1]);
+        double my = ORC_MIN(d[id - extX], d[id + extX]);
+        double mz = ORC_MIN(d[id - extXY], d[id + extXY]);
+        double u = orc_godunov_d64(init[id], mx, my, mz, dx, dy, dz);
+        if (u != d[id]) {
+          double df = fabs(d[id] - u);
+          if (df != df) df = INFINITY;
+          if (df > md) md = df;
+        }
+        d[id] = u;
+      }
+    }
+  }
+  return md;
+}
+
+int orc_polish64(double *d, const double *init, int X, int Y, int Z, double dx, double dy,
+                 double dz, double tol, int max_passes, double *last_maxdiff)
+{
+  int passes = 0;
+  double md = 0.0;
+  while (passes < max_passes) {
+    md = 0.0;
+    for (int sn = 1; sn <= 8; sn++) {
+      double m = orc_sweep_polish64(d, init, X, Y, Z, dx, dy, dz, sn);
+      if (m > md) md = m;
+    }
+    passes++;
+    if (md <= tol) break;
+  }
+  if (last_maxdiff) *last_maxdiff = md;
+  return passes;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Convergence loops (new: the reference runs exactly one pass,             */
 /* single_gpu_cuda/src/phi3D.c:118, mpi_openacc/src/block_fsm.c:296).       */

@@ -33,6 +33,12 @@
  *                        terrain column heights) and determinism.
  *   orc_judge_f32        planted defects (values below truth) are flagged;
  *                        a converged field passes.
+ *   orc_sweep_polish64 / orc_polish64
+ *                        its limit equals the monotone FP64 solve's
+ *                        (orc_solve_d64) to 1e-12; the exact fixed point is
+ *                        left unchanged; a region planted 0.5 h below the
+ *                        solution is raised back (a planted min(u, G)
+ *                        update fails this); point-source axis values n h.
  */
 #ifndef PDFS_FSM_ORACLE_H
 #define PDFS_FSM_ORACLE_H
@@ -61,6 +67,15 @@ void orc_sweep_f32(float *d, int X, int Y, int Z, double dx, double dy,
                    double dz, int sn);
 void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
                    double dz, int sn);
+
+/* FP64 polish: one ordering of u = min(init, G64(u)) (difference form; may
+ * raise values), returning the largest change of any update; and passes of
+ * it until a pass's largest change is <= tol (at most max_passes), returning
+ * the passes run. init: padded FP64 (ghosts unused). */
+double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, double dx,
+                          double dy, double dz, int sn);
+int orc_polish64(double *d, const double *init, int X, int Y, int Z, double dx, double dy,
+                 double dz, double tol, int max_passes, double *last_maxdiff);
 
 /* Passes of sn = 1..8 until converged (or max_passes). Return passes run. */
 int orc_solve_ref64(double *d, int X, int Y, int Z, double dx, double dy,

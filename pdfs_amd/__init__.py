@@ -119,7 +119,8 @@ class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_passes", C.c_int), ("fixed_passes", C.c_int),
                 ("tol", C.c_double), ("judge", C.c_int), ("tile", C.c_int),
                 ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int),
-                ("certify", C.c_int), ("certify_from", C.c_int)]
+                ("certify", C.c_int), ("certify_from", C.c_int), ("polish", C.c_int),
+                ("polish_tol", C.c_double), ("tile64", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -128,7 +129,9 @@ class Stats(C.Structure):
                 ("judge_flagged", C.c_longlong), ("ms_sweeps", C.c_double),
                 ("ms_judge", C.c_double), ("ms_total", C.c_double),
                 ("ms_sweep_kernels", C.c_double), ("tile", C.c_int), ("judge_runs", C.c_int),
-                ("fallbacks", C.c_int), ("ms_warm", C.c_double)]
+                ("fallbacks", C.c_int), ("ms_warm", C.c_double), ("polish_passes", C.c_int),
+                ("polish_orderings", C.c_int), ("polish_maxdiff", C.c_double),
+                ("ms_polish", C.c_double), ("polish_skipped", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -19,4 +19,19 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
                       const int *dims, uint32_t *const *flags, double dx, double dy, double dz,
                       long long *flagged);
 
+/* FP64 polish of FP32 fields with an init (those without are left alone):
+ * passes of u = min(init, G64(u)) until a pass changes no value by more than
+ * o->polish_tol * hmin, the result rounded back to FP32; adds to
+ * s->polish_* and s->ms_polish. */
+int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
+                 const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
+                 pdfs_stats *s);
+
+/* pdfs_solve_batch_f32_dev with per-field outcomes: field_flagged[f] = the
+ * final judge's failing points of field f (0 when certified during the
+ * passes or without an init), -1 when the field did not converge. */
+int pdfsi_batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
+                    const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
+                    pdfs_stats *st, int *field_passes, long long *field_flagged);
+
 #endif

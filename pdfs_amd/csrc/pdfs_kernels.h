@@ -24,6 +24,25 @@ typedef struct {
   int iso;
 } pdfs_sp32;
 
+/* FP64 spacing constants of the difference form, computed on the host in
+ * the oracle's operation order (h2 = h h, 2 h2, 3 h2, 1/3; w = 1/d^2,
+ * W = w0 + w1 + w2). */
+typedef struct {
+  double d[3], w[3], W;
+  double h, two_h2, three_h2, third;
+  int iso;
+} pdfs_sp64;
+
+/* Sweep modes of the column pipeline. */
+enum {
+  PDFSK_MODE_F32_ISO = 0,      /* FP32 difference form, u = min(u, G(u)) */
+  PDFSK_MODE_F32_ANISO = 1,
+  PDFSK_MODE_F64_REF = 2,      /* FP64 reference arithmetic, u = min(u, G(u)) */
+  PDFSK_MODE_F64_POL_ISO = 3,  /* FP64 difference form, u = min(init, G(u)) */
+  PDFSK_MODE_F64_POL_ANISO = 4,
+  PDFSK_MODE_COUNT = 5
+};
+
 /* Entries of padding before and after each edge array: prefetch indices of
  * the column kernel stay inside the allocation. */
 #define PDFSK_EDGE_PAD 4096
@@ -36,12 +55,16 @@ typedef struct {
  * batch of one). Dense FP32 field [Z][Y][X]; its east/north edge blocks
  * start at entry ex_off / ey_off of the edge allocations. */
 typedef struct {
-  float *f;
+  void *f;            /* float (FP32 modes) or double (FP64 modes) */
+  const float *init;  /* polish modes: dense FP32 init */
   int X, Y, Z;
   int nTx, nTy;
   uint32_t ex_off, ey_off;
   int pad;
 } pdfsk_field;
+
+/* Largest dense FP64 field of the column kernel (x*y*z*8 < 2^31). */
+#define PDFSK_MAX_POINTS64 ((1ll << 31) / 8 - 1)
 
 /* Device state of the batched column sweep. state: [0] ticket counter,
  * [1] epoch, [2] ticket base, [3] finished workgroups, [4] column tickets;
@@ -55,6 +78,7 @@ typedef struct {
   unsigned *streak;      /* [nfields][2]: counter value after the last ordering, and the
                             number of consecutive orderings that changed nothing */
   int nfields;
+  unsigned long long *maxdiff; /* [nfields] FP64 modes: max |old - new| bits (accumulate) */
   unsigned *err;         /* bounded-spin timeout flag */
   uint64_t *edge_x, *edge_y;        /* face parts (index -PDFSK_EDGE_PAD) of one allocation
                                        that starts at edge_x; edge_y follows the x part */
@@ -80,13 +104,17 @@ enum {
 };
 void pdfsk_tile_dims(int variant, int *tx, int *ty);
 
-/* Resident workgroups of the column kernel for a tile variant (0 on error). */
-int pdfsk_col_max_resident(int variant, int iso);
+/* Whether a tile variant runs a sweep mode (FP64 modes: 16-wide tiles). */
+int pdfsk_tile_supports(int variant, int mode);
+/* Resident workgroups of the column kernel for a tile variant and mode (0 if
+ * unsupported or on error). */
+int pdfsk_col_max_resident(int variant, int mode);
 
 /* One ordering sn (1..8) over every field of the batch, in place; grid =
- * persistent workgroups (at most the resident count). */
-int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp32 *sp, int variant,
-                          int grid, hipStream_t st);
+ * persistent workgroups (at most the resident count). sp: FP32 constants
+ * (FP32 modes), sp64: FP64 constants (FP64 modes). */
+int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *sp,
+                      const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st);
 
 /* One ordering sn of the reference hyperplane schedule, FP64, on the padded
  * array, one launch per level (reference arithmetic, bitwise parity).
@@ -111,6 +139,23 @@ unsigned pdfsk_judge_tiles(int X, int Y, int Z);
  * the caller); tab on the device, ntiles = tab[nf].tile0 */
 int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long ntiles, double dx, double dy,
                           double dz, unsigned long long *counts, hipStream_t st);
+
+/* Padded FP64 <-> dense FP64 (interior; unpack leaves the ghosts alone). */
+int pdfsk_pack64(const double *padded, double *dense, int X, int Y, int Z, hipStream_t st);
+int pdfsk_unpack64(const double *dense, double *padded, int X, int Y, int Z, hipStream_t st);
+/* *count += ghost cells of a padded field whose bits are not +inf (count
+ * zeroed by the caller). */
+int pdfsk_count_ghost_not_inf(const double *padded, int X, int Y, int Z,
+                              unsigned long long *count, hipStream_t st);
+/* Element-wise conversions over the fields of a table, one launch:
+ * dir 0: dst64[i] = (double)src32[i]; dir 1: dst32[i] = (float)src64[i]
+ * (round to nearest even). */
+typedef struct {
+  float *f32;
+  double *f64;
+  long long n;
+} pdfsk_conv;
+int pdfsk_convert_batch(const pdfsk_conv *tab, int nf, long long nmax, int dir, hipStream_t st);
 
 /* Layout conversions: padded FP64 <-> dense FP32 and dense copies. */
 int pdfsk_pack_f64(const double *padded, float *dense, int X, int Y, int Z,

@@ -20,6 +20,7 @@
 // routine, so the FP32 and FP64 results match the CPU oracle bit for bit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "pdfs_kernels.h"
 
@@ -273,9 +274,10 @@ __device__ __forceinline__ uint64_t ld_edge(const uint64_t *p) {
 // constant clock). SLEEP is the pause between polls in units of 64 clocks;
 // the shared error flag is read on every 16th poll only, so a waiting lane
 // puts little traffic on the memory side. On timeout the error flag is
-// raised and the stale value used, so every wave still exits.
+// raised and the stale word used, so every wave still exits. Returns the
+// entry's 32-bit payload (an FP32 value, or one half of an FP64 value).
 template <int SLEEP>
-__device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsigned *err) {
+__device__ __noinline__ uint32_t wait_edge(const uint64_t *p, uint32_t epoch, unsigned *err) {
   const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
   uint64_t e = ld_edge(p);
   unsigned n = 0;
@@ -290,7 +292,7 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
       }
     }
   }
-  return __uint_as_float((uint32_t)e);
+  return (uint32_t)e;
 }
 
 // ---------------------------------------------------------------------------
@@ -306,7 +308,55 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
 #define PDFSK_GATE_EXTRA 2
 #endif
 
-// Column-pipeline sweep, FP32, one ordering.
+// Isotropic FP64 difference form with its own sort: the operations of
+// god_diff_iso<double> (the oracle's orc_godunov_d64 for equal spacings),
+// with the 1-D / 2-D / 3-D choice made by selects.
+__device__ __forceinline__ double god_iso64(double cur, double mx, double my, double mz, double h,
+                                            double two_h2, double three_h2, double third) {
+#pragma clang fp contract(off)
+  const double lo = mx < my ? mx : my;
+  const double hi = mx > my ? mx : my;
+  const double m0 = lo < mz ? lo : mz;
+  const double mid = lo > mz ? lo : mz;
+  const double m1 = hi < mid ? hi : mid;
+  const double m2 = hi > mid ? hi : mid;
+  return god_iso64_sorted(cur, m0, m1, m2, h, two_h2, three_h2, third);
+}
+
+// Sweep modes of the column pipeline (PDFSK_MODE_*):
+//   F32_ISO / F32_ANISO   FP32 difference form, u = min(u, G(u))
+//   F64_REF               FP64, the reference's arithmetic
+//                         (single_gpu_cuda/src/phi3D_fsm.cu:249-295),
+//                         u = min(u, G(u)): bit-identical to the reference
+//   F64_POL_ISO / _ANISO  FP64 difference form, u = min(init, G(u)): a
+//                         Gauss-Seidel step of the fixed-point map itself,
+//                         which may raise a value as well as lower it
+// FP64 values cross column boundaries as two 32-bit halves, each in its own
+// epoch-tagged entry (two edge-wave lanes per value).
+template <bool F64> struct ValT { typedef float T; };
+template <> struct ValT<true> { typedef double T; };
+template <int M> struct SMode {
+  static constexpr bool F64 = M >= PDFSK_MODE_F64_REF;
+  static constexpr bool POL = M >= PDFSK_MODE_F64_POL_ISO;
+  static constexpr bool ISO = M == PDFSK_MODE_F32_ISO || M == PDFSK_MODE_F64_POL_ISO;
+  static constexpr int W = F64 ? 2 : 1;  // 32-bit words per value
+  typedef typename ValT<F64>::T T;
+};
+
+__device__ __forceinline__ float buf_ld(float, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double buf_ld(double, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+}
+
+// Column-pipeline sweep, one ordering.
 //
 // Virtual coordinates make every ordering (+,+,+): virtual v in [1, dim]
 // maps to physical flip ? dim - v : v - 1. Workgroup = one TX x TY column
@@ -333,10 +383,12 @@ __device__ __noinline__ float wait_edge(const uint64_t *p, uint32_t epoch, unsig
 // lane branches.
 // ---------------------------------------------------------------------------
 // Field table entry (one per field of a batch; a single solve is a batch
-// of one). Dense FP32 field [Z][Y][X]; edge blocks start at ex_off / ey_off
-// (entries from the face parts' index 0, padding included).
+// of one). Dense field [Z][Y][X] of the mode's value type; polish modes read
+// the dense FP32 init; edge blocks start at ex_off / ey_off (entries from the
+// face parts' index 0, padding included).
 struct FieldDesc {
-  float *f;
+  void *f;
+  const float *init;
   int X, Y, Z;
   int nTx, nTy;
   uint32_t ex_off, ey_off;
@@ -358,41 +410,52 @@ struct ColArgs {
   uint32_t edge_x_bytes, edge_y_bytes;  // whole allocation from edge_x; y part alone
   unsigned *changed;  // per field
   unsigned *streak;   // per field: [2f] counter after the last ordering, [2f+1] unchanged run
+  unsigned long long *maxdiff;  // per field, FP64 modes: max |old - new| (bits) of any update
   int nfields;
   unsigned *err;
   pdfs_sp32 sp;
+  pdfs_sp64 sp64;
 };
 
 
-template <int TX, int TY>
+template <int TX, int TY, int W>
 struct ColGeom {
   static constexpr int NT = TX * TY;         // compute threads
   static constexpr int NB = NT + 64;         // plus the edge wave
   static constexpr int R = TX + TY + 2;      // ring depth (z-planes)
   static constexpr int PW = TX + 2;          // row pitch: x = -1 .. TX
   static constexpr int PA = PW * (TY + 2);   // rows y = -1 .. TY
-  static constexpr int PS = PA + ((2 - PA % 32) + 32) % 32;  // plane stride = 2 mod 32 banks
+  // plane stride in values: 2 mod 32 LDS banks (in 32-bit words)
+  static constexpr int PSW = PA * W + ((2 - (PA * W) % 32) + 32) % 32;
+  static constexpr int PS = PSW / W;
   static constexpr int DUMMY = R * PS;       // one dummy cell per thread
-  static constexpr int LDS_FLOATS = DUMMY + NB;
+  static constexpr int LDS_VALUES = DUMMY + NB;
   static constexpr int cell(int x, int y) { return (y + 1) * PW + x + 1; }
 };
 
-template <int TX, int TY, int U, bool ISO>
+template <int TX, int TY, int U, int M>
 __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
+  using S = SMode<M>;
+  using T = typename S::T;
+  constexpr int W = S::W;
   // queue depths: plane loads U steps ahead (HBM latency), edge entries UE
   // steps ahead; GATE = start lag beyond the inherent skew
   constexpr int UE = PDFSK_UE, GATE = PDFSK_UE + PDFSK_GATE_EXTRA;
-  using G = ColGeom<TX, TY>;
+  using G = ColGeom<TX, TY, W>;
   constexpr int NT = G::NT, R = G::R, PW = G::PW, PS = G::PS;
-  static_assert(NT % 64 == 0 && TX + TY <= 64, "compute waves and edge-wave lanes");
-  __shared__ float lds[G::LDS_FLOATS];
+  static_assert(NT % 64 == 0 && W * (TX + TY) <= 64, "compute waves and edge-wave lanes");
+  static_assert(G::PSW % W == 0, "plane stride in whole values");
+  __shared__ T lds[G::LDS_VALUES];
+  // polish: per-thread delay line of init values (plane vz in slot vz mod R)
+  __shared__ float iring[S::POL ? R * NT : 1];
   __shared__ int s_col;
   __shared__ unsigned s_changed;
-  float *const ring = lds;
+  T *const ring = lds;
+  uint32_t *const lds32 = (uint32_t *)lds;
 
   const int tid = threadIdx.x;
   const bool edge_wave = tid >= NT;  // wave-uniform: NT is a multiple of 64
-  const float INF = __builtin_inff();
+  const T INF = (T)__builtin_inff();
   const unsigned ncols = __hip_atomic_load(a.state + ST_NCOLS, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t epoch = __hip_atomic_load(a.state + ST_EPOCH, __ATOMIC_RELAXED,
@@ -441,7 +504,11 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       const uint64_t fp = (uint64_t)fd.f;
       const uint32_t flo = __builtin_amdgcn_readfirstlane((uint32_t)fp);
       const uint32_t fhi = __builtin_amdgcn_readfirstlane((uint32_t)(fp >> 32));
-      fd.f = (float *)(((uint64_t)fhi << 32) | flo);
+      fd.f = (void *)(((uint64_t)fhi << 32) | flo);
+      const uint64_t ip = (uint64_t)fd.init;
+      const uint32_t ilo = __builtin_amdgcn_readfirstlane((uint32_t)ip);
+      const uint32_t ihi = __builtin_amdgcn_readfirstlane((uint32_t)(ip >> 32));
+      fd.init = (const float *)(((uint64_t)ihi << 32) | ilo);
       fd.X = __builtin_amdgcn_readfirstlane(fd.X);
       fd.Y = __builtin_amdgcn_readfirstlane(fd.Y);
       fd.Z = __builtin_amdgcn_readfirstlane(fd.Z);
@@ -453,8 +520,9 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     const int NL = Z + TX + TY - 2;  // steps 0..NL; the last one only stores
     const int NS = Z + TX + TY - 1;  // steps (entry rows) per column
     // Edge entries, byte offsets from edge_x. A column's east-face block
-    // holds one row of TY entries per column step: the lane (TX-1, r) result
-    // of step l' is entry l' * TY + r (north face alike, TX entries per
+    // holds one row of TY values per column step: the lane (TX-1, r) result
+    // of step l' is value l' * TY + r, stored as W entries (its 32-bit
+    // words), so entry (l' * TY + r) * W + w (north face alike, TX values per
     // step). Lane (0, r) of the downstream column needs at its step s the
     // value of plane s + 1 - r, which its upwind column computed at step
     // s + TX - 1 for every r: a step's entries are written together and read
@@ -464,20 +532,24 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     const int self = ca * nTy + cb;
 
     if (edge_wave) {
-      // ---- edge wave: lane e < TX serves column c = e of the south / north
-      // faces, lane TX + r row r of the west / east faces; other lanes idle
+      // ---- edge wave: value lane v < TX serves column c = v of the south /
+      // north faces, value lane TX + r row r of the west / east faces; lane
+      // e = v * W + w moves word w of its value; other lanes idle
       const int e = tid - NT;
-      const bool isS = e < TX, isW = !isS && e < TX + TY;
-      const int k = isS ? e : (isW ? e - TX : 0);  // the served lane's skew (c or r)
+      const int ev = e / W, ew = e % W;
+      const bool isS = ev < TX, isW = !isS && ev < TX + TY;
+      const int k = isS ? ev : (isW ? ev - TX : 0);  // the served lane's skew (c or r)
       const bool up = isS ? cb > 0 : (isW && ca > 0);  // an upwind column exists
+      // halo word of a boundary lane (+inf in the value's type)
+      const uint32_t infw = W == 1 ? 0x7F800000u : (ew ? 0x7FF00000u : 0u);
       // entry of step s: coff + s * cinc
       uint32_t coff = 0xFFFFFFF0u, cinc = 0u;
       if (isS && cb > 0) {
-        coff = eyb + (uint32_t)(((ca * nTy + cb - 1) * NS + TY - 1) * TX + k) * 8u;
-        cinc = TX * 8u;
+        coff = eyb + (uint32_t)((((ca * nTy + cb - 1) * NS + TY - 1) * TX + k) * W + ew) * 8u;
+        cinc = TX * W * 8u;
       } else if (isW && ca > 0) {
-        coff = exb + (uint32_t)((((ca - 1) * nTy + cb) * NS + TX - 1) * TY + k) * 8u;
-        cinc = TY * 8u;
+        coff = exb + (uint32_t)(((((ca - 1) * nTy + cb) * NS + TX - 1) * TY + k) * W + ew) * 8u;
+        cinc = TY * W * 8u;
       }
       // halo cell written (plane of step s: s + 1 - k) and result cell
       // published (plane of producer step l': l' + 1 - pk)
@@ -485,10 +557,11 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       const int pcl = isS ? G::cell(k, TY - 1) : (isW ? G::cell(TX - 1, k) : G::DUMMY + tid);
       const int pk = isS ? k + TY - 1 : TX - 1 + k;
       const uint32_t pslot = (isS || isW) ? PS : 0u;  // slot stride (idle lanes: fixed dummy)
-      uint32_t poff = isS ? eyb + (uint32_t)(self * NS * TX + k) * 8u
-                          : (isW ? exb + (uint32_t)(self * NS * TY + k) * 8u : 0xFFFFFFF0u);
-      const uint32_t pinc = isS ? TX * 8u : (isW ? TY * 8u : 0u);
-      // slot (in floats) of plane p: (p mod R) * PS
+      uint32_t poff = isS ? eyb + (uint32_t)((self * NS * TX + k) * W + ew) * 8u
+                          : (isW ? exb + (uint32_t)((self * NS * TY + k) * W + ew) * 8u
+                                 : 0xFFFFFFF0u);
+      const uint32_t pinc = isS ? TX * W * 8u : (isW ? TY * W * 8u : 0u);
+      // slot (in values) of plane p: (p mod R) * PS
       auto slot_of = [&](int p) -> uint32_t { return (uint32_t)(((p % R) + R) % R) * pslot; };
       uint32_t hs = slot_of(1 - k);   // halo slot of step 0
       uint32_t ps = slot_of(1 - pk);  // result slot of producer step 0
@@ -513,18 +586,18 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       // consume the entry of step s (loaded UE steps earlier) into the halo
       // cell; a stale entry (its upwind column is behind) is waited for
       auto consume = [&](const int s, const u32x2 q) __attribute__((always_inline)) {
-        float v = __uint_as_float(q.x);
+        uint32_t v = q.x;
         if (__builtin_expect(up && (unsigned)(s - k) < (unsigned)Z && q.y != epoch, 0))
           v = wait_edge<1>(a.edge_x + (coff + (uint32_t)s * cinc) / 8u, epoch, a.err);
-        lds[hcl + hs] = up ? v : INF;
+        lds32[(hcl + hs) * W + ew] = up ? v : infw;
         hs = next(hs);
       };
       // publish the result of producer step lp (complete since the barrier
       // that ended step lp)
       auto publish = [&](const int lp) __attribute__((always_inline)) {
-        const float v = lds[pcl + ps];
+        const uint32_t v = lds32[(pcl + ps) * W + ew];
         ps = next(ps);
-        const u32x2 q = {__float_as_uint(v), epoch};
+        const u32x2 q = {v, epoch};
         const bool live = (isS || isW) && (unsigned)(lp - pk) < (unsigned)Z;
         __builtin_amdgcn_raw_buffer_store_b64(q, re, opq(live ? poff : 0xFFFFFFF0u), 0, PDFSK_SC1);
         poff += pinc;
@@ -558,15 +631,15 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     } else {
       // ---- compute waves
       const int tx = tid % TX, ty = tid / TX;
-      const uint32_t XYb = (uint32_t)X * (uint32_t)Y * 4u;
+      const uint32_t XYb = (uint32_t)X * (uint32_t)Y * (uint32_t)sizeof(T);
       const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
       const __amdgpu_buffer_rsrc_t fr = __builtin_amdgcn_make_buffer_rsrc(
           fd.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
       // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
       auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
-      auto xyb = [&](int vx, int vy) -> uint32_t {
+      auto xyi = [&](int vx, int vy) -> uint32_t {
         const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
-        return (uint32_t)(py * X + px) * 4u;
+        return (uint32_t)(py * X + px);
       };
       const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
       const bool dom = vx <= X && vy <= Y;
@@ -574,7 +647,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       const int lo = G::cell(tx, ty);  // ring cell of this thread
 
       // interior plane loads / stores
-      const uint32_t lxy = dom ? xyb(vx, vy) : 0x80000000u;
+      const uint32_t lxy = dom ? xyi(vx, vy) * (uint32_t)sizeof(T) : 0x80000000u;
       uint32_t ld = zb(3) + lxy;  // next plane to load: 3
       uint32_t st = dom ? zb(1) + lxy : 0xFFFFFFF0u;
       const uint32_t st_inc = dom ? dZ : 0u;
@@ -590,64 +663,88 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         hcell = G::cell(TX, tid);
         hmul = PS;
         hval = hx <= X && hy <= Y;
-        if (hval) hxy = xyb(hx, hy);
+        if (hval) hxy = xyi(hx, hy) * (uint32_t)sizeof(T);
       } else if (tid < TY + TX) {
         const int q = tid - TY;
         const int hx = ca * TX + q + 1, hy = cb * TY + TY + 1;
         hcell = G::cell(q, TY);
         hmul = PS;
         hval = hx <= X && hy <= Y;
-        if (hval) hxy = xyb(hx, hy);
+        if (hval) hxy = xyi(hx, hy) * (uint32_t)sizeof(T);
       }
       uint32_t hl = zb(3) + hxy;
       const int hmask = hmul ? -1 : 0;  // slot offset applies to halo lanes only
+
+      // polish: the init of this thread's column, loaded plane by plane
+      // with the field (coalesced) and kept in its LDS delay line until the
+      // thread's hyperplane reaches it
+      const uint32_t XY4 = (uint32_t)X * (uint32_t)Y * 4u;
+      const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)fd.init, (short)0, S::POL ? (int)(XY4 * (uint32_t)Z) : 0, 0x00020000);
+      const uint32_t dZI = a.fz ? (uint32_t)(0u - XY4) : XY4;
+      auto zbi = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XY4; };
+      const uint32_t lxyi = dom ? xyi(vx, vy) * 4u : 0x80000000u;
+      uint32_t ldi = zbi(3) + lxyi;
 
       // prologue: planes 0 (ghost), 1 and 2 into the ring, then the plane
       // queue (planes 3 .. U+2)
       ring[lo] = INF;
       lds[hcell] = INF;
       {
-        const float v1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + lxy, 0, 0));
-        const float h1 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(1) + hxy, 0, 0));
-        const float v2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + lxy, 0, 0));
-        const float h2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, zb(2) + hxy, 0, 0));
+        const T v1 = buf_ld(T(), fr, zb(1) + lxy);
+        const T h1 = buf_ld(T(), fr, zb(1) + hxy);
+        const T v2 = buf_ld(T(), fr, zb(2) + lxy);
+        const T h2 = buf_ld(T(), fr, zb(2) + hxy);
         const bool live2 = Z >= 2;
         ring[PS + lo] = dom ? v1 : INF;
         lds[hcell + hmul] = hval ? h1 : INF;
         ring[2 * PS + lo] = (dom && live2) ? v2 : INF;
         lds[hcell + 2 * hmul] = (hval && live2) ? h2 : INF;
+        if (S::POL) {
+          iring[NT + tid] = buf_ld(float(), ir, zbi(1) + lxyi);
+          iring[2 * NT + tid] = buf_ld(float(), ir, zbi(2) + lxyi);
+        }
       }
-      float qP[U], qH[U];
+      T qP[U], qH[U];
+      float qI[S::POL ? U : 1];
 #pragma unroll
       for (int j = 0; j < U; j++) {
-        qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
-        qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+        qP[j] = buf_ld(T(), fr, ld);
+        qH[j] = buf_ld(T(), fr, hl);
         ld += dZ;
         hl += dZ;
+        if (S::POL) {
+          qI[j] = buf_ld(float(), ir, ldi);
+          ldi += dZI;
+        }
       }
       __syncthreads();
       // Ring addressing: plane vz of a cell lives in slot vz mod R. so_c is
-      // the float offset of slot (vz mod R) for this thread's current plane
-      // vz = l + 1 - skew; the step wraps it with an unsigned min.
-      constexpr uint32_t RPS = (uint32_t)R * PS;
+      // the offset of slot (vz mod R) for this thread's current plane
+      // vz = l + 1 - skew; the step wraps it with an unsigned min (si_c
+      // alike in the init delay line).
+      constexpr uint32_t RPS = (uint32_t)R * PS, RNT = (uint32_t)R * NT;
       auto wrap_next = [](uint32_t s) -> uint32_t {
         const uint32_t n = s + PS;
         return __builtin_elementwise_min(n, n - RPS);
       };
-      uint32_t so_c = (uint32_t)(((1 - skew) % R + R) % R) * PS;
+      const uint32_t slot0 = (uint32_t)(((1 - skew) % R + R) % R);
+      uint32_t so_c = slot0 * PS;
+      uint32_t si_c = slot0 * NT;
       // registers of step 0: own plane vz (c) and vz + 1 (zp), the east and
       // north cells' plane vz (xp, yp); zm is the own result of the step
       // before
-      float c, zp, xp, yp;
+      T c, zp, xp, yp;
       {
-        const float *pc = ring + so_c + lo;
+        const T *pc = ring + so_c + lo;
         c = pc[0];
         xp = pc[1];
         yp = pc[PW];
         zp = ring[wrap_next(so_c) + lo];
       }
-      float zm = INF;
+      T zm = INF;
       unsigned nchg = 0;
+      T md = (T)0;  // FP64 modes: largest |old - new| of this thread's updates
       // uniform slots: plane l+3 is written at step l, plane l-(TX+TY-2) is
       // stored at step l
       int slot_w = 3 % R, slot_s = (R - (TX + TY - 2) % R) % R;
@@ -664,28 +761,54 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       auto step = [&](const int l, const int j, const bool steady)
                       __attribute__((always_inline)) {
         // (a)
-        float *const pc = ring + so_c + lo;
-        const float xm = pc[-1], ym = pc[-PW];
-        const float sval = ring[slot_s * PS + lo];
+        T *const pc = ring + so_c + lo;
+        const T xm = pc[-1], ym = pc[-PW];
+        const T sval = ring[slot_s * PS + lo];
         // (b)
         {
           const bool live = steady || l + 3 <= Z;
           ring[slot_w * PS + lo] = (dom && live) ? qP[j] : INF;
           lds[hcell + ((slot_w * PS) & hmask)] = (hval && live) ? qH[j] : INF;
         }
-        qP[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, ld, 0, 0));
-        qH[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fr, hl, 0, 0));
+        qP[j] = buf_ld(T(), fr, ld);
+        qH[j] = buf_ld(T(), fr, hl);
         ld += dZ;
         hl += dZ;
+        float ini = 0.0f;
+        if (S::POL) {
+          iring[slot_w * NT + tid] = qI[j];
+          qI[j] = buf_ld(float(), ir, ldi);
+          ldi += dZI;
+          ini = iring[si_c + tid];
+        }
         // (d)
         const uint32_t so_n = wrap_next(so_c);
-        const float *pn1 = ring + so_n + lo;
-        const float xp1 = pn1[1], yp1 = pn1[PW];
-        const float zp1 = ring[wrap_next(so_n) + lo];
+        const T *pn1 = ring + so_n + lo;
+        const T xp1 = pn1[1], yp1 = pn1[PW];
+        const T zp1 = ring[wrap_next(so_n) + lo];
         // (c)
         const int vz = l + 1 - skew;
         const bool act = steady ? dom : (dom && (unsigned)(vz - 1) < (unsigned)Z);
-        const float u = god32<ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
+        T u;
+        if constexpr (!S::F64) {
+          u = god32<S::ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
+        } else {
+          // per-axis minima in the physical order of the reference's
+          // min(lower index, higher index) (mpi_openacc/src/utilities.h:35)
+          const T mx = a.fx ? (xp < xm ? xp : xm) : (xm < xp ? xm : xp);
+          const T my = a.fy ? (yp < ym ? yp : ym) : (ym < yp ? ym : yp);
+          const T mz = a.fz ? (zp < zm ? zp : zm) : (zm < zp ? zm : zp);
+          if constexpr (M == PDFSK_MODE_F64_REF)
+            u = god_ref64(c, mx, my, mz, a.sp64.d[0], a.sp64.d[1], a.sp64.d[2]);
+          else if constexpr (S::ISO)
+            u = god_iso64((double)ini, mx, my, mz, a.sp64.h, a.sp64.two_h2, a.sp64.three_h2,
+                          a.sp64.third);
+          else
+            u = god_diff_aniso<double>((double)ini, mx, my, mz, a.sp64.d, a.sp64.w, a.sp64.W);
+          const T df = __builtin_fabs(c - u);
+          const T dd = df == df ? df : (T)__builtin_inff();
+          md = (act && u != c && dd > md) ? dd : md;
+        }
         nchg += (act && u != c) ? 1u : 0u;
         // lanes outside the field keep their cells at +inf (the ghost
         // values their neighbours read); inactive lanes inside it write a
@@ -693,8 +816,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         pc[0] = dom ? u : INF;
         {
           const bool sto = steady || l >= TX + TY - 1;
-          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sval), fr,
-                                                steady ? st : opq(sto ? st : 0xFFFFFFF0u), 0, 0);
+          buf_st(sval, fr, steady ? st : opq(sto ? st : 0xFFFFFFF0u));
           st += sto ? st_inc : 0u;
         }
         zm = (steady || act) ? u : INF;
@@ -703,6 +825,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         xp = xp1;
         yp = yp1;
         so_c = so_n;
+        if (S::POL) {
+          const uint32_t n = si_c + NT;
+          si_c = __builtin_elementwise_min(n, n - RNT);
+        }
         slot_w = slot_w == R - 1 ? 0 : slot_w + 1;
         slot_s = slot_s == R - 1 ? 0 : slot_s + 1;
         __syncthreads();
@@ -728,6 +854,15 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       }
       if (l0 <= NL) block(l0, true, false);
       if (nchg) atomicAdd(&s_changed, nchg);
+      if constexpr (S::F64) {
+        // wave max of the change (non-negative doubles order as their bits)
+        unsigned long long b = (unsigned long long)__double_as_longlong(md);
+        for (int o = 32; o > 0; o >>= 1) {
+          const unsigned long long t = __shfl_xor(b, o);
+          b = t > b ? t : b;
+        }
+        if ((tid & 63) == 0 && b) atomicMax(a.maxdiff + fi, b);
+      }
     }
     __syncthreads();
     if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);
@@ -1010,6 +1145,59 @@ __global__ void k_unpack_f64(const float *__restrict__ dn, double *__restrict__ 
   }
 }
 
+__global__ void k_pack64(const double *__restrict__ pd, double *__restrict__ dn, int X, int Y,
+                         int Z) {
+  const long N = (long)X * Y * Z;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(p % X);
+    const long q = p / X;
+    const int y = (int)(q % Y), z = (int)(q / Y);
+    dn[p] = pd[((long)(z + 1) * (Y + 2) + (y + 1)) * (X + 2) + (x + 1)];
+  }
+}
+
+__global__ void k_unpack64(const double *__restrict__ dn, double *__restrict__ pd, int X, int Y,
+                           int Z) {
+  const long N = (long)X * Y * Z;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(p % X);
+    const long q = p / X;
+    const int y = (int)(q % Y), z = (int)(q / Y);
+    pd[((long)(z + 1) * (Y + 2) + (y + 1)) * (X + 2) + (x + 1)] = dn[p];
+  }
+}
+
+// Ghost cells (any index 0 or dim + 1) whose bits are not those of +inf.
+__global__ void k_count_ghost_not_inf(const double *__restrict__ pd, int X, int Y, int Z,
+                                      unsigned long long *count) {
+  const long n = (long)(X + 2) * (Y + 2) * (Z + 2);
+  unsigned long long bad = 0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(p % (X + 2));
+    const long q = p / (X + 2);
+    const int j = (int)(q % (Y + 2)), i = (int)(q / (Y + 2));
+    const bool ghost = i == 0 || i == Z + 1 || j == 0 || j == Y + 1 || k == 0 || k == X + 1;
+    bad += ghost && __double_as_longlong(pd[p]) != 0x7FF0000000000000ll;
+  }
+  if (bad) atomicAdd(count, bad);
+}
+
+// dir 0: widen FP32 -> FP64 (exact); dir 1: narrow FP64 -> FP32 (nearest
+// even). Field f is handled by the workgroups of row blockIdx.y.
+__global__ void k_convert_batch(const pdfsk_conv *__restrict__ tab, int dir) {
+  const pdfsk_conv t = tab[blockIdx.y];
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
+       p += (long long)gridDim.x * blockDim.x) {
+    if (dir == 0)
+      t.f64[p] = (double)t.f32[p];
+    else
+      t.f32[p] = (float)t.f64[p];
+  }
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -1167,10 +1355,10 @@ extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
 #define PDFSK_U 8
 #endif
 
-template <int TX, int TY, bool ISO>
+template <int TX, int TY, int M>
 static int max_resident(int per_cu) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, PDFSK_U, ISO>,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, PDFSK_U, M>,
                                                    TX * TY + 64, 0) != hipSuccess)
     return 0;
   int dev = 0, ncu = 0;
@@ -1181,31 +1369,56 @@ static int max_resident(int per_cu) {
   return nb * ncu;
 }
 
-extern "C" int pdfsk_col_max_resident(int variant, int iso) {
-  if ((unsigned)variant >= PDFSK_TILE_COUNT) return 0;
+// Tile shapes per mode: FP32 modes take every shape; FP64 values need two
+// edge-wave lanes each (TX + TY <= 32) and twice the ring, so FP64 modes
+// take the 16-wide shapes only.
+extern "C" int pdfsk_tile_supports(int variant, int mode) {
+  if ((unsigned)variant >= PDFSK_TILE_COUNT || (unsigned)mode >= PDFSK_MODE_COUNT) return 0;
+  return mode < PDFSK_MODE_F64_REF || kVariants[variant].tx == 16;
+}
+
+template <int M>
+static int max_resident_m(const ColVariant &v) {
+  if (v.tx == 16 && v.ty == 8) return max_resident<16, 8, M>(v.per_cu);
+  if (v.tx == 16) return max_resident<16, 16, M>(v.per_cu);
+  if (M >= PDFSK_MODE_F64_REF) return 0;
+  if (v.ty == 8) return max_resident<32, 8, M < PDFSK_MODE_F64_REF ? M : 0>(v.per_cu);
+  return max_resident<32, 16, M < PDFSK_MODE_F64_REF ? M : 0>(v.per_cu);
+}
+
+extern "C" int pdfsk_col_max_resident(int variant, int mode) {
+  if (!pdfsk_tile_supports(variant, mode)) return 0;
   const ColVariant &v = kVariants[variant];
-  if (v.tx == 32 && v.ty == 8)
-    return iso ? max_resident<32, 8, true>(v.per_cu) : max_resident<32, 8, false>(v.per_cu);
+  switch (mode) {
+    case PDFSK_MODE_F32_ISO: return max_resident_m<PDFSK_MODE_F32_ISO>(v);
+    case PDFSK_MODE_F32_ANISO: return max_resident_m<PDFSK_MODE_F32_ANISO>(v);
+    case PDFSK_MODE_F64_REF: return max_resident_m<PDFSK_MODE_F64_REF>(v);
+    case PDFSK_MODE_F64_POL_ISO: return max_resident_m<PDFSK_MODE_F64_POL_ISO>(v);
+    default: return max_resident_m<PDFSK_MODE_F64_POL_ANISO>(v);
+  }
+}
+
+template <int TX, int TY, int M>
+static void launch_col(const ColArgs &a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, M>), dim3(grid), dim3(TX * TY + 64), 0, st, a);
+}
+
+template <int M>
+static void launch_col_m(const ColVariant &v, const ColArgs &a, int grid, hipStream_t st) {
   if (v.tx == 16 && v.ty == 8)
-    return iso ? max_resident<16, 8, true>(v.per_cu) : max_resident<16, 8, false>(v.per_cu);
-  if (v.tx == 32 && v.ty == 16)
-    return iso ? max_resident<32, 16, true>(v.per_cu) : max_resident<32, 16, false>(v.per_cu);
-  return iso ? max_resident<16, 16, true>(v.per_cu) : max_resident<16, 16, false>(v.per_cu);
-}
-
-template <int TX, int TY>
-static void launch_col(const ColArgs &a, int grid, hipStream_t st, int iso) {
-  if (iso)
-    hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, true>), dim3(grid), dim3(TX * TY + 64), 0, st,
-                       a);
+    launch_col<16, 8, M>(a, grid, st);
+  else if (v.tx == 16)
+    launch_col<16, 16, M>(a, grid, st);
+  else if (v.ty == 8)
+    launch_col<32, 8, M < PDFSK_MODE_F64_REF ? M : 0>(a, grid, st);
   else
-    hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, false>), dim3(grid), dim3(TX * TY + 64), 0,
-                       st, a);
+    launch_col<32, 16, M < PDFSK_MODE_F64_REF ? M : 0>(a, grid, st);
 }
 
-extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp32 *sp,
-                                     int variant, int grid, hipStream_t st) {
-  if (sn < 1 || sn > 8 || grid < 1) return -1;
+extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *sp,
+                                 const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st) {
+  if (sn < 1 || sn > 8 || grid < 1 || !pdfsk_tile_supports(variant, mode)) return -1;
+  if (mode >= PDFSK_MODE_F64_REF && !b->maxdiff) return -1;
   static_assert(sizeof(FieldDesc) == sizeof(pdfsk_field), "field table layout");
   // Ordering sn as (+,+,+) in virtual coordinates: physical direction per
   // axis (mpi_openacc/src/phi3D_fsm.c:206-218 reflections x{4,8} y{2,6}
@@ -1213,6 +1426,7 @@ extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp
   const int desc = (sn == 2 || sn == 5 || sn == 7 || sn == 8);
   const int rx = (sn == 4 || sn == 8), ry = (sn == 2 || sn == 6), rz = (sn == 3 || sn == 7);
   ColArgs a;
+  memset(&a, 0, sizeof a);
   a.fields = (const FieldDesc *)b->fields;
   a.cols = b->cols;
   a.state = b->state;
@@ -1225,19 +1439,19 @@ extern "C" int pdfsk_sweep_batch_f32(const pdfsk_batch *b, int sn, const pdfs_sp
   a.edge_y_bytes = (uint32_t)b->edge_y_bytes;
   a.changed = b->changed;
   a.streak = b->streak;
+  a.maxdiff = b->maxdiff;
   a.nfields = b->nfields;
   a.err = b->err;
-  a.sp = *sp;
-  int tx = 16, ty = 16;
-  pdfsk_tile_dims(variant, &tx, &ty);
-  if (tx == 32 && ty == 8)
-    launch_col<32, 8>(a, grid, st, sp->iso);
-  else if (tx == 16 && ty == 8)
-    launch_col<16, 8>(a, grid, st, sp->iso);
-  else if (tx == 32 && ty == 16)
-    launch_col<32, 16>(a, grid, st, sp->iso);
-  else
-    launch_col<16, 16>(a, grid, st, sp->iso);
+  if (sp) a.sp = *sp;
+  if (sp64) a.sp64 = *sp64;
+  const ColVariant &v = kVariants[variant];
+  switch (mode) {
+    case PDFSK_MODE_F32_ISO: launch_col_m<PDFSK_MODE_F32_ISO>(v, a, grid, st); break;
+    case PDFSK_MODE_F32_ANISO: launch_col_m<PDFSK_MODE_F32_ANISO>(v, a, grid, st); break;
+    case PDFSK_MODE_F64_REF: launch_col_m<PDFSK_MODE_F64_REF>(v, a, grid, st); break;
+    case PDFSK_MODE_F64_POL_ISO: launch_col_m<PDFSK_MODE_F64_POL_ISO>(v, a, grid, st); break;
+    default: launch_col_m<PDFSK_MODE_F64_POL_ANISO>(v, a, grid, st); break;
+  }
   return hip_rc(hipGetLastError());
 }
 
@@ -1321,6 +1535,38 @@ extern "C" int pdfsk_check_sqrt_cr(uint32_t lo, uint32_t hi, unsigned long long 
                                    hipStream_t st) {
   if (hi <= lo) return -1;
   hipLaunchKernelGGL(k_check_sqrt_cr, dim3(256 * 64), dim3(256), 0, st, lo, hi - lo, count);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_pack64(const double *padded, double *dense, int X, int Y, int Z,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_pack64, dim3(grid_for((long)X * Y * Z, 256)), dim3(256), 0, st, padded,
+                     dense, X, Y, Z);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_unpack64(const double *dense, double *padded, int X, int Y, int Z,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_unpack64, dim3(grid_for((long)X * Y * Z, 256)), dim3(256), 0, st, dense,
+                     padded, X, Y, Z);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_count_ghost_not_inf(const double *padded, int X, int Y, int Z,
+                                         unsigned long long *count, hipStream_t st) {
+  const long n = (long)(X + 2) * (Y + 2) * (Z + 2);
+  hipLaunchKernelGGL(k_count_ghost_not_inf, dim3(grid_for(n, 256) < 4096 ? grid_for(n, 256) : 4096),
+                     dim3(256), 0, st, padded, X, Y, Z, count);
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_convert_batch(const pdfsk_conv *tab, int nf, long long nmax, int dir,
+                                   hipStream_t st) {
+  if (nf < 1 || nf > 65535 || nmax < 1 || (dir != 0 && dir != 1)) return -1;
+  long long gx = (nmax + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  hipLaunchKernelGGL(k_convert_batch, dim3((unsigned)gx, (unsigned)nf), dim3(256), 0, st, tab,
+                     dir);
   return hip_rc(hipGetLastError());
 }
 

@@ -722,7 +722,9 @@ int pdfs_warm_start_batch_f32_dev(pdfs_net *n, int nf, const float *const *inits
 /* Warm-started certified solve: field = warm start, FP32 passes until the
  * FP64 judge passes; if the judge still flags points (the warm start was
  * below the solution somewhere) or the pass cap is hit, the field is reset
- * to its init and solved plainly (a fallback). */
+ * to its init and solved plainly (a fallback). The FP64 polish (opts.polish)
+ * runs once, on the accepted result: after the certified solve, or within
+ * the plain re-solve. */
 int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float *init, int x,
                             int y, int z, double dx, double dy, double dz, const pdfs_opts *o,
                             pdfs_stats *st)
@@ -737,21 +739,26 @@ int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float 
   hipStream_t s = (hipStream_t)pdfs_ctx_stream(c);
   hipEvent_t e0, e1;
   HCK(hipEventCreate(&e0));
-  HCK(hipEventCreate(&e1));
-  HCK(hipEventRecord(e0, s));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return pdfsi_fail(PDFS_EHIP, "pdfs_solve_warm_f32_dev: hipEventCreate failed");
+  }
   double hmax = dx > dy ? dx : dy;
   if (dz > hmax) hmax = dz;
-  int rc = pdfs_warm_start_f32_dev(n, init, field, x, y, z, hmax);
-  if (rc) return rc;
-  HCK(hipEventRecord(e1, s));
-  HCK(hipEventSynchronize(e1));
   float wms = 0.f;
-  hipEventElapsedTime(&wms, e0, e1);
+  int rc = PDFS_OK;
+  if (hipEventRecord(e0, s) != hipSuccess) rc = pdfsi_fail(PDFS_EHIP, "hipEventRecord failed");
+  if (rc == PDFS_OK) rc = pdfs_warm_start_f32_dev(n, init, field, x, y, z, hmax);
+  if (rc == PDFS_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+    rc = pdfsi_fail(PDFS_EHIP, "pdfs_solve_warm_f32_dev: event failed");
+  if (rc == PDFS_OK) hipEventElapsedTime(&wms, e0, e1);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (rc) return rc;
   pdfs_opts ow = oo;
   ow.certify = 1;
   ow.judge = 1;
+  ow.polish = 0;
   pdfs_stats sw;
   rc = pdfs_solve_f32_dev(c, field, init, x, y, z, dx, dy, dz, &ow, &sw);
   if (rc && rc != PDFS_ENOTCONV) return rc;
@@ -770,20 +777,26 @@ int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float 
     sp.judge_runs += sw.judge_runs;
     sp.fallbacks = 1;
     sp.ms_warm = wms;
-    sp.ms_total = sp.ms_sweeps + sp.ms_judge + wms;
+    sp.ms_total = sp.ms_sweeps + sp.ms_judge + sp.ms_polish + wms;
     if (st) *st = sp;
     return rc;
   }
-  sw.ms_total += wms;
+  if (oo.polish) {
+    const int dims[3] = {x, y, z};
+    rc = pdfsi_polish(c, 1, &field, &init, dims, dx, dy, dz, &oo, &sw);
+    if (rc && rc != PDFS_ENOTCONV) return rc;
+  }
+  sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + wms;
   if (st) *st = sw;
-  return PDFS_OK;
+  return rc;
 }
 
 /* Batched warm-started certified solve: every field gets its warm start, the
- * batch runs in shared persistent launches with certified stop, then each
- * field is judged; flagged (or unconverged) fields are reset to their init
- * and solved plainly in a second batch. field_passes / field_fallback
- * (optional) report per field. */
+ * batch runs in shared persistent launches with certified stop and a final
+ * judge of the fields the passes did not certify; flagged (or unconverged)
+ * fields are reset to their init and solved plainly in a second batch. The
+ * FP64 polish (opts.polish) runs once per field, on the accepted results.
+ * field_passes / field_fallback (optional) report per field. */
 int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const *fields,
                                   const float *const *inits, const int *dims, double dx,
                                   double dy, double dz, const pdfs_opts *o, pdfs_stats *st,
@@ -809,14 +822,18 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   float wms = 0.f;
   pdfs_opts ow = oo;
   ow.certify = 1;
-  ow.judge = 0;
+  ow.judge = 1;
+  ow.polish = 0;
   pdfs_stats sw;
+  memset(&sw, 0, sizeof sw);
   int *passes = (int *)calloc((size_t)nf, sizeof(int));
   int *bad = (int *)calloc((size_t)nf, sizeof(int));
+  long long *fl = (long long *)calloc((size_t)nf, sizeof(long long));
   float **rf = (float **)calloc((size_t)nf, sizeof(float *));
   const float **ri = (const float **)calloc((size_t)nf, sizeof(float *));
+  const float **gi = (const float **)calloc((size_t)nf, sizeof(float *));
   int *rd = (int *)calloc((size_t)3 * nf, sizeof(int));
-  if (!passes || !bad || !rf || !ri || !rd) {
+  if (!passes || !bad || !fl || !rf || !ri || !gi || !rd) {
     rc = pdfsi_fail(PDFS_ENOMEM, "pdfs_solve_batch_warm_f32_dev: out of memory");
     goto out;
   }
@@ -826,38 +843,26 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   HGO(hipEventRecord(e1, s));
   HGO(hipEventSynchronize(e1));
   hipEventElapsedTime(&wms, e0, e1);
-  rc = pdfs_solve_batch_f32_dev(c, nf, fields, inits, dims, dx, dy, dz, &ow, &sw, passes, NULL);
+  /* certified batch; fields the passes did not certify are judged once at
+   * the end (field_flagged: their failing points, -1 if unconverged) */
+  rc = pdfsi_batch_f32(c, nf, fields, inits, dims, dx, dy, dz, &ow, &sw, passes, fl);
   if (rc && rc != PDFS_ENOTCONV) goto out;
-  int notconv = rc == PDFS_ENOTCONV;
   rc = 0;
-  /* judge every field (certified ones pass by construction; this also
-   * catches fields that stopped at the pass cap) */
-  HGO(hipEventRecord(e0, s));
   int nbad = 0;
-  {
-    long long *fl = (long long *)calloc((size_t)nf, sizeof(long long));
-    if (!fl) {
-      rc = pdfsi_fail(PDFS_ENOMEM, "pdfs_solve_batch_warm_f32_dev: out of memory");
-      goto out;
-    }
-    rc = pdfsi_judge_batch(c, nf, (const float *const *)fields, inits, dims, NULL, dx, dy, dz, fl);
-    for (int f = 0; f < nf && rc == PDFS_OK; f++) {
-      if (fl[f] > 0 || notconv) {
-        bad[f] = 1;
-        nbad++;
-      }
-    }
-    free(fl);
-    if (rc) goto out;
-    sw.judge_runs += nf;
+  for (int f = 0; f < nf; f++) {
+    bad[f] = fl[f] != 0;
+    nbad += bad[f];
+    gi[f] = bad[f] ? NULL : inits[f];
   }
-  HGO(hipEventRecord(e1, s));
-  HGO(hipEventSynchronize(e1));
-  float jms = 0.f;
-  hipEventElapsedTime(&jms, e0, e1);
-  sw.ms_judge += jms;
   sw.fallbacks = 0;
   sw.judge_flagged = 0;
+  /* polish of the accepted fields */
+  if (oo.polish && nbad < nf) {
+    rc = pdfsi_polish(c, nf, fields, gi, dims, dx, dy, dz, &oo, &sw);
+    if (rc && rc != PDFS_ENOTCONV) goto out;
+  }
+  int rc_pol = rc;
+  rc = 0;
   if (nbad) {
     int k = 0;
     for (int f = 0; f < nf; f++) {
@@ -891,10 +896,16 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
     sw.ms_judge += sp.ms_judge;
     sw.judge_runs += sp.judge_runs;
     sw.judge_flagged = sp.judge_flagged;
+    if (sp.polish_passes > sw.polish_passes) sw.polish_passes = sp.polish_passes;
+    sw.polish_orderings += sp.polish_orderings;
+    if (sp.polish_maxdiff > sw.polish_maxdiff) sw.polish_maxdiff = sp.polish_maxdiff;
+    sw.polish_skipped += sp.polish_skipped;
+    sw.ms_polish += sp.ms_polish;
     sw.fallbacks = nbad;
   }
+  if (rc == PDFS_OK) rc = rc_pol;
   sw.ms_warm = wms;
-  sw.ms_total = sw.ms_sweeps + sw.ms_judge + wms;
+  sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + wms;
   if (st) *st = sw;
   for (int f = 0; f < nf; f++) {
     if (field_passes) field_passes[f] = passes[f];
@@ -905,8 +916,10 @@ out:
   hipEventDestroy(e1);
   free(passes);
   free(bad);
+  free(fl);
   free(rf);
   free(ri);
+  free(gi);
   free(rd);
   return rc;
 }

@@ -16,20 +16,29 @@
 struct pdfs_ctx {
   int dev;
   hipStream_t st;
-  int resident[PDFSK_TILE_COUNT][2];
+  int resident[PDFSK_TILE_COUNT][PDFSK_MODE_COUNT];
   /* batched column sweep: persistent device state, edge allocations, plan */
   pdfsk_batch bt;
   size_t cap_ex, cap_ey;           /* edge entries excluding padding */
   pdfsk_field *h_fields;           /* host copy of the field table */
   int *h_cols;
   int nf, ncols, tile, cap_f, cap_cols;
+  int words;                       /* 32-bit words per value of the current plan (1 or 2) */
   unsigned *h_changed, *prev_changed; /* pinned readback, last values */
   unsigned *h_streak;              /* per field: counter, run of unchanged orderings */
   unsigned *h_err;
   hipGraphExec_t gexec[3];         /* captured pass (orderings 1-8), halves 1-4 and 5-8 */
   long long tune_keys[32];         /* tile autotuner cache: size class -> tile */
   int tune_tiles[32], ntune;
-  int g_tile[3], g_iso[3], g_grid[3];
+  int g_tile[3], g_mode[3], g_grid[3];
+  /* FP64 column sweeps: per-field max change (device, pinned readback),
+   * dense FP64 working fields, conversion table */
+  unsigned long long *h_maxdiff;
+  int cap_md;
+  double *pbuf;
+  size_t pcap;
+  pdfsk_conv *conv_host, *conv_dev;
+  int cap_conv;
   /* FP64 path / judge */
   unsigned long long *d_u64; /* [0] maxdiff bits */
   unsigned long long *h_pin64;
@@ -104,6 +113,16 @@ void *pdfs_ctx_stream(pdfs_ctx *c) { return c ? (void *)c->st : NULL; }
     }                                                                                \
   } while (0)
 
+/* kernel launch inside a function that owns host allocations */
+#define KGO(call)                                                                    \
+  do {                                                                               \
+    int r_ = (call);                                                                 \
+    if (r_ != 0) {                                                                   \
+      rc = fail(PDFS_EHIP, "%s:%d %s: launch failed (%d)", __FILE__, __LINE__, #call, r_); \
+      goto out;                                                                      \
+    }                                                                                \
+  } while (0)
+
 void pdfs_default_opts(pdfs_opts *o)
 {
   memset(o, 0, sizeof *o);
@@ -115,6 +134,9 @@ void pdfs_default_opts(pdfs_opts *o)
   o->graph = 0;
   o->certify = 0;
   o->certify_from = 1;
+  o->polish = 1;
+  o->polish_tol = 1e-5;
+  o->tile64 = -1;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -150,7 +172,7 @@ pdfs_ctx *pdfs_ctx_create(void)
     return NULL;
   }
   for (int v = 0; v < PDFSK_TILE_COUNT; v++)
-    for (int iso = 0; iso < 2; iso++) c->resident[v][iso] = pdfsk_col_max_resident(v, iso);
+    for (int m = 0; m < PDFSK_MODE_COUNT; m++) c->resident[v][m] = pdfsk_col_max_resident(v, m);
   c->tile = -1;
   return c;
 }
@@ -166,6 +188,11 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->bt.cols);
   hipFree(c->bt.changed);
   hipFree(c->bt.streak);
+  hipFree(c->bt.maxdiff);
+  hipHostFree(c->h_maxdiff);
+  hipFree(c->pbuf);
+  hipFree(c->conv_dev);
+  free(c->conv_host);
   free(c->h_streak);
   hipFree(c->bt.state);
   hipFree(c->d_u64);
@@ -203,6 +230,28 @@ static void make_sp32(double dx, double dy, double dz, pdfs_sp32 *s)
   s->d[2] = (float)dz;
   for (int i = 0; i < 3; i++) s->w[i] = 1.0f / (s->d[i] * s->d[i]);
   s->W = s->w[0] + s->w[1] + s->w[2];
+}
+
+static int mode32(const pdfs_sp32 *sp)
+{
+  return sp->iso ? PDFSK_MODE_F32_ISO : PDFSK_MODE_F32_ANISO;
+}
+
+/* FP64 spacing constants in the oracle's operation order. */
+static void make_sp64(double dx, double dy, double dz, pdfs_sp64 *s)
+{
+  memset(s, 0, sizeof *s);
+  s->iso = (dx == dy && dy == dz);
+  s->d[0] = dx;
+  s->d[1] = dy;
+  s->d[2] = dz;
+  for (int i = 0; i < 3; i++) s->w[i] = 1.0 / (s->d[i] * s->d[i]);
+  s->W = s->w[0] + s->w[1] + s->w[2];
+  s->h = dx;
+  const double h2 = dx * dx;
+  s->two_h2 = 2.0 * h2;
+  s->three_h2 = 3.0 * h2;
+  s->third = 1.0 / 3.0;
 }
 
 /* Size class of a batch for the tile cache: field count and log2 buckets of
@@ -318,9 +367,9 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
 }
 
 /* Face entries of a batch's edge allocation for a tile (one row per column
- * step, z + TX + TY - 1 steps); 0 if the batch does not fit the kernel's
- * 32-bit buffer offsets (2 GiB with padding). */
-static int plan_fits(int nf, const int *dims, int tile)
+ * step, z + TX + TY - 1 steps, `words` entries per value); 0 if the batch
+ * does not fit the kernel's 32-bit buffer offsets (2 GiB with padding). */
+static int plan_fits_w(int nf, const int *dims, int tile, int words)
 {
   int TX, TY;
   pdfsk_tile_dims(tile, &TX, &TY);
@@ -328,21 +377,24 @@ static int plan_fits(int nf, const int *dims, int tile)
   for (int f = 0; f < nf; f++) {
     const int x = dims[3 * f], y = dims[3 * f + 1], z = dims[3 * f + 2];
     const size_t nc = (size_t)((x + TX - 1) / TX) * (size_t)((y + TY - 1) / TY);
-    ex += nc * (size_t)(z + TX + TY - 1) * TY;
-    ey += nc * (size_t)(z + TX + TY - 1) * TX;
+    ex += nc * (size_t)(z + TX + TY - 1) * TY * words;
+    ey += nc * (size_t)(z + TX + TY - 1) * TX * words;
   }
   return (ex + ey + 4 * PDFSK_EDGE_PAD) * 8 < (1ull << 31);
 }
 
+static int plan_fits(int nf, const int *dims, int tile) { return plan_fits_w(nf, dims, tile, 1); }
+
 /* keep: re-tile the same batch mid-solve, keeping the per-field change
- * counters and unchanged-ordering runs (they do not depend on the tile) */
-static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile,
-                         int keep)
+ * counters and unchanged-ordering runs (they do not depend on the tile).
+ * words: 1 for FP32 fields, 2 for FP64 fields; inits (polish) or NULL. */
+static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *const *inits,
+                         const int *dims, int tile, int keep, int words)
 {
   if (nf < 1) return fail(PDFS_EINVAL, "empty batch");
   int TX, TY;
   pdfsk_tile_dims(tile, &TX, &TY);
-  if (!plan_fits(nf, dims, tile))
+  if (!plan_fits_w(nf, dims, tile, words))
     return fail(PDFS_EINVAL, "batch too large for one sweep (edge buffers exceed 2 GiB)");
   if (nf > c->nf || !c->h_fields) {
     free(c->h_fields);
@@ -357,17 +409,19 @@ static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *d
         hipHostMalloc((void **)&c->h_changed, (size_t)nf * sizeof(unsigned), 0) != hipSuccess)
       return fail(PDFS_ENOMEM, "plan: host alloc");
   }
+  const long long maxpts = words == 2 ? PDFSK_MAX_POINTS64 : PDFSK_MAX_POINTS;
   size_t ex = 0, ey = 0;
   int ncols = 0;
   for (int f = 0; f < nf; f++) {
     int x = dims[3 * f], y = dims[3 * f + 1], z = dims[3 * f + 2];
     if (!fields[f] || x < 1 || y < 1 || z < 1)
       return fail(PDFS_EINVAL, "field %d: bad pointer or dims", f);
-    if ((long long)x * y * z > PDFSK_MAX_POINTS)
-      return fail(PDFS_EINVAL, "field %d: %lld points exceed the FP32 sweep limit %lld", f,
-                  (long long)x * y * z, (long long)PDFSK_MAX_POINTS);
+    if ((long long)x * y * z > maxpts)
+      return fail(PDFS_EINVAL, "field %d: %lld points exceed the %s sweep limit %lld", f,
+                  (long long)x * y * z, words == 2 ? "FP64" : "FP32", maxpts);
     pdfsk_field *fd = &c->h_fields[f];
     fd->f = fields[f];
+    fd->init = inits ? inits[f] : NULL;
     fd->X = x;
     fd->Y = y;
     fd->Z = z;
@@ -378,18 +432,22 @@ static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *d
     fd->pad = 0;
     size_t nc = (size_t)fd->nTx * fd->nTy;
     /* one row of face entries per column step (z + TX + TY - 1 steps) */
-    ex += nc * (size_t)(z + TX + TY - 1) * TY;
-    ey += nc * (size_t)(z + TX + TY - 1) * TX;
+    ex += nc * (size_t)(z + TX + TY - 1) * TY * words;
+    ey += nc * (size_t)(z + TX + TY - 1) * TX * words;
     ncols += (int)nc;
   }
   if (ex > c->cap_ex || ey > c->cap_ey) {
     /* one allocation: the x-face part, then the y-face part, each with a
      * padding of PDFSK_EDGE_PAD entries on both sides; the kernel's single
      * buffer descriptor covers both, so one instruction moves a column's
-     * west and south entries */
+     * west and south entries. Entries carry the epoch they were written
+     * in, which only grows, so entries of earlier plans are never taken
+     * for current ones. */
+    const size_t cx = ex > c->cap_ex ? ex : c->cap_ex, cy = ey > c->cap_ey ? ey : c->cap_ey;
+    HIPCHK(hipStreamSynchronize(c->st));
     if (c->bt.edge_x) hipFree(c->bt.edge_x);
     c->bt.edge_x = c->bt.edge_y = NULL;
-    const size_t cx = ex > c->cap_ex ? ex : c->cap_ex, cy = ey > c->cap_ey ? ey : c->cap_ey;
+    c->cap_ex = c->cap_ey = 0;
     const size_t nx = cx + 2 * PDFSK_EDGE_PAD, ny = cy + 2 * PDFSK_EDGE_PAD;
     HIPCHK(hipMalloc((void **)&c->bt.edge_x, (nx + ny) * sizeof(uint64_t)));
     HIPCHK(hipMemsetAsync(c->bt.edge_x, 0, (nx + ny) * sizeof(uint64_t), c->st));
@@ -405,18 +463,33 @@ static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *d
       grow_dev((void **)&c->bt.streak, &cap_s, 2 * nf, sizeof(unsigned)) ||
       grow_dev((void **)&c->bt.cols, &c->cap_cols, 3 * ncols, sizeof(int)))
     return fail(PDFS_ENOMEM, "plan: device alloc");
+  if (nf > c->cap_md) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    hipFree(c->bt.maxdiff);
+    hipHostFree(c->h_maxdiff);
+    c->bt.maxdiff = NULL;
+    c->h_maxdiff = NULL;
+    c->cap_md = 0;
+    if (hipMalloc((void **)&c->bt.maxdiff, (size_t)nf * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_maxdiff, (size_t)nf * sizeof(unsigned long long), 0) !=
+            hipSuccess)
+      return fail(PDFS_ENOMEM, "plan: maxdiff alloc");
+    c->cap_md = nf;
+  }
   free(c->h_cols);
   c->h_cols = (int *)malloc((size_t)3 * ncols * sizeof(int));
   if (!c->h_cols) return fail(PDFS_ENOMEM, "plan: host alloc");
   c->nf = nf;
   c->tile = tile;
+  c->words = words;
   HIPCHK(hipMemcpyAsync(c->bt.fields, c->h_fields, (size_t)nf * sizeof(pdfsk_field),
                         hipMemcpyHostToDevice, c->st));
   if (!keep) {
-    HIPCHK(hipMemsetAsync(c->bt.changed, 0, (size_t)nf * sizeof(unsigned), c->st));
-    HIPCHK(hipMemsetAsync(c->bt.streak, 0, (size_t)2 * nf * sizeof(unsigned), c->st));
+    KCHK(pdfsk_fill_u32(c->bt.changed, 0u, (size_t)nf, c->st));
+    KCHK(pdfsk_fill_u32(c->bt.streak, 0u, (size_t)2 * nf, c->st));
     memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
   }
+  KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)2 * nf, c->st));
   c->bt.nfields = nf;
   for (int g = 0; g < 3; g++)
     if (c->gexec[g]) {
@@ -428,26 +501,27 @@ static int plan_build_ex(pdfs_ctx *c, int nf, float *const *fields, const int *d
 
 static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile)
 {
-  return plan_build_ex(c, nf, fields, dims, tile, 0);
+  return plan_build_ex(c, nf, (void *const *)fields, NULL, dims, tile, 0, 1);
 }
 
-static int launch_grid(pdfs_ctx *c, int iso)
+static int launch_grid(pdfs_ctx *c, int mode)
 {
-  int grid = c->resident[c->tile][iso];
+  int grid = c->resident[c->tile][mode];
   if (grid <= 0) return -1;
   return grid > c->ncols ? (c->ncols > 0 ? c->ncols : 1) : grid;
 }
 
-/* Orderings sn0..sn1 of the current plan, optionally profiled per launch
- * or replayed from a captured graph (a whole pass, or one of its halves). */
-static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int profile,
-                         int graph, double *kernel_ms)
+/* Orderings sn0..sn1 of the current plan in a sweep mode, optionally
+ * profiled per launch or replayed from a captured graph (a whole pass, or
+ * one of its halves). */
+static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_sp64 *sp64,
+                         int sn0, int sn1, int profile, int graph, double *kernel_ms)
 {
-  int grid = launch_grid(c, sp->iso);
+  int grid = launch_grid(c, mode);
   if (grid <= 0) return fail(PDFS_EHIP, "column kernel cannot be resident");
   const int gi = sn0 == 1 && sn1 == 8 ? 0 : sn0 == 1 && sn1 == 4 ? 1 : sn0 == 5 && sn1 == 8 ? 2 : -1;
   if (graph && gi >= 0 && !profile) {
-    if (!c->gexec[gi] || c->g_tile[gi] != c->tile || c->g_iso[gi] != sp->iso ||
+    if (!c->gexec[gi] || c->g_tile[gi] != c->tile || c->g_mode[gi] != mode ||
         c->g_grid[gi] != grid) {
       if (c->gexec[gi]) {
         hipGraphExecDestroy(c->gexec[gi]);
@@ -456,7 +530,7 @@ static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
       for (int sn = sn0; sn <= sn1; sn++)
-        if (pdfsk_sweep_batch_f32(&c->bt, sn, sp, c->tile, grid, c->st)) {
+        if (pdfsk_sweep_batch(&c->bt, sn, mode, sp, sp64, c->tile, grid, c->st)) {
           hipStreamEndCapture(c->st, &g);
           return fail(PDFS_EHIP, "launch during capture failed");
         }
@@ -465,7 +539,7 @@ static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int
       hipGraphDestroy(g);
       if (e != hipSuccess) return fail(PDFS_EHIP, "graph instantiate: %s", hipGetErrorString(e));
       c->g_tile[gi] = c->tile;
-      c->g_iso[gi] = sp->iso;
+      c->g_mode[gi] = mode;
       c->g_grid[gi] = grid;
     }
     HIPCHK(hipGraphLaunch(c->gexec[gi], c->st));
@@ -473,7 +547,7 @@ static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int
   }
   for (int sn = sn0; sn <= sn1; sn++) {
     if (profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1)], c->st));
-    KCHK(pdfsk_sweep_batch_f32(&c->bt, sn, sp, c->tile, grid, c->st));
+    KCHK(pdfsk_sweep_batch(&c->bt, sn, mode, sp, sp64, c->tile, grid, c->st));
     if (profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1) + 1], c->st));
   }
   if (profile) {
@@ -487,18 +561,27 @@ static int run_orderings(pdfs_ctx *c, const pdfs_sp32 *sp, int sn0, int sn1, int
   return PDFS_OK;
 }
 
-/* Per-field points changed since the last read; returns the timeout flag. */
-static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
+/* Per-field points changed since the last read; returns the timeout flag.
+ * maxdiff (FP64 plans, or NULL): per-field largest change since the last
+ * read, which is then reset. */
+static int read_changed_md(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff)
 {
   HIPCHK(hipMemcpyAsync(c->h_changed, c->bt.changed, (size_t)c->nf * sizeof(unsigned),
                         hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(c->h_err, c->bt.err, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(c->h_streak, c->bt.streak, (size_t)2 * c->nf * sizeof(unsigned),
                         hipMemcpyDeviceToHost, c->st));
+  if (maxdiff) {
+    HIPCHK(hipMemcpyAsync(c->h_maxdiff, c->bt.maxdiff,
+                          (size_t)c->nf * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->st));
+    KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)2 * c->nf, c->st));
+  }
   HIPCHK(hipStreamSynchronize(c->st));
   for (int f = 0; f < c->nf; f++) {
     delta[f] = (long long)(unsigned)(c->h_changed[f] - c->prev_changed[f]);
     c->prev_changed[f] = c->h_changed[f];
+    if (maxdiff) memcpy(&maxdiff[f], &c->h_maxdiff[f], sizeof(double));
   }
   *err = c->h_err[0];
   if (*err) {
@@ -507,6 +590,275 @@ static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
     hipStreamSynchronize(c->st);
   }
   return PDFS_OK;
+}
+
+static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
+{
+  return read_changed_md(c, delta, err, NULL);
+}
+
+/* FP64 column variant: explicit (a 16-wide tile), else 16x16. */
+static int tile64_of(const pdfs_opts *o)
+{
+  if (o && o->tile64 >= 0 && o->tile64 < PDFSK_TILE_COUNT &&
+      pdfsk_tile_supports(o->tile64, PDFSK_MODE_F64_REF))
+    return o->tile64;
+  return PDFSK_TILE_16x16;
+}
+
+/* FP64 column sweeps of nf dense FP64 fields in mode PDFSK_MODE_F64_*.
+ * fixed_passes > 0: exactly that many passes; else passes until a pass's
+ * largest change is <= tol, at most maxp (PDFS_ENOTCONV beyond). A field
+ * whose last ordering changed nothing is at its fixed point: the device
+ * skips its later orderings, and the pass after it, which would change
+ * nothing, is counted but not launched. field_passes / field_maxdiff (or
+ * NULL) receive each field's pass count and its last pass's largest change.
+ * st: passes (the largest), orderings, point_updates, last_maxdiff (the
+ * largest of the fields' last passes), ms_sweeps. */
+static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
+                         const float *const *inits, const int *dims, const pdfs_sp64 *sp64,
+                         int tile, int fixed_passes, double tol, int maxp, int graph,
+                         pdfs_stats *st, int *field_passes, double *field_maxdiff)
+{
+  pdfs_stats s;
+  memset(&s, 0, sizeof s);
+  int rc = plan_build_ex(c, nf, (void *const *)fields, inits, dims, tile, 0, 2);
+  if (rc) return rc;
+  long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
+  double *md = (double *)calloc((size_t)nf, sizeof(double));
+  double *fmd = (double *)calloc((size_t)nf, sizeof(double));
+  int *active = (int *)malloc((size_t)nf * sizeof(int));
+  int *fp = (int *)calloc((size_t)nf, sizeof(int));
+  if (!delta || !md || !fmd || !active || !fp) {
+    rc = fail(PDFS_ENOMEM, "sweep64: host alloc");
+    goto out;
+  }
+  for (int f = 0; f < nf; f++) active[f] = 1;
+  if (fixed_passes > 0) maxp = fixed_passes;
+  if (maxp < 1) maxp = 1;
+  int nactive = nf;
+  HIPGO(hipEventRecord(c->ev0, c->st));
+  for (int p = 0; p < maxp && nactive > 0; p++) {
+    long long npts = 0;
+    for (int f = 0; f < nf; f++)
+      if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+    if ((rc = run_orderings(c, mode, NULL, sp64, 1, 8, 0, graph, NULL))) break;
+    unsigned err = 0;
+    if ((rc = read_changed_md(c, delta, &err, md))) break;
+    if (err) {
+      rc = fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+      break;
+    }
+    s.orderings += 8;
+    s.point_updates += 8 * npts;
+    int changed_set = 0;
+    for (int f = 0; f < nf; f++) {
+      if (!active[f]) continue;
+      const long long nfp = (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+      const unsigned run = c->h_streak[2 * f + 1];
+      long long skipped = run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
+      if (skipped > 8) skipped = 8;
+      s.point_updates -= skipped * nfp;
+      fp[f] = p + 1;
+      fmd[f] = md[f];
+      int done = fixed_passes > 0 ? p + 1 >= fixed_passes : md[f] <= tol;
+      if (!done && run >= PDFSK_CONVERGED_RUN) {
+        /* at its fixed point: every later pass changes nothing */
+        if (fixed_passes > 0) {
+          fp[f] = fixed_passes;
+          done = 1;
+        } else if (p + 2 <= maxp) {
+          fp[f] = p + 2;
+          fmd[f] = 0.0;
+          done = 1;
+        }
+      }
+      if (done) {
+        active[f] = 0;
+        nactive--;
+        changed_set = 1;
+      }
+    }
+    if (changed_set && nactive > 0 && (rc = plan_upload_cols(c, active))) break;
+  }
+  HIPGO(hipEventRecord(c->ev1, c->st));
+  HIPGO(hipEventSynchronize(c->ev1));
+  {
+    float t = 0.f;
+    hipEventElapsedTime(&t, c->ev0, c->ev1);
+    s.ms_sweeps = t;
+    s.ms_total = t;
+  }
+  for (int f = 0; f < nf; f++) {
+    if (fp[f] > s.passes) s.passes = fp[f];
+    if (fmd[f] > s.last_maxdiff || fmd[f] != fmd[f]) s.last_maxdiff = fmd[f];
+    if (field_passes) field_passes[f] = fp[f];
+    if (field_maxdiff) field_maxdiff[f] = fmd[f];
+  }
+  s.tile = tile;
+  if (rc == PDFS_OK && nactive > 0)
+    rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, maxp);
+out:
+  if (st) *st = s;
+  free(delta);
+  free(md);
+  free(fmd);
+  free(active);
+  free(fp);
+  return rc;
+}
+
+/* Dense FP64 working fields in the context's buffer: offsets (in doubles,
+ * 256-byte aligned) of n fields; grows the buffer as needed. */
+static int pbuf_layout(pdfs_ctx *c, int n, const long long *npts, size_t *off)
+{
+  size_t tot = 0;
+  for (int i = 0; i < n; i++) {
+    off[i] = tot;
+    tot += ((size_t)npts[i] + 31) / 32 * 32;
+  }
+  if (tot > c->pcap) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    hipFree(c->pbuf);
+    c->pbuf = NULL;
+    c->pcap = 0;
+    if (hipMalloc((void **)&c->pbuf, tot * sizeof(double)) != hipSuccess)
+      return fail(PDFS_ENOMEM, "FP64 working fields: %zu bytes", tot * sizeof(double));
+    c->pcap = tot;
+  }
+  return PDFS_OK;
+}
+
+/* FP64 polish of one group of fields that fits one FP64 plan. */
+static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *const *inits,
+                        const int *dims, const pdfs_sp64 *sp64, double tol, int maxp, int tile,
+                        int graph, pdfs_stats *ps)
+{
+  long long *np = (long long *)malloc((size_t)k * sizeof(long long));
+  size_t *off = (size_t *)malloc((size_t)k * sizeof(size_t));
+  double **f64 = (double **)malloc((size_t)k * sizeof(double *));
+  int rc = PDFS_OK;
+  long long nmax = 0;
+  if (!np || !off || !f64) {
+    rc = fail(PDFS_ENOMEM, "polish: host alloc");
+    goto out;
+  }
+  for (int i = 0; i < k; i++) {
+    np[i] = (long long)dims[3 * i] * dims[3 * i + 1] * dims[3 * i + 2];
+    if (np[i] > nmax) nmax = np[i];
+  }
+  if ((rc = pbuf_layout(c, k, np, off))) goto out;
+  if (k > c->cap_conv) {
+    HIPGO(hipStreamSynchronize(c->st));
+    hipFree(c->conv_dev);
+    free(c->conv_host);
+    c->conv_dev = NULL;
+    c->cap_conv = 0;
+    c->conv_host = (pdfsk_conv *)malloc((size_t)k * sizeof(pdfsk_conv));
+    if (!c->conv_host) {
+      rc = fail(PDFS_ENOMEM, "polish: host alloc");
+      goto out;
+    }
+    HIPGO(hipMalloc((void **)&c->conv_dev, (size_t)k * sizeof(pdfsk_conv)));
+    c->cap_conv = k;
+  }
+  for (int i = 0; i < k; i++) {
+    f64[i] = c->pbuf + off[i];
+    c->conv_host[i].f32 = fields[i];
+    c->conv_host[i].f64 = f64[i];
+    c->conv_host[i].n = np[i];
+  }
+  HIPGO(hipMemcpyAsync(c->conv_dev, c->conv_host, (size_t)k * sizeof(pdfsk_conv),
+                       hipMemcpyHostToDevice, c->st));
+  HIPGO(hipStreamSynchronize(c->st));
+  KGO(pdfsk_convert_batch(c->conv_dev, k, nmax, 0, c->st));
+  {
+    const int mode = sp64->iso ? PDFSK_MODE_F64_POL_ISO : PDFSK_MODE_F64_POL_ANISO;
+    rc = sweep64_batch(c, mode, k, f64, inits, dims, sp64, tile, 0, tol, maxp, graph, ps, NULL,
+                       NULL);
+    if (rc && rc != PDFS_ENOTCONV) goto out;
+    const int rc0 = rc;
+    rc = PDFS_OK;
+    KGO(pdfsk_convert_batch(c->conv_dev, k, nmax, 1, c->st));
+    HIPGO(hipStreamSynchronize(c->st));
+    rc = rc0;
+  }
+out:
+  free(np);
+  free(off);
+  free(f64);
+  return rc;
+}
+
+/* FP64 polish of converged FP32 fields (those with an init): widen to FP64,
+ * passes of u = min(init, G64(u)) on the column pipeline until a pass
+ * changes no value by more than polish_tol * hmin, then round back to FP32.
+ * It removes the storage rounding that FP32 sweeps accumulate along
+ * characteristics. Fields go in groups whose FP64 edge entries fit one plan;
+ * a field beyond the FP64 column limit (2^28 points) is left as it is and
+ * counted in polish_skipped. */
+int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
+                 const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
+                 pdfs_stats *s)
+{
+  if (!inits) return PDFS_OK;
+  const int tile = tile64_of(o);
+  pdfs_sp64 sp64;
+  make_sp64(dx, dy, dz, &sp64);
+  const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
+  const double tol = (o->polish_tol > 0 ? o->polish_tol : 1e-5) * hmin;
+  const int maxp = o->max_passes > 0 ? o->max_passes : 100;
+  float **gf = (float **)malloc((size_t)nf * sizeof(float *));
+  const float **gi = (const float **)malloc((size_t)nf * sizeof(float *));
+  int *gd = (int *)malloc((size_t)3 * nf * sizeof(int));
+  int rc = PDFS_OK, k = 0;
+  float tms = 0.f;
+  if (!gf || !gi || !gd) {
+    rc = fail(PDFS_ENOMEM, "polish: host alloc");
+    goto out;
+  }
+  HIPGO(hipEventRecord(c->kev[14], c->st));
+  for (int f = 0; f <= nf; f++) {
+    if (f < nf) {
+      if (!inits[f]) continue;
+      if ((long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2] > PDFSK_MAX_POINTS64 ||
+          !plan_fits_w(1, dims + 3 * f, tile, 2)) {
+        s->polish_skipped++;
+        continue;
+      }
+      memcpy(gd + 3 * k, dims + 3 * f, 3 * sizeof(int));
+    }
+    /* run the group when field f does not fit with it, or at the end */
+    if (k > 0 && (f == nf || !plan_fits_w(k + 1, gd, tile, 2))) {
+      pdfs_stats ps;
+      memset(&ps, 0, sizeof ps);
+      const int r = polish_group(c, k, gf, gi, gd, &sp64, tol, maxp, tile, o->graph, &ps);
+      if (ps.passes > s->polish_passes) s->polish_passes = ps.passes;
+      s->polish_orderings += ps.orderings;
+      if (ps.last_maxdiff > s->polish_maxdiff) s->polish_maxdiff = ps.last_maxdiff;
+      if (r != PDFS_OK && r != PDFS_ENOTCONV) {
+        rc = r;
+        goto out;
+      }
+      if (r == PDFS_ENOTCONV) rc = r;
+      if (f < nf) memcpy(gd, dims + 3 * f, 3 * sizeof(int));
+      k = 0;
+    }
+    if (f < nf) {
+      gf[k] = fields[f];
+      gi[k] = inits[f];
+      k++;
+    }
+  }
+  HIPGO(hipEventRecord(c->kev[15], c->st));
+  HIPGO(hipEventSynchronize(c->kev[15]));
+  hipEventElapsedTime(&tms, c->kev[14], c->kev[15]);
+  s->ms_polish += tms;
+out:
+  free(gf);
+  free(gi);
+  free(gd);
+  return rc;
 }
 
 int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx, double dy,
@@ -519,7 +871,7 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
   make_sp32(dx, dy, dz, &sp);
   int dims[3] = {x, y, z}, rc;
   if ((rc = plan_build(c, 1, &field, dims, tile))) return rc;
-  if ((rc = run_orderings(c, &sp, sn, sn, 0, 0, NULL))) return rc;
+  if ((rc = run_orderings(c, mode32(&sp), &sp, NULL, sn, sn, 0, 0, NULL))) return rc;
   long long d = 0;
   unsigned err = 0;
   if ((rc = read_changed(c, &d, &err))) return rc;
@@ -557,6 +909,10 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
     const int x = dims[3 * i], y = dims[3 * i + 1], z = dims[3 * i + 2];
     if (!v[i] || !init[i] || x < 1 || y < 1 || z < 1)
       return fail(PDFS_EINVAL, "judge: field %d: bad pointer or dims", i);
+    /* 32-bit element offsets in the judge kernel */
+    if ((long long)x * y * z > PDFSK_MAX_POINTS)
+      return fail(PDFS_EINVAL, "judge: field %d: %lld points exceed the limit %lld", i,
+                  (long long)x * y * z, (long long)PDFSK_MAX_POINTS);
     pdfsk_judge_field *t = &c->jt_host[i];
     t->v = v[i];
     t->init = init[i];
@@ -615,7 +971,8 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
  * the caller's spacings as the reference does). */
 static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
                      const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
-                     pdfs_stats *st, int *field_passes, double *field_ms);
+                     pdfs_stats *st, int *field_passes, double *field_ms,
+                     long long *field_flagged);
 
 int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
                              const float *const *inits, const int *dims, double dx, double dy,
@@ -624,12 +981,21 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
 {
   if (!c || nf < 1 || !fields || !dims || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
-  return batch_f32(c, nf, fields, inits, dims, dx, dy, dz, o, st, field_passes, field_ms);
+  return batch_f32(c, nf, fields, inits, dims, dx, dy, dz, o, st, field_passes, field_ms, NULL);
+}
+
+int pdfsi_batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
+                    const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
+                    pdfs_stats *st, int *field_passes, long long *field_flagged)
+{
+  return batch_f32(c, nf, fields, inits, dims, dx, dy, dz, o, st, field_passes, NULL,
+                   field_flagged);
 }
 
 static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
                      const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
-                     pdfs_stats *st, int *field_passes, double *field_ms)
+                     pdfs_stats *st, int *field_passes, double *field_ms,
+                     long long *field_flagged)
 {
   if (!c || nf < 1 || !fields || !dims)
     return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
@@ -663,6 +1029,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   }
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
+  const int m32 = mode32(&sp);
   int rc;
   if ((rc = plan_build(c, nf, fields, dims, tile))) return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
@@ -698,6 +1065,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   for (int f = 0; f < nf; f++) {
     active[f] = 1;
     lastfl[f] = -1;  /* not judged yet */
+    if (field_flagged) field_flagged[f] = 0;
     if (field_passes) field_passes[f] = 0;
     if (field_ms) field_ms[f] = 0.0;
   }
@@ -731,17 +1099,17 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
       if (tuning) {
         /* ordering 1 untimed, candidate v on ordering v + 2, the best on 8 */
         double cand_ms[PDFSK_TILE_COUNT], tk = 0.0;
-        if ((rc = run_orderings(c, &sp, 1, 1, o->profile, 0, &kms))) break;
+        if ((rc = run_orderings(c, m32, &sp, NULL, 1, 1, o->profile, 0, &kms))) break;
         for (int v = 0; v < PDFSK_TILE_COUNT && !rc; v++) {
           cand_ms[v] = 1e30;
           if (!plan_fits(nf, dims, v)) {
             /* its ordering still runs, on the current tile */
-            rc = run_orderings(c, &sp, v + 2, v + 2, o->profile, 0, &kms);
+            rc = run_orderings(c, m32, &sp, NULL, v + 2, v + 2, o->profile, 0, &kms);
             continue;
           }
-          if (c->tile != v && (rc = plan_build_ex(c, nf, fields, dims, v, 1))) break;
+          if (c->tile != v && (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, dims, v, 1, 1))) break;
           tk = 0.0;
-          rc = run_orderings(c, &sp, v + 2, v + 2, 1, 0, &tk);
+          rc = run_orderings(c, m32, &sp, NULL, v + 2, v + 2, 1, 0, &tk);
           cand_ms[v] = tk;
           if (o->profile) kms += tk;
         }
@@ -752,9 +1120,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
         tuning = 0;
         tune_store(c, key, best);
         tile = best;
-        if (c->tile != best && (rc = plan_build_ex(c, nf, fields, dims, best, 1))) break;
-        if ((rc = run_orderings(c, &sp, PDFSK_TILE_COUNT + 2, 8, o->profile, 0, &kms))) break;
-      } else if ((rc = run_orderings(c, &sp, sn0, sn1, o->profile, o->graph, &kms))) {
+        if (c->tile != best && (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, dims, best, 1, 1))) break;
+        if ((rc = run_orderings(c, m32, &sp, NULL, PDFSK_TILE_COUNT + 2, 8, o->profile, 0, &kms))) break;
+      } else if ((rc = run_orderings(c, m32, &sp, NULL, sn0, sn1, o->profile, o->graph, &kms))) {
         break;
       }
       HIPGO(hipEventRecord(c->ev1, c->st));
@@ -858,6 +1226,13 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   s.last_changed = last_total;
   s.ms_sweeps = ms - s.ms_judge;
   s.ms_sweep_kernels = kms;
+  /* FP64 polish of the converged FP32 fields (they keep their FP32 tile
+   * choice; the plan is rebuilt for the FP64 sweeps) */
+  if ((rc == PDFS_OK || rc == PDFS_ENOTCONV) && o->polish && inits && o->fixed_passes <= 0) {
+    const int rc0 = rc;
+    rc = pdfsi_polish(c, nf, fields, inits, dims, dx, dy, dz, o, &s);
+    if (rc == PDFS_OK) rc = rc0;
+  }
   if (rc == PDFS_OK && o->judge && inits) {
     float tj_before = (float)s.ms_judge;
     int judged = 0;
@@ -865,13 +1240,17 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     int nj = 0;
     for (int f = 0; f < nf; f++) {
       if (!inits[f] || certified[f]) continue;
+      jidx[nj] = f;
       jv[nj] = fields[f];
       ji[nj] = inits[f];
       memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
       nj++;
     }
     if (nj > 0 && (rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl)) == PDFS_OK) {
-      for (int k = 0; k < nj; k++) s.judge_flagged += jfl[k];
+      for (int k = 0; k < nj; k++) {
+        s.judge_flagged += jfl[k];
+        if (field_flagged) field_flagged[jidx[k]] = jfl[k];
+      }
       judged = nj;
     }
     HIPGO(hipEventRecord(c->ev1, c->st));
@@ -881,8 +1260,11 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     s.ms_judge = tj_before + t;
     s.judge_runs += judged;
   }
-  s.ms_total = s.ms_sweeps + s.ms_judge;
+  s.ms_total = s.ms_sweeps + s.ms_judge + s.ms_polish;
   if (st) *st = s;
+  if (field_flagged)
+    for (int f = 0; f < nf; f++)
+      if (active[f] && o->fixed_passes <= 0) field_flagged[f] = -1;
   if (rc == PDFS_OK && o->fixed_passes <= 0 && nactive > 0)
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, s.passes);
 out:
@@ -907,7 +1289,7 @@ static int single_f32(pdfs_ctx *c, float *field, const float *init, int x, int y
   if (!c || !field || x < 1 || y < 1 || z < 1)
     return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
   int dims[3] = {x, y, z};
-  return batch_f32(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL);
+  return batch_f32(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL, NULL);
 }
 
 int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
@@ -918,14 +1300,60 @@ int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int 
   return single_f32(c, field, init, x, y, z, dx, dy, dz, o, st);
 }
 
+/* Whether a padded FP64 field runs on the column pipeline: its ghost layer
+ * is all +inf (the reference's solver input, single_gpu_cuda/src/
+ * vti_parser.c:137-180; the column kernel's ghosts are implicit +inf) and it
+ * fits the kernel's buffer offsets. Other fields run the per-level kernel,
+ * which reads the ghost cells. */
+static int col64_ok(pdfs_ctx *c, const double *padded, int x, int y, int z, int tile, int *ok)
+{
+  *ok = 0;
+  const int dims[3] = {x, y, z};
+  if ((long long)x * y * z > PDFSK_MAX_POINTS64 || !plan_fits_w(1, dims, tile, 2)) return PDFS_OK;
+  HIPCHK(hipMemsetAsync(c->d_u64 + 1, 0, sizeof(unsigned long long), c->st));
+  KCHK(pdfsk_count_ghost_not_inf(padded, x, y, z, c->d_u64 + 1, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_pin64 + 1, c->d_u64 + 1, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  *ok = c->h_pin64[1] == 0;
+  return PDFS_OK;
+}
+
+/* Reference-arithmetic FP64 solve of a padded device field, in place:
+ * `passes` passes (> 0), or passes until the largest change of a pass is
+ * <= tol, at most maxp (PDFS_ENOTCONV beyond). On the column pipeline
+ * (bit-identical to the reference's per-level schedule) when col64_ok;
+ * with a slowness field, or otherwise, on the per-level kernel. */
 static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, int y, int z,
-                       double dx, double dy, double dz, int passes, double tol, pdfs_stats *st)
+                       double dx, double dy, double dz, int passes, double tol, int maxp,
+                       int tile, pdfs_stats *st)
 {
   pdfs_stats s;
   memset(&s, 0, sizeof s);
-  int maxp = passes > 0 ? passes : 1000;
+  if (maxp < 1) maxp = 1000;
   long long npts = (long long)x * y * z;
+  int ok = 0, rc;
+  if (!slow && (rc = col64_ok(c, padded, x, y, z, tile, &ok))) return rc;
+  if (ok) {
+    size_t off = 0;
+    if ((rc = pbuf_layout(c, 1, &npts, &off))) return rc;
+    double *dn = c->pbuf;
+    const int dims[3] = {x, y, z};
+    pdfs_sp64 sp64;
+    make_sp64(dx, dy, dz, &sp64);
+    KCHK(pdfsk_pack64(padded, dn, x, y, z, c->st));
+    rc = sweep64_batch(c, PDFSK_MODE_F64_REF, 1, &dn, NULL, dims, &sp64, tile,
+                       passes > 0 ? passes : 0, tol, maxp, 0, &s, NULL, NULL);
+    if (rc && rc != PDFS_ENOTCONV) return rc;
+    KCHK(pdfsk_unpack64(dn, padded, x, y, z, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (st) *st = s;
+    return rc;
+  }
+  int fixed = passes > 0;
+  if (fixed) maxp = passes;
   float ms = 0.f;
+  double md = 0.0;
   for (int p = 0; p < maxp; p++) {
     HIPCHK(hipMemsetAsync(c->d_u64, 0, sizeof(unsigned long long), c->st));
     HIPCHK(hipEventRecord(c->ev0, c->st));
@@ -938,18 +1366,20 @@ static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, i
     float t = 0.f;
     hipEventElapsedTime(&t, c->ev0, c->ev1);
     ms += t;
-    double md;
     unsigned long long b = c->h_pin64[0];
     memcpy(&md, &b, sizeof md);
     s.passes++;
     s.orderings += 8;
     s.point_updates += 8 * npts;
     s.last_maxdiff = md;
-    if (passes <= 0 && md <= tol) break;
+    if (!fixed && md <= tol) break;
   }
   s.ms_sweeps = ms;
   s.ms_total = ms;
   if (st) *st = s;
+  if (!fixed && !(md <= tol))
+    return fail(PDFS_ENOTCONV, "FP64 solve not converged after %d passes (max change %g)", maxp,
+                md);
   return PDFS_OK;
 }
 
@@ -958,7 +1388,8 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
 {
   if (!c || !padded || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_ref64_dev: bad arguments");
-  return solve_ref64(c, padded, NULL, x, y, z, dx, dy, dz, passes, tol, st);
+  return solve_ref64(c, padded, NULL, x, y, z, dx, dy, dz, passes, tol, 1000, PDFSK_TILE_16x16,
+                     st);
 }
 
 int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
@@ -967,7 +1398,8 @@ int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, 
 {
   if (!c || !padded || !slowness || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_speed64_dev: bad arguments");
-  return solve_ref64(c, padded, slowness, x, y, z, dx, dy, dz, passes, tol, st);
+  return solve_ref64(c, padded, slowness, x, y, z, dx, dy, dz, passes, tol, 1000,
+                     PDFSK_TILE_16x16, st);
 }
 
 /* ---- host padded FP64 entry points ---- */
@@ -1015,7 +1447,8 @@ static int solve_host(double *distance, int x, int y, int z, double dx, double d
   }
   if (o->precision == PDFS_F64_REF) {
     rc = solve_ref64(c, d, NULL, x, y, z, dx, dy, dz, o->fixed_passes > 0 ? o->fixed_passes : 0,
-                     o->tol, st);
+                     o->tol, o->max_passes, tile64_of(o), st);
+    if (rc && rc != PDFS_ENOTCONV) goto out;
   } else {
     if (hipMalloc((void **)&f, n * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&f0, n * sizeof(float)) != hipSuccess) {
@@ -1054,6 +1487,7 @@ void run_fsm(Phi *pf, int iterations)
   pdfs_default_opts(&o);
   o.precision = PDFS_F64_REF;
   o.fixed_passes = iterations > 0 ? iterations : 0;
+  o.max_passes = 1000;
   o.tol = 1e-12;
   pdfs_stats s;
   hipEvent_t a, b;
@@ -1064,7 +1498,8 @@ void run_fsm(Phi *pf, int iterations)
   hipEventRecord(b, 0);
   hipEventSynchronize(b);
   if (rc != PDFS_OK) {
-    printf("\nHIP failure in run_fsm: '%s'\n", pdfs_last_error());
+    printf("\n%s in run_fsm: '%s'\n", rc == PDFS_ENOTCONV ? "No convergence" : "HIP failure",
+           pdfs_last_error());
     exit(EXIT_FAILURE);
   }
   float ms = 0.f;
@@ -1072,6 +1507,38 @@ void run_fsm(Phi *pf, int iterations)
   hipEventDestroy(a);
   hipEventDestroy(b);
   printf("Parallel FSM time: %f s.\n", ms / 1000.0);
+}
+
+/* One reference ordering sn of a padded device field, in place: on the
+ * column pipeline when col64_ok, else on the per-level kernel. */
+static int ordering_ref64(pdfs_ctx *c, double *padded, int x, int y, int z, double dx, double dy,
+                          double dz, int sn)
+{
+  int ok = 0, rc;
+  if ((rc = col64_ok(c, padded, x, y, z, PDFSK_TILE_16x16, &ok))) return rc;
+  if (!ok) {
+    KCHK(pdfsk_sweep_ref64(padded, NULL, x, y, z, sn, dx, dy, dz, NULL, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return PDFS_OK;
+  }
+  long long npts = (long long)x * y * z;
+  size_t off = 0;
+  if ((rc = pbuf_layout(c, 1, &npts, &off))) return rc;
+  double *dn = c->pbuf;
+  const int dims[3] = {x, y, z};
+  pdfs_sp64 sp64;
+  make_sp64(dx, dy, dz, &sp64);
+  KCHK(pdfsk_pack64(padded, dn, x, y, z, c->st));
+  if ((rc = plan_build_ex(c, 1, (void *const *)&dn, NULL, dims, PDFSK_TILE_16x16, 0, 2))) return rc;
+  if ((rc = run_orderings(c, PDFSK_MODE_F64_REF, NULL, &sp64, sn, sn, 0, 0, NULL))) return rc;
+  long long d = 0;
+  unsigned err = 0;
+  double md = 0.0;
+  if ((rc = read_changed_md(c, &d, &err, &md))) return rc;
+  if (err) return fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
+  KCHK(pdfsk_unpack64(dn, padded, x, y, z, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return PDFS_OK;
 }
 
 /* mpi_openacc/src/phi3D_fsm.c:67: one ordering sn, in place. */
@@ -1086,10 +1553,9 @@ void runFSM(PhiACC *pf, int sn)
   double *d = NULL;
   if (hipMalloc((void **)&d, np * sizeof(double)) != hipSuccess ||
       hipMemcpy(d, pf->distance, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      pdfsk_sweep_ref64(d, NULL, pf->x, pf->y, pf->z, sn, pf->dx, pf->dy, pf->dz, NULL, c->st) != 0 ||
-      hipStreamSynchronize(c->st) != hipSuccess ||
+      ordering_ref64(c, d, pf->x, pf->y, pf->z, pf->dx, pf->dy, pf->dz, sn) != PDFS_OK ||
       hipMemcpy(pf->distance, d, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
-    printf("\nHIP failure in runFSM\n");
+    printf("\nHIP failure in runFSM: '%s'\n", pdfs_last_error());
     exit(EXIT_FAILURE);
   }
   hipFree(d);
@@ -1207,7 +1673,8 @@ int pdfs_calc_dist_field(Phi *pf, int passes, int precision)
     goto out;
   }
   if (precision == PDFS_F64_REF) {
-    if ((rc = solve_ref64(c, d, NULL, x, y, z, pf->dx, pf->dy, pf->dz, passes, 1e-12, NULL)))
+    if ((rc = solve_ref64(c, d, NULL, x, y, z, pf->dx, pf->dy, pf->dz, passes, 1e-12, 1000,
+                          PDFSK_TILE_16x16, NULL)))
       goto out;
   } else {
     pdfs_opts o;

@@ -63,6 +63,10 @@ def lib():
         L.orc_judge_f32.restype = C.c_long
         L.orc_residual_f64.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D]
         L.orc_residual_f64.restype = _D
+        L.orc_sweep_polish64.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D, _I]
+        L.orc_sweep_polish64.restype = _D
+        L.orc_polish64.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D, _D, _I, C.POINTER(_D)]
+        L.orc_polish64.restype = _I
         _lib = L
     return _lib
 
@@ -139,6 +143,31 @@ def solve_f32(a, h=(1.0, 1.0, 1.0), max_passes=100):
     ch = C.c_long()
     p = lib().orc_solve_f32(a, *dims(a), *h, max_passes, C.byref(ch))
     return p, ch.value
+
+
+def sweep_polish64(a, init, sn, h=(1.0, 1.0, 1.0)):
+    """One ordering of u = min(init, G64(u)) in place; returns the largest change."""
+    return lib().orc_sweep_polish64(a, init, *dims(a), *h, sn)
+
+
+def polish64(a, init, h=(1.0, 1.0, 1.0), tol=1e-5, max_passes=100):
+    """Polish passes until a pass changes nothing by more than tol (in place);
+    returns (passes, last pass's largest change)."""
+    md = _D()
+    p = lib().orc_polish64(a, init, *dims(a), *h, tol, max_passes, C.byref(md))
+    return p, md.value
+
+
+def solve_f32_polished(init, h=(1.0, 1.0, 1.0), tol_h=1e-5, max_passes=100):
+    """The FP32 solve followed by the FP64 polish, as the GPU's default FP32
+    solve runs it: returns (FP32 result (padded), FP32 passes, polish passes).
+    init: padded FP64 whose interior values are FP32-representable."""
+    u = init.astype(np.float32)
+    p32, _ = solve_f32(u, h=h, max_passes=max_passes)
+    v = u.astype(np.float64)
+    i64 = init.astype(np.float32).astype(np.float64)
+    pp, _ = polish64(v, i64, h=h, tol=tol_h * min(h), max_passes=max_passes)
+    return v.astype(np.float32), p32, pp
 
 
 def gen_point(X, Y, Z, seed):

@@ -183,10 +183,9 @@ def test_warm_solve_good_bound_certifies_without_fallback(ctx, net, dev):
 def test_warm_solve_bad_bound_falls_back(ctx, net, dev):
     """An underestimating warm start (everything 0.5 h) leaves a wrong field;
     the judge flags it and the solver falls back to the plain solve, whose
-    result equals the plain FP32 oracle bit for bit."""
+    result equals the oracle's FP32 solve + FP64 polish bit for bit."""
     a = _spheres(91)
-    ref32 = a.astype(np.float32)
-    O.solve_f32(ref32)
+    ref32, _, _ = O.solve_f32_polished(a)
     init = torch.from_numpy(np.ascontiguousarray(O.interior(a)).astype(np.float32)).to(dev)
     net.set_calibration(1.0 / 32, 0.0, 0.5)
     f = init.clone()

@@ -439,3 +439,70 @@ def test_speed_monotone_in_slowness():
     inner = (slice(1, -1),) * 3
     assert np.all(u1[inner] <= u2[inner])
     assert np.any(u1[inner] < u2[inner])
+
+
+# ---------------------------------------------------------------- FP64 polish
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.7, 1.3)])
+def test_polish_limit_is_the_fp64_solution(h):
+    """The polish (u = min(init, G64(u)), which may raise values) started
+    from the FP32 solution converges to the monotone FP64 difference-form
+    solve's limit (a different iteration of the same fixed-point equation)."""
+    a = O.gen_spheres(30, 26, 22, seed=91, n=5, r=(2.0, 5.0))
+    u64 = a.copy()
+    O.solve_d64(u64, h=h, tol=0.0, max_passes=200)
+    v = a.astype(np.float32)
+    O.solve_f32(v, h=h)
+    w = v.astype(np.float64)
+    passes, md = O.polish64(w, a, h=h, tol=0.0, max_passes=200)
+    assert md == 0.0 and passes < 200
+    diff = np.abs(O.interior(w) - O.interior(u64))
+    assert diff.max() <= 1e-12 * max(1.0, float(np.abs(O.interior(u64)).max()))
+
+
+def test_polish_raises_a_planted_region_where_min_sweeps_cannot():
+    """A box planted 0.5 h below the FP64 solution comes back to it under the
+    polish; the monotone min(u, G(u)) difference-form sweeps leave it low
+    (which a planted monotone update in the polish would do too)."""
+    a = O.gen_spheres(34, 30, 26, seed=92, n=4, r=(2.0, 5.0))
+    u64 = a.copy()
+    O.solve_d64(u64, tol=0.0, max_passes=200)
+    bad = u64.copy()
+    bad[8:14, 9:15, 10:18] -= 0.5
+    pol = bad.copy()
+    O.polish64(pol, a, tol=0.0, max_passes=200)
+    assert np.abs(O.interior(pol) - O.interior(u64)).max() <= 1e-9
+    mono = bad.copy()
+    O.solve_d64(mono, tol=0.0, max_passes=200)
+    assert np.abs(O.interior(mono) - O.interior(u64)).max() >= 0.4
+
+
+def test_polish_point_source_axis_exact():
+    """Along the axis lines through a point source the polished FP32 field
+    holds n h exactly (1-D updates from the exact source)."""
+    X, Y, Z = 40, 36, 30
+    a = O.gen_point(X, Y, Z, seed=93)
+    i, j, k = (int(v) for v in np.argwhere(a == 0.0)[0])
+    want, _, pp = O.solve_f32_polished(a)
+    assert pp >= 1
+    for kk in range(1, X + 1):
+        assert want[i, j, kk] == abs(kk - k)
+    for jj in range(1, Y + 1):
+        assert want[i, jj, k] == abs(jj - j)
+    for ii in range(1, Z + 1):
+        assert want[ii, j, k] == abs(ii - i)
+
+
+def test_polish_cuts_the_fp32_storage_error():
+    """On a field with distances near 100 h the FP32 fixed point carries the
+    storage rounding accumulated along characteristics; the polish brings it
+    to the 1e-5 h polish tolerance and below the FP32 error."""
+    a = O.gen_spheres(96, 96, 96, seed=94, n=3, r=(2.0, 4.0))
+    u64 = a.copy()
+    O.solve_d64(u64, tol=1e-12, max_passes=200)
+    raw = a.astype(np.float32)
+    O.solve_f32(raw)
+    pol, _, pp = O.solve_f32_polished(a)
+    e_raw = np.abs(O.interior(raw).astype(np.float64) - O.interior(u64)).max()
+    e_pol = np.abs(O.interior(pol).astype(np.float64) - O.interior(u64)).max()
+    print(f"FP32 {e_raw:.3e} h, polished {e_pol:.3e} h ({pp} polish passes)")
+    assert e_pol <= 1e-5 and e_pol < e_raw

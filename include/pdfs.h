@@ -168,6 +168,21 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
 int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
                            int z, double dx, double dy, double dz, int passes, double tol,
                            pdfs_stats *st);
+/* One ordering over a dense FP32 field, in place, with no device
+ * allocation. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, allocation-free
+ * serving loops). Same sweep and result as pdfs_sweep_f32_dev; its scratch
+ * (launch state, tickets, counters, edge entries) is one 64 MiB device array
+ * per GPU, held until the process ends. Fields whose edge entries exceed
+ * 63 MiB (or more than 87381 columns) are rejected with PDFS_EINVAL (for
+ * 16x16 tiles, about 384^3; tile as in pdfs_sweep_f32_dev). Launches on
+ * `stream` (a hipStream_t, NULL: the null stream) without synchronising:
+ * result (device, 2 words) gets result[0] += points changed and result[1] |=
+ * 1 if an edge wait timed out. Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU race, so issue them on one stream. */
+int pdfs_sweep_f32_noalloc(float *field, int x, int y, int z, double dx, double dy, double dz,
+                           int sn, int tile, void *stream, unsigned long long *result);
+
 /* FP64 residual judge: a point fails when |v - min(init, G64(v))| >
  * |e| 2^-22 + hmin 2^-21 (e = min(init, G64(v)); equal infinities pass, an
  * infinity against a finite value fails, and so does a point whose value or

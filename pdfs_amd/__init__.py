@@ -39,7 +39,7 @@ EXPORTED = [
     "pdfs_net_features_dev", "pdfs_net_target_dev", "pdfs_net_forward_dev",
     "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_warm_start_batch_f32_dev",
     "pdfs_solve_warm_f32_dev",
-    "pdfs_solve_batch_warm_f32_dev",
+    "pdfs_solve_batch_warm_f32_dev", "pdfs_sweep_f32_noalloc",
 ]
 WEIGHTS_PATH = os.path.join(HERE, "weights", "warm_unet.bin")
 CLI_PATH = os.path.join(HERE, "bin", "PDFS")
@@ -240,6 +240,8 @@ def lib():
         L.pdfs_solve_batch_warm_f32_dev.argtypes = [vp, vp, I, vp, vp, vp, D, D, D,
                                                     C.POINTER(Opts), C.POINTER(Stats), vp, vp]
         L.pdfs_solve_batch_warm_f32_dev.restype = I
+        L.pdfs_sweep_f32_noalloc.argtypes = [vp, I, I, I, D, D, D, I, I, vp, vp]
+        L.pdfs_sweep_f32_noalloc.restype = I
         _lib = L
     return _lib
 
@@ -500,6 +502,20 @@ class Context:
                                                  C.byref(o), C.byref(st), passes, fb)
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
         return st.as_dict(), list(passes), list(fb)
+
+
+def sweep_f32_noalloc(field, sn, result, h=(1.0, 1.0, 1.0), tile=-1, stream=None):
+    """One FP32 ordering sn in place with no device allocation
+    (pdfs_sweep_f32_noalloc), on `stream` (a torch.cuda.Stream, default the
+    current one). result: int64 cuda tensor of 2 elements, [0] += points
+    changed, [1] |= the edge-wait timeout flag; both written on the stream."""
+    import torch
+    assert field.dtype == torch.float32 and field.dim() == 3 and field.is_contiguous()
+    assert result.dtype == torch.int64 and result.numel() >= 2 and result.is_cuda
+    Z, Y, X = field.shape
+    st = stream if stream is not None else torch.cuda.current_stream()
+    _check(lib().pdfs_sweep_f32_noalloc(field.data_ptr(), X, Y, Z, *h, sn, tile, st.cuda_stream,
+                                        result.data_ptr()))
 
 
 def net_grid(X, Y, Z):

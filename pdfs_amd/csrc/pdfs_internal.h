@@ -5,6 +5,11 @@
 #include <stdint.h>
 
 #include "pdfs.h"
+#include "pdfs_kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
 
 /* set the thread's last-error message (pdfs_last_error) and return code */
 int pdfsi_fail(int code, const char *fmt, ...);
@@ -34,4 +39,12 @@ int pdfsi_batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *cons
                     const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
                     pdfs_stats *st, int *field_passes, long long *field_flagged);
 
+/* FP32 spacing constants in the oracle's operation order. */
+void pdfsi_make_sp32(double dx, double dy, double dz, pdfs_sp32 *s);
+/* default column variant of an FP32 sweep of nf fields (x, y of the first) */
+int pdfsi_default_tile(int nf, int x, int y);
+
+#ifdef __cplusplus
+}
+#endif
 #endif

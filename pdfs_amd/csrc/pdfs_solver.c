@@ -232,6 +232,8 @@ static void make_sp32(double dx, double dy, double dz, pdfs_sp32 *s)
   s->W = s->w[0] + s->w[1] + s->w[2];
 }
 
+void pdfsi_make_sp32(double dx, double dy, double dz, pdfs_sp32 *s) { make_sp32(dx, dy, dz, s); }
+
 static int mode32(const pdfs_sp32 *sp)
 {
   return sp->iso ? PDFSK_MODE_F32_ISO : PDFSK_MODE_F32_ANISO;
@@ -278,6 +280,8 @@ static int default_tile(int nf, int x, int y)
   if (nf > 1 || (long long)x * y >= 512LL * 512) return PDFSK_TILE_32x16;
   return PDFSK_TILE_16x16;
 }
+
+int pdfsi_default_tile(int nf, int x, int y) { return default_tile(nf, x, y); }
 
 /* the online tuner gives each candidate one of orderings 2..7 */
 _Static_assert(PDFSK_TILE_COUNT <= 6, "tile candidates must fit orderings 2..7 of a pass");

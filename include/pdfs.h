@@ -88,12 +88,12 @@ typedef struct {
                       also runs after every half pass (orderings 4 and 8) */
   int polish;      /* FP32 (converged solves of fields with an init; default 1): after the
                       FP32 stop, FP64 passes of u = min(init, G64(u)) from the FP32 field
-                      until a pass changes no value by more than polish_tol * hmin, then
-                      the result is rounded to FP32. FP32 storage rounding accumulates
-                      along characteristics (at 512^3 up to ~3e-4 h); the polish brings
-                      the result to the FP64 solution's FP32 rounding. 0: the FP32
-                      fixed point itself. */
-  double polish_tol; /* default 1e-5 */
+                      until the last ordering of a pass changes no value by more than
+                      polish_tol * hmin, then the result is rounded to FP32. FP32 storage
+                      rounding accumulates along characteristics (at 512^3 up to ~3e-4 h);
+                      one polish pass leaves ~1e-6 h (the last ordering's largest change
+                      is ~10x the error it leaves). 0: the FP32 fixed point itself. */
+  double polish_tol; /* default 2e-5 */
   int tile64;      /* FP64 column variant (16-wide tiles: 0 16x16, 2 16x8, 4, 5); -1 default */
 } pdfs_opts;
 
@@ -123,6 +123,8 @@ typedef struct {
   double ms_polish;          /* polish time, conversions included */
   int polish_skipped;        /* fields left unpolished: 2^28 points or more (FP64 column
                                 limit) */
+  long long polish_point_updates; /* interior points x polish orderings executed */
+  double ms_polish_kernels;  /* profile: sum of per-launch polish kernel times */
 } pdfs_stats;
 
 void pdfs_default_opts(pdfs_opts *o);

@@ -367,15 +367,16 @@ int orc_polish64(double *d, const double *init, int X, int Y, int Z, double dx, 
                  double dz, double tol, int max_passes, double *last_maxdiff)
 {
   int passes = 0;
-  double md = 0.0;
+  double md = 0.0, m8 = 0.0;
   while (passes < max_passes) {
     md = 0.0;
     for (int sn = 1; sn <= 8; sn++) {
       double m = orc_sweep_polish64(d, init, X, Y, Z, dx, dy, dz, sn);
       if (m > md) md = m;
+      m8 = m;
     }
     passes++;
-    if (md <= tol) break;
+    if (m8 <= tol) break;
   }
   if (last_maxdiff) *last_maxdiff = md;
   return passes;

@@ -70,8 +70,9 @@ void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
 
 /* FP64 polish: one ordering of u = min(init, G64(u)) (difference form; may
  * raise values), returning the largest change of any update; and passes of
- * it until a pass's largest change is <= tol (at most max_passes), returning
- * the passes run. init: padded FP64 (ghosts unused). */
+ * it until the largest change of a pass's last ordering is <= tol (at most
+ * max_passes), returning the passes run and (last_maxdiff) the last pass's
+ * largest change. init: padded FP64 (ghosts unused). */
 double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, double dx,
                           double dy, double dz, int sn);
 int orc_polish64(double *d, const double *init, int X, int Y, int Z, double dx, double dy,

@@ -131,7 +131,8 @@ class Stats(C.Structure):
                 ("ms_sweep_kernels", C.c_double), ("tile", C.c_int), ("judge_runs", C.c_int),
                 ("fallbacks", C.c_int), ("ms_warm", C.c_double), ("polish_passes", C.c_int),
                 ("polish_orderings", C.c_int), ("polish_maxdiff", C.c_double),
-                ("ms_polish", C.c_double), ("polish_skipped", C.c_int)]
+                ("ms_polish", C.c_double), ("polish_skipped", C.c_int),
+                ("polish_point_updates", C.c_longlong), ("ms_polish_kernels", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

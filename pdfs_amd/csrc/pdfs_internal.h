@@ -25,8 +25,8 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
                       long long *flagged);
 
 /* FP64 polish of FP32 fields with an init (those without are left alone):
- * passes of u = min(init, G64(u)) until a pass changes no value by more than
- * o->polish_tol * hmin, the result rounded back to FP32; adds to
+ * passes of u = min(init, G64(u)) until the last ordering of a pass changes no
+ * value by more than o->polish_tol * hmin, the result rounded back to FP32; adds to
  * s->polish_* and s->ms_polish. */
 int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
                  const int *dims, double dx, double dy, double dz, const pdfs_opts *o,

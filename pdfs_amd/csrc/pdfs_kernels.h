@@ -79,6 +79,7 @@ typedef struct {
                             number of consecutive orderings that changed nothing */
   int nfields;
   unsigned long long *maxdiff; /* [nfields] FP64 modes: max |old - new| bits (accumulate) */
+  unsigned long long *maxdiff_last; /* [nfields] or NULL: ordering 8 accumulates here instead */
   unsigned *err;         /* bounded-spin timeout flag */
   uint64_t *edge_x, *edge_y;        /* face parts (index -PDFSK_EDGE_PAD) of one allocation
                                        that starts at edge_x; edge_y follows the x part */

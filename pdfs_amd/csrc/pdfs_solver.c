@@ -48,7 +48,7 @@ struct pdfs_ctx {
   int cap_jt;
   double *d_params;
   hipEvent_t ev0, ev1;
-  hipEvent_t kev[16];
+  hipEvent_t kev[18];   /* [0..15] per-ordering profile, [16..17] polish */
 };
 
 static __thread char g_err[512];
@@ -135,7 +135,7 @@ void pdfs_default_opts(pdfs_opts *o)
   o->certify = 0;
   o->certify_from = 1;
   o->polish = 1;
-  o->polish_tol = 1e-5;
+  o->polish_tol = 2e-5;
   o->tile64 = -1;
 }
 
@@ -157,7 +157,7 @@ pdfs_ctx *pdfs_ctx_create(void)
       hipHostMalloc((void **)&c->h_pin64, 4 * sizeof(unsigned long long), 0) != hipSuccess ||
       hipMalloc((void **)&c->d_params, 1024 * sizeof(double)) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      create_events(c->kev, 16) != 0) {
+      create_events(c->kev, 18) != 0) {
     fail(PDFS_EHIP, "pdfs_ctx_create: HIP initialisation failed");
     free(c);
     return NULL;
@@ -209,7 +209,7 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   free(c->h_cols);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
-  for (int i = 0; i < 16; i++) hipEventDestroy(c->kev[i]);
+  for (int i = 0; i < 18; i++) hipEventDestroy(c->kev[i]);
   hipStreamDestroy(c->st);
   free(c);
 }
@@ -474,8 +474,9 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     c->bt.maxdiff = NULL;
     c->h_maxdiff = NULL;
     c->cap_md = 0;
-    if (hipMalloc((void **)&c->bt.maxdiff, (size_t)nf * sizeof(unsigned long long)) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_maxdiff, (size_t)nf * sizeof(unsigned long long), 0) !=
+    if (hipMalloc((void **)&c->bt.maxdiff, (size_t)2 * nf * sizeof(unsigned long long)) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&c->h_maxdiff, (size_t)2 * nf * sizeof(unsigned long long), 0) !=
             hipSuccess)
       return fail(PDFS_ENOMEM, "plan: maxdiff alloc");
     c->cap_md = nf;
@@ -493,7 +494,8 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     KCHK(pdfsk_fill_u32(c->bt.streak, 0u, (size_t)2 * nf, c->st));
     memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
   }
-  KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)2 * nf, c->st));
+  c->bt.maxdiff_last = c->bt.maxdiff + c->cap_md;
+  KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)4 * c->cap_md, c->st));
   c->bt.nfields = nf;
   for (int g = 0; g < 3; g++)
     if (c->gexec[g]) {
@@ -567,8 +569,9 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
 
 /* Per-field points changed since the last read; returns the timeout flag.
  * maxdiff (FP64 plans, or NULL): per-field largest change since the last
- * read, which is then reset. */
-static int read_changed_md(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff)
+ * read, and last (or NULL) that of ordering 8 alone; both are then reset. */
+static int read_changed_md2(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff,
+                            double *last)
 {
   HIPCHK(hipMemcpyAsync(c->h_changed, c->bt.changed, (size_t)c->nf * sizeof(unsigned),
                         hipMemcpyDeviceToHost, c->st));
@@ -579,13 +582,22 @@ static int read_changed_md(pdfs_ctx *c, long long *delta, unsigned *err, double 
     HIPCHK(hipMemcpyAsync(c->h_maxdiff, c->bt.maxdiff,
                           (size_t)c->nf * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c->st));
-    KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)2 * c->nf, c->st));
+    HIPCHK(hipMemcpyAsync(c->h_maxdiff + c->cap_md, c->bt.maxdiff_last,
+                          (size_t)c->nf * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->st));
+    KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)4 * c->cap_md, c->st));
   }
   HIPCHK(hipStreamSynchronize(c->st));
   for (int f = 0; f < c->nf; f++) {
     delta[f] = (long long)(unsigned)(c->h_changed[f] - c->prev_changed[f]);
     c->prev_changed[f] = c->h_changed[f];
-    if (maxdiff) memcpy(&maxdiff[f], &c->h_maxdiff[f], sizeof(double));
+    if (maxdiff) {
+      double a, b;
+      memcpy(&a, &c->h_maxdiff[f], sizeof(double));
+      memcpy(&b, &c->h_maxdiff[c->cap_md + f], sizeof(double));
+      maxdiff[f] = b > a ? b : a;
+      if (last) last[f] = b;
+    }
   }
   *err = c->h_err[0];
   if (*err) {
@@ -596,9 +608,14 @@ static int read_changed_md(pdfs_ctx *c, long long *delta, unsigned *err, double 
   return PDFS_OK;
 }
 
+static int read_changed_md(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff)
+{
+  return read_changed_md2(c, delta, err, maxdiff, NULL);
+}
+
 static int read_changed(pdfs_ctx *c, long long *delta, unsigned *err)
 {
-  return read_changed_md(c, delta, err, NULL);
+  return read_changed_md2(c, delta, err, NULL, NULL);
 }
 
 /* FP64 column variant: explicit (a 16-wide tile), else 16x16. */
@@ -612,7 +629,8 @@ static int tile64_of(const pdfs_opts *o)
 
 /* FP64 column sweeps of nf dense FP64 fields in mode PDFSK_MODE_F64_*.
  * fixed_passes > 0: exactly that many passes; else passes until a pass's
- * largest change is <= tol, at most maxp (PDFS_ENOTCONV beyond). A field
+ * largest change (stop_on_last: that of its last ordering) is <= tol, at
+ * most maxp (PDFS_ENOTCONV beyond). A field
  * whose last ordering changed nothing is at its fixed point: the device
  * skips its later orderings, and the pass after it, which would change
  * nothing, is counted but not launched. field_passes / field_maxdiff (or
@@ -621,15 +639,18 @@ static int tile64_of(const pdfs_opts *o)
  * largest of the fields' last passes), ms_sweeps. */
 static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
                          const float *const *inits, const int *dims, const pdfs_sp64 *sp64,
-                         int tile, int fixed_passes, double tol, int maxp, int graph,
-                         pdfs_stats *st, int *field_passes, double *field_maxdiff)
+                         int tile, int fixed_passes, double tol, int stop_on_last, int maxp,
+                         int graph, int profile, pdfs_stats *st, int *field_passes,
+                         double *field_maxdiff)
 {
+  double kms = 0.0;
   pdfs_stats s;
   memset(&s, 0, sizeof s);
   int rc = plan_build_ex(c, nf, (void *const *)fields, inits, dims, tile, 0, 2);
   if (rc) return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
-  double *md = (double *)calloc((size_t)nf, sizeof(double));
+  double *md = (double *)calloc((size_t)2 * nf, sizeof(double));
+  double *md8 = md ? md + nf : NULL;
   double *fmd = (double *)calloc((size_t)nf, sizeof(double));
   int *active = (int *)malloc((size_t)nf * sizeof(int));
   int *fp = (int *)calloc((size_t)nf, sizeof(int));
@@ -646,9 +667,9 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
     long long npts = 0;
     for (int f = 0; f < nf; f++)
       if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
-    if ((rc = run_orderings(c, mode, NULL, sp64, 1, 8, 0, graph, NULL))) break;
+    if ((rc = run_orderings(c, mode, NULL, sp64, 1, 8, profile, graph, &kms))) break;
     unsigned err = 0;
-    if ((rc = read_changed_md(c, delta, &err, md))) break;
+    if ((rc = read_changed_md2(c, delta, &err, md, md8))) break;
     if (err) {
       rc = fail(PDFS_ETIMEOUT, "column pipeline: edge wait timed out");
       break;
@@ -665,7 +686,8 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
       s.point_updates -= skipped * nfp;
       fp[f] = p + 1;
       fmd[f] = md[f];
-      int done = fixed_passes > 0 ? p + 1 >= fixed_passes : md[f] <= tol;
+      int done =
+          fixed_passes > 0 ? p + 1 >= fixed_passes : (stop_on_last ? md8[f] : md[f]) <= tol;
       if (!done && run >= PDFSK_CONVERGED_RUN) {
         /* at its fixed point: every later pass changes nothing */
         if (fixed_passes > 0) {
@@ -700,6 +722,7 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
     if (field_maxdiff) field_maxdiff[f] = fmd[f];
   }
   s.tile = tile;
+  s.ms_sweep_kernels = kms;
   if (rc == PDFS_OK && nactive > 0)
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, maxp);
 out:
@@ -736,7 +759,7 @@ static int pbuf_layout(pdfs_ctx *c, int n, const long long *npts, size_t *off)
 /* FP64 polish of one group of fields that fits one FP64 plan. */
 static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *const *inits,
                         const int *dims, const pdfs_sp64 *sp64, double tol, int maxp, int tile,
-                        int graph, pdfs_stats *ps)
+                        int graph, int profile, pdfs_stats *ps)
 {
   long long *np = (long long *)malloc((size_t)k * sizeof(long long));
   size_t *off = (size_t *)malloc((size_t)k * sizeof(size_t));
@@ -778,8 +801,8 @@ static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *c
   KGO(pdfsk_convert_batch(c->conv_dev, k, nmax, 0, c->st));
   {
     const int mode = sp64->iso ? PDFSK_MODE_F64_POL_ISO : PDFSK_MODE_F64_POL_ANISO;
-    rc = sweep64_batch(c, mode, k, f64, inits, dims, sp64, tile, 0, tol, maxp, graph, ps, NULL,
-                       NULL);
+    rc = sweep64_batch(c, mode, k, f64, inits, dims, sp64, tile, 0, tol, 1, maxp, graph, profile,
+                       ps, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) goto out;
     const int rc0 = rc;
     rc = PDFS_OK;
@@ -796,7 +819,8 @@ out:
 
 /* FP64 polish of converged FP32 fields (those with an init): widen to FP64,
  * passes of u = min(init, G64(u)) on the column pipeline until a pass
- * changes no value by more than polish_tol * hmin, then round back to FP32.
+ * changes no value by more than polish_tol * hmin in its last ordering, then
+ * round back to FP32.
  * It removes the storage rounding that FP32 sweeps accumulate along
  * characteristics. Fields go in groups whose FP64 edge entries fit one plan;
  * a field beyond the FP64 column limit (2^28 points) is left as it is and
@@ -810,7 +834,7 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
   pdfs_sp64 sp64;
   make_sp64(dx, dy, dz, &sp64);
   const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
-  const double tol = (o->polish_tol > 0 ? o->polish_tol : 1e-5) * hmin;
+  const double tol = (o->polish_tol > 0 ? o->polish_tol : 2e-5) * hmin;
   const int maxp = o->max_passes > 0 ? o->max_passes : 100;
   float **gf = (float **)malloc((size_t)nf * sizeof(float *));
   const float **gi = (const float **)malloc((size_t)nf * sizeof(float *));
@@ -821,7 +845,7 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
     rc = fail(PDFS_ENOMEM, "polish: host alloc");
     goto out;
   }
-  HIPGO(hipEventRecord(c->kev[14], c->st));
+  HIPGO(hipEventRecord(c->kev[16], c->st));
   for (int f = 0; f <= nf; f++) {
     if (f < nf) {
       if (!inits[f]) continue;
@@ -836,9 +860,12 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
     if (k > 0 && (f == nf || !plan_fits_w(k + 1, gd, tile, 2))) {
       pdfs_stats ps;
       memset(&ps, 0, sizeof ps);
-      const int r = polish_group(c, k, gf, gi, gd, &sp64, tol, maxp, tile, o->graph, &ps);
+      const int r =
+          polish_group(c, k, gf, gi, gd, &sp64, tol, maxp, tile, o->graph, o->profile, &ps);
       if (ps.passes > s->polish_passes) s->polish_passes = ps.passes;
       s->polish_orderings += ps.orderings;
+      s->polish_point_updates += ps.point_updates;
+      s->ms_polish_kernels += ps.ms_sweep_kernels;
       if (ps.last_maxdiff > s->polish_maxdiff) s->polish_maxdiff = ps.last_maxdiff;
       if (r != PDFS_OK && r != PDFS_ENOTCONV) {
         rc = r;
@@ -854,9 +881,9 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
       k++;
     }
   }
-  HIPGO(hipEventRecord(c->kev[15], c->st));
-  HIPGO(hipEventSynchronize(c->kev[15]));
-  hipEventElapsedTime(&tms, c->kev[14], c->kev[15]);
+  HIPGO(hipEventRecord(c->kev[17], c->st));
+  HIPGO(hipEventSynchronize(c->kev[17]));
+  hipEventElapsedTime(&tms, c->kev[16], c->kev[17]);
   s->ms_polish += tms;
 out:
   free(gf);
@@ -1347,7 +1374,7 @@ static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, i
     make_sp64(dx, dy, dz, &sp64);
     KCHK(pdfsk_pack64(padded, dn, x, y, z, c->st));
     rc = sweep64_batch(c, PDFSK_MODE_F64_REF, 1, &dn, NULL, dims, &sp64, tile,
-                       passes > 0 ? passes : 0, tol, maxp, 0, &s, NULL, NULL);
+                       passes > 0 ? passes : 0, tol, 0, maxp, 0, 0, &s, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) return rc;
     KCHK(pdfsk_unpack64(dn, padded, x, y, z, c->st));
     HIPCHK(hipStreamSynchronize(c->st));

@@ -150,15 +150,15 @@ def sweep_polish64(a, init, sn, h=(1.0, 1.0, 1.0)):
     return lib().orc_sweep_polish64(a, init, *dims(a), *h, sn)
 
 
-def polish64(a, init, h=(1.0, 1.0, 1.0), tol=1e-5, max_passes=100):
-    """Polish passes until a pass changes nothing by more than tol (in place);
-    returns (passes, last pass's largest change)."""
+def polish64(a, init, h=(1.0, 1.0, 1.0), tol=2e-5, max_passes=100):
+    """Polish passes until a pass's last ordering changes nothing by more than
+    tol (in place); returns (passes, last pass's largest change)."""
     md = _D()
     p = lib().orc_polish64(a, init, *dims(a), *h, tol, max_passes, C.byref(md))
     return p, md.value
 
 
-def solve_f32_polished(init, h=(1.0, 1.0, 1.0), tol_h=1e-5, max_passes=100):
+def solve_f32_polished(init, h=(1.0, 1.0, 1.0), tol_h=2e-5, max_passes=100):
     """The FP32 solve followed by the FP64 polish, as the GPU's default FP32
     solve runs it: returns (FP32 result (padded), FP32 passes, polish passes).
     init: padded FP64 whose interior values are FP32-representable."""

@@ -21,8 +21,9 @@ PDFS_OK, PDFS_EINVAL, PDFS_EHIP, PDFS_ENOMEM, PDFS_ETIMEOUT, PDFS_ENOTCONV = 0, 
 PDFS_F64_REF, PDFS_F32 = 0, 1
 GEN_POINT, GEN_TERRAIN, GEN_SPHERES = 0, 1, 2
 TILE_16x16, TILE_32x8, TILE_16x8, TILE_AUTO = 0, 1, 2, -2
-TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU = 3, 4, 5
-TILES = (TILE_16x16, TILE_32x8, TILE_16x8, TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU)
+TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU, TILE_8x8, TILE_16x4 = 3, 4, 5, 6, 7
+TILES = (TILE_16x16, TILE_32x8, TILE_16x8, TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU, TILE_8x8,
+         TILE_16x4)
 
 EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",

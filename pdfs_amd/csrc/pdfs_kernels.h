@@ -101,7 +101,13 @@ enum {
   PDFSK_TILE_32x16 = 3,     /* 512 threads, 1 per CU (LDS) */
   PDFSK_TILE_16x16_1CU = 4, /* 256 threads, 1 per CU */
   PDFSK_TILE_16x8_2CU = 5,  /* 128 threads, 2 per CU */
-  PDFSK_TILE_COUNT = 6
+  PDFSK_TILE_8x8 = 6,       /* 64 threads, max per CU */
+  PDFSK_TILE_16x4 = 7,      /* 64 threads, max per CU */
+  PDFSK_TILE_COUNT = 8
+};
+/* variants the online FP32 tuner tries (one per ordering 2..7 of a pass) */
+enum {
+  PDFSK_TUNE_COUNT = 6
 };
 void pdfsk_tile_dims(int variant, int *tx, int *ty);
 

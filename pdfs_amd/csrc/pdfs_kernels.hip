@@ -1366,7 +1366,8 @@ struct ColVariant {
   int tx, ty, per_cu;
 };
 static const ColVariant kVariants[PDFSK_TILE_COUNT] = {
-    {16, 16, 0}, {32, 8, 0}, {16, 8, 0}, {32, 16, 0}, {16, 16, 1}, {16, 8, 2}};
+    {16, 16, 0}, {32, 8, 0}, {16, 8, 0}, {32, 16, 0}, {16, 16, 1}, {16, 8, 2}, {8, 8, 0},
+    {16, 4, 0}};
 
 extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
   const ColVariant &v = kVariants[(unsigned)variant < PDFSK_TILE_COUNT ? variant : 0];
@@ -1377,11 +1378,17 @@ extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
 #ifndef PDFSK_U
 #define PDFSK_U 8
 #endif
+#ifndef PDFSK_U64
+#define PDFSK_U64 8
+#endif
+// plane-queue depth of a mode (measured at 512^3: depth 4 for the FP64
+// modes, with fewer registers, is no faster than 8)
+template <int M> constexpr int queue_depth() { return M >= PDFSK_MODE_F64_REF ? PDFSK_U64 : PDFSK_U; }
 
 template <int TX, int TY, int M>
 static int max_resident(int per_cu) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, PDFSK_U, M>,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, queue_depth<M>(), M>,
                                                    TX * TY + 64, 0) != hipSuccess)
     return 0;
   int dev = 0, ncu = 0;
@@ -1397,11 +1404,13 @@ static int max_resident(int per_cu) {
 // take the 16-wide shapes only.
 extern "C" int pdfsk_tile_supports(int variant, int mode) {
   if ((unsigned)variant >= PDFSK_TILE_COUNT || (unsigned)mode >= PDFSK_MODE_COUNT) return 0;
-  return mode < PDFSK_MODE_F64_REF || kVariants[variant].tx == 16;
+  return mode < PDFSK_MODE_F64_REF || kVariants[variant].tx + kVariants[variant].ty <= 32;
 }
 
 template <int M>
 static int max_resident_m(const ColVariant &v) {
+  if (v.tx == 8) return max_resident<8, 8, M>(v.per_cu);
+  if (v.tx == 16 && v.ty == 4) return max_resident<16, 4, M>(v.per_cu);
   if (v.tx == 16 && v.ty == 8) return max_resident<16, 8, M>(v.per_cu);
   if (v.tx == 16) return max_resident<16, 16, M>(v.per_cu);
   if (M >= PDFSK_MODE_F64_REF) return 0;
@@ -1423,12 +1432,17 @@ extern "C" int pdfsk_col_max_resident(int variant, int mode) {
 
 template <int TX, int TY, int M>
 static void launch_col(const ColArgs &a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_sweep_col<TX, TY, PDFSK_U, M>), dim3(grid), dim3(TX * TY + 64), 0, st, a);
+  hipLaunchKernelGGL((k_sweep_col<TX, TY, queue_depth<M>(), M>), dim3(grid), dim3(TX * TY + 64), 0,
+                     st, a);
 }
 
 template <int M>
 static void launch_col_m(const ColVariant &v, const ColArgs &a, int grid, hipStream_t st) {
-  if (v.tx == 16 && v.ty == 8)
+  if (v.tx == 8)
+    launch_col<8, 8, M>(a, grid, st);
+  else if (v.tx == 16 && v.ty == 4)
+    launch_col<16, 4, M>(a, grid, st);
+  else if (v.tx == 16 && v.ty == 8)
     launch_col<16, 8, M>(a, grid, st);
   else if (v.tx == 16)
     launch_col<16, 16, M>(a, grid, st);

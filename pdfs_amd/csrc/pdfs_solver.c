@@ -284,7 +284,7 @@ static int default_tile(int nf, int x, int y)
 int pdfsi_default_tile(int nf, int x, int y) { return default_tile(nf, x, y); }
 
 /* the online tuner gives each candidate one of orderings 2..7 */
-_Static_assert(PDFSK_TILE_COUNT <= 6, "tile candidates must fit orderings 2..7 of a pass");
+_Static_assert(PDFSK_TUNE_COUNT <= 6, "tile candidates must fit orderings 2..7 of a pass");
 
 static int tune_lookup(pdfs_ctx *c, long long key)
 {
@@ -1052,7 +1052,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     if (tile < 0) {
       /* the first candidate whose edge buffers fit */
       tuning = 1;
-      for (tile = 0; tile < PDFSK_TILE_COUNT - 1 && !plan_fits(nf, dims, tile); tile++) {
+      for (tile = 0; tile < PDFSK_TUNE_COUNT - 1 && !plan_fits(nf, dims, tile); tile++) {
       }
     }
   } else {
@@ -1129,9 +1129,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
         if (active[f]) npts += (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
       if (tuning) {
         /* ordering 1 untimed, candidate v on ordering v + 2, the best on 8 */
-        double cand_ms[PDFSK_TILE_COUNT], tk = 0.0;
+        double cand_ms[PDFSK_TUNE_COUNT], tk = 0.0;
         if ((rc = run_orderings(c, m32, &sp, NULL, 1, 1, o->profile, 0, &kms))) break;
-        for (int v = 0; v < PDFSK_TILE_COUNT && !rc; v++) {
+        for (int v = 0; v < PDFSK_TUNE_COUNT && !rc; v++) {
           cand_ms[v] = 1e30;
           if (!plan_fits(nf, dims, v)) {
             /* its ordering still runs, on the current tile */
@@ -1146,13 +1146,14 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
         }
         if (rc) break;
         int best = 0;
-        for (int v = 1; v < PDFSK_TILE_COUNT; v++)
+        for (int v = 1; v < PDFSK_TUNE_COUNT; v++)
           if (cand_ms[v] < cand_ms[best]) best = v;
         tuning = 0;
         tune_store(c, key, best);
         tile = best;
         if (c->tile != best && (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, dims, best, 1, 1))) break;
-        if ((rc = run_orderings(c, m32, &sp, NULL, PDFSK_TILE_COUNT + 2, 8, o->profile, 0, &kms))) break;
+        if ((rc = run_orderings(c, m32, &sp, NULL, PDFSK_TUNE_COUNT + 2, 8, o->profile, 0, &kms)))
+          break;
       } else if ((rc = run_orderings(c, m32, &sp, NULL, sn0, sn1, o->profile, o->graph, &kms))) {
         break;
       }

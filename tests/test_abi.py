@@ -6,6 +6,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 import pdfs_amd
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -105,3 +107,21 @@ def test_entry_points_reject_bad_arguments_before_touching_the_gpu():
     for name, call in calls:
         assert call() == -1, name
         assert name in err(), (name, err())
+
+
+REF_HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                        "_ref", "PDFS_ref_host")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HOST), reason="reference host program not built")
+def test_reference_host_program_links_run_fsm_from_libpdfs():
+    """The reference's single-GPU host C, compiled unmodified, takes run_fsm
+    from lib/libpdfs.so: the symbol is undefined in the program and the
+    library resolves (the GPU test runs it)."""
+    import subprocess
+    nm = subprocess.run(["nm", "-D", REF_HOST], capture_output=True, text=True, check=True).stdout
+    assert any(l.split()[-1] == "run_fsm" and " U " in f" {l} " for l in nm.splitlines())
+    ldd = subprocess.run(["ldd", REF_HOST], capture_output=True, text=True, check=True).stdout
+    line = next(l for l in ldd.splitlines() if "libpdfs.so" in l)
+    assert "not found" not in line and os.path.samefile(line.split("=>")[1].split()[0],
+                                                        pdfs_amd.LIB_PATH)

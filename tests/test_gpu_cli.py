@@ -79,6 +79,37 @@ def test_cli_end_to_end(tmp_path):
     assert got == want
 
 
+REF_HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                        "_ref", "PDFS_ref_host")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HOST), reason="reference host program not built")
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.75, 1.25)])
+def test_reference_host_program_linked_to_libpdfs(tmp_path, h):
+    """The drop-in as the reference builds it: single_gpu_cuda/src/{main,
+    phi3D,vti_parser,file_writer}.c compiled unmodified (oracle/Makefile
+    `dropin`) and linked against lib/libpdfs.so in place of the nvcc object
+    (single_gpu_cuda/Makefile:19-29). Its run_fsm call (phi3D.c:119) goes
+    through this build's C-ABI: the output file equals bin/PDFS's and the
+    oracle pipeline's byte for byte, and stdout has the reference's lines."""
+    loc, dist = random_cli_input(12, 9, 7, 47)
+    inp = str(tmp_path / "field.vti")
+    write_input_vti(inp, loc, dist, h)
+    r = subprocess.run([REF_HOST, inp, str(tmp_path / "ref")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Parallel FSM time:" in r.stdout and "x: 12\tdx:" in r.stdout
+    q = subprocess.run([pdfs_amd.CLI_PATH, inp, str(tmp_path / "ours")], capture_output=True,
+                       text=True, timeout=120)
+    assert q.returncode == 0, q.stdout + q.stderr
+    L, D = _padded(loc, dist)
+    want = reference_vti_text(oracle_pipeline(L, D, 1, h), h)
+    got_ref = open(str(tmp_path / "ref_distfield.vti")).read()
+    got_ours = open(str(tmp_path / "ours_distfield.vti")).read()
+    assert got_ref == want
+    assert got_ours == got_ref
+
+
 @pytest.mark.skipif(not os.path.exists(pdfs_amd.NC_LIB_PATH), reason="no lib/libpdfs_nc.so")
 @pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.75, 1.25)])
 def test_cli_netcdf_end_to_end(tmp_path, h):

@@ -248,6 +248,13 @@ def lib():
     return _lib
 
 
+def _sync_inputs():
+    """Wait for the work torch queued on its current stream (the inputs of a
+    library call). Only that stream: other threads' contexts keep running."""
+    import torch
+    torch.cuda.current_stream().synchronize()
+
+
 def _check(rc, ok=(PDFS_OK,)):
     if rc not in ok:
         raise PdfsError(rc, lib().pdfs_last_error().decode(errors="replace"))
@@ -356,7 +363,7 @@ class Context:
         Z, Y, X = field.shape
         o = default_opts(precision=PDFS_F32, **kw)
         st = Stats()
-        torch.cuda.synchronize()
+        _sync_inputs()
         rc = lib().pdfs_solve_f32_dev(self.c, self._ptr(field), self._ptr(init), X, Y, Z, *h,
                                       C.byref(o), C.byref(st))
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
@@ -380,7 +387,7 @@ class Context:
         ms = (C.c_double * n)()
         o = default_opts(precision=PDFS_F32, **kw)
         st = Stats()
-        torch.cuda.synchronize()
+        _sync_inputs()
         rc = lib().pdfs_solve_batch_f32_dev(self.c, n, ptrs, iptrs, dims, *h, C.byref(o),
                                             C.byref(st), passes, ms)
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
@@ -390,7 +397,7 @@ class Context:
         import torch
         Z, Y, X = field.shape
         ch = C.c_longlong(0)
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_sweep_f32_dev(self.c, self._ptr(field), X, Y, Z, *h, sn, tile,
                                         C.byref(ch)))
         return ch.value
@@ -400,7 +407,7 @@ class Context:
         assert padded.dtype == torch.float64 and padded.dim() == 3
         Z, Y, X = (s - 2 for s in padded.shape)
         st = Stats()
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_solve_ref64_dev(self.c, self._ptr(padded), X, Y, Z, *h, passes, tol,
                                           C.byref(st)))
         return st.as_dict()
@@ -414,7 +421,7 @@ class Context:
         assert slowness.dtype == torch.float64 and slowness.shape == padded.shape
         Z, Y, X = (s - 2 for s in padded.shape)
         st = Stats()
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_solve_speed64_dev(self.c, self._ptr(padded), self._ptr(slowness), X, Y,
                                             Z, *h, passes, tol, C.byref(st)))
         return st.as_dict()
@@ -430,42 +437,44 @@ class Context:
         import torch
         Z, Y, X = field.shape
         n = C.c_longlong(0)
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_judge_f32_dev(self.c, self._ptr(field), self._ptr(init), X, Y, Z, *h,
                                         self._ptr(flags), C.byref(n)))
         return n.value
 
     def generate(self, kind, seed, X, Y, Z, h=1.0, dense=None, padded=None):
         import torch
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_generate_dev(self.c, kind, seed, X, Y, Z, h, self._ptr(dense),
                                        self._ptr(padded)))
 
     def generate_ex(self, kind, seed, X, Y, Z, h=1.0, nobj=0, p=(0.0, 0.0), q=(0.0, 0.0),
                     dense=None, padded=None):
         import torch
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_generate_ex_dev(self.c, kind, seed, X, Y, Z, h, nobj, p[0], p[1],
                                           q[0], q[1], self._ptr(dense), self._ptr(padded)))
 
     def pack(self, padded, dense):
         import torch
         Z, Y, X = dense.shape
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_pack_dev(self.c, self._ptr(padded), self._ptr(dense), X, Y, Z))
         self._sync()
 
     def unpack(self, dense, padded):
         import torch
         Z, Y, X = dense.shape
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_unpack_dev(self.c, self._ptr(dense), self._ptr(padded), X, Y, Z))
         self._sync()
 
     def _sync(self):
-        # device-wide sync: drains the library's own stream as well
+        """Wait for the context's own stream (calls that return before their
+        kernels finish: pack, unpack, the network's feature and target
+        kernels, warm starts)."""
         import torch
-        torch.cuda.synchronize()
+        torch.cuda.ExternalStream(lib().pdfs_ctx_stream(self.c)).synchronize()
 
     def solve_warm_f32(self, net, field, init, h=(1.0, 1.0, 1.0), allow_notconv=False, **kw):
         """Warm-started certified solve (in place); falls back to a plain
@@ -475,7 +484,7 @@ class Context:
         Z, Y, X = field.shape
         o = default_opts(precision=PDFS_F32, **kw)
         st = Stats()
-        torch.cuda.synchronize()
+        _sync_inputs()
         rc = lib().pdfs_solve_warm_f32_dev(self.c, net.n, self._ptr(field), self._ptr(init), X, Y,
                                            Z, *h, C.byref(o), C.byref(st))
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
@@ -499,7 +508,7 @@ class Context:
         fb = (C.c_int * n)()
         o = default_opts(precision=PDFS_F32, **kw)
         st = Stats()
-        torch.cuda.synchronize()
+        _sync_inputs()
         rc = lib().pdfs_solve_batch_warm_f32_dev(self.c, net.n, n, ptrs, iptrs, dims, *h,
                                                  C.byref(o), C.byref(st), passes, fb)
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
@@ -595,20 +604,22 @@ class Net:
     def features(self, init, mask):
         import torch
         Z, Y, X = init.shape
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_net_features_dev(self.n, init.data_ptr(), X, Y, Z, mask.data_ptr()))
+        self.ctx._sync()
 
     def target(self, u, t):
         import torch
         Z, Y, X = u.shape
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_net_target_dev(self.n, u.data_ptr(), X, Y, Z, t.data_ptr()))
+        self.ctx._sync()
 
     def forward(self, mask, pred=None):
         import torch
         nb, Dz, Dy, Dx, ch = mask.shape
         assert ch == 16 and mask.is_contiguous()
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_net_forward_dev(self.n, mask.data_ptr(), nb, Dz, Dy, Dx,
                                           None if pred is None else pred.data_ptr()))
         return pred
@@ -619,7 +630,7 @@ class Net:
         assert ch == 16 and mask.is_contiguous() and target.is_contiguous()
         assert tuple(target.shape) == (nb, Dz, Dy, Dx) and target.dtype == torch.float32
         loss = C.c_float(0)
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_net_train_step_dev(self.n, mask.data_ptr(), target.data_ptr(), nb, Dz,
                                              Dy, Dx, lr, C.byref(loss)))
         return loss.value
@@ -627,9 +638,9 @@ class Net:
     def warm_start(self, init, w, hmax=1.0):
         import torch
         Z, Y, X = init.shape
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_warm_start_f32_dev(self.n, init.data_ptr(), w.data_ptr(), X, Y, Z, hmax))
-        torch.cuda.synchronize()
+        self.ctx._sync()
 
     def warm_start_batch(self, inits, ws, hmax=1.0):
         """Warm starts of uneven fields in one launch per layer (ws[i] gets
@@ -643,6 +654,6 @@ class Net:
         for i, t in enumerate(inits):
             assert t.dtype == torch.float32 and t.is_contiguous() and ws[i].shape == t.shape
             dims[3 * i], dims[3 * i + 1], dims[3 * i + 2] = t.shape[2], t.shape[1], t.shape[0]
-        torch.cuda.synchronize()
+        _sync_inputs()
         _check(lib().pdfs_warm_start_batch_f32_dev(self.n, nf, ip, wp, dims, hmax))
-        torch.cuda.synchronize()
+        self.ctx._sync()

@@ -580,8 +580,8 @@ def test_certified_batch_retires_fields_mid_pass(ctx, dev):
 def test_two_contexts_in_two_threads(dev):
     """One context per host thread: two threads solve at the same time, each
     on its own context and stream (a single field in one, a batch of two in
-    the other, twice each); every result equals the same solve run alone, bit
-    for bit, and both judges pass."""
+    the other, six times each); their solves overlap in time, every result
+    equals the same solve run alone, bit for bit, and both judges pass."""
     import threading
 
     shapes = [(64, 48, 40), (56, 72, 36), (40, 40, 64)]
@@ -595,13 +595,19 @@ def test_two_contexts_in_two_threads(dev):
         want.append(f)
     alone.close()
     results, errors = {}, []
+    spans = {"single": [], "batch": []}
+    start = threading.Barrier(2, timeout=60)
+    import time
 
     def single():
         c = pdfs_amd.Context()
         try:
-            for k in range(2):
+            start.wait()
+            for k in range(6):
                 f = inits[0].clone()
+                t0 = time.perf_counter()
                 st = c.solve_f32(f, inits[0])
+                spans["single"].append((t0, time.perf_counter()))
                 results[("single", k)] = (f, st["judge_flagged"])
         except Exception as e:  # reported by the main thread
             errors.append(e)
@@ -611,9 +617,12 @@ def test_two_contexts_in_two_threads(dev):
     def batch():
         c = pdfs_amd.Context()
         try:
-            for k in range(2):
+            start.wait()
+            for k in range(6):
                 fs = [inits[1].clone(), inits[2].clone()]
+                t0 = time.perf_counter()
                 st, _, _ = c.solve_batch_f32(fs, [inits[1], inits[2]])
+                spans["batch"].append((t0, time.perf_counter()))
                 results[("batch", k)] = (fs, st["judge_flagged"])
         except Exception as e:
             errors.append(e)
@@ -628,7 +637,13 @@ def test_two_contexts_in_two_threads(dev):
     assert not any(t.is_alive() for t in th)
     assert not errors, errors
     torch.cuda.synchronize()
-    for k in range(2):
+    # the two threads' solves ran at the same time (a context waits on its
+    # own stream and torch's current one, not on the other context)
+    overlap = sum(1 for a in spans["single"] for b in spans["batch"]
+                  if min(a[1], b[1]) > max(a[0], b[0]))
+    print(f"overlapping solve pairs: {overlap}")
+    assert overlap >= 1
+    for k in range(6):
         f, fl = results[("single", k)]
         assert fl == 0 and torch.equal(f.view(torch.int32), want[0].view(torch.int32))
         fs, fl = results[("batch", k)]

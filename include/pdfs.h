@@ -170,6 +170,29 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
 int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
                            int z, double dx, double dy, double dz, int passes, double tol,
                            pdfs_stats *st);
+/* Spatially varying speed on the FP32 column pipeline (no counterpart in
+ * the reference, which declares Phi.F and SPEED, phi3D.h:37, 54, but never
+ * uses them): pdfs_solve_f32_dev for the Eikonal |grad u| = s with a dense
+ * FP32 slowness s = 1/F per point (device, [z][y][x] like the field; values
+ * finite and > 0). Each update is the FP32 difference-form solve with the
+ * point's spacings d_i s; the FP64 polish (fields with an init) and the
+ * residual judge use the same scaling, the judge's spacing tolerance being
+ * hmin s 2^-21. Options and statistics as pdfs_solve_f32_dev, except that
+ * the column variant 32x16 (tile 3) and PDFS_TILE_AUTO fall back to the
+ * default variant (the slowness delay line does not fit 32x16's LDS). */
+int pdfs_solve_speed_f32_dev(pdfs_ctx *c, float *field, const float *init_or_null,
+                             const float *slowness, int x, int y, int z, double dx, double dy,
+                             double dz, const pdfs_opts *o, pdfs_stats *st);
+/* Batch form: slowness[f] for fields[f]; otherwise pdfs_solve_batch_f32_dev. */
+int pdfs_solve_batch_speed_f32_dev(pdfs_ctx *c, int nfields, float *const *fields,
+                                   const float *const *inits, const float *const *slowness,
+                                   const int *dims, double dx, double dy, double dz,
+                                   const pdfs_opts *o, pdfs_stats *st, int *field_passes,
+                                   double *field_ms);
+/* pdfs_judge_f32_dev of |grad u| = s (the judge pdfs_solve_speed_f32_dev runs). */
+int pdfs_judge_speed_f32_dev(pdfs_ctx *c, const float *field, const float *init,
+                             const float *slowness, int x, int y, int z, double dx, double dy,
+                             double dz, uint32_t *flags, long long *flagged);
 /* One ordering over a dense FP32 field, in place, with no device
  * allocation. No counterpart in the reference: for callers that cannot
  * allocate device memory (code under HIP graph capture, allocation-free

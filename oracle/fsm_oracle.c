@@ -84,12 +84,15 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
 /* 2^-48 and cannot move any value >= 2^-24), so the GPU square root never  */
 /* sees a denormal or tiny input. FP64 clamps at 0.                         */
 /* The HIP kernels evaluate exactly these operations in the same order.     */
+/* Slowness sl (the _speed variants; 1 otherwise, where every scaling below */
+/* is exact): the spacings become d_i sl, the isotropic h sl, and the       */
+/* anisotropic weights 1/d_i^2 and their sum are scaled by 1/sl^2.          */
 /* ------------------------------------------------------------------------ */
 #define DIFF_FORM_BODY(T, SQRT, FLUSH)                                          \
   T m[3] = {mx, my, mz};                                                        \
   T d[3] = {(T)dx, (T)dy, (T)dz};                                               \
   if (dx == dy && dy == dz) {                                                   \
-    T h = (T)dx;                                                                \
+    T h = (T)dx * sl;                                                           \
     T h2 = h * h;                                                               \
     T two_h2 = (T)2 * h2;                                                       \
     T three_h2 = (T)3 * h2;                                                     \
@@ -119,6 +122,12 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
   {                                                                             \
     T w[3] = {(T)1 / (d[0] * d[0]), (T)1 / (d[1] * d[1]), (T)1 / (d[2] * d[2])}; \
     T W = w[0] + w[1] + w[2];                                                   \
+    const T r = (T)1 / (sl * sl);                                               \
+    for (int i = 0; i < 3; i++) {                                               \
+      d[i] = d[i] * sl;                                                         \
+      w[i] = w[i] * r;                                                          \
+    }                                                                           \
+    W = W * r;                                                                  \
     for (int i = 1; i < 3; i++) {                                               \
       for (int j = 0; j < 3 - i; j++) {                                         \
         if (m[j] > m[j + 1]) {                                                  \
@@ -150,16 +159,28 @@ double orc_godunov_ref(double cur, double mx, double my, double mz,
     return cur < u ? cur : u;                                                   \
   }
 
+float orc_godunov_f32_speed(float cur, float mx, float my, float mz, double dx, double dy,
+                            double dz, float sl)
+{
+  DIFF_FORM_BODY(float, sqrtf, 0x1p-96f)
+}
+
+double orc_godunov_d64_speed(double cur, double mx, double my, double mz, double dx,
+                             double dy, double dz, double sl)
+{
+  DIFF_FORM_BODY(double, sqrt, 0.0)
+}
+
 float orc_godunov_f32(float cur, float mx, float my, float mz, double dx,
                       double dy, double dz)
 {
-  DIFF_FORM_BODY(float, sqrtf, 0x1p-96f)
+  return orc_godunov_f32_speed(cur, mx, my, mz, dx, dy, dz, 1.0f);
 }
 
 double orc_godunov_d64(double cur, double mx, double my, double mz, double dx,
                        double dy, double dz)
 {
-  DIFF_FORM_BODY(double, sqrt, 0.0)
+  return orc_godunov_d64_speed(cur, mx, my, mz, dx, dy, dz, 1.0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -292,7 +313,7 @@ void orc_sweep_lex64(double *d, int X, int Y, int Z, double dx, double dy,
   }
 }
 
-#define LEX_SWEEP_BODY(T, SOLVE)                                             \
+#define LEX_SWEEP_BODY(T, SOLVE, SL)                                         \
   int sx, sy, sz;                                                            \
   order_dirs(sn, &sx, &sy, &sz);                                             \
   size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);               \
@@ -306,7 +327,7 @@ void orc_sweep_lex64(double *d, int X, int Y, int Z, double dx, double dy,
         T mx = ORC_MIN(d[id - 1], d[id + 1]);                                \
         T my = ORC_MIN(d[id - extX], d[id + extX]);                          \
         T mz = ORC_MIN(d[id - extXY], d[id + extXY]);                        \
-        d[id] = SOLVE(d[id], mx, my, mz, dx, dy, dz);                        \
+        d[id] = SOLVE(d[id], mx, my, mz, dx, dy, dz, SL);                    \
       }                                                                      \
     }                                                                        \
   }
@@ -314,13 +335,29 @@ void orc_sweep_lex64(double *d, int X, int Y, int Z, double dx, double dy,
 void orc_sweep_f32(float *d, int X, int Y, int Z, double dx, double dy,
                    double dz, int sn)
 {
-  LEX_SWEEP_BODY(float, orc_godunov_f32)
+  LEX_SWEEP_BODY(float, orc_godunov_f32_speed, 1.0f)
+}
+
+/* FP32 sweep of |grad u| = s (new; the reference declares Phi.F / SPEED,  */
+/* phi3D.h:37, 54, and never uses them): orc_sweep_f32 with each point's    */
+/* spacings scaled by its slowness s (padded FP32, interior read).          */
+void orc_sweep_f32_speed(float *d, const float *s, int X, int Y, int Z, double dx, double dy,
+                         double dz, int sn)
+{
+  LEX_SWEEP_BODY(float, orc_godunov_f32_speed, s[id])
 }
 
 void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
                    double dz, int sn)
 {
-  LEX_SWEEP_BODY(double, orc_godunov_d64)
+  LEX_SWEEP_BODY(double, orc_godunov_d64_speed, 1.0)
+}
+
+/* FP64 difference-form sweep of |grad u| = s (s padded FP64). */
+void orc_sweep_d64_speed(double *d, const double *s, int X, int Y, int Z, double dx, double dy,
+                         double dz, int sn)
+{
+  LEX_SWEEP_BODY(double, orc_godunov_d64_speed, s[id])
 }
 
 /* FP64 polish (new; no reference counterpart). A Gauss-Seidel step of the */
@@ -336,6 +373,14 @@ void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
 double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, double dx,
                           double dy, double dz, int sn)
 {
+  return orc_sweep_polish64_speed(d, init, NULL, X, Y, Z, dx, dy, dz, sn);
+}
+
+/* orc_sweep_polish64 of |grad u| = s: s (padded FP32, or NULL for 1) scales
+ * each point's spacings as in orc_sweep_f32_speed. */
+double orc_sweep_polish64_speed(double *d, const double *init, const float *s, int X, int Y,
+                                int Z, double dx, double dy, double dz, int sn)
+{
   int sx, sy, sz;
   order_dirs(sn, &sx, &sy, &sz);
   size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
@@ -350,7 +395,8 @@ double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, do
         double mx = ORC_MIN(d[id - 1], d[id + 1]);
         double my = ORC_MIN(d[id - extX], d[id + extX]);
         double mz = ORC_MIN(d[id - extXY], d[id + extXY]);
-        double u = orc_godunov_d64(init[id], mx, my, mz, dx, dy, dz);
+        double u = orc_godunov_d64_speed(init[id], mx, my, mz, dx, dy, dz,
+                                         s ? (double)s[id] : 1.0);
         if (u != d[id]) {
           double df = fabs(d[id] - u);
           if (df != df) df = INFINITY;
@@ -366,12 +412,19 @@ double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, do
 int orc_polish64(double *d, const double *init, int X, int Y, int Z, double dx, double dy,
                  double dz, double tol, int max_passes, double *last_maxdiff)
 {
+  return orc_polish64_speed(d, init, NULL, X, Y, Z, dx, dy, dz, tol, max_passes, last_maxdiff);
+}
+
+int orc_polish64_speed(double *d, const double *init, const float *s, int X, int Y, int Z,
+                       double dx, double dy, double dz, double tol, int max_passes,
+                       double *last_maxdiff)
+{
   int passes = 0;
   double md = 0.0, m8 = 0.0;
   while (passes < max_passes) {
     md = 0.0;
     for (int sn = 1; sn <= 8; sn++) {
-      double m = orc_sweep_polish64(d, init, X, Y, Z, dx, dy, dz, sn);
+      double m = orc_sweep_polish64_speed(d, init, s, X, Y, Z, dx, dy, dz, sn);
       if (m > md) md = m;
       m8 = m;
     }
@@ -457,13 +510,26 @@ int orc_solve_d64(double *d, int X, int Y, int Z, double dx, double dy,
 int orc_solve_f32(float *d, int X, int Y, int Z, double dx, double dy,
                   double dz, int max_passes, long *last_changed)
 {
+  return orc_solve_f32_speed(d, NULL, X, Y, Z, dx, dy, dz, max_passes, last_changed);
+}
+
+/* Passes of orc_sweep_f32_speed (s NULL: orc_sweep_f32) until a pass
+ * changes no value. */
+int orc_solve_f32_speed(float *d, const float *s, int X, int Y, int Z, double dx, double dy,
+                        double dz, int max_passes, long *last_changed)
+{
   size_t n = (size_t)(X + 2) * (Y + 2) * (Z + 2);
   float *prev = (float *)malloc(n * sizeof(float));
   int passes = 0;
   long changed = 0;
   while (passes < max_passes) {
     memcpy(prev, d, n * sizeof(float));
-    for (int sn = 1; sn <= 8; sn++) orc_sweep_f32(d, X, Y, Z, dx, dy, dz, sn);
+    for (int sn = 1; sn <= 8; sn++) {
+      if (s)
+        orc_sweep_f32_speed(d, s, X, Y, Z, dx, dy, dz, sn);
+      else
+        orc_sweep_f32(d, X, Y, Z, dx, dy, dz, sn);
+    }
     passes++;
     changed = 0;
     for (size_t p = 0; p < n; p++) changed += memcmp(&prev[p], &d[p], sizeof(float)) != 0;
@@ -626,6 +692,15 @@ void orc_gen_spheres(double *d, int X, int Y, int Z, double h, uint64_t seed,
 long orc_judge_f32(const float *v, const float *init, int X, int Y, int Z,
                    double dx, double dy, double dz, uint8_t *flags)
 {
+  return orc_judge_f32_speed(v, init, NULL, X, Y, Z, dx, dy, dz, flags);
+}
+
+/* orc_judge_f32 of |grad u| = s (s padded FP32, or NULL for 1): the FP64
+ * solve scales the point's spacings by s, and the spacing part of the
+ * tolerance becomes hmin s 2^-21. */
+long orc_judge_f32_speed(const float *v, const float *init, const float *s, int X, int Y, int Z,
+                         double dx, double dy, double dz, uint8_t *flags)
+{
   size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
   double hmin = ORC_MIN(dx, ORC_MIN(dy, dz));
   long bad = 0;
@@ -636,11 +711,12 @@ long orc_judge_f32(const float *v, const float *init, int X, int Y, int Z,
         double mx = ORC_MIN((double)v[id - 1], (double)v[id + 1]);
         double my = ORC_MIN((double)v[id - extX], (double)v[id + extX]);
         double mz = ORC_MIN((double)v[id - extXY], (double)v[id + extXY]);
-        double e = orc_godunov_d64((double)init[id], mx, my, mz, dx, dy, dz);
+        const double sl = s ? (double)s[id] : 1.0;
+        double e = orc_godunov_d64_speed((double)init[id], mx, my, mz, dx, dy, dz, sl);
         double val = (double)v[id];
         int ok;
         if (isinf(e) || isinf(val)) ok = (e == val);
-        else ok = fabs(val - e) <= fabs(e) * 0x1.0p-22 + hmin * 0x1.0p-21;
+        else ok = fabs(val - e) <= fabs(e) * 0x1.0p-22 + hmin * sl * 0x1.0p-21;
         if (flags) flags[id] = (uint8_t)!ok;
         bad += !ok;
       }

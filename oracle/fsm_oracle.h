@@ -67,6 +67,19 @@ void orc_sweep_f32(float *d, int X, int Y, int Z, double dx, double dy,
                    double dz, int sn);
 void orc_sweep_d64(double *d, int X, int Y, int Z, double dx, double dy,
                    double dz, int sn);
+/* |grad u| = s, s = slowness (1 / speed) per point: the FP32 / FP64
+ * difference-form solves with the point's spacings d_i s (anisotropic
+ * weights 1/d_i^2 and their sum scaled by 1/s^2). s padded, interior read. */
+float orc_godunov_f32_speed(float cur, float mx, float my, float mz, double dx, double dy,
+                            double dz, float sl);
+double orc_godunov_d64_speed(double cur, double mx, double my, double mz, double dx,
+                             double dy, double dz, double sl);
+void orc_sweep_f32_speed(float *d, const float *s, int X, int Y, int Z, double dx, double dy,
+                         double dz, int sn);
+void orc_sweep_d64_speed(double *d, const double *s, int X, int Y, int Z, double dx, double dy,
+                         double dz, int sn);
+int orc_solve_f32_speed(float *d, const float *s, int X, int Y, int Z, double dx, double dy,
+                        double dz, int max_passes, long *last_changed);
 
 /* FP64 polish: one ordering of u = min(init, G64(u)) (difference form; may
  * raise values), returning the largest change of any update; and passes of
@@ -77,6 +90,12 @@ double orc_sweep_polish64(double *d, const double *init, int X, int Y, int Z, do
                           double dy, double dz, int sn);
 int orc_polish64(double *d, const double *init, int X, int Y, int Z, double dx, double dy,
                  double dz, double tol, int max_passes, double *last_maxdiff);
+/* the polish of |grad u| = s (s padded FP32, or NULL for 1) */
+double orc_sweep_polish64_speed(double *d, const double *init, const float *s, int X, int Y,
+                                int Z, double dx, double dy, double dz, int sn);
+int orc_polish64_speed(double *d, const double *init, const float *s, int X, int Y, int Z,
+                       double dx, double dy, double dz, double tol, int max_passes,
+                       double *last_maxdiff);
 
 /* Passes of sn = 1..8 until converged (or max_passes). Return passes run. */
 int orc_solve_ref64(double *d, int X, int Y, int Z, double dx, double dy,
@@ -110,6 +129,9 @@ void orc_gen_spheres(double *d, int X, int Y, int Z, double h, uint64_t seed,
  * |v - min(init, G64(v))| > tol. Returns the number of flagged points. */
 long orc_judge_f32(const float *v, const float *init, int X, int Y, int Z,
                    double dx, double dy, double dz, uint8_t *flags);
+/* with slowness s (padded FP32, or NULL): spacing tolerance hmin s 2^-21 */
+long orc_judge_f32_speed(const float *v, const float *init, const float *s, int X, int Y, int Z,
+                         double dx, double dy, double dz, uint8_t *flags);
 /* max |v - min(init, G(v))| over the interior, FP64 difference form. */
 double orc_residual_f64(const double *v, const double *init, int X, int Y,
                         int Z, double dx, double dy, double dz);

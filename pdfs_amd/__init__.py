@@ -40,7 +40,8 @@ EXPORTED = [
     "pdfs_net_features_dev", "pdfs_net_target_dev", "pdfs_net_forward_dev",
     "pdfs_net_train_step_dev", "pdfs_warm_start_f32_dev", "pdfs_warm_start_batch_f32_dev",
     "pdfs_solve_warm_f32_dev",
-    "pdfs_solve_batch_warm_f32_dev", "pdfs_sweep_f32_noalloc",
+    "pdfs_solve_batch_warm_f32_dev", "pdfs_sweep_f32_noalloc", "pdfs_solve_speed_f32_dev",
+    "pdfs_solve_batch_speed_f32_dev", "pdfs_judge_speed_f32_dev",
 ]
 WEIGHTS_PATH = os.path.join(HERE, "weights", "warm_unet.bin")
 CLI_PATH = os.path.join(HERE, "bin", "PDFS")
@@ -193,6 +194,15 @@ def lib():
         L.pdfs_solve_batch_f32_dev.argtypes = [vp, I, vp, vp, vp, D, D, D, C.POINTER(Opts),
                                                C.POINTER(Stats), vp, vp]
         L.pdfs_solve_batch_f32_dev.restype = I
+        L.pdfs_solve_speed_f32_dev.argtypes = [vp, vp, vp, vp, I, I, I, D, D, D,
+                                               C.POINTER(Opts), C.POINTER(Stats)]
+        L.pdfs_solve_speed_f32_dev.restype = I
+        L.pdfs_solve_batch_speed_f32_dev.argtypes = [vp, I, vp, vp, vp, vp, D, D, D,
+                                                     C.POINTER(Opts), C.POINTER(Stats), vp, vp]
+        L.pdfs_solve_batch_speed_f32_dev.restype = I
+        L.pdfs_judge_speed_f32_dev.argtypes = [vp, vp, vp, vp, I, I, I, D, D, D, vp,
+                                               C.POINTER(LL)]
+        L.pdfs_judge_speed_f32_dev.restype = I
         L.pdfs_calc_dist_field.argtypes = [C.POINTER(Phi), I, I]
         L.pdfs_calc_dist_field.restype = I
         L.pdfs_prepost.argtypes = [I, vp, vp, I, I, I]
@@ -392,6 +402,58 @@ class Context:
                                             C.byref(st), passes, ms)
         _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
         return st.as_dict(), list(passes), list(ms)
+
+    def solve_speed_f32(self, field, slowness, init=None, h=(1.0, 1.0, 1.0),
+                        allow_notconv=False, **kw):
+        """|grad u| = slowness (1/F per point) on a dense float32 field
+        [Z, Y, X], in place; slowness: float32 cuda tensor of the same
+        shape. Options and statistics as solve_f32."""
+        import torch
+        assert field.dtype == torch.float32 and field.dim() == 3
+        assert slowness.dtype == torch.float32 and slowness.shape == field.shape
+        Z, Y, X = field.shape
+        o = default_opts(precision=PDFS_F32, **kw)
+        st = Stats()
+        _sync_inputs()
+        rc = lib().pdfs_solve_speed_f32_dev(self.c, self._ptr(field), self._ptr(init),
+                                            self._ptr(slowness), X, Y, Z, *h, C.byref(o),
+                                            C.byref(st))
+        _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+        return st.as_dict()
+
+    def solve_batch_speed_f32(self, fields, slownesses, inits=None, h=(1.0, 1.0, 1.0),
+                              allow_notconv=False, **kw):
+        """solve_batch_f32 of |grad u| = s with slownesses[i] for fields[i]."""
+        import torch
+        n = len(fields)
+        assert len(slownesses) == n
+        ptrs = (C.c_void_p * n)(*[self._ptr(f) for f in fields])
+        iptrs = (C.c_void_p * n)(*[self._ptr(t) for t in inits]) if inits else None
+        sptrs = (C.c_void_p * n)(*[self._ptr(t) for t in slownesses])
+        dims = (C.c_int * (3 * n))()
+        for i, f in enumerate(fields):
+            assert f.dtype == torch.float32 and f.dim() == 3
+            assert slownesses[i].dtype == torch.float32 and slownesses[i].shape == f.shape
+            Z, Y, X = f.shape
+            dims[3 * i], dims[3 * i + 1], dims[3 * i + 2] = X, Y, Z
+        passes = (C.c_int * n)()
+        ms = (C.c_double * n)()
+        o = default_opts(precision=PDFS_F32, **kw)
+        st = Stats()
+        _sync_inputs()
+        rc = lib().pdfs_solve_batch_speed_f32_dev(self.c, n, ptrs, iptrs, sptrs, dims, *h,
+                                                  C.byref(o), C.byref(st), passes, ms)
+        _check(rc, (PDFS_OK, PDFS_ENOTCONV) if allow_notconv else (PDFS_OK,))
+        return st.as_dict(), list(passes), list(ms)
+
+    def judge_speed_f32(self, field, init, slowness, h=(1.0, 1.0, 1.0), flags=None):
+        Z, Y, X = field.shape
+        n = C.c_longlong(0)
+        _sync_inputs()
+        _check(lib().pdfs_judge_speed_f32_dev(self.c, self._ptr(field), self._ptr(init),
+                                              self._ptr(slowness), X, Y, Z, *h,
+                                              self._ptr(flags), C.byref(n)))
+        return n.value
 
     def sweep_f32(self, field, sn, h=(1.0, 1.0, 1.0), tile=-1):
         import torch

@@ -19,18 +19,20 @@ int pdfsi_bad_h(double dx, double dy, double dz);
 
 /* FP64 residual judge of n dense FP32 fields (dims[3i..3i+2] = x, y, z) in
  * one launch on the context's stream; flagged[i] = failing points of field
- * i; flags[i] (optional, may be NULL) receive one bit per point. */
+ * i; flags[i] (optional, may be NULL) receive one bit per point. slow (or
+ * NULL for unit speed): dense FP32 slowness per field. */
 int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *const *init,
-                      const int *dims, uint32_t *const *flags, double dx, double dy, double dz,
-                      long long *flagged);
+                      const float *const *slow, const int *dims, uint32_t *const *flags,
+                      double dx, double dy, double dz, long long *flagged);
 
 /* FP64 polish of FP32 fields with an init (those without are left alone):
  * passes of u = min(init, G64(u)) until the last ordering of a pass changes no
  * value by more than o->polish_tol * hmin, the result rounded back to FP32; adds to
- * s->polish_* and s->ms_polish. */
+ * s->polish_* and s->ms_polish. slows (or NULL): per-field FP32 slowness of
+ * |grad u| = s. */
 int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
-                 const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
-                 pdfs_stats *s);
+                 const float *const *slows, const int *dims, double dx, double dy, double dz,
+                 const pdfs_opts *o, pdfs_stats *s);
 
 /* pdfs_solve_batch_f32_dev with per-field outcomes: field_flagged[f] = the
  * final judge's failing points of field f (0 when certified during the

@@ -40,8 +40,14 @@ enum {
   PDFSK_MODE_F64_REF = 2,      /* FP64 reference arithmetic, u = min(u, G(u)) */
   PDFSK_MODE_F64_POL_ISO = 3,  /* FP64 difference form, u = min(init, G(u)) */
   PDFSK_MODE_F64_POL_ANISO = 4,
-  PDFSK_MODE_COUNT = 5
+  PDFSK_MODE_F32_SPEED = 5,    /* FP32 difference form, spacings scaled by the point's slowness */
+  PDFSK_MODE_F64_POL_SPEED = 6,/* FP64 polish, spacings scaled by the point's slowness */
+  PDFSK_MODE_COUNT = 7
 };
+/* modes whose values are FP64 */
+#define PDFSK_MODE_IS_F64(m) \
+  ((m) == PDFSK_MODE_F64_REF || (m) == PDFSK_MODE_F64_POL_ISO || (m) == PDFSK_MODE_F64_POL_ANISO || \
+   (m) == PDFSK_MODE_F64_POL_SPEED)
 
 /* Entries of padding before and after each edge array: prefetch indices of
  * the column kernel stay inside the allocation. */
@@ -57,6 +63,7 @@ enum {
 typedef struct {
   void *f;            /* float (FP32 modes) or double (FP64 modes) */
   const float *init;  /* polish modes: dense FP32 init */
+  const float *slow;  /* speed modes: dense FP32 slowness (1 / speed) */
   int X, Y, Z;
   int nTx, nTy;
   uint32_t ex_off, ey_off;
@@ -137,6 +144,7 @@ int pdfsk_sweep_ref64(double *d, const double *slow, int X, int Y, int Z, int sn
  * [ceil(N/32)] words zeroed by the caller) optional per field. */
 typedef struct {
   const float *v, *init;
+  const float *slow;  /* dense FP32 slowness, or NULL for unit speed (same for every field) */
   uint32_t *flags;
   int X, Y, Z, tile0;
 } pdfsk_judge_field;
@@ -145,7 +153,7 @@ unsigned pdfsk_judge_tiles(int X, int Y, int Z);
 /* counts[f] += points of field f failing the FP64 residual (counts zeroed by
  * the caller); tab on the device, ntiles = tab[nf].tile0 */
 int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long ntiles, double dx, double dy,
-                          double dz, unsigned long long *counts, hipStream_t st);
+                          double dz, int speed, unsigned long long *counts, hipStream_t st);
 
 /* Padded FP64 <-> dense FP64 (interior; unpack leaves the ghosts alone). */
 int pdfsk_pack64(const double *padded, double *dense, int X, int Y, int Z, hipStream_t st);

@@ -19,6 +19,7 @@
 // Compiled with -ffp-contract=off semantics via pragmas on every arithmetic
 // routine, so the FP32 and FP64 results match the CPU oracle bit for bit.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 #include <string.h>
 
@@ -257,6 +258,20 @@ __device__ __forceinline__ double god_d64(double cur, double mx, double my, doub
   return god_diff_aniso<double>(cur, mx, my, mz, dd, ww, W);
 }
 
+// Anisotropic FP64 solve with the spacings scaled by the slowness s: the
+// weights 1/d_i^2 and their sum are scaled by r = 1/s^2, in the oracle's
+// operation order.
+__device__ __forceinline__ double god_d64_speed(double cur, double mx, double my, double mz,
+                                                double dx, double dy, double dz, double s) {
+#pragma clang fp contract(off)
+  const double w0 = 1.0 / (dx * dx), w1 = 1.0 / (dy * dy), w2 = 1.0 / (dz * dz);
+  const double W = w0 + w1 + w2;
+  const double r = 1.0 / (s * s);
+  double dd[3] = {dx * s, dy * s, dz * s};
+  double ww[3] = {w0 * r, w1 * r, w2 * r};
+  return god_diff_aniso<double>(cur, mx, my, mz, dd, ww, W * r);
+}
+
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 // Buffer-op cache policy bit sc1 (gfx940+): the access is coherent at agent
 // scope (all XCDs); 64-bit aligned accesses are single-copy atomic.
@@ -352,15 +367,27 @@ __device__ __forceinline__ double god_iso64_fminmax(double cur, double xm, doubl
 //   F64_POL_ISO / _ANISO  FP64 difference form, u = min(init, G(u)): a
 //                         Gauss-Seidel step of the fixed-point map itself,
 //                         which may raise a value as well as lower it
+//   F32_SPEED, F64_POL_SPEED
+//                         the FP32 and polish solves of |grad u| = s with a
+//                         slowness s per point: the point's spacings are
+//                         d_i s (isotropic: h s; anisotropic: d_i s and
+//                         weights w_i / s^2, W / s^2)
 // FP64 values cross column boundaries as two 32-bit halves, each in its own
-// epoch-tagged entry (two edge-wave lanes per value).
+// epoch-tagged entry (two edge-wave lanes per value). The per-point inputs
+// (init, slowness) are "aux" streams: loaded with the field, plane by
+// plane, and kept in a per-thread LDS delay line until the thread's
+// hyperplane reaches them.
 template <bool F64> struct ValT { typedef float T; };
 template <> struct ValT<true> { typedef double T; };
 template <int M> struct SMode {
-  static constexpr bool F64 = M >= PDFSK_MODE_F64_REF;
-  static constexpr bool POL = M >= PDFSK_MODE_F64_POL_ISO;
+  static constexpr bool F64 = PDFSK_MODE_IS_F64(M);
+  static constexpr bool POL = M == PDFSK_MODE_F64_POL_ISO || M == PDFSK_MODE_F64_POL_ANISO ||
+                              M == PDFSK_MODE_F64_POL_SPEED;
+  static constexpr bool SPEED = M == PDFSK_MODE_F32_SPEED || M == PDFSK_MODE_F64_POL_SPEED;
   static constexpr bool ISO = M == PDFSK_MODE_F32_ISO || M == PDFSK_MODE_F64_POL_ISO;
   static constexpr int W = F64 ? 2 : 1;  // 32-bit words per value
+  static constexpr int NAUX = (POL ? 1 : 0) + (SPEED ? 1 : 0);  // init, then slowness
+  static constexpr int AUX_SLOW = POL ? 1 : 0;
   typedef typename ValT<F64>::T T;
 };
 
@@ -410,6 +437,7 @@ __device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, uint3
 struct FieldDesc {
   void *f;
   const float *init;
+  const float *slow;
   int X, Y, Z;
   int nTx, nTy;
   uint32_t ex_off, ey_off;
@@ -467,8 +495,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
   static_assert(NT % 64 == 0 && W * (TX + TY) <= 64, "compute waves and edge-wave lanes");
   static_assert(G::PSW % W == 0, "plane stride in whole values");
   __shared__ T lds[G::LDS_VALUES];
-  // polish: per-thread delay line of init values (plane vz in slot vz mod R)
-  __shared__ float iring[S::POL ? R * NT : 1];
+  // aux streams (init, slowness): per-thread delay lines (plane vz in slot
+  // vz mod R)
+  constexpr int NA = S::NAUX;
+  __shared__ float iring[NA ? NA * R * NT : 1];
   __shared__ int s_col;
   __shared__ unsigned s_changed;
   T *const ring = lds;
@@ -530,6 +560,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       const uint32_t ilo = __builtin_amdgcn_readfirstlane((uint32_t)ip);
       const uint32_t ihi = __builtin_amdgcn_readfirstlane((uint32_t)(ip >> 32));
       fd.init = (const float *)(((uint64_t)ihi << 32) | ilo);
+      const uint64_t sp = (uint64_t)fd.slow;
+      const uint32_t slo = __builtin_amdgcn_readfirstlane((uint32_t)sp);
+      const uint32_t shi = __builtin_amdgcn_readfirstlane((uint32_t)(sp >> 32));
+      fd.slow = (const float *)(((uint64_t)shi << 32) | slo);
       fd.X = __builtin_amdgcn_readfirstlane(fd.X);
       fd.Y = __builtin_amdgcn_readfirstlane(fd.Y);
       fd.Z = __builtin_amdgcn_readfirstlane(fd.Z);
@@ -696,12 +730,16 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       uint32_t hl = zb(3) + hxy;
       const int hmask = hmul ? -1 : 0;  // slot offset applies to halo lanes only
 
-      // polish: the init of this thread's column, loaded plane by plane
-      // with the field (coalesced) and kept in its LDS delay line until the
-      // thread's hyperplane reaches it
+      // aux streams (polish: the init; speed: the slowness) of this thread's
+      // column, loaded plane by plane with the field (coalesced) and kept in
+      // its LDS delay line until the thread's hyperplane reaches them
       const uint32_t XY4 = (uint32_t)X * (uint32_t)Y * 4u;
-      const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc(
-          (void *)fd.init, (short)0, S::POL ? (int)(XY4 * (uint32_t)Z) : 0, 0x00020000);
+      __amdgpu_buffer_rsrc_t ir[NA ? NA : 1];
+#pragma unroll
+      for (int k = 0; k < NA; k++)
+        ir[k] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(S::POL && k == 0 ? fd.init : fd.slow), (short)0, (int)(XY4 * (uint32_t)Z),
+            0x00020000);
       const uint32_t dZI = a.fz ? (uint32_t)(0u - XY4) : XY4;
       auto zbi = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XY4; };
       const uint32_t lxyi = dom ? xyi(vx, vy) * 4u : 0x80000000u;
@@ -721,23 +759,23 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         lds[hcell + hmul] = hval ? h1 : INF;
         ring[2 * PS + lo] = (dom && live2) ? v2 : INF;
         lds[hcell + 2 * hmul] = (hval && live2) ? h2 : INF;
-        if (S::POL) {
-          iring[NT + tid] = buf_ld(float(), ir, zbi(1) + lxyi);
-          iring[2 * NT + tid] = buf_ld(float(), ir, zbi(2) + lxyi);
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+          iring[(k * R + 1) * NT + tid] = buf_ld(float(), ir[k], zbi(1) + lxyi);
+          iring[(k * R + 2) * NT + tid] = buf_ld(float(), ir[k], zbi(2) + lxyi);
         }
       }
       T qP[U], qH[U];
-      float qI[S::POL ? U : 1];
+      float qI[NA ? NA : 1][U];
 #pragma unroll
       for (int j = 0; j < U; j++) {
         qP[j] = buf_ld(T(), fr, ld);
         qH[j] = buf_ld(T(), fr, hl);
         ld += dZ;
         hl += dZ;
-        if (S::POL) {
-          qI[j] = buf_ld(float(), ir, ldi);
-          ldi += dZI;
-        }
+#pragma unroll
+        for (int k = 0; k < NA; k++) qI[k][j] = buf_ld(float(), ir[k], ldi);
+        ldi += dZI;
       }
       __syncthreads();
       // Ring addressing: plane vz of a cell lives in slot vz mod R. so_c is
@@ -795,13 +833,15 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         qH[j] = buf_ld(T(), fr, hl);
         ld += dZ;
         hl += dZ;
-        float ini = 0.0f;
-        if (S::POL) {
-          iring[slot_w * NT + tid] = qI[j];
-          qI[j] = buf_ld(float(), ir, ldi);
-          ldi += dZI;
-          ini = iring[si_c + tid];
+        float aux[NA ? NA : 1];
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+          iring[k * R * NT + slot_w * NT + tid] = qI[k][j];
+          qI[k][j] = buf_ld(float(), ir[k], ldi);
+          aux[k] = iring[k * R * NT + si_c + tid];
         }
+        ldi += NA ? dZI : 0u;
+        const float ini = S::POL ? aux[0] : 0.0f;
         // (d)
         const uint32_t so_n = wrap_next(so_c);
         const T *pn1 = ring + so_n + lo;
@@ -811,8 +851,36 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         const int vz = l + 1 - skew;
         const bool act = steady ? dom : (dom && (unsigned)(vz - 1) < (unsigned)Z);
         T u;
-        if constexpr (!S::F64) {
+        if constexpr (M == PDFSK_MODE_F32_SPEED) {
+          // spacings scaled by the point's slowness s
+          const float sl = aux[S::AUX_SLOW];
+          if (a.sp.iso) {
+            const float hs = a.sp.h * sl, h2 = hs * hs;
+            u = god_iso32(c, xm, xp, ym, yp, zm, zp, hs, 2.0f * h2, 3.0f * h2, a.sp.third);
+          } else {
+            const float r = 1.0f / (sl * sl);
+            const float dd[3] = {a.sp.d[0] * sl, a.sp.d[1] * sl, a.sp.d[2] * sl};
+            const float ww[3] = {a.sp.w[0] * r, a.sp.w[1] * r, a.sp.w[2] * r};
+            u = god_diff_aniso<float>(c, xm < xp ? xm : xp, ym < yp ? ym : yp, zm < zp ? zm : zp,
+                                      dd, ww, a.sp.W * r);
+          }
+        } else if constexpr (!S::F64) {
           u = god32<S::ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
+        } else if constexpr (M == PDFSK_MODE_F64_POL_SPEED) {
+          const double sl = (double)aux[S::AUX_SLOW];
+          if (a.sp64.iso) {
+            const double hs = a.sp64.h * sl, h2 = hs * hs;
+            u = god_iso64_fminmax((double)ini, xm, xp, ym, yp, zm, zp, hs, 2.0 * h2, 3.0 * h2,
+                                  a.sp64.third);
+          } else {
+            const double mx = a.fx ? (xp < xm ? xp : xm) : (xm < xp ? xm : xp);
+            const double my = a.fy ? (yp < ym ? yp : ym) : (ym < yp ? ym : yp);
+            const double mz = a.fz ? (zp < zm ? zp : zm) : (zm < zp ? zm : zp);
+            const double r = 1.0 / (sl * sl);
+            const double dd[3] = {a.sp64.d[0] * sl, a.sp64.d[1] * sl, a.sp64.d[2] * sl};
+            const double ww[3] = {a.sp64.w[0] * r, a.sp64.w[1] * r, a.sp64.w[2] * r};
+            u = god_diff_aniso<double>((double)ini, mx, my, mz, dd, ww, a.sp64.W * r);
+          }
         } else if constexpr (M == PDFSK_MODE_F64_POL_ISO) {
           u = god_iso64_fminmax((double)ini, xm, xp, ym, yp, zm, zp, a.sp64.h, a.sp64.two_h2,
                                 a.sp64.three_h2, a.sp64.third);
@@ -848,7 +916,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         xp = xp1;
         yp = yp1;
         so_c = so_n;
-        if (S::POL) {
+        if (NA) {
           const uint32_t n = si_c + NT;
           si_c = __builtin_elementwise_min(n, n - RNT);
         }
@@ -1008,14 +1076,15 @@ struct JudgeLane {
 };
 // The operands of one point: its value, its six neighbours' and its init.
 struct JudgeOps {
-  float zm, cur, zp, xm, xp, ym, yp, ini;
+  float zm, cur, zp, xm, xp, ym, yp, ini, sl;
 };
 
 // Operands of plane zl (clamped to the field: the planes past its end are
 // loaded but never judged). Out-of-field neighbours load the point itself
 // and are replaced by +inf in judge_plane.
+template <bool SPEED>
 __device__ __forceinline__ JudgeOps judge_load(const float *v, const float *init,
-                                               const JudgeLane &L, int zl) {
+                                               const float *slow, const JudgeLane &L, int zl) {
   const int zc = zl < L.Z ? zl : L.Z - 1;
   const uint32_t p = (uint32_t)zc * L.XY + L.col;
   JudgeOps o;
@@ -1027,12 +1096,13 @@ __device__ __forceinline__ JudgeOps judge_load(const float *v, const float *init
   o.ym = ld_global(v, p - L.oym);
   o.yp = ld_global(v, p + L.oyp);
   o.ini = ld_global(init, p);
+  o.sl = SPEED ? ld_global(slow, p) : 1.0f;
   return o;
 }
 
 // Judge the point of plane z (< Z) whose operands are o: sets its flag bit
 // when it fails and returns the number of failing points of the wave.
-template <bool ISO>
+template <bool ISO, bool SPEED>
 __device__ __forceinline__ unsigned judge_plane(const JudgeOps &o, const JudgeLane &L, int z,
                                                 double dx, double dy, double dz,
                                                 uint32_t *flags) {
@@ -1048,12 +1118,17 @@ __device__ __forceinline__ unsigned judge_plane(const JudgeOps &o, const JudgeLa
   const float my = __builtin_elementwise_minimum(ym, yp);
   const float mz = __builtin_elementwise_minimum(zm, zp);
   double e;
+  // speed: the point's spacings scaled by its slowness
+  const double sl = SPEED ? (double)o.sl : 1.0;
   if (ISO) {
     const float m0 = __builtin_elementwise_minimum(__builtin_elementwise_minimum(mx, my), mz);
     const float m2 = __builtin_elementwise_maximum(__builtin_elementwise_maximum(mx, my), mz);
     const float m1 = __builtin_amdgcn_fmed3f(mx, my, mz);
-    e = god_iso64_sorted((double)o.ini, (double)m0, (double)m1, (double)m2, dx,
-                         2.0 * (dx * dx), 3.0 * (dx * dx), 1.0 / 3.0);
+    const double h = SPEED ? dx * sl : dx;
+    e = god_iso64_sorted((double)o.ini, (double)m0, (double)m1, (double)m2, h, 2.0 * (h * h),
+                         3.0 * (h * h), 1.0 / 3.0);
+  } else if (SPEED) {
+    e = god_d64_speed((double)o.ini, (double)mx, (double)my, (double)mz, dx, dy, dz, sl);
   } else {
     e = god_d64((double)o.ini, (double)mx, (double)my, (double)mz, dx, dy, dz);
   }
@@ -1063,7 +1138,8 @@ __device__ __forceinline__ unsigned judge_plane(const JudgeOps &o, const JudgeLa
   const double dv = (double)o.cur;
   const double ae = __builtin_fabs(e) < 0x1.fffffffffffffp+1023 ? __builtin_fabs(e)
                                                                 : 0x1.fffffffffffffp+1023;
-  const bool bad = L.in && !(dv == e) && !(__builtin_fabs(dv - e) <= ae * 0x1.0p-22 + L.htol);
+  const double htol = SPEED ? L.htol * sl : L.htol;
+  const bool bad = L.in && !(dv == e) && !(__builtin_fabs(dv - e) <= ae * 0x1.0p-22 + htol);
   if (bad && flags) {
     const uint32_t p = (uint32_t)z * L.XY + L.col;
     // a global (not flat) atomic: a flat one in the loop would make the
@@ -1077,7 +1153,7 @@ __device__ __forceinline__ unsigned judge_plane(const JudgeOps &o, const JudgeLa
 
 // Fields of a batch: field f owns tiles [tab[f].tile0, tab[f+1].tile0)
 // (tab[nf].tile0 = total); flags optional per field.
-template <bool ISO>
+template <bool ISO, bool SPEED>
 __global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__restrict__ tab,
                                                    int nf, double dx, double dy, double dz,
                                                    unsigned long long *counts) {
@@ -1096,6 +1172,7 @@ __global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__re
   const pdfsk_judge_field fd = tab[f];
   const float *__restrict__ v = fd.v;
   const float *__restrict__ init = fd.init;
+  const float *__restrict__ slow = fd.slow;
   const int X = fd.X, Y = fd.Y, Z = fd.Z;
   t -= (uint32_t)fd.tile0;
   const int ntx = (X + 63) >> 6, nty = (Y + 3) >> 2;
@@ -1129,15 +1206,15 @@ __global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__re
   unsigned long long nbad = 0;
   // Two operand sets, alternately loaded one plane ahead of the plane being
   // judged: the loads of plane z + 1 are in flight while plane z is solved.
-  JudgeOps A = judge_load(v, init, L, z0), B;
+  JudgeOps A = judge_load<SPEED>(v, init, slow, L, z0), B;
   int z = z0;
   for (; z + 1 < z1; z += 2) {
-    B = judge_load(v, init, L, z + 1);
-    nbad += judge_plane<ISO>(A, L, z, dx, dy, dz, fd.flags);
-    A = judge_load(v, init, L, z + 2);
-    nbad += judge_plane<ISO>(B, L, z + 1, dx, dy, dz, fd.flags);
+    B = judge_load<SPEED>(v, init, slow, L, z + 1);
+    nbad += judge_plane<ISO, SPEED>(A, L, z, dx, dy, dz, fd.flags);
+    A = judge_load<SPEED>(v, init, slow, L, z + 2);
+    nbad += judge_plane<ISO, SPEED>(B, L, z + 1, dx, dy, dz, fd.flags);
   }
-  if (z < z1) nbad += judge_plane<ISO>(A, L, z, dx, dy, dz, fd.flags);
+  if (z < z1) nbad += judge_plane<ISO, SPEED>(A, L, z, dx, dy, dz, fd.flags);
   if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(counts + f, nbad);
 }
 
@@ -1383,79 +1460,93 @@ extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
 #endif
 // plane-queue depth of a mode (measured at 512^3: depth 4 for the FP64
 // modes, with fewer registers, is no faster than 8)
-template <int M> constexpr int queue_depth() { return M >= PDFSK_MODE_F64_REF ? PDFSK_U64 : PDFSK_U; }
+template <int M> constexpr int queue_depth() { return SMode<M>::F64 ? PDFSK_U64 : PDFSK_U; }
 
+// Whether a tile shape can run a mode: an FP64 value takes two edge-wave
+// lanes (W (TX + TY) <= 64) and the LDS (ring, aux delay lines) must fit the
+// CU's 160 KB. FP32 modes take every shape but 32 x 16 with an aux stream;
+// FP64 modes the 16-wide shapes, and not 16 x 16 with two aux streams.
 template <int TX, int TY, int M>
-static int max_resident(int per_cu) {
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_col<TX, TY, queue_depth<M>(), M>,
-                                                   TX * TY + 64, 0) != hipSuccess)
-    return 0;
-  int dev = 0, ncu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    return 0;
-  if (per_cu > 0 && per_cu < nb) nb = per_cu;
-  return nb * ncu;
+constexpr bool col_fits() {
+  using S = SMode<M>;
+  using G = ColGeom<TX, TY, S::W>;
+  const long lds = (long)G::LDS_VALUES * (long)sizeof(typename S::T) +
+                   4L * (S::NAUX ? S::NAUX * G::R * G::NT : 1) + 8;
+  return S::W * (TX + TY) <= 64 && lds <= 160L * 1024;
 }
 
-// Tile shapes per mode: FP32 modes take every shape; FP64 values need two
-// edge-wave lanes each (TX + TY <= 32) and twice the ring, so FP64 modes
-// take the 16-wide shapes only.
+// The tile shapes of kVariants, as template arguments: calls f with an
+// object whose TX / TY members are the shape's, or returns dflt for a shape
+// that cannot run mode M.
+template <int X_, int Y_> struct TileShape {
+  static constexpr int TX = X_, TY = Y_;
+};
+template <int M, typename R, typename F>
+static R with_tile(const ColVariant &v, R dflt, F f) {
+  auto go = [&](auto shape) -> R {
+    using Sh = decltype(shape);
+    if constexpr (col_fits<Sh::TX, Sh::TY, M>())
+      return f(shape);
+    else
+      return dflt;
+  };
+  if (v.tx == 8 && v.ty == 8) return go(TileShape<8, 8>());
+  if (v.tx == 16 && v.ty == 4) return go(TileShape<16, 4>());
+  if (v.tx == 16 && v.ty == 8) return go(TileShape<16, 8>());
+  if (v.tx == 16 && v.ty == 16) return go(TileShape<16, 16>());
+  if (v.tx == 32 && v.ty == 8) return go(TileShape<32, 8>());
+  if (v.tx == 32 && v.ty == 16) return go(TileShape<32, 16>());
+  return dflt;
+}
+
+// f<M>() for a runtime mode
+template <typename F>
+static auto with_mode(int mode, F f) {
+  switch (mode) {
+    case PDFSK_MODE_F32_ISO: return f(std::integral_constant<int, PDFSK_MODE_F32_ISO>());
+    case PDFSK_MODE_F32_ANISO: return f(std::integral_constant<int, PDFSK_MODE_F32_ANISO>());
+    case PDFSK_MODE_F64_REF: return f(std::integral_constant<int, PDFSK_MODE_F64_REF>());
+    case PDFSK_MODE_F64_POL_ISO: return f(std::integral_constant<int, PDFSK_MODE_F64_POL_ISO>());
+    case PDFSK_MODE_F64_POL_ANISO:
+      return f(std::integral_constant<int, PDFSK_MODE_F64_POL_ANISO>());
+    case PDFSK_MODE_F32_SPEED: return f(std::integral_constant<int, PDFSK_MODE_F32_SPEED>());
+    default: return f(std::integral_constant<int, PDFSK_MODE_F64_POL_SPEED>());
+  }
+}
+
 extern "C" int pdfsk_tile_supports(int variant, int mode) {
   if ((unsigned)variant >= PDFSK_TILE_COUNT || (unsigned)mode >= PDFSK_MODE_COUNT) return 0;
-  return mode < PDFSK_MODE_F64_REF || kVariants[variant].tx + kVariants[variant].ty <= 32;
-}
-
-template <int M>
-static int max_resident_m(const ColVariant &v) {
-  if (v.tx == 8) return max_resident<8, 8, M>(v.per_cu);
-  if (v.tx == 16 && v.ty == 4) return max_resident<16, 4, M>(v.per_cu);
-  if (v.tx == 16 && v.ty == 8) return max_resident<16, 8, M>(v.per_cu);
-  if (v.tx == 16) return max_resident<16, 16, M>(v.per_cu);
-  if (M >= PDFSK_MODE_F64_REF) return 0;
-  if (v.ty == 8) return max_resident<32, 8, M < PDFSK_MODE_F64_REF ? M : 0>(v.per_cu);
-  return max_resident<32, 16, M < PDFSK_MODE_F64_REF ? M : 0>(v.per_cu);
+  return with_mode(mode, [&](auto m) {
+    return with_tile<decltype(m)::value, int>(kVariants[variant], 0, [](auto) { return 1; });
+  });
 }
 
 extern "C" int pdfsk_col_max_resident(int variant, int mode) {
   if (!pdfsk_tile_supports(variant, mode)) return 0;
   const ColVariant &v = kVariants[variant];
-  switch (mode) {
-    case PDFSK_MODE_F32_ISO: return max_resident_m<PDFSK_MODE_F32_ISO>(v);
-    case PDFSK_MODE_F32_ANISO: return max_resident_m<PDFSK_MODE_F32_ANISO>(v);
-    case PDFSK_MODE_F64_REF: return max_resident_m<PDFSK_MODE_F64_REF>(v);
-    case PDFSK_MODE_F64_POL_ISO: return max_resident_m<PDFSK_MODE_F64_POL_ISO>(v);
-    default: return max_resident_m<PDFSK_MODE_F64_POL_ANISO>(v);
-  }
-}
-
-template <int TX, int TY, int M>
-static void launch_col(const ColArgs &a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_sweep_col<TX, TY, queue_depth<M>(), M>), dim3(grid), dim3(TX * TY + 64), 0,
-                     st, a);
-}
-
-template <int M>
-static void launch_col_m(const ColVariant &v, const ColArgs &a, int grid, hipStream_t st) {
-  if (v.tx == 8)
-    launch_col<8, 8, M>(a, grid, st);
-  else if (v.tx == 16 && v.ty == 4)
-    launch_col<16, 4, M>(a, grid, st);
-  else if (v.tx == 16 && v.ty == 8)
-    launch_col<16, 8, M>(a, grid, st);
-  else if (v.tx == 16)
-    launch_col<16, 16, M>(a, grid, st);
-  else if (v.ty == 8)
-    launch_col<32, 8, M < PDFSK_MODE_F64_REF ? M : 0>(a, grid, st);
-  else
-    launch_col<32, 16, M < PDFSK_MODE_F64_REF ? M : 0>(a, grid, st);
+  return with_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    return with_tile<M, int>(v, 0, [&](auto shape) {
+      using Sh = decltype(shape);
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, k_sweep_col<Sh::TX, Sh::TY, queue_depth<M>(), M>, Sh::TX * Sh::TY + 64,
+              0) != hipSuccess)
+        return 0;
+      int dev = 0, ncu = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+      if (v.per_cu > 0 && v.per_cu < nb) nb = v.per_cu;
+      return nb * ncu;
+    });
+  });
 }
 
 extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *sp,
                                  const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st) {
   if (sn < 1 || sn > 8 || grid < 1 || !pdfsk_tile_supports(variant, mode)) return -1;
-  if (mode >= PDFSK_MODE_F64_REF && !b->maxdiff) return -1;
+  if (PDFSK_MODE_IS_F64(mode) && !b->maxdiff) return -1;
   static_assert(sizeof(FieldDesc) == sizeof(pdfsk_field), "field table layout");
   // Ordering sn as (+,+,+) in virtual coordinates: physical direction per
   // axis (mpi_openacc/src/phi3D_fsm.c:206-218 reflections x{4,8} y{2,6}
@@ -1481,14 +1572,15 @@ extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const p
   a.err = b->err;
   if (sp) a.sp = *sp;
   if (sp64) a.sp64 = *sp64;
-  const ColVariant &v = kVariants[variant];
-  switch (mode) {
-    case PDFSK_MODE_F32_ISO: launch_col_m<PDFSK_MODE_F32_ISO>(v, a, grid, st); break;
-    case PDFSK_MODE_F32_ANISO: launch_col_m<PDFSK_MODE_F32_ANISO>(v, a, grid, st); break;
-    case PDFSK_MODE_F64_REF: launch_col_m<PDFSK_MODE_F64_REF>(v, a, grid, st); break;
-    case PDFSK_MODE_F64_POL_ISO: launch_col_m<PDFSK_MODE_F64_POL_ISO>(v, a, grid, st); break;
-    default: launch_col_m<PDFSK_MODE_F64_POL_ANISO>(v, a, grid, st); break;
-  }
+  with_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    return with_tile<M, int>(kVariants[variant], 0, [&](auto shape) {
+      using Sh = decltype(shape);
+      hipLaunchKernelGGL((k_sweep_col<Sh::TX, Sh::TY, queue_depth<M>(), M>), dim3(grid),
+                         dim3(Sh::TX * Sh::TY + 64), 0, st, a);
+      return 0;
+    });
+  });
   return hip_rc(hipGetLastError());
 }
 
@@ -1525,16 +1617,18 @@ extern "C" int pdfsk_sweep_ref64(double *d, const double *slow, int X, int Y, in
 extern "C" unsigned pdfsk_judge_tiles(int X, int Y, int Z) { return judge_tiles(X, Y, Z); }
 
 extern "C" int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long ntiles, double dx,
-                                     double dy, double dz, unsigned long long *counts,
+                                     double dy, double dz, int speed, unsigned long long *counts,
                                      hipStream_t st) {
   if (nf < 1 || ntiles < 1 || ntiles >= (1L << 31) - 8) return -1;
   const long g = (ntiles + 7) / 8 * 8;  // a multiple of the 8 XCDs
-  if (dx == dy && dy == dz)
-    hipLaunchKernelGGL(k_judge_f32<true>, dim3((unsigned)g), dim3(256), 0, st, tab, nf, dx, dy, dz,
-                       counts);
+  const bool iso = dx == dy && dy == dz;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(256), 0, st, tab, nf, dx, dy, dz, counts);
+  };
+  if (speed)
+    iso ? go(k_judge_f32<true, true>) : go(k_judge_f32<false, true>);
   else
-    hipLaunchKernelGGL(k_judge_f32<false>, dim3((unsigned)g), dim3(256), 0, st, tab, nf, dx, dy,
-                       dz, counts);
+    iso ? go(k_judge_f32<true, false>) : go(k_judge_f32<false, false>);
   return hip_rc(hipGetLastError());
 }
 

@@ -783,7 +783,7 @@ int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float 
   }
   if (oo.polish) {
     const int dims[3] = {x, y, z};
-    rc = pdfsi_polish(c, 1, &field, &init, dims, dx, dy, dz, &oo, &sw);
+    rc = pdfsi_polish(c, 1, &field, &init, NULL, dims, dx, dy, dz, &oo, &sw);
     if (rc && rc != PDFS_ENOTCONV) return rc;
   }
   sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + wms;
@@ -858,7 +858,7 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   sw.judge_flagged = 0;
   /* polish of the accepted fields */
   if (oo.polish && nbad < nf) {
-    rc = pdfsi_polish(c, nf, fields, gi, dims, dx, dy, dz, &oo, &sw);
+    rc = pdfsi_polish(c, nf, fields, gi, NULL, dims, dx, dy, dz, &oo, &sw);
     if (rc && rc != PDFS_ENOTCONV) goto out;
   }
   int rc_pol = rc;

@@ -87,6 +87,7 @@ __global__ void k_na_setup(unsigned char *base, float *f, int X, int Y, int Z, i
     pdfsk_field *fd = (pdfsk_field *)(base + NA_FIELD);
     fd->f = f;
     fd->init = NULL;
+    fd->slow = NULL;
     fd->X = X;
     fd->Y = Y;
     fd->Z = Z;

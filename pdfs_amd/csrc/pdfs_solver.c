@@ -393,7 +393,8 @@ static int plan_fits(int nf, const int *dims, int tile) { return plan_fits_w(nf,
  * counters and unchanged-ordering runs (they do not depend on the tile).
  * words: 1 for FP32 fields, 2 for FP64 fields; inits (polish) or NULL. */
 static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *const *inits,
-                         const int *dims, int tile, int keep, int words)
+                         const float *const *slows, const int *dims, int tile, int keep,
+                         int words)
 {
   if (nf < 1) return fail(PDFS_EINVAL, "empty batch");
   int TX, TY;
@@ -426,6 +427,8 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     pdfsk_field *fd = &c->h_fields[f];
     fd->f = fields[f];
     fd->init = inits ? inits[f] : NULL;
+    fd->slow = slows ? slows[f] : NULL;
+    if (slows && !fd->slow) return fail(PDFS_EINVAL, "field %d: no slowness", f);
     fd->X = x;
     fd->Y = y;
     fd->Z = z;
@@ -507,7 +510,7 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
 
 static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims, int tile)
 {
-  return plan_build_ex(c, nf, (void *const *)fields, NULL, dims, tile, 0, 1);
+  return plan_build_ex(c, nf, (void *const *)fields, NULL, NULL, dims, tile, 0, 1);
 }
 
 static int launch_grid(pdfs_ctx *c, int mode)
@@ -640,7 +643,8 @@ static int tile64_of(const pdfs_opts *o)
  * st: passes (the largest), orderings, point_updates, last_maxdiff (the
  * largest of the fields' last passes), ms_sweeps. */
 static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
-                         const float *const *inits, const int *dims, const pdfs_sp64 *sp64,
+                         const float *const *inits, const float *const *slows, const int *dims,
+                         const pdfs_sp64 *sp64,
                          int tile, int fixed_passes, double tol, int stop_on_last, int maxp,
                          int graph, int profile, pdfs_stats *st, int *field_passes,
                          double *field_maxdiff)
@@ -648,7 +652,7 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
   double kms = 0.0;
   pdfs_stats s;
   memset(&s, 0, sizeof s);
-  int rc = plan_build_ex(c, nf, (void *const *)fields, inits, dims, tile, 0, 2);
+  int rc = plan_build_ex(c, nf, (void *const *)fields, inits, slows, dims, tile, 0, 2);
   if (rc) return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
   double *md = (double *)calloc((size_t)2 * nf, sizeof(double));
@@ -760,8 +764,8 @@ static int pbuf_layout(pdfs_ctx *c, int n, const long long *npts, size_t *off)
 
 /* FP64 polish of one group of fields that fits one FP64 plan. */
 static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *const *inits,
-                        const int *dims, const pdfs_sp64 *sp64, double tol, int maxp, int tile,
-                        int graph, int profile, pdfs_stats *ps)
+                        const float *const *slows, const int *dims, const pdfs_sp64 *sp64,
+                        double tol, int maxp, int tile, int graph, int profile, pdfs_stats *ps)
 {
   long long *np = (long long *)malloc((size_t)k * sizeof(long long));
   size_t *off = (size_t *)malloc((size_t)k * sizeof(size_t));
@@ -802,9 +806,10 @@ static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *c
   HIPGO(hipStreamSynchronize(c->st));
   KGO(pdfsk_convert_batch(c->conv_dev, k, nmax, 0, c->st));
   {
-    const int mode = sp64->iso ? PDFSK_MODE_F64_POL_ISO : PDFSK_MODE_F64_POL_ANISO;
-    rc = sweep64_batch(c, mode, k, f64, inits, dims, sp64, tile, 0, tol, 1, maxp, graph, profile,
-                       ps, NULL, NULL);
+    const int mode = slows ? PDFSK_MODE_F64_POL_SPEED
+                           : sp64->iso ? PDFSK_MODE_F64_POL_ISO : PDFSK_MODE_F64_POL_ANISO;
+    rc = sweep64_batch(c, mode, k, f64, inits, slows, dims, sp64, tile, 0, tol, 1, maxp, graph,
+                       profile, ps, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) goto out;
     const int rc0 = rc;
     rc = PDFS_OK;
@@ -828,11 +833,12 @@ out:
  * a field beyond the FP64 column limit (2^28 points) is left as it is and
  * counted in polish_skipped. */
 int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
-                 const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
-                 pdfs_stats *s)
+                 const float *const *slows, const int *dims, double dx, double dy, double dz,
+                 const pdfs_opts *o, pdfs_stats *s)
 {
   if (!inits) return PDFS_OK;
-  const int tile = tile64_of(o);
+  int tile = tile64_of(o);
+  if (slows && !pdfsk_tile_supports(tile, PDFSK_MODE_F64_POL_SPEED)) tile = PDFSK_TILE_8x8;
   pdfs_sp64 sp64;
   make_sp64(dx, dy, dz, &sp64);
   const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
@@ -840,10 +846,11 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
   const int maxp = o->max_passes > 0 ? o->max_passes : 100;
   float **gf = (float **)malloc((size_t)nf * sizeof(float *));
   const float **gi = (const float **)malloc((size_t)nf * sizeof(float *));
+  const float **gs = (const float **)malloc((size_t)nf * sizeof(float *));
   int *gd = (int *)malloc((size_t)3 * nf * sizeof(int));
   int rc = PDFS_OK, k = 0;
   float tms = 0.f;
-  if (!gf || !gi || !gd) {
+  if (!gf || !gi || !gs || !gd) {
     rc = fail(PDFS_ENOMEM, "polish: host alloc");
     goto out;
   }
@@ -862,8 +869,8 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
     if (k > 0 && (f == nf || !plan_fits_w(k + 1, gd, tile, 2))) {
       pdfs_stats ps;
       memset(&ps, 0, sizeof ps);
-      const int r =
-          polish_group(c, k, gf, gi, gd, &sp64, tol, maxp, tile, o->graph, o->profile, &ps);
+      const int r = polish_group(c, k, gf, gi, slows ? gs : NULL, gd, &sp64, tol, maxp, tile,
+                                 o->graph, o->profile, &ps);
       if (ps.passes > s->polish_passes) s->polish_passes = ps.passes;
       s->polish_orderings += ps.orderings;
       s->polish_point_updates += ps.point_updates;
@@ -880,6 +887,7 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
     if (f < nf) {
       gf[k] = fields[f];
       gi[k] = inits[f];
+      gs[k] = slows ? slows[f] : NULL;
       k++;
     }
   }
@@ -890,6 +898,7 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
 out:
   free(gf);
   free(gi);
+  free(gs);
   free(gd);
   return rc;
 }
@@ -916,8 +925,8 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
 /* FP64 residual judge of n fields in one launch; flagged[i] = failing points
  * of field i. flags[i] (optional) receive one bit per point. */
 int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *const *init,
-                      const int *dims, uint32_t *const *flags, double dx, double dy, double dz,
-                      long long *flagged)
+                      const float *const *slow, const int *dims, uint32_t *const *flags,
+                      double dx, double dy, double dz, long long *flagged)
 {
   if (!c || n < 1 || !v || !init || !dims) return fail(PDFS_EINVAL, "judge: bad arguments");
   if (n + 1 > c->cap_jt) {
@@ -940,7 +949,7 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
   long long tiles = 0;
   for (int i = 0; i < n; i++) {
     const int x = dims[3 * i], y = dims[3 * i + 1], z = dims[3 * i + 2];
-    if (!v[i] || !init[i] || x < 1 || y < 1 || z < 1)
+    if (!v[i] || !init[i] || (slow && !slow[i]) || x < 1 || y < 1 || z < 1)
       return fail(PDFS_EINVAL, "judge: field %d: bad pointer or dims", i);
     /* 32-bit element offsets in the judge kernel */
     if ((long long)x * y * z > PDFSK_MAX_POINTS)
@@ -949,6 +958,7 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
     pdfsk_judge_field *t = &c->jt_host[i];
     t->v = v[i];
     t->init = init[i];
+    t->slow = slow ? slow[i] : NULL;
     t->flags = flags ? flags[i] : NULL;
     t->X = x;
     t->Y = y;
@@ -964,7 +974,8 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
   HIPCHK(hipMemcpyAsync(c->jt_dev, c->jt_host, (size_t)(n + 1) * sizeof(pdfsk_judge_field),
                         hipMemcpyHostToDevice, c->st));
   KCHK(pdfsk_fill_u32((uint32_t *)c->jc_dev, 0u, (size_t)2 * n, c->st));
-  KCHK(pdfsk_judge_batch_f32(c->jt_dev, n, (long)tiles, dx, dy, dz, c->jc_dev, c->st));
+  KCHK(pdfsk_judge_batch_f32(c->jt_dev, n, (long)tiles, dx, dy, dz, slow != NULL, c->jc_dev,
+                             c->st));
   HIPCHK(hipMemcpyAsync(c->jc_host, c->jc_dev, (size_t)n * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, c->st));
   /* also keeps the host table from being rewritten while its copy is pending */
@@ -995,7 +1006,8 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
     return fail(PDFS_EINVAL, "pdfs_judge_f32_dev: bad arguments");
   const int dims[3] = {x, y, z};
   long long fl = 0;
-  int rc = pdfsi_judge_batch(c, 1, &field, &init, dims, flags ? &flags : NULL, dx, dy, dz, &fl);
+  int rc =
+      pdfsi_judge_batch(c, 1, &field, &init, NULL, dims, flags ? &flags : NULL, dx, dy, dz, &fl);
   if (rc == PDFS_OK && flagged) *flagged = fl;
   return rc;
 }
@@ -1003,9 +1015,9 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
 /* The FP32 batch solve without the spacing check (run_fsm and the CLI take
  * the caller's spacings as the reference does). */
 static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
-                     const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
-                     pdfs_stats *st, int *field_passes, double *field_ms,
-                     long long *field_flagged);
+                     const float *const *slows, const int *dims, double dx, double dy,
+                     double dz, const pdfs_opts *o, pdfs_stats *st, int *field_passes,
+                     double *field_ms, long long *field_flagged);
 
 int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
                              const float *const *inits, const int *dims, double dx, double dy,
@@ -1014,21 +1026,34 @@ int pdfs_solve_batch_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
 {
   if (!c || nf < 1 || !fields || !dims || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
-  return batch_f32(c, nf, fields, inits, dims, dx, dy, dz, o, st, field_passes, field_ms, NULL);
+  return batch_f32(c, nf, fields, inits, NULL, dims, dx, dy, dz, o, st, field_passes, field_ms,
+                   NULL);
+}
+
+int pdfs_solve_batch_speed_f32_dev(pdfs_ctx *c, int nf, float *const *fields,
+                                   const float *const *inits, const float *const *slowness,
+                                   const int *dims, double dx, double dy, double dz,
+                                   const pdfs_opts *o, pdfs_stats *st, int *field_passes,
+                                   double *field_ms)
+{
+  if (!c || nf < 1 || !fields || !slowness || !dims || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_solve_batch_speed_f32_dev: bad arguments");
+  return batch_f32(c, nf, fields, inits, slowness, dims, dx, dy, dz, o, st, field_passes,
+                   field_ms, NULL);
 }
 
 int pdfsi_batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
                     const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
                     pdfs_stats *st, int *field_passes, long long *field_flagged)
 {
-  return batch_f32(c, nf, fields, inits, dims, dx, dy, dz, o, st, field_passes, NULL,
+  return batch_f32(c, nf, fields, inits, NULL, dims, dx, dy, dz, o, st, field_passes, NULL,
                    field_flagged);
 }
 
 static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
-                     const int *dims, double dx, double dy, double dz, const pdfs_opts *o,
-                     pdfs_stats *st, int *field_passes, double *field_ms,
-                     long long *field_flagged)
+                     const float *const *slows, const int *dims, double dx, double dy,
+                     double dz, const pdfs_opts *o, pdfs_stats *st, int *field_passes,
+                     double *field_ms, long long *field_flagged)
 {
   if (!c || nf < 1 || !fields || !dims)
     return fail(PDFS_EINVAL, "pdfs_solve_batch_f32_dev: bad arguments");
@@ -1047,8 +1072,15 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   long long key = tune_key(nf, dims);
   int tuning = 0;
   int tile;
-  if (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT) {
+  /* a slowness field: the speed mode, whose column variants exclude 32x16
+   * (its aux delay line does not fit the LDS); no online tuning */
+  const int speed = slows != NULL;
+  if (o->tile >= 0 && o->tile < PDFSK_TILE_COUNT &&
+      (!speed || pdfsk_tile_supports(o->tile, PDFSK_MODE_F32_SPEED))) {
     tile = o->tile;
+  } else if (speed) {
+    tile = default_tile(nf, dims[0], dims[1]);
+    if (!pdfsk_tile_supports(tile, PDFSK_MODE_F32_SPEED)) tile = PDFSK_TILE_32x8;
   } else if (o->tile == PDFS_TILE_AUTO) {
     tile = tune_lookup(c, key);
     if (tile < 0) {
@@ -1062,9 +1094,10 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   }
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
-  const int m32 = mode32(&sp);
+  const int m32 = speed ? PDFSK_MODE_F32_SPEED : mode32(&sp);
   int rc;
-  if ((rc = plan_build(c, nf, fields, dims, tile))) return rc;
+  if ((rc = plan_build_ex(c, nf, (void *const *)fields, NULL, slows, dims, tile, 0, 1)))
+    return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
   /* per-field change count of the current pass (summed over its segments),
    * and the points the field's last judge flagged */
@@ -1079,9 +1112,10 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   int *jd = (int *)malloc((size_t)3 * nf * sizeof(int));
   const float **jv = (const float **)malloc((size_t)nf * sizeof(float *));
   const float **ji = (const float **)malloc((size_t)nf * sizeof(float *));
+  const float **js = (const float **)malloc((size_t)nf * sizeof(float *));
   long long *jfl = (long long *)malloc((size_t)nf * sizeof(long long));
   if (!delta || !pdel || !lastfl || !active || !certified || !jidx || !jdone || !jd || !jv ||
-      !ji || !jfl) {
+      !ji || !js || !jfl) {
     free(delta);
     free(pdel);
     free(lastfl);
@@ -1092,6 +1126,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     free(jd);
     free(jv);
     free(ji);
+    free(js);
     free(jfl);
     return fail(PDFS_ENOMEM, "batch: host alloc");
   }
@@ -1140,7 +1175,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
             rc = run_orderings(c, m32, &sp, NULL, v + 2, v + 2, o->profile, 0, &kms);
             continue;
           }
-          if (c->tile != v && (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, dims, v, 1, 1))) break;
+          if (c->tile != v &&
+              (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, NULL, dims, v, 1, 1)))
+            break;
           tk = 0.0;
           rc = run_orderings(c, m32, &sp, NULL, v + 2, v + 2, 1, 0, &tk);
           cand_ms[v] = tk;
@@ -1153,7 +1190,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
         tuning = 0;
         tune_store(c, key, best);
         tile = best;
-        if (c->tile != best && (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, dims, best, 1, 1))) break;
+        if (c->tile != best &&
+            (rc = plan_build_ex(c, nf, (void *const *)fields, NULL, NULL, dims, best, 1, 1)))
+          break;
         if ((rc = run_orderings(c, m32, &sp, NULL, PDFSK_TUNE_COUNT + 2, 8, o->profile, 0, &kms)))
           break;
       } else if ((rc = run_orderings(c, m32, &sp, NULL, sn0, sn1, o->profile, o->graph, &kms))) {
@@ -1218,13 +1257,16 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
           jidx[nj] = f;
           jv[nj] = fields[f];
           ji[nj] = inits[f];
+          js[nj] = speed ? slows[f] : NULL;
           memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
           nj++;
         }
       }
       /* all fields to certify in one judge launch */
       if (nj > 0) {
-        if ((rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl))) break;
+        if ((rc = pdfsi_judge_batch(c, nj, jv, ji, speed ? js : NULL, jd, NULL, dx, dy, dz,
+                                    jfl)))
+          break;
         s.judge_runs += nj;
         for (int k = 0; k < nj; k++) {
           jdone[jidx[k]] = jfl[k] == 0;
@@ -1264,7 +1306,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
    * choice; the plan is rebuilt for the FP64 sweeps) */
   if ((rc == PDFS_OK || rc == PDFS_ENOTCONV) && o->polish && inits && o->fixed_passes <= 0) {
     const int rc0 = rc;
-    rc = pdfsi_polish(c, nf, fields, inits, dims, dx, dy, dz, o, &s);
+    rc = pdfsi_polish(c, nf, fields, inits, slows, dims, dx, dy, dz, o, &s);
     if (rc == PDFS_OK) rc = rc0;
   }
   if (rc == PDFS_OK && o->judge && inits) {
@@ -1277,10 +1319,12 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
       jidx[nj] = f;
       jv[nj] = fields[f];
       ji[nj] = inits[f];
+      js[nj] = speed ? slows[f] : NULL;
       memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
       nj++;
     }
-    if (nj > 0 && (rc = pdfsi_judge_batch(c, nj, jv, ji, jd, NULL, dx, dy, dz, jfl)) == PDFS_OK) {
+    if (nj > 0 && (rc = pdfsi_judge_batch(c, nj, jv, ji, speed ? js : NULL, jd, NULL, dx, dy, dz,
+                                          jfl)) == PDFS_OK) {
       for (int k = 0; k < nj; k++) {
         s.judge_flagged += jfl[k];
         if (field_flagged) field_flagged[jidx[k]] = jfl[k];
@@ -1312,6 +1356,7 @@ out:
   free(jd);
   free(jv);
   free(ji);
+  free(js);
   free(jfl);
   return rc;
 }
@@ -1323,7 +1368,7 @@ static int single_f32(pdfs_ctx *c, float *field, const float *init, int x, int y
   if (!c || !field || x < 1 || y < 1 || z < 1)
     return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
   int dims[3] = {x, y, z};
-  return batch_f32(c, 1, &field, &init, dims, dx, dy, dz, o, st, NULL, NULL, NULL);
+  return batch_f32(c, 1, &field, &init, NULL, dims, dx, dy, dz, o, st, NULL, NULL, NULL);
 }
 
 int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int y, int z,
@@ -1332,6 +1377,30 @@ int pdfs_solve_f32_dev(pdfs_ctx *c, float *field, const float *init, int x, int 
   if (!c || !field || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_f32_dev: bad arguments");
   return single_f32(c, field, init, x, y, z, dx, dy, dz, o, st);
+}
+
+int pdfs_solve_speed_f32_dev(pdfs_ctx *c, float *field, const float *init,
+                             const float *slowness, int x, int y, int z, double dx, double dy,
+                             double dz, const pdfs_opts *o, pdfs_stats *st)
+{
+  if (!c || !field || !slowness || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_solve_speed_f32_dev: bad arguments");
+  const int dims[3] = {x, y, z};
+  return batch_f32(c, 1, &field, &init, &slowness, dims, dx, dy, dz, o, st, NULL, NULL, NULL);
+}
+
+int pdfs_judge_speed_f32_dev(pdfs_ctx *c, const float *field, const float *init,
+                             const float *slowness, int x, int y, int z, double dx, double dy,
+                             double dz, uint32_t *flags, long long *flagged)
+{
+  if (!c || !field || !init || !slowness || bad_h(dx, dy, dz))
+    return fail(PDFS_EINVAL, "pdfs_judge_speed_f32_dev: bad arguments");
+  const int dims[3] = {x, y, z};
+  long long fl = 0;
+  int rc = pdfsi_judge_batch(c, 1, &field, &init, &slowness, dims, flags ? &flags : NULL, dx, dy,
+                             dz, &fl);
+  if (rc == PDFS_OK && flagged) *flagged = fl;
+  return rc;
 }
 
 /* Whether a padded FP64 field runs on the column pipeline: its ghost layer
@@ -1376,7 +1445,7 @@ static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, i
     pdfs_sp64 sp64;
     make_sp64(dx, dy, dz, &sp64);
     KCHK(pdfsk_pack64(padded, dn, x, y, z, c->st));
-    rc = sweep64_batch(c, PDFSK_MODE_F64_REF, 1, &dn, NULL, dims, &sp64, tile,
+    rc = sweep64_batch(c, PDFSK_MODE_F64_REF, 1, &dn, NULL, NULL, dims, &sp64, tile,
                        passes > 0 ? passes : 0, tol, 0, maxp, 0, 0, &s, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) return rc;
     KCHK(pdfsk_unpack64(dn, padded, x, y, z, c->st));
@@ -1563,7 +1632,8 @@ static int ordering_ref64(pdfs_ctx *c, double *padded, int x, int y, int z, doub
   pdfs_sp64 sp64;
   make_sp64(dx, dy, dz, &sp64);
   KCHK(pdfsk_pack64(padded, dn, x, y, z, c->st));
-  if ((rc = plan_build_ex(c, 1, (void *const *)&dn, NULL, dims, tile64_of(NULL), 0, 2))) return rc;
+  if ((rc = plan_build_ex(c, 1, (void *const *)&dn, NULL, NULL, dims, tile64_of(NULL), 0, 2)))
+    return rc;
   if ((rc = run_orderings(c, PDFSK_MODE_F64_REF, NULL, &sp64, sn, sn, 0, 0, NULL))) return rc;
   long long d = 0;
   unsigned err = 0;

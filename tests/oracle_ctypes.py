@@ -67,6 +67,25 @@ def lib():
         L.orc_sweep_polish64.restype = _D
         L.orc_polish64.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D, _D, _I, C.POINTER(_D)]
         L.orc_polish64.restype = _I
+        L.orc_godunov_f32_speed.argtypes = [C.c_float] * 4 + [_D] * 3 + [C.c_float]
+        L.orc_godunov_f32_speed.restype = C.c_float
+        L.orc_godunov_d64_speed.argtypes = [_D] * 8
+        L.orc_godunov_d64_speed.restype = _D
+        L.orc_sweep_f32_speed.argtypes = [_fp, _fp, _I, _I, _I, _D, _D, _D, _I]
+        L.orc_sweep_f32_speed.restype = None
+        L.orc_sweep_d64_speed.argtypes = [_dp, _dp, _I, _I, _I, _D, _D, _D, _I]
+        L.orc_sweep_d64_speed.restype = None
+        L.orc_solve_f32_speed.argtypes = [_fp, C.c_void_p, _I, _I, _I, _D, _D, _D, _I,
+                                          C.POINTER(C.c_long)]
+        L.orc_solve_f32_speed.restype = _I
+        L.orc_sweep_polish64_speed.argtypes = [_dp, _dp, C.c_void_p, _I, _I, _I, _D, _D, _D, _I]
+        L.orc_sweep_polish64_speed.restype = _D
+        L.orc_polish64_speed.argtypes = [_dp, _dp, C.c_void_p, _I, _I, _I, _D, _D, _D, _D, _I,
+                                         C.POINTER(_D)]
+        L.orc_polish64_speed.restype = _I
+        L.orc_judge_f32_speed.argtypes = [_fp, _fp, C.c_void_p, _I, _I, _I, _D, _D, _D,
+                                          C.c_void_p]
+        L.orc_judge_f32_speed.restype = C.c_long
         _lib = L
     return _lib
 
@@ -168,6 +187,70 @@ def solve_f32_polished(init, h=(1.0, 1.0, 1.0), tol_h=2e-5, max_passes=100):
     i64 = init.astype(np.float32).astype(np.float64)
     pp, _ = polish64(v, i64, h=h, tol=tol_h * min(h), max_passes=max_passes)
     return v.astype(np.float32), p32, pp
+
+
+def _fptr(s):
+    if s is None:
+        return None
+    assert s.dtype == np.float32 and s.flags["C_CONTIGUOUS"]
+    return s.ctypes.data
+
+
+def sweep_f32_speed(a, s, sn, h=(1.0, 1.0, 1.0)):
+    """One FP32 ordering of |grad u| = s in place (a, s padded float32)."""
+    lib().orc_sweep_f32_speed(a, s, *dims(a), *h, sn)
+    return a
+
+
+def sweep_d64_speed(a, s, sn, h=(1.0, 1.0, 1.0)):
+    lib().orc_sweep_d64_speed(a, s, *dims(a), *h, sn)
+    return a
+
+
+def solve_f32_speed(a, s, h=(1.0, 1.0, 1.0), max_passes=100):
+    """FP32 passes of |grad u| = s until a pass changes nothing (s padded
+    float32, or None for 1); returns (passes, points changed in the last)."""
+    ch = C.c_long()
+    p = lib().orc_solve_f32_speed(a, _fptr(s), *dims(a), *h, max_passes, C.byref(ch))
+    return p, ch.value
+
+
+def solve_d64_speed(a, s, h=(1.0, 1.0, 1.0), tol=0.0, max_passes=100):
+    """FP64 difference-form passes of |grad u| = s (s padded float64) until a
+    pass changes nothing by more than tol; returns the passes run."""
+    for p in range(1, max_passes + 1):
+        prev = a.copy()
+        for sn in range(1, 9):
+            sweep_d64_speed(a, s, sn, h)
+        with np.errstate(invalid="ignore"):
+            d = np.abs(a - prev)
+        d[a == prev] = 0.0
+        if not (np.nanmax(d) <= tol):
+            continue
+        return p
+    return max_passes
+
+
+def polish64_speed(a, init, s, h=(1.0, 1.0, 1.0), tol=2e-5, max_passes=100):
+    md = _D()
+    p = lib().orc_polish64_speed(a, init, _fptr(s), *dims(a), *h, tol, max_passes, C.byref(md))
+    return p, md.value
+
+
+def solve_f32_speed_polished(init, s, h=(1.0, 1.0, 1.0), tol_h=2e-5, max_passes=100):
+    """The FP32 speed solve followed by the FP64 speed polish, as the GPU's
+    default speed solve with an init runs it: returns (FP32 result (padded),
+    FP32 passes, polish passes). s padded float32."""
+    u = init.astype(np.float32)
+    p32, _ = solve_f32_speed(u, s, h=h, max_passes=max_passes)
+    v = u.astype(np.float64)
+    i64 = init.astype(np.float32).astype(np.float64)
+    pp, _ = polish64_speed(v, i64, s, h=h, tol=tol_h * min(h), max_passes=max_passes)
+    return v.astype(np.float32), p32, pp
+
+
+def judge_f32_speed(v, init, s, h=(1.0, 1.0, 1.0)):
+    return lib().orc_judge_f32_speed(v, init, _fptr(s), *dims(v), *h, None)
 
 
 def gen_point(X, Y, Z, seed):

@@ -103,6 +103,13 @@ def test_entry_points_reject_bad_arguments_before_touching_the_gpu():
                                                                       1.0)),
         ("pdfs_solve_warm_f32_dev", lambda: L.pdfs_solve_warm_f32_dev(
             None, None, None, None, 4, 4, 4, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st))),
+        ("pdfs_solve_speed_f32_dev", lambda: L.pdfs_solve_speed_f32_dev(
+            None, None, None, None, 4, 4, 4, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st))),
+        ("pdfs_solve_batch_speed_f32_dev", lambda: L.pdfs_solve_batch_speed_f32_dev(
+            None, 1, None, None, None, None, 1.0, 1.0, 1.0, C.byref(opts), C.byref(st), None,
+            None)),
+        ("pdfs_judge_speed_f32_dev", lambda: L.pdfs_judge_speed_f32_dev(
+            None, None, None, None, 4, 4, 4, 1.0, 1.0, 1.0, None, None)),
     ]
     for name, call in calls:
         assert call() == -1, name

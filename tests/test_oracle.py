@@ -510,3 +510,137 @@ def test_polish_cuts_the_fp32_storage_error():
     e_pol = np.abs(O.interior(pol).astype(np.float64) - O.interior(u64)).max()
     print(f"FP32 {e_raw:.3e} h, polished {e_pol:.3e} h ({pp} polish passes)")
     assert e_pol <= 1e-5 and e_pol < e_raw
+
+
+# ------------------------------------------- spatially varying speed, FP32 path
+def _smooth_slowness(shape, seed, lo=0.5, hi=2.0):
+    """Padded slowness: a few random plane waves mapped into [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    f = np.zeros(shape)
+    for _ in range(4):
+        k = rng.uniform(-0.4, 0.4, size=3)
+        f += np.sin(k[0] * x + k[1] * y + k[2] * z + rng.uniform(0, 6.3))
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.75, 1.25)])
+def test_speed_f32_scaled_slowness_is_scaled_spacing(h):
+    """FP32: slowness 2 (and 0.5) everywhere is the unit-speed sweep with
+    doubled (halved) spacings bit for bit: the spacings d s and the weights
+    1 / (d s)^2 = (1 / d^2) / s^2 are exact power-of-two scalings. A weight
+    scaled by 1/s instead of 1/s^2, or the spacing left unscaled, fails."""
+    a = O.gen_spheres(10, 8, 7, seed=12, n=3, r=(1.0, 2.5)).astype(np.float32)
+    for sc in (2.0, 0.5):
+        s = np.full(a.shape, sc, np.float32)
+        for sn in range(1, 9):
+            b, c = a.copy(), a.copy()
+            O.sweep_f32_speed(b, s, sn, h)
+            O.sweep_f32(c, sn, tuple(sc * v for v in h))
+            assert np.array_equal(b.view(np.uint32), c.view(np.uint32)), (sc, sn)
+        b = a.copy()
+        O.sweep_f32_speed(b, np.ones_like(a), 1, h)
+        c = O.sweep_f32(a.copy(), 1, h)
+        assert np.array_equal(b.view(np.uint32), c.view(np.uint32))
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_speed_f32_layered_medium_exact_recursion(axis):
+    """A source plane in a medium layered along one axis, with anisotropic
+    spacings: every update is the 1-D one, so the FP32 result is exactly the
+    recursion u_k = u_{k-1} + fl(d_axis s_k) in FP32 along that axis (the
+    other axes' spacings must not enter)."""
+    X, Y, Z = 7, 6, 11
+    h = (0.75, 1.0, 0.5)
+    a = O.padded(X, Y, Z, dtype=np.float32)
+    a[1:-1, 1:-1, 1:-1] = 1e6
+    s = np.ones(a.shape, np.float32)
+    n = (Z, Y, X)[axis]
+    layers = np.random.default_rng(5 + axis).uniform(0.25, 4.0, size=n).astype(np.float32)
+    d = np.float32(h[2 - axis])
+    if axis == 0:
+        a[1, 1:-1, 1:-1] = 0.0
+        for k in range(n):
+            s[k + 1] = layers[k]
+    else:
+        a[1:-1, 1:-1, 1] = 0.0
+        for k in range(n):
+            s[:, :, k + 1] = layers[k]
+    O.solve_f32_speed(a, s, h)
+    want = [np.float32(0.0)]
+    for k in range(1, n):
+        want.append(np.float32(want[-1] + np.float32(d * layers[k])))
+    for k in range(n):
+        got = a[k + 1, 1:-1, 1:-1] if axis == 0 else a[1:-1, 1:-1, k + 1]
+        assert np.all(got == want[k]), k
+
+
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.6, 0.8, 1.1)])
+def test_speed_d64_difference_form_matches_reference_form(h):
+    """The FP64 difference form with scaled spacings converges to the
+    reference-form speed solve (orc_solve_ref64_speed, itself pinned by the
+    unit / doubled / layered checks above) to 1e-10 relative: the same
+    equation written two ways. A wrong per-point scaling of the weights
+    moves the 2-D / 3-D roots by far more."""
+    a = O.gen_spheres(18, 15, 13, seed=31, n=4, r=(1.5, 3.5))
+    s = _smooth_slowness(a.shape, 2)
+    r64 = a.copy()
+    O.solve_ref64_speed(r64, s, h, tol=0.0, max_passes=200)
+    d64 = a.copy()
+    O.solve_d64_speed(d64, s, h, tol=0.0, max_passes=200)
+    inner = (slice(1, -1),) * 3
+    ref, got = r64[inner], d64[inner]
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(got))
+    assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(ref[fin], min(h))) < 1e-10
+    # the slowness matters: the unit-speed solve is far away
+    u1 = a.copy()
+    O.solve_ref64(u1, h, tol=0.0)
+    assert np.max(np.abs(u1[inner][fin] - ref[fin])) > 1.0
+
+
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.6, 0.8, 1.1)])
+def test_speed_polish_limit_and_f32_accuracy(h):
+    """The FP64 speed polish from the FP32 speed solution converges to the
+    FP64 speed solution (reference form) to 1e-9 h; the FP32 speed solve
+    alone is within 1e-4 h of it on this small field."""
+    # an init exact in FP32, so that both solves start from the same values
+    a = O.gen_spheres(20, 17, 15, seed=44, n=4, r=(1.5, 3.5)).astype(np.float32).astype(np.float64)
+    s32 = _smooth_slowness(a.shape, 6).astype(np.float32)
+    s64 = s32.astype(np.float64)
+    r64 = a.copy()
+    O.solve_ref64_speed(r64, s64, h, tol=0.0, max_passes=200)
+    u = a.astype(np.float32)
+    O.solve_f32_speed(u, s32, h)
+    inner = (slice(1, -1),) * 3
+    fin = np.isfinite(r64[inner])
+    err32 = np.max(np.abs(u[inner][fin].astype(np.float64) - r64[inner][fin]))
+    assert err32 < 1e-4 * min(h)
+    v = u.astype(np.float64)
+    init = a.astype(np.float32).astype(np.float64)
+    p, _ = O.polish64_speed(v, init, s32, h, tol=0.0, max_passes=100)
+    assert p < 100
+    assert np.max(np.abs(v[inner][fin] - r64[inner][fin])) < 1e-9 * min(h)
+    # unit-slowness polish of the same field lands elsewhere
+    w = u.astype(np.float64)
+    O.polish64_speed(w, init, None, h, tol=0.0, max_passes=100)
+    assert np.max(np.abs(w[inner][fin] - r64[inner][fin])) > 0.5
+
+
+def test_speed_judge_uses_the_slowness():
+    """The speed judge passes the FP64 speed solution rounded to FP32, flags
+    a value planted 1e-3 below it, and the unit-speed judge rejects the same
+    solution (the slowness enters the judged equation)."""
+    h = (1.0, 1.0, 1.0)
+    a = O.gen_spheres(16, 14, 12, seed=52, n=3, r=(1.5, 3.0))
+    s32 = _smooth_slowness(a.shape, 9).astype(np.float32)
+    r64 = a.copy()
+    O.solve_ref64_speed(r64, s32.astype(np.float64), h, tol=0.0, max_passes=200)
+    v = r64.astype(np.float32)
+    init = a.astype(np.float32)
+    assert O.judge_f32_speed(v, init, s32, h) == 0
+    assert O.judge_f32_speed(v, init, None, h) > 100
+    w = v.copy()
+    w[7, 6, 9] -= np.float32(1e-3)
+    assert O.judge_f32_speed(w, init, s32, h) >= 1

@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--batch512", type=int, default=8,
                     help="fields of the same-size batch leg (0: skip)")
     ap.add_argument("--no-warm", action="store_true", help="skip the certified / warm-start legs")
+    ap.add_argument("--no-speed", action="store_true",
+                    help="skip the spatially varying speed leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run plain 1 GiB copies (for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -181,6 +183,8 @@ def main():
         del tinit, tu64
     del u64
 
+    if not args.no_speed:
+        extra["speed_512"] = speed_leg(ctx, init, field, args)
     if not args.no_oracle:
         extra["parity"] = parity_leg(ctx, args)
     if args.batch512 > 0:
@@ -341,6 +345,84 @@ def solve_modes(ctx, net, init, field, tile, plain_ms, u64=None):
                            "fallbacks": sw["fallbacks"], "judge_flagged": int(sw["judge_flagged"]),
                            "speedup_vs_plain": round(plain_ms / wms, 3)})
     log(f"[bench] modes {tuple(init.shape)}: {out}")
+    return out
+
+
+def slowness_field(Z, Y, X, dev):
+    """Slowness (1 / speed) in [0.5, 2] for the speed leg: four layers in z,
+    modulated by two smooth lateral waves (float32 [Z, Y, X])."""
+    import torch
+    z = torch.arange(Z, device=dev, dtype=torch.float32).view(Z, 1, 1)
+    y = torch.arange(Y, device=dev, dtype=torch.float32).view(1, Y, 1)
+    x = torch.arange(X, device=dev, dtype=torch.float32).view(1, 1, X)
+    layer = torch.floor(z * 4.0 / Z)
+    wave = torch.sin(0.031 * x + 0.017 * y + 0.4) + torch.cos(0.023 * y - 0.011 * z)
+    f = layer / 3.0 * 0.6 + (wave + 2.0) / 4.0 * 0.4
+    return (0.5 + 1.5 * f).contiguous()
+
+
+def speed_leg(ctx, init, field, args):
+    """Spatially varying speed at the headline size: the multi-sphere field
+    with a layered, laterally varying slowness in [0.5, 2]. The default
+    solve (FP32 speed sweeps to the fixed point, FP64 speed polish, judge)
+    timed best of 3; the FP32 fixed point alone; and both against the
+    converged FP64 speed solution (reference arithmetic with the point's
+    spacings scaled by its slowness, pdfs_solve_speed64_dev)."""
+    import torch
+    Z, Y, X = init.shape
+    dev = init.device
+    s = slowness_field(Z, Y, X, dev)
+    out = {"slowness": "[0.5, 2]: 4 z-layers x smooth lateral waves"}
+
+    def timed(fn, reps=3):
+        best, st = None, None
+        for _ in range(reps):
+            field.copy_(init)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            dt = 1000.0 * (time.perf_counter() - t0)
+            if best is None or dt < best:
+                best, st = dt, r
+        return best, st
+
+    ms, st = timed(lambda: ctx.solve_speed_f32(field, s, init=init, tile=args.tile, profile=0))
+    res = field.clone()
+    ms_raw, sr = timed(lambda: ctx.solve_speed_f32(field, s, init=init, tile=args.tile, polish=0),
+                       reps=1)
+    raw = field.clone()
+    field.copy_(init)
+    pst = ctx.solve_speed_f32(field, s, init=init, tile=args.tile, profile=1)
+    launches = 8 * ((pst["orderings"] + 7) // 8)
+    out.update({"ms": round(ms, 3), "fields_per_s": round(1000.0 / ms, 3), "passes": st["passes"],
+                "orderings": st["orderings"], "polish_passes": st["polish_passes"],
+                "judge_flagged": int(st["judge_flagged"]), "tile": st["tile"],
+                "ms_fp32_only": round(ms_raw, 3),
+                "fp32_ms_per_launch": round(pst["ms_sweep_kernels"] / launches, 4),
+                "polish_ms_per_launch": round(pst["ms_polish_kernels"]
+                                              / max(1, pst["polish_orderings"]), 4)})
+    if not args.no_oracle:
+        pad = torch.full((Z + 2, Y + 2, X + 2), float("inf"), dtype=torch.float64, device=dev)
+        pad[1:-1, 1:-1, 1:-1] = init.double()
+        sp = torch.ones_like(pad)
+        sp[1:-1, 1:-1, 1:-1] = s.double()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r64 = ctx.solve_speed64(pad, sp, passes=0, tol=1e-12)
+        torch.cuda.synchronize()
+        out["speed64_seconds"] = round(time.perf_counter() - t0, 3)
+        out["speed64_passes"] = r64["passes"]
+        u64 = pad[1:-1, 1:-1, 1:-1]
+        for name, v in (("", res), ("fp32_only_", raw)):
+            e = (v.double() - u64).abs()
+            out[name + "max_err"] = float(e.max())
+            out[name + "rms_err"] = float(e.pow(2).mean().sqrt())
+        out["within_1e-4h"] = bool(out["max_err"] <= 1e-4)
+        del pad, sp, u64
+    del s, res, raw
+    torch.cuda.empty_cache()
+    log(f"[bench] speed {tuple(init.shape)}: {out}")
     return out
 
 

@@ -474,9 +474,8 @@ static int layer_bwd(pdfs_net *n, int l, const uint16_t *x, const uint16_t *gy, 
                      int lvl, int Dz, int Dy, int Dx)
 {
   const int dz = Dz >> lvl, dy = Dy >> lvl, dx = Dx >> lvl;
-  const long M = (long)nb * dz * dy * dx;
-  NCK(pdfsnn_conv3_wgrad(x, CIN[l], gy, COUT[l], n->G + n->woff[l], nb, dz, dy, dx, n->st));
-  NCK(pdfsnn_bias_grad(gy, COUT[l], M, n->G + n->boff[l], n->st));
+  NCK(pdfsnn_conv3_wgrad(x, CIN[l], gy, COUT[l], n->G + n->woff[l], n->G + n->boff[l], nb, dz,
+                         dy, dx, n->st));
   if (gx)
     NCK(pdfsnn_conv3(gy, COUT[l], n->WT + n->woff[l], NULL, CIN[l], NULL, gx, nb, dz, dy, dx, 0, 1,
                      n->st));

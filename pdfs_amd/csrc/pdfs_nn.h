@@ -20,9 +20,9 @@ int pdfsnn_conv3(const uint16_t *in, int Cin, const uint16_t *W, const float *bi
                  uint16_t *out, float *out_f32, int n, int Dz, int Dy, int Dx, int relu, int flip,
                  hipStream_t st);
 /* dW[tap][co][ci] += sum_p dY[p][co] * X[p + shift(tap)][ci]; (Cout/16)(Cin/16) <= 32 */
-int pdfsnn_conv3_wgrad(const uint16_t *X, int Cin, const uint16_t *dY, int Cout, float *dW, int n,
-                       int Dz, int Dy, int Dx, hipStream_t st);
-int pdfsnn_bias_grad(const uint16_t *dY, int Cout, long M, float *db, hipStream_t st);
+/* dW += the weight gradient; db (or NULL) += the bias gradient */
+int pdfsnn_conv3_wgrad(const uint16_t *X, int Cin, const uint16_t *dY, int Cout, float *dW,
+                       float *db, int n, int Dz, int Dy, int Dx, hipStream_t st);
 int pdfsnn_pool2(const uint16_t *in, uint16_t *out, int n, int Dz, int Dy, int Dx, int C,
                  hipStream_t st);
 /* din (fine dims Dz,Dy,Dx) = or += dout / 8 */

@@ -1,19 +1,22 @@
 // gfx950 kernels of the learned warm start: a 3-D conv encoder-decoder in
-// BF16 with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).
+// BF16 with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16).
 //
 // Activations: channels-last [n][z][y][x][C] BF16, C a multiple of 16.
 // Weights: [27][Cout][Cin] BF16 (tap = (dz*3 + dy)*3 + dx, cin fastest),
 // an FP32 master copy lives with the optimizer.
 //
-// MFMA 16x16x16 operand layout (wave64): lane l = c + 16 g (c = l % 16,
-// g = l / 16). A (16 x 16, M x K): lane holds A[c][4g .. 4g+3]. B (K x N):
-// lane holds B[4g .. 4g+3][c]. D (M x N, FP32): lane holds D[4g .. 4g+3][c].
+// MFMA 16x16x32 operand layout (wave64): lane l = c + 16 g (c = l % 16,
+// g = l / 16). A (16 x 32, M x K): lane holds A[c][8g .. 8g+7]. B (K x N):
+// lane holds B[8g .. 8g+7][c]. D (M x N, FP32): lane holds D[4g .. 4g+3][c].
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pdfs_nn.h"
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));  // the same 16 bytes, as raw bits
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -22,6 +25,11 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
 static inline int hip_rc(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
@@ -34,85 +42,147 @@ static inline unsigned grid1(long n, int bs) {
 // ---------------------------------------------------------------------------
 // 3x3x3 convolution, stride 1, zero padding 1, as an implicit GEMM:
 // M = positions, N = Cout, K = 27 taps x Cin, on LDS-staged input tiles.
-// Block (4 waves): an 8 x 8 x 4 tile of output positions of one sample, as
-// 16 groups of 16 positions (8 along x by 2 along y); wave w owns groups
-// 4w .. 4w+3, each against all Cout (NT tiles of 16). Per chunk of 16 input
-// channels the input tile with a one-point halo, 10 x 10 x 6 positions x 16
-// BF16 = 19.2 KB, is staged in LDS (zero outside the grid); all 27 taps then
-// read their A fragments from LDS, the weight fragments come from L2.
+// Block (4 waves): an 8 x 8 x TZ tile of output positions of one sample, as
+// 4 TZ groups of 16 positions (8 along x by 2 along y); wave w owns groups
+// w TZ .. w TZ + TZ - 1, each against all Cout (NT tiles of 16), so every
+// weight fragment feeds TZ MFMAs. TZ = 4, or 1 on the coarse levels (more
+// blocks). The input tile with a
+// one-point halo (10 x 10 x (TZ + 2) positions) is staged in LDS one channel
+// chunk at a time:
+// * K32 (Cin a multiple of 32): chunks of 32 channels; one MFMA step is one
+//   tap x 32 channels, lane group g taking channels 8g .. 8g+7;
+// * otherwise chunks of 16 channels and taps in pairs: lane groups 0, 1 take
+//   tap 2p (channels 8g ..), groups 2, 3 tap 2p + 1 (tap 27 is zero), so the
+//   32-deep K of one MFMA spans two taps.
+// The K steps are unrolled with two register sets: the A fragments (LDS)
+// and weight fragments (L2) of step s + 1 load while step s's MFMAs run.
 // out = act(conv + bias); act: 0 linear, 1 ReLU. out (BF16) and out_f32
 // each optional. With flip = 1 the taps are mirrored (dz,dy,dx) ->
 // (2-dz, 2-dy, 2-dx): with transposed weights this is the input gradient.
 // ---------------------------------------------------------------------------
-constexpr int CT_HX = 10, CT_HY = 10, CT_HZ = 6, CT_HP = CT_HX * CT_HY * CT_HZ;
+constexpr int CT_HX = 10, CT_HY = 10;
+// LDS tile of a (TZ, K32) kernel, in 16-byte vectors: one plane of positions
+// per 8-channel vector, plane pitch HP + 2 vectors ((HP + 2) * 16 B is 32 B
+// mod 128 B for HP = 100 (TZ + 2), so consecutive planes start 8 banks
+// apart); an A-fragment read takes consecutive vectors of one plane.
+template <int TZ> struct ConvTile {
+  static constexpr int HZ = TZ + 2, HP = CT_HX * CT_HY * HZ, HPS = HP + 2;
+};
+template <int TZ, bool K32> constexpr int conv_tile_vecs() { return ConvTile<TZ>::HPS * (K32 ? 4 : 2); }
+
+__device__ __forceinline__ int tap_shift(int tap, int flip) {
+  const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+  return flip ? ((2 - dz) * CT_HY + (2 - dy)) * CT_HX + (2 - dx)
+              : (dz * CT_HY + dy) * CT_HX + dx;
+}
 
 // One output tile (bx, by, bz) of sample b; tile = the block's LDS staging
-// area ([z][y][x][channel quad], CT_HP * 4 entries).
-template <int NT>
+// area.
+template <int NT, bool K32, int TZ>
 __device__ __forceinline__ void conv3_tile_body(const uint16_t *__restrict__ in, int Cin,
                                                 const uint16_t *__restrict__ W,
                                                 const float *__restrict__ bias, int Cout,
                                                 uint16_t *__restrict__ out,
                                                 float *__restrict__ out_f32, long b, int Dz,
                                                 int Dy, int Dx, int bx, int by, int bz, int relu,
-                                                int flip, bf16x4 *tile) {
-  constexpr int HX = CT_HX, HY = CT_HY, HP = CT_HP;
+                                                int flip, s16x8 *tile) {
+  constexpr int HX = CT_HX, HY = CT_HY, HP = ConvTile<TZ>::HP, HPS = ConvTile<TZ>::HPS;
+  constexpr int CQ = K32 ? 4 : 2;       // 8-channel vectors per position and chunk
+  constexpr int CK = 8 * CQ;            // channels per chunk
+  constexpr int NSTEP = K32 ? 27 : 14;  // MFMA K steps per chunk
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, g = lane >> 4;
-  const int x0 = bx * 8, y0 = by * 8, z0 = bz * 4;
-  f32x4 acc[4][NT];
+  const int x0 = bx * 8, y0 = by * 8, z0 = bz * TZ;
+  f32x4 acc[TZ][NT];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < TZ; i++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[i][t] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  // this lane's A row in each of its groups: x = c & 7, y = 2 (q & 3) + (c >> 3), z = q >> 2
-  int arow[4];
+  // this lane's A row in each of its groups q: x = c & 7, y = 2 (q & 3) + (c >> 3), z = q >> 2
+  int arow[TZ];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int q = 4 * wave + i;
+  for (int i = 0; i < TZ; i++) {
+    const int q = TZ * wave + i;
     arow[i] = ((q >> 2) * HY + 2 * (q & 3) + (c >> 3)) * HX + (c & 7);
   }
-  const int nchunk = Cin >> 4;
+  const s16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  // K step s of chunk k for this lane: its tap, and its 8-channel vector of the chunk
+  auto step_tap = [&](int s) -> int { return K32 ? s : 2 * s + (g >> 1); };
+  const int vq = K32 ? g : (g & 1);
+  // weight fragments of step s (tap 27, past the last pair, is zero: its
+  // load reads tap 26 and is discarded)
+  const uint16_t *const wl = W + (long)c * Cin + 8 * vq;
+  auto load_w = [&](int k, int s, s16x8 *bw) __attribute__((always_inline)) {
+    const int tap = step_tap(s);
+    const int tc = tap < 27 ? tap : 26;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const s16x8 v = *(const s16x8 *)(wl + ((long)tc * Cout + 16 * t) * Cin + CK * k);
+      bw[t] = tap < 27 ? v : zero8;
+    }
+  };
+  auto load_a = [&](int s, s16x8 *a) __attribute__((always_inline)) {
+    const int tap = step_tap(s);
+    const int sh = tap_shift(tap < 27 ? tap : 26, flip);
+#pragma unroll
+    for (int i = 0; i < TZ; i++) a[i] = tile[vq * HPS + arow[i] + sh];
+  };
+  const int nchunk = Cin / CK;
   for (int k = 0; k < nchunk; k++) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < HP * 4; i += 256) {
-      const int v = i & 3, p = i >> 2;
+    s16x8 a[2][TZ], w[2][NT];
+    load_w(k, 0, w[0]);
+    // stage the chunk in batches of up to SB vectors per thread: a batch's
+    // loads all issue before its LDS writes (the first batch's before the
+    // barrier that ends the previous chunk's reads)
+    constexpr int NV = (HP * CQ + 255) / 256, SB = 6;
+    auto stage_load = [&](int j) -> s16x8 {
+      const int i = (int)threadIdx.x + 256 * j;
+      const int v = i % CQ, p = i / CQ;
       const int hx = p % HX, hy = (p / HX) % HY, hz = p / (HX * HY);
       const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-      bf16x4 val = {0, 0, 0, 0};
-      if ((unsigned)gx < (unsigned)Dx && (unsigned)gy < (unsigned)Dy && (unsigned)gz < (unsigned)Dz)
-        val = *(const bf16x4 *)(in + (((b * Dz + gz) * Dy + gy) * (long)Dx + gx) * Cin + 16 * k +
-                                4 * v);
-      tile[i] = val;
+      s16x8 r = zero8;
+      if (i < HP * CQ && (unsigned)gx < (unsigned)Dx && (unsigned)gy < (unsigned)Dy &&
+          (unsigned)gz < (unsigned)Dz)
+        r = *(const s16x8 *)(in + (((b * Dz + gz) * Dy + gy) * (long)Dx + gx) * Cin + CK * k +
+                             8 * v);
+      return r;
+    };
+    auto stage_store = [&](int j, s16x8 v) __attribute__((always_inline)) {
+      const int i = (int)threadIdx.x + 256 * j;
+      if (i < HP * CQ) tile[(i % CQ) * HPS + i / CQ] = v;
+    };
+#pragma unroll
+    for (int j0 = 0; j0 < NV; j0 += SB) {
+      s16x8 val[SB];
+#pragma unroll
+      for (int j = 0; j < SB; j++)
+        if (j0 + j < NV) val[j] = stage_load(j0 + j);
+      if (j0 == 0) __syncthreads();
+#pragma unroll
+      for (int j = 0; j < SB; j++)
+        if (j0 + j < NV) stage_store(j0 + j, val[j]);
     }
     __syncthreads();
-    for (int tap = 0; tap < 27; tap++) {
-      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-      const int sh = flip ? ((2 - dz) * HY + (2 - dy)) * HX + (2 - dx) : (dz * HY + dy) * HX + dx;
-      // output tiles beyond Cout (Cout = 48 or 96 with NT = 4 or 8) are skipped
-      bf16x4 bw[NT];
+    load_a(0, a[0]);
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        bw[t] = 16 * t < Cout
-                    ? *(const bf16x4 *)(W + ((long)tap * Cout + 16 * t + c) * Cin + 16 * k + 4 * g)
-                    : (bf16x4){0, 0, 0, 0};
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const bf16x4 a = tile[(arow[i] + sh) * 4 + g];
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-          if (16 * t < Cout)
-            acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[i][t], 0, 0, 0);
+    for (int s = 0; s < NSTEP; s++) {
+      const int cur = s & 1;
+      if (s + 1 < NSTEP) {
+        load_w(k, s + 1, w[cur ^ 1]);
+        load_a(s + 1, a[cur ^ 1]);
       }
+#pragma unroll
+      for (int i = 0; i < TZ; i++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[i][t] = mfma32(a[cur][i], w[cur][t], acc[i][t]);
     }
   }
   // D: lane (c, g) holds rows 4g .. 4g+3 (positions of the group) of column c
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int q = 4 * wave + i;
+  for (int i = 0; i < TZ; i++) {
+    const int q = TZ * wave + i;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      if (16 * t >= Cout) continue;
       const int co = 16 * t + c;
       const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
@@ -131,15 +201,15 @@ __device__ __forceinline__ void conv3_tile_body(const uint16_t *__restrict__ in,
 }
 
 
-template <int NT>
+template <int NT, bool K32, int TZ>
 __global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t *__restrict__ in, int Cin,
                                                     const uint16_t *__restrict__ W,
                                                     const float *__restrict__ bias, int Cout,
                                                     uint16_t *__restrict__ out,
                                                     float *__restrict__ out_f32, int n, int Dz,
                                                     int Dy, int Dx, int relu, int flip) {
-  __shared__ bf16x4 tile[CT_HP * 4];
-  const int ntx = (Dx + 7) >> 3, nty = (Dy + 7) >> 3, ntz = (Dz + 3) >> 2;
+  __shared__ s16x8 tile[conv_tile_vecs<TZ, K32>()];
+  const int ntx = (Dx + 7) >> 3, nty = (Dy + 7) >> 3, ntz = (Dz + TZ - 1) / TZ;
   int bid = blockIdx.x;
   const int bx = bid % ntx;
   bid /= ntx;
@@ -147,8 +217,8 @@ __global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t *__restrict__
   bid /= nty;
   const int bz = bid % ntz;
   const long b = bid / ntz;
-  conv3_tile_body<NT>(in, Cin, W, bias, Cout, out, out_f32, b, Dz, Dy, Dx, bx, by, bz, relu, flip,
-                      tile);
+  conv3_tile_body<NT, K32, TZ>(in, Cin, W, bias, Cout, out, out_f32, b, Dz, Dy, Dx, bx, by, bz,
+                               relu, flip, tile);
 }
 
 // Uneven batch: segment f owns blocks [seg[f].blk, seg[f+1].blk) and the
@@ -162,7 +232,8 @@ __device__ __forceinline__ int seg_by_blk(const pdfsnn_seg *seg, int nf, int v) 
   return lo;
 }
 
-template <int NT>
+// The batch's blocks are 8 x 8 x 4 tiles (its block tables count them so).
+template <int NT, bool K32>
 __global__ __launch_bounds__(256) void k_conv3_tile_rag(const uint16_t *__restrict__ in, int Cin,
                                                         const uint16_t *__restrict__ W,
                                                         const float *__restrict__ bias, int Cout,
@@ -170,7 +241,7 @@ __global__ __launch_bounds__(256) void k_conv3_tile_rag(const uint16_t *__restri
                                                         float *__restrict__ out_f32,
                                                         const pdfsnn_seg *__restrict__ seg,
                                                         int nf, int relu) {
-  __shared__ bf16x4 tile[CT_HP * 4];
+  __shared__ s16x8 tile[conv_tile_vecs<4, K32>()];
   const int f = seg_by_blk(seg, nf, (int)blockIdx.x);
   const pdfsnn_seg sg = seg[f];
   const int ntx = (sg.Dx + 7) >> 3, nty = (sg.Dy + 7) >> 3;
@@ -180,23 +251,30 @@ __global__ __launch_bounds__(256) void k_conv3_tile_rag(const uint16_t *__restri
   const int by = bid % nty;
   const int bz = bid / nty;
   const long p0 = sg.pos;
-  conv3_tile_body<NT>(in + p0 * Cin, Cin, W, bias, Cout, out ? out + p0 * Cout : nullptr,
-                      out_f32 ? out_f32 + p0 * Cout : nullptr, 0, sg.Dz, sg.Dy, sg.Dx, bx, by, bz,
-                      relu, 0, tile);
+  conv3_tile_body<NT, K32, 4>(in + p0 * Cin, Cin, W, bias, Cout, out ? out + p0 * Cout : nullptr,
+                              out_f32 ? out_f32 + p0 * Cout : nullptr, 0, sg.Dz, sg.Dy, sg.Dx,
+                              bx, by, bz, relu, 0, tile);
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[tap][co][ci] += sum_p dY[p][co] X[p + s(tap)][ci].
-// Block: one tap, a run of positions; 64-position chunks staged in LDS,
-// then every wave accumulates its (co, ci) tiles with MFMA (M = co, N = ci,
-// K = positions), and adds them to the FP32 dW at the end.
+// Weight gradient: dW[tap][co][ci] += sum_p dY[p][co] X[p + s(tap)][ci], and
+// (db non-null) the bias gradient db[co] += sum_p dY[p][co], summed by the
+// centre tap's blocks from their staged dY.
+// Block: one tap, a run of positions; 64-position chunks staged in LDS
+// transposed (channel-major: [co][pos], [ci][pos]), so an operand fragment of
+// 8 consecutive positions is one 16-byte read; every wave accumulates its
+// (co, ci) tiles with MFMA (M = co, N = ci, K = positions, two 32-position
+// steps per chunk), and adds them to the FP32 dW at the end.
 // ---------------------------------------------------------------------------
+constexpr int WG_PP = 64 + 8;  // position pitch of the transposed chunks (bf16), 16-byte aligned
+
 __global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict__ X, int Cin,
                                                      const uint16_t *__restrict__ dY, int Cout,
-                                                     float *__restrict__ dW, int n, int Dz,
+                                                     float *__restrict__ dW,
+                                                     float *__restrict__ db, int n, int Dz,
                                                      int Dy, int Dx, int chunks_per_block) {
-  __shared__ uint16_t sY[64 * 64];    // [pos][co]
-  __shared__ uint16_t sX[64 * 128];   // [pos][ci]
+  __shared__ __attribute__((aligned(16))) uint16_t sY[64 * WG_PP];   // [co][pos]
+  __shared__ __attribute__((aligned(16))) uint16_t sX[128 * WG_PP];  // [ci][pos]
   const int tap = blockIdx.y;
   const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -207,20 +285,28 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict_
   f32x4 acc[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) acc[i] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const bool bsum = db && tap == 13 && (int)threadIdx.x < Cout;
+  float bacc = 0.f;
   for (int ch = 0; ch < chunks_per_block; ch++) {
     const long pbase = ((long)blockIdx.x * chunks_per_block + ch) * 64;
     if (pbase >= M) break;
     __syncthreads();
-    // stage dY[pbase .. +64][0..Cout) and X[shifted][0..Cin)
-    for (int i = threadIdx.x; i < 64 * Cout; i += 256) {
-      const int r = i / Cout, co = i % Cout;
+    // stage dY[pbase .. +64][0..Cout) and X[shifted][0..Cin), transposed:
+    // a thread loads 8 channels (16 bytes) of one position and writes them
+    // to 8 channel rows
+    const int nvy = Cout / 8, nvx = Cin / 8;
+    for (int i = threadIdx.x; i < 64 * nvy; i += 256) {
+      const int r = i / nvy, c8 = 8 * (i % nvy);
       const long p = pbase + r;
-      sY[r * Cout + co] = p < M ? dY[p * Cout + co] : (uint16_t)0;
+      s16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (p < M) v = *(const s16x8 *)(dY + p * Cout + c8);
+#pragma unroll
+      for (int j = 0; j < 8; j++) sY[(c8 + j) * WG_PP + r] = (uint16_t)v[j];
     }
-    for (int i = threadIdx.x; i < 64 * Cin; i += 256) {
-      const int r = i / Cin, ci = i % Cin;
+    for (int i = threadIdx.x; i < 64 * nvx; i += 256) {
+      const int r = i / nvx, c8 = 8 * (i % nvx);
       const long p = pbase + r;
-      uint16_t v = 0;
+      s16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
       if (p < M) {
         const int x = (int)(p % Dx);
         long q = p / Dx;
@@ -231,26 +317,24 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict_
         const int zz = z + dz - 1, yy = y + dy - 1, xx = x + dx - 1;
         if ((unsigned)zz < (unsigned)Dz && (unsigned)yy < (unsigned)Dy &&
             (unsigned)xx < (unsigned)Dx)
-          v = X[(((b * Dz + zz) * Dy + yy) * Dx + xx) * (long)Cin + ci];
+          v = *(const s16x8 *)(X + (((b * Dz + zz) * Dy + yy) * Dx + xx) * (long)Cin + c8);
       }
-      sX[r * Cin + ci] = v;
+#pragma unroll
+      for (int j = 0; j < 8; j++) sX[(c8 + j) * WG_PP + r] = (uint16_t)v[j];
     }
     __syncthreads();
+    if (bsum)
+      for (int r = 0; r < 64; r++) bacc += bf2f(sY[threadIdx.x * WG_PP + r]);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int tl = wave + 4 * i;
       if (tl >= ntile) break;
       const int tco = tl / nci, tci = tl % nci;
 #pragma unroll
-      for (int kk = 0; kk < 4; kk++) {
-        bf16x4 a, b;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int r = 16 * kk + 4 * g + e;
-          a[e] = (short)sY[r * Cout + 16 * tco + c];
-          b[e] = (short)sX[r * Cin + 16 * tci + c];
-        }
-        acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
+      for (int kk = 0; kk < 2; kk++) {
+        const s16x8 a = *(const s16x8 *)(sY + (16 * tco + c) * WG_PP + 32 * kk + 8 * g);
+        const s16x8 b = *(const s16x8 *)(sX + (16 * tci + c) * WG_PP + 32 * kk + 8 * g);
+        acc[i] = mfma32(a, b, acc[i]);
       }
     }
   }
@@ -265,23 +349,7 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t *__restrict_
       atomicAdd(&dW[((long)tap * Cout + co) * Cin + ci], acc[i][e]);
     }
   }
-}
-
-// bias gradient: db[co] += sum_p dY[p][co]
-__global__ void k_bias_grad(const uint16_t *__restrict__ dY, int Cout, long M, float *db) {
-  __shared__ float part[256];
-  const int co = blockIdx.y;
-  float s = 0.f;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < M;
-       p += (long)gridDim.x * blockDim.x)
-    s += bf2f(dY[p * Cout + co]);
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(&db[co], part[0]);
+  if (bsum) atomicAdd(&db[threadIdx.x], bacc);
 }
 
 // ---------------------------------------------------------------------------
@@ -333,14 +401,17 @@ __global__ void k_pool2_bwd(const float *dout, float *din, int n, int Dz, int Dy
 
 // nearest x2 upsample of src (coarse, Cs channels) into channels [0, Cs) of
 // dst (fine, Cd channels), and copy skip (fine, Ck channels) into [Cs, Cs+Ck)
+// One thread per 8 channels (16 bytes) of an output position; Cs and Ck are
+// multiples of 16.
 __global__ void k_up2_concat(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
                              uint16_t *dst, int n, int Dz, int Dy, int Dx) {
-  const int Cd = Cs + Ck;
-  const long N = (long)n * Dz * Dy * Dx * Cd;
+  const int Cv = (Cs + Ck) / 8;
+  const long N = (long)n * Dz * Dy * Dx * Cv;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % Cd);
-    const long p = i / Cd;
+    const int c = 8 * (int)(i % Cv);
+    const long p = i / Cv;
+    s16x8 v;
     if (c < Cs) {
       const int x = (int)(p % Dx);
       long q = p / Dx;
@@ -348,11 +419,12 @@ __global__ void k_up2_concat(const uint16_t *src, int Cs, const uint16_t *skip, 
       q /= Dy;
       const int z = (int)(q % Dz);
       const long b = q / Dz;
-      dst[i] = src[((((b * (Dz / 2) + z / 2) * (Dy / 2) + y / 2) * (Dx / 2)) + x / 2) * (long)Cs +
-                   c];
+      v = *(const s16x8 *)(src + ((((b * (Dz / 2) + z / 2) * (Dy / 2) + y / 2) * (Dx / 2)) + x / 2) *
+                                     (long)Cs + c);
     } else {
-      dst[i] = skip[p * Ck + (c - Cs)];
+      v = *(const s16x8 *)(skip + p * Ck + (c - Cs));
     }
+    *(s16x8 *)(dst + 8 * i) = v;
   }
 }
 
@@ -578,25 +650,29 @@ __global__ __launch_bounds__(256) void k_up2_concat_rag(const uint16_t *src, int
                                                         const uint16_t *skip, int Ck,
                                                         uint16_t *dst, const pdfsnn_seg *sf,
                                                         const pdfsnn_seg *sc, int nf) {
-  const int Cd = Cs + Ck;
+  const int Cd = Cs + Ck, Cv = Cd / 8;
   const int f = seg_by_eblk(sf, nf, (int)blockIdx.x);
   const pdfsnn_seg a = sf[f];
   const long cpos = sc[f].pos;
   const int p0 = ((int)blockIdx.x - a.eblk) * RAG_EP;
   const int np = min(RAG_EP, a.Dz * a.Dy * a.Dx - p0);
   const long g0 = (long)a.pos + p0;  // packed position of the block's first point
-  for (int e = threadIdx.x; e < np * Cd; e += 256) {
-    const int c = e % Cd;
-    const int pl = e / Cd;
+  // one thread per 8 channels (16 bytes) of a position
+  for (int e = threadIdx.x; e < np * Cv; e += 256) {
+    const int c = 8 * (e % Cv);
+    const int pl = e / Cv;
+    s16x8 v;
     if (c < Cs) {
       int q = p0 + pl;
       const int x = q % a.Dx;
       q /= a.Dx;
       const int y = q % a.Dy, z = q / a.Dy;
-      dst[g0 * Cd + e] = src[(cpos + ((z / 2) * (a.Dy / 2) + y / 2) * (a.Dx / 2) + x / 2) * Cs + c];
+      v = *(const s16x8 *)(src + (cpos + ((z / 2) * (a.Dy / 2) + y / 2) * (a.Dx / 2) + x / 2) * Cs +
+                           c);
     } else {
-      dst[g0 * Cd + e] = skip[(g0 + pl) * Ck + (c - Cs)];
+      v = *(const s16x8 *)(skip + (g0 + pl) * Ck + (c - Cs));
     }
+    *(s16x8 *)(dst + (g0 + pl) * Cd + c) = v;
   }
 }
 
@@ -681,44 +757,72 @@ __global__ __launch_bounds__(256) void k_warm_up4_rag(const float *pred, int C,
 // ---------------------------------------------------------------------------
 // C-ABI launchers
 // ---------------------------------------------------------------------------
+// f(NT, K32) for a conv's channel counts: NT = Cout / 16 output tiles (1,
+// 2, 3, 4, 6 or 8: the layers' counts), K32 when Cin is a multiple of 32.
+// Returns -1 for another Cout.
+template <typename F>
+static int conv3_dispatch(int Cin, int Cout, F f) {
+  const int nt = Cout / 16;
+  auto with_nt = [&](auto k32) {
+    switch (nt) {
+      case 1: f(std::integral_constant<int, 1>(), k32); return 0;
+      case 2: f(std::integral_constant<int, 2>(), k32); return 0;
+      case 3: f(std::integral_constant<int, 3>(), k32); return 0;
+      case 4: f(std::integral_constant<int, 4>(), k32); return 0;
+      case 6: f(std::integral_constant<int, 6>(), k32); return 0;
+      case 8: f(std::integral_constant<int, 8>(), k32); return 0;
+      default: return -1;
+    }
+  };
+  if (Cin % 32 == 0) return with_nt(std::integral_constant<bool, true>());
+  return with_nt(std::integral_constant<bool, false>());
+}
+
 extern "C" int pdfsnn_conv3(const uint16_t *in, int Cin, const uint16_t *W, const float *bias,
                             int Cout, uint16_t *out, float *out_f32, int n, int Dz, int Dy,
                             int Dx, int relu, int flip, hipStream_t st) {
   if (Cin % 16 || Cout % 16 || Cout > 128 || Cin > 128 || Cin < 16 || Cout < 16) return -1;
   if (n < 1 || Dz < 1 || Dy < 1 || Dx < 1) return -1;
-  const long blocks = (long)n * ((Dz + 3) / 4) * ((Dy + 7) / 8) * ((Dx + 7) / 8);
+  // output tile depth 4, or 1 on the coarse levels (4x the blocks) when the
+  // grid would otherwise give under 64 blocks or the conv has one channel
+  // chunk and under 512 blocks. Measured at a 128^3 forward: depth 1 takes
+  // the 16^3 layers from 45 to 30 us and the 32^3 first layer from 31 to 21
+  // us, but its other 32^3 layers (2 and 4 chunks) from 46 / 77 to 55 / 103
+  // us; depth 8 was slower everywhere (register-bound occupancy).
+  auto nblocks = [&](int tz) {
+    return (long)n * ((Dz + tz - 1) / tz) * ((Dy + 7) / 8) * ((Dx + 7) / 8);
+  };
+  const long nb4 = nblocks(4);
+  const int tz = (nb4 < 64 || (nb4 < 512 && Cin <= 32)) ? 1 : 4;
+  const long blocks = nblocks(tz);
   if (blocks >= (1L << 31)) return -1;
-  const int nt = Cout / 16;
   const dim3 grid((unsigned)blocks), blk(256);
-  if (nt == 1)
-    hipLaunchKernelGGL(k_conv3_tile<1>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
-                       Dz, Dy, Dx, relu, flip);
-  else if (nt == 2)
-    hipLaunchKernelGGL(k_conv3_tile<2>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
-                       Dz, Dy, Dx, relu, flip);
-  else if (nt <= 4)
-    hipLaunchKernelGGL(k_conv3_tile<4>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
-                       Dz, Dy, Dx, relu, flip);
-  else
-    hipLaunchKernelGGL(k_conv3_tile<8>, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n,
-                       Dz, Dy, Dx, relu, flip);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, n, Dz, Dy,
+                       Dx, relu, flip);
+  };
+  if (conv3_dispatch(Cin, Cout, [&](auto nt, auto k32) {
+        constexpr int NT = decltype(nt)::value;
+        constexpr bool K32 = decltype(k32)::value;
+        if (tz == 4)
+          go(k_conv3_tile<NT, K32, 4>);
+        else
+          go(k_conv3_tile<NT, K32, 1>);
+      }))
+    return -1;
   return hip_rc(hipGetLastError());
 }
 
 extern "C" int pdfsnn_conv3_wgrad(const uint16_t *X, int Cin, const uint16_t *dY, int Cout,
-                                  float *dW, int n, int Dz, int Dy, int Dx, hipStream_t st) {
+                                  float *dW, float *db, int n, int Dz, int Dy, int Dx,
+                                  hipStream_t st) {
   if (Cin % 16 || Cout % 16 || Cout > 64 || Cin > 128 || (Cout / 16) * (Cin / 16) > 32) return -1;
   const long M = (long)n * Dz * Dy * Dx;
   const long nchunks = (M + 63) / 64;
   const int per = 16;
   const unsigned bx = (unsigned)((nchunks + per - 1) / per);
-  hipLaunchKernelGGL(k_conv3_wgrad, dim3(bx, 27), dim3(256), 0, st, X, Cin, dY, Cout, dW, n, Dz,
-                     Dy, Dx, per);
-  return hip_rc(hipGetLastError());
-}
-
-extern "C" int pdfsnn_bias_grad(const uint16_t *dY, int Cout, long M, float *db, hipStream_t st) {
-  hipLaunchKernelGGL(k_bias_grad, dim3(64, Cout), dim3(256), 0, st, dY, Cout, M, db);
+  hipLaunchKernelGGL(k_conv3_wgrad, dim3(bx, 27), dim3(256), 0, st, X, Cin, dY, Cout, dW, db, n,
+                     Dz, Dy, Dx, per);
   return hip_rc(hipGetLastError());
 }
 
@@ -738,7 +842,8 @@ extern "C" int pdfsnn_pool2_bwd(const float *dout, float *din, int n, int Dz, in
 
 extern "C" int pdfsnn_up2_concat(const uint16_t *src, int Cs, const uint16_t *skip, int Ck,
                                  uint16_t *dst, int n, int Dz, int Dy, int Dx, hipStream_t st) {
-  hipLaunchKernelGGL(k_up2_concat, dim3(grid1((long)n * Dz * Dy * Dx * (Cs + Ck), 256)),
+  if (Cs % 16 || Ck % 16) return -1;
+  hipLaunchKernelGGL(k_up2_concat, dim3(grid1((long)n * Dz * Dy * Dx * (Cs + Ck) / 8, 256)),
                      dim3(256), 0, st, src, Cs, skip, Ck, dst, n, Dz, Dy, Dx);
   return hip_rc(hipGetLastError());
 }
@@ -816,20 +921,15 @@ extern "C" int pdfsnn_conv3_rag(const uint16_t *in, int Cin, const uint16_t *W, 
                                 int nf, int nblocks, int relu, hipStream_t st) {
   if (Cin % 16 || Cout % 16 || Cout > 128 || Cin > 128 || Cin < 16 || Cout < 16) return -1;
   if (nf < 1 || nblocks < 1) return -1;
-  const int nt = Cout / 16;
   const dim3 grid((unsigned)nblocks), blk(256);
-  if (nt == 1)
-    hipLaunchKernelGGL(k_conv3_tile_rag<1>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
-                       out_f32, seg, nf, relu);
-  else if (nt == 2)
-    hipLaunchKernelGGL(k_conv3_tile_rag<2>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
-                       out_f32, seg, nf, relu);
-  else if (nt <= 4)
-    hipLaunchKernelGGL(k_conv3_tile_rag<4>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
-                       out_f32, seg, nf, relu);
-  else
-    hipLaunchKernelGGL(k_conv3_tile_rag<8>, grid, blk, 0, st, in, Cin, W, bias, Cout, out,
-                       out_f32, seg, nf, relu);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, blk, 0, st, in, Cin, W, bias, Cout, out, out_f32, seg, nf,
+                       relu);
+  };
+  if (conv3_dispatch(Cin, Cout, [&](auto nt, auto k32) {
+        go(k_conv3_tile_rag<decltype(nt)::value, decltype(k32)::value>);
+      }))
+    return -1;
   return hip_rc(hipGetLastError());
 }
 
@@ -846,7 +946,7 @@ extern "C" int pdfsnn_up2_concat_rag(const uint16_t *src, int Cs, const uint16_t
                                      uint16_t *dst, const pdfsnn_seg *seg_fine,
                                      const pdfsnn_seg *seg_coarse, int nf, int eblocks_fine,
                                      hipStream_t st) {
-  if (nf < 1 || eblocks_fine < 1) return -1;
+  if (nf < 1 || eblocks_fine < 1 || Cs % 16 || Ck % 16) return -1;
   hipLaunchKernelGGL(k_up2_concat_rag, dim3((unsigned)eblocks_fine), dim3(256), 0, st, src, Cs,
                      skip, Ck, dst, seg_fine, seg_coarse, nf);
   return hip_rc(hipGetLastError());

@@ -211,3 +211,35 @@ def test_speed_terrain_full_size_within_tolerance(ctx, dev):
             assert st["judge_flagged"] == 0, name
     print("speed terrain max errors (h):", errs, flush=True)
     assert errs["plain"] <= F32_VS_F64_TOL and errs["certified"] <= F32_VS_F64_TOL
+
+
+def test_speed_graph_replay_and_certified_batch(ctx, dev):
+    """Passes replayed from a captured hipGraph give the eager speed solve's
+    bits, on every replay (several solves in a row); the certified batch
+    stop leaves every field within the speed judge's tolerance (judge
+    flags 0) and no later than the plain stop."""
+    X, Y, Z = 48, 40, 30
+    a = O.gen_spheres(X, Y, Z, seed=71, n=4, r=(1.5, 4.0))
+    s = _dense(_slowness(X, Y, Z, seed=71), dev)
+    want = _dense(a, dev)
+    ctx.solve_speed_f32(want, s, init=want.clone(), graph=0)
+    for rep in range(3):
+        t = _dense(a, dev)
+        print(f"speed graph replay {rep}", flush=True)
+        ctx.solve_speed_f32(t, s, init=t.clone(), graph=1)
+        assert torch.equal(t.view(torch.int32), want.view(torch.int32)), rep
+    rng = np.random.default_rng(9)
+    fields, slows = [], []
+    for i in range(5):
+        X, Y, Z = (int(v) for v in rng.integers(8, 48, 3))
+        fields.append(O.gen_terrain(X, Y, Z, seed=900 + i, amp=(1.0, 6.0), sig=(2.0, 9.0)))
+        slows.append(_dense(_slowness(X, Y, Z, seed=900 + i), dev))
+    ts = [_dense(a, dev) for a in fields]
+    print("speed certified batch", flush=True)
+    stc, pc, _ = ctx.solve_batch_speed_f32(ts, slows, inits=[t.clone() for t in ts], certify=1)
+    assert stc["judge_flagged"] == 0
+    for t, sl, a in zip(ts, slows, fields):
+        assert ctx.judge_speed_f32(t, _dense(a, dev), sl) == 0
+    tp = [_dense(a, dev) for a in fields]
+    stp, pp, _ = ctx.solve_batch_speed_f32(tp, slows, inits=[t.clone() for t in tp])
+    assert all(c <= p for c, p in zip(pc, pp)), (pc, pp)

@@ -1458,9 +1458,15 @@ extern "C" void pdfsk_tile_dims(int variant, int *tx, int *ty) {
 #ifndef PDFSK_U64
 #define PDFSK_U64 8
 #endif
-// plane-queue depth of a mode (measured at 512^3: depth 4 for the FP64
-// modes, with fewer registers, is no faster than 8)
-template <int M> constexpr int queue_depth() { return SMode<M>::F64 ? PDFSK_U64 : PDFSK_U; }
+// plane-queue depth of a mode. Measured at 512^3 on 8x8 tiles: depth 4 for
+// the FP64 modes with one aux stream, with fewer registers, is no faster
+// than 8 (1.63 against 1.58 ms per polish ordering: both at two waves per
+// SIMD), and depth 2 (three waves per SIMD) is slower (1.80 ms). The speed
+// polish (two aux streams) needs 256 VGPRs at depth 8, one wave per SIMD;
+// at depth 4 it takes 201, two waves per SIMD: 2.15 -> 1.83 ms.
+template <int M> constexpr int queue_depth() {
+  return SMode<M>::F64 ? (SMode<M>::NAUX > 1 && PDFSK_U64 > 4 ? 4 : PDFSK_U64) : PDFSK_U;
+}
 
 // Whether a tile shape can run a mode: an FP64 value takes two edge-wave
 // lanes (W (TX + TY) <= 64) and the LDS (ring, aux delay lines) must fit the

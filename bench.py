@@ -57,9 +57,15 @@ def main():
                     help="skip the spatially varying speed leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run plain 1 GiB copies (for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--profile-batch", choices=("512", "c4"),
+                    help="only run one batch's plain solves (8 x 512^3 spheres, or the c4 batch of "
+                         "--batch fields) for a rocprofv3 run; prints their statistics")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile_batch:
+        profile_batch(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -256,6 +262,52 @@ def copy_calibration():
         torch.add(a, 0.0, out=b)
     torch.cuda.synchronize()
     print(f"copy calibration: {n * 4} bytes read and written per kernel", flush=True)
+
+
+def profile_batch(args):
+    """The plain batched solve of the 8 x 512^3 batch (seeds seed .. seed + 7)
+    or of the c4 batch, twice: the first tunes the tile (its first pass tries
+    each variant), the second runs on the tuned tile with per-launch HIP
+    events. Prints the second solve's statistics as one JSON line: sweep and
+    polish launches, their average times and algorithmic GB/s (8 B per
+    executed FP32 point-update, 20 B per polish point-update)."""
+    import torch
+
+    import pdfs_amd
+    from pdfs_amd.workloads import c4_specs, generate_c4
+
+    ctx = pdfs_amd.Context()
+    dev = torch.device("cuda:0")
+    if args.profile_batch == "512":
+        inits = []
+        for i in range(8):
+            t = torch.empty((512, 512, 512), dtype=torch.float32, device=dev)
+            ctx.generate(pdfs_amd.GEN_SPHERES, args.seed + i, 512, 512, 512, 1.0, dense=t)
+            inits.append(t)
+        name = "8 x 512^3 spheres"
+    else:
+        inits = generate_c4(ctx, c4_specs(seed=4, nfields=args.batch))
+        name = f"c4, {args.batch} uneven fields"
+    fields = [t.clone() for t in inits]
+    ctx.solve_batch_f32(fields, inits, tile=pdfs_amd.TILE_AUTO, judge=0)
+    for f, t in zip(fields, inits):
+        f.copy_(t)
+    st, passes, _ = ctx.solve_batch_f32(fields, inits, tile=pdfs_amd.TILE_AUTO, judge=0, profile=1)
+    launches = 8 * ((st["orderings"] + 7) // 8)
+    sweep_ms = st["ms_sweep_kernels"] / max(1, launches)
+    pol_n = max(1, st["polish_orderings"])
+    pol_ms = st["ms_polish_kernels"] / pol_n
+    out = {"batch": name, "points": sum(t.numel() for t in inits), "tile": st["tile"],
+           "passes_max": max(passes), "orderings_swept": st["orderings"], "sweep_launches": launches,
+           "sweep_avg_launch_ms": round(sweep_ms, 4),
+           "sweep_gbs": round(8.0 * st["point_updates"] / (st["ms_sweep_kernels"] * 1e-3) / 1e9, 1),
+           "polish_launches": st["polish_orderings"], "polish_avg_launch_ms": round(pol_ms, 4),
+           "polish_gbs": round(20.0 * st["polish_point_updates"] /
+                               (st["ms_polish_kernels"] * 1e-3) / 1e9, 1)
+           if st["ms_polish_kernels"] > 0 else None,
+           "ms_sweeps": round(st["ms_sweeps"], 3), "ms_polish": round(st["ms_polish"], 3)}
+    print(json.dumps(out), flush=True)
+    ctx.close()
 
 
 def measured_traffic(kernel="sweep"):

@@ -70,6 +70,9 @@ enum { PDFS_OK = 0, PDFS_EINVAL = -1, PDFS_EHIP = -2, PDFS_ENOMEM = -3, PDFS_ETI
 enum { PDFS_TILE_AUTO = -2 };
 enum { PDFS_F64_REF = 0, /* FP64, reference arithmetic and schedule */
        PDFS_F32 = 1 };   /* FP32 difference form, column pipeline */
+/* Storage of the FP32 path's fields while they are swept (opts.storage). */
+enum { PDFS_STORE_F32 = 0,   /* FP32 values (then the FP64 polish, opts.polish) */
+       PDFS_STORE_FIXED = 1 }; /* 32-bit fixed point (default; see pdfs_opts.storage) */
 
 typedef struct {
   int precision;   /* PDFS_F64_REF or PDFS_F32 */
@@ -95,6 +98,22 @@ typedef struct {
                       is ~10x the error it leaves). 0: the FP32 fixed point itself. */
   double polish_tol; /* default 2e-5 */
   int tile64;      /* FP64 column variant (16-wide tiles: 0 16x16, 2 16x8, 4, 5); -1 default */
+  int storage;     /* FP32 path, converged solves (fixed_passes <= 0) with equal spacings and unit
+                      speed: PDFS_STORE_FIXED (default) stores each field while it is swept as
+                      32-bit fixed point, value = base + q h 2^-k with base the field's smallest
+                      value and k = 32 - ceil(log2(x + y + z)) (q = 2^32 - 1 is +inf; values
+                      beyond the range, which no converged value reaches, are +inf while they
+                      last). The local solve is the FP32 difference form on the integer
+                      differences, rounded to the nearest quantum, so storage adds no error
+                      that grows with the distance: at 512^3 the result is within ~3e-5 h of
+                      the FP64 solution with no polish (FP32 storage: ~3e-4 h before the
+                      polish). Fields are converted in place (FP32 in and out); NaN in the
+                      start field is treated as +inf. The polish does not run; the judge (and
+                      certification) tests |value - min(init, G64(value))| <= judge_tol h +
+                      4 quanta on the fixed-point values. Falls back to PDFS_STORE_F32 for
+                      other solves and for a start field holding -inf. */
+  double judge_tol;/* fixed-point storage: judge tolerance in units of h (default 2e-6; the
+                      certified stop at 512^3 leaves about 10x this much error) */
 } pdfs_opts;
 
 typedef struct {
@@ -125,6 +144,8 @@ typedef struct {
                                 limit) */
   long long polish_point_updates; /* interior points x polish orderings executed */
   double ms_polish_kernels;  /* profile: sum of per-launch polish kernel times */
+  int storage;               /* PDFS_STORE_FIXED if the fields were swept in fixed point */
+  double ms_convert;         /* fixed point: FP32 <-> fixed conversions (with the base readback) */
 } pdfs_stats;
 
 void pdfs_default_opts(pdfs_opts *o);
@@ -218,6 +239,14 @@ int pdfs_sweep_f32_noalloc(float *field, int x, int y, int z, double dx, double 
 int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x, int y, int z,
                        double dx, double dy, double dz, uint32_t *flags,
                        long long *flagged);
+/* The judge of a field held in fixed point (no counterpart in the reference;
+ * the judge that fixed-point solves run, pdfs_opts.storage): q (device,
+ * dense [z][y][x]) holds value = base + q d, q = 2^32 - 1 for +inf; init is
+ * the dense FP32 init; spacing h. A point fails when
+ * |value - min(init, G64(value))| > tol (equal infinities pass). */
+int pdfs_judge_fixed_dev(pdfs_ctx *c, const uint32_t *q, const float *init, int x, int y, int z,
+                         double h, double base, double d, double tol, uint32_t *flags,
+                         long long *flagged);
 
 /* Self-test (no counterpart in the reference): the FP32 sweep's square root
  * (hardware estimate plus a one-ulp correction) against the correctly

@@ -742,3 +742,140 @@ double orc_residual_f64(const double *v, const double *init, int X, int Y,
       }
   return worst;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Fixed-point storage (new; no reference counterpart). Values are unsigned */
+/* multiples q of a quantum d = h 2^-k above a base (the start field's      */
+/* smallest value), UINT32_MAX standing for +inf. The local solve is        */
+/* orc_godunov_f32's isotropic difference form evaluated in FP32 on the     */
+/* integer differences a = m1 - m0, b = m2 - m0 in units of the quantum     */
+/* (Hq = 2^k), its root offset rounded to the nearest quantum; additions    */
+/* saturate at +inf.                                                        */
+/* ------------------------------------------------------------------------ */
+#define ORC_FX_INF 0xFFFFFFFFu
+
+int orc_fx_bits(int X, int Y, int Z)
+{
+  long long s = (long long)X + Y + Z;
+  int c = 0;
+  while ((1ll << c) < s) c++;
+  return 32 - c;
+}
+
+float orc_fx_min(const float *v, size_t n)
+{
+  float m = INFINITY;
+  for (size_t i = 0; i < n; i++)
+    if (v[i] == v[i] && v[i] < m) m = v[i];
+  return m;
+}
+
+void orc_fx_encode(const float *v, uint32_t *q, size_t n, double base, double d)
+{
+  for (size_t i = 0; i < n; i++) {
+    double r = rint(((double)v[i] - base) / d);
+    q[i] = r < 4294967295.0 ? (uint32_t)r : ORC_FX_INF;
+  }
+}
+
+void orc_fx_decode(const uint32_t *q, float *v, size_t n, double base, double d)
+{
+  for (size_t i = 0; i < n; i++)
+    v[i] = q[i] == ORC_FX_INF ? INFINITY : (float)(base + (double)q[i] * d);
+}
+
+static inline uint32_t fx_add_sat(uint32_t a, uint32_t b)
+{
+  uint64_t s = (uint64_t)a + b;
+  return s > ORC_FX_INF ? ORC_FX_INF : (uint32_t)s;
+}
+
+uint32_t orc_godunov_fx(uint32_t cur, uint32_t mx, uint32_t my, uint32_t mz, int k)
+{
+  const uint32_t Hq = 1u << k;
+  const float H = (float)Hq;
+  const float two_h2 = 2.0f * (H * H), three_h2 = 3.0f * (H * H);
+  uint32_t lo = ORC_MIN(mx, my), hi = ORC_MAX(mx, my);
+  uint32_t m0 = ORC_MIN(lo, mz), mid = ORC_MAX(lo, mz);
+  uint32_t m1 = ORC_MIN(hi, mid), m2 = ORC_MAX(hi, mid);
+  uint32_t u = fx_add_sat(m0, Hq);
+  if (u > m1) {
+    float a = (float)(m1 - m0);
+    float b = (float)(m2 - m0);
+    float r2 = two_h2 - a * a;
+    float e = b + (b - a);
+    float R = r2, P = a, K = 0.5f;
+    if (r2 > e * e) {
+      float D = three_h2 - 2.0f * ((a * a + b * b) - a * b);
+      R = D > 0x1p-96f ? D : 0.0f;
+      P = a + b;
+      K = 1.0f / 3.0f;
+    }
+    float t = (P + sqrtf(R)) * K;
+    uint32_t tq = t >= 0.0f ? (uint32_t)rintf(t) : 0u;
+    u = fx_add_sat(m0, tq);
+  }
+  return cur < u ? cur : u;
+}
+
+long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn)
+{
+  int sx, sy, sz;
+  order_dirs(sn, &sx, &sy, &sz);
+  size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
+  long changed = 0;
+  for (int a = 1; a <= Z; a++) {
+    int i = sz > 0 ? a : Z + 1 - a;
+    for (int b = 1; b <= Y; b++) {
+      int j = sy > 0 ? b : Y + 1 - b;
+      for (int c = 1; c <= X; c++) {
+        int x = sx > 0 ? c : X + 1 - c;
+        size_t id = idx3(X, Y, i, j, x);
+        uint32_t mx = ORC_MIN(q[id - 1], q[id + 1]);
+        uint32_t my = ORC_MIN(q[id - extX], q[id + extX]);
+        uint32_t mz = ORC_MIN(q[id - extXY], q[id + extXY]);
+        uint32_t u = orc_godunov_fx(q[id], mx, my, mz, k);
+        changed += u != q[id];
+        q[id] = u;
+      }
+    }
+  }
+  return changed;
+}
+
+int orc_solve_fx(uint32_t *q, int X, int Y, int Z, int k, int max_passes, long *last_changed)
+{
+  int passes = 0;
+  long changed = 0;
+  while (passes < max_passes) {
+    changed = 0;
+    for (int sn = 1; sn <= 8; sn++) changed += orc_sweep_fx(q, X, Y, Z, k, sn);
+    passes++;
+    if (changed == 0) break;
+  }
+  if (last_changed) *last_changed = changed;
+  return passes;
+}
+
+long orc_judge_fx(const uint32_t *q, const float *init, int X, int Y, int Z, double h,
+                  double base, double d, double tol, uint8_t *flags)
+{
+  size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
+  long bad = 0;
+#define FXV(w) ((w) == ORC_FX_INF ? INFINITY : base + (double)(w) * d)
+  for (int i = 1; i <= Z; i++)
+    for (int j = 1; j <= Y; j++)
+      for (int k = 1; k <= X; k++) {
+        size_t id = idx3(X, Y, i, j, k);
+        double mx = ORC_MIN(FXV(q[id - 1]), FXV(q[id + 1]));
+        double my = ORC_MIN(FXV(q[id - extX]), FXV(q[id + extX]));
+        double mz = ORC_MIN(FXV(q[id - extXY]), FXV(q[id + extXY]));
+        double e = orc_godunov_d64((double)init[id], mx, my, mz, h, h, h);
+        double val = FXV(q[id]);
+        int ok = (val == e) || fabs(val - e) <= tol;
+        if (flags) flags[id] = (uint8_t)!ok;
+        bad += !ok;
+      }
+#undef FXV
+  return bad;
+}

@@ -39,6 +39,16 @@
  *                        left unchanged; a region planted 0.5 h below the
  *                        solution is raised back (a planted min(u, G)
  *                        update fails this); point-source axis values n h.
+ *   orc_godunov_fx / orc_sweep_fx / orc_solve_fx (fixed-point storage)
+ *                        1-D updates exact (axis values n 2^k quanta),
+ *                        2-D / 3-D equal-minima roots the nearest quantum to
+ *                        h/sqrt2, h/sqrt3, general roots within 1.5 quanta
+ *                        of an independent root finder, +inf saturation;
+ *                        converged fields within 5e-5 h of the FP64 solve
+ *                        (orc_solve_d64); base-shift invariance; the order
+ *                        key and encode / decode against numpy.
+ *   orc_judge_fx         a converged fixed-point field passes, planted
+ *                        defects of a few quanta beyond the tolerance fail.
  */
 #ifndef PDFS_FSM_ORACLE_H
 #define PDFS_FSM_ORACLE_H
@@ -137,6 +147,25 @@ double orc_residual_f64(const double *v, const double *init, int X, int Y,
                         int Z, double dx, double dy, double dz);
 
 uint64_t orc_splitmix64(uint64_t *state);
+
+/* Fixed-point storage (new; no reference counterpart): value = base + q d,
+ * d = h 2^-k, q = UINT32_MAX for +inf; k = 32 - ceil(log2(X + Y + Z)).
+ * Arrays of q are padded like the FP arrays, ghosts UINT32_MAX. */
+int orc_fx_bits(int X, int Y, int Z);
+/* smallest non-NaN value of n FP32 values (+inf if none) */
+float orc_fx_min(const float *v, size_t n);
+void orc_fx_encode(const float *v, uint32_t *q, size_t n, double base, double d);
+void orc_fx_decode(const uint32_t *q, float *v, size_t n, double base, double d);
+/* local solve on quanta: Hq = 2^k */
+uint32_t orc_godunov_fx(uint32_t cur, uint32_t mx, uint32_t my, uint32_t mz, int k);
+/* one ordering in place; returns the points changed */
+long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn);
+/* passes until a pass changes nothing (or max_passes); returns passes run */
+int orc_solve_fx(uint32_t *q, int X, int Y, int Z, int k, int max_passes, long *last_changed);
+/* FP64 residual judge of a fixed-point field (spacing h): a point fails when
+ * |value - min(init, G64(value))| > tol; returns the failing points. */
+long orc_judge_fx(const uint32_t *q, const float *init, int X, int Y, int Z, double h,
+                  double base, double d, double tol, uint8_t *flags);
 
 #ifdef __cplusplus
 }

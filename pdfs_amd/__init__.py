@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpdfs.so")
 
 PDFS_OK, PDFS_EINVAL, PDFS_EHIP, PDFS_ENOMEM, PDFS_ETIMEOUT, PDFS_ENOTCONV = 0, -1, -2, -3, -4, -5
 PDFS_F64_REF, PDFS_F32 = 0, 1
+PDFS_STORE_F32, PDFS_STORE_FIXED = 0, 1
 GEN_POINT, GEN_TERRAIN, GEN_SPHERES = 0, 1, 2
 TILE_16x16, TILE_32x8, TILE_16x8, TILE_AUTO = 0, 1, 2, -2
 TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU, TILE_8x8, TILE_16x4 = 3, 4, 5, 6, 7
@@ -29,6 +30,7 @@ EXPORTED = [
     "run_fsm", "runFSM", "pdfs_default_opts", "pdfs_solve", "pdfs_ctx_create",
     "pdfs_ctx_destroy", "pdfs_last_error", "pdfs_solve_f32_dev", "pdfs_sweep_f32_dev",
     "pdfs_solve_ref64_dev", "pdfs_solve_speed64_dev", "pdfs_judge_f32_dev",
+    "pdfs_judge_fixed_dev",
     "pdfs_selftest_sqrt_f32", "pdfs_generate_dev",
     "pdfs_generate_ex_dev",
     "pdfs_pack_dev",
@@ -122,7 +124,8 @@ class Opts(C.Structure):
                 ("tol", C.c_double), ("judge", C.c_int), ("tile", C.c_int),
                 ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int),
                 ("certify", C.c_int), ("certify_from", C.c_int), ("polish", C.c_int),
-                ("polish_tol", C.c_double), ("tile64", C.c_int)]
+                ("polish_tol", C.c_double), ("tile64", C.c_int), ("storage", C.c_int),
+                ("judge_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -134,7 +137,8 @@ class Stats(C.Structure):
                 ("fallbacks", C.c_int), ("ms_warm", C.c_double), ("polish_passes", C.c_int),
                 ("polish_orderings", C.c_int), ("polish_maxdiff", C.c_double),
                 ("ms_polish", C.c_double), ("polish_skipped", C.c_int),
-                ("polish_point_updates", C.c_longlong), ("ms_polish_kernels", C.c_double)]
+                ("polish_point_updates", C.c_longlong), ("ms_polish_kernels", C.c_double),
+                ("storage", C.c_int), ("ms_convert", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -181,6 +185,8 @@ def lib():
         L.pdfs_solve_speed64_dev.restype = I
         L.pdfs_judge_f32_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, vp, C.POINTER(LL)]
         L.pdfs_judge_f32_dev.restype = I
+        L.pdfs_judge_fixed_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, D, vp, C.POINTER(LL)]
+        L.pdfs_judge_fixed_dev.restype = I
         L.pdfs_selftest_sqrt_f32.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.pdfs_selftest_sqrt_f32.restype = I
         L.pdfs_generate_dev.argtypes = [vp, I, C.c_uint64, I, I, I, D, vp, vp]
@@ -365,6 +371,20 @@ class Context:
             return None
         assert t.is_cuda and t.is_contiguous()
         return t.data_ptr()
+
+    def judge_fixed(self, q, init, h, base, d, tol, flags=None):
+        """The FP64 judge of a fixed-point field: q an int32/uint32 cuda tensor
+        [Z, Y, X] of quanta (value = base + q d, all ones = +inf), init float32
+        of the same shape. Returns the number of failing points."""
+        import torch
+        assert q.dtype in (torch.int32, torch.uint32) and q.dim() == 3
+        assert init.dtype == torch.float32 and init.shape == q.shape
+        Z, Y, X = q.shape
+        n = C.c_longlong()
+        _sync_inputs()
+        _check(lib().pdfs_judge_fixed_dev(self.c, self._ptr(q), self._ptr(init), X, Y, Z, h,
+                                          base, d, tol, self._ptr(flags), C.byref(n)))
+        return n.value
 
     def solve_f32(self, field, init=None, h=(1.0, 1.0, 1.0), allow_notconv=False, **kw):
         """field: dense float32 cuda tensor [Z, Y, X], solved in place."""

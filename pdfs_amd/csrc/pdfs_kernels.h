@@ -42,7 +42,9 @@ enum {
   PDFSK_MODE_F64_POL_ANISO = 4,
   PDFSK_MODE_F32_SPEED = 5,    /* FP32 difference form, spacings scaled by the point's slowness */
   PDFSK_MODE_F64_POL_SPEED = 6,/* FP64 polish, spacings scaled by the point's slowness */
-  PDFSK_MODE_COUNT = 7
+  PDFSK_MODE_FX_ISO = 7,       /* 32-bit fixed-point values (quantum h 2^-fxk, UINT32_MAX = +inf),
+                                  FP32 difference form on their differences, u = min(u, G(u)) */
+  PDFSK_MODE_COUNT = 8
 };
 /* modes whose values are FP64 */
 #define PDFSK_MODE_IS_F64(m) \
@@ -61,14 +63,39 @@ enum {
  * batch of one). Dense FP32 field [Z][Y][X]; its east/north edge blocks
  * start at entry ex_off / ey_off of the edge allocations. */
 typedef struct {
-  void *f;            /* float (FP32 modes) or double (FP64 modes) */
+  void *f;            /* float (FP32 modes), double (FP64 modes) or uint32 (fixed point) */
   const float *init;  /* polish modes: dense FP32 init */
   const float *slow;  /* speed modes: dense FP32 slowness (1 / speed) */
   int X, Y, Z;
   int nTx, nTy;
   uint32_t ex_off, ey_off;
-  int pad;
+  int fxk;            /* fixed-point mode: the field's quantum is h 2^-fxk (0 otherwise) */
 } pdfsk_field;
+
+/* Fixed-point storage of a field (PDFSK_MODE_FX_ISO): value = base + q d with
+ * d = h 2^-k, q = UINT32_MAX for +inf. k = 32 - ceil(log2(X + Y + Z)): every
+ * converged value lies in [base, base + (X + Y + Z - 3) h] when base is the
+ * smallest value of the start field (an axis-parallel path of 1-D updates
+ * bounds it), so it is below UINT32_MAX d. */
+int pdfsk_fx_bits(int X, int Y, int Z);
+/* Per-field work of the fixed-point conversions, one launch each over all
+ * fields: f the field (FP32 in, uint32 while solving, FP32 out), n points,
+ * base and d as above; key: per-field order key of the smallest non-NaN
+ * value (pdfsk_fx_min: UINT32_MAX-initialised by the caller). */
+typedef struct {
+  float *f;
+  long long n;
+  double base, d;
+  uint32_t *key;
+} pdfsk_fx;
+/* key = min over the field of an order-preserving key of its non-NaN FP32
+ * values; pdfsk_fx_key_value inverts it on the host (UINT32_MAX: no value). */
+int pdfsk_fx_min_batch(const pdfsk_fx *tab, int nf, long long nmax, hipStream_t st);
+float pdfsk_fx_key_value(uint32_t key);
+/* In place: q = rint((v - base) / d) when below UINT32_MAX, else (and for
+ * NaN) UINT32_MAX; back: v = (float)(base + q d), UINT32_MAX -> +inf. */
+int pdfsk_fx_encode_batch(const pdfsk_fx *tab, int nf, long long nmax, hipStream_t st);
+int pdfsk_fx_decode_batch(const pdfsk_fx *tab, int nf, long long nmax, hipStream_t st);
 
 /* Largest dense FP64 field of the column kernel (x*y*z*8 < 2^31). */
 #define PDFSK_MAX_POINTS64 ((1ll << 31) / 8 - 1)
@@ -147,6 +174,9 @@ typedef struct {
   const float *slow;  /* dense FP32 slowness, or NULL for unit speed (same for every field) */
   uint32_t *flags;
   int X, Y, Z, tile0;
+  /* fixed-point fields (pdfsk_judge_batch_f32 with fx): v holds q of
+   * value = fx_base + q fx_d, and a point passes when |value - e| <= fx_tol */
+  double fx_base, fx_d, fx_tol;
 } pdfsk_judge_field;
 /* judge tiles of one field (the tile0 stride) */
 unsigned pdfsk_judge_tiles(int X, int Y, int Z);
@@ -154,6 +184,9 @@ unsigned pdfsk_judge_tiles(int X, int Y, int Z);
  * the caller); tab on the device, ntiles = tab[nf].tile0 */
 int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long ntiles, double dx, double dy,
                           double dz, int speed, unsigned long long *counts, hipStream_t st);
+/* the same for fixed-point fields (isotropic unit speed) */
+int pdfsk_judge_batch_fx(const pdfsk_judge_field *tab, int nf, long ntiles, double h,
+                         unsigned long long *counts, hipStream_t st);
 
 /* Padded FP64 <-> dense FP64 (interior; unpack leaves the ghosts alone). */
 int pdfsk_pack64(const double *padded, double *dense, int X, int Y, int Z, hipStream_t st);

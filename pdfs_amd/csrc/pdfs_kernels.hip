@@ -166,6 +166,46 @@ __device__ __forceinline__ float god_iso32(float c, float xm, float xp, float ym
   return c < u ? c : u;
 }
 
+// Fixed-point isotropic solve. Values are unsigned multiples of a quantum
+// (the field's h 2^-k); UINT32_MAX is +inf. The per-axis minima and their
+// sort are exact integer min / max; the differences a = m1 - m0 and
+// b = m2 - m0 go to FP32 (exact below 2^24 quanta), and the root offset t of
+// god_iso32's difference form is evaluated in FP32 in units of the quantum
+// (Hq = 2^k, 2 Hq^2 and 3 Hq^2 exact) and rounded to the nearest quantum;
+// u = m0 + t saturates at +inf. The 1-D candidate m0 + Hq is exact, so the
+// stored values carry no rounding from their magnitude: the quantum bounds
+// what one update adds, wherever the value lies. Lanes whose root candidate
+// is not selected may compute a NaN t; it is converted as 0.
+__device__ __forceinline__ uint32_t god_fx_iso(uint32_t c, uint32_t xm, uint32_t xp,
+                                               uint32_t ym, uint32_t yp, uint32_t zm,
+                                               uint32_t zp, uint32_t Hq, float two_h2,
+                                               float three_h2, float third) {
+#pragma clang fp contract(off)
+  const uint32_t mx = xm < xp ? xm : xp, my = ym < yp ? ym : yp, mz = zm < zp ? zm : zp;
+  const uint32_t lo = mx < my ? mx : my, hi = mx > my ? mx : my;
+  const uint32_t m0 = lo < mz ? lo : mz;
+  const uint32_t mid = lo > mz ? lo : mz;
+  const uint32_t m1 = hi < mid ? hi : mid;
+  const uint32_t m2 = hi > mid ? hi : mid;
+  const uint32_t u1 = __builtin_elementwise_add_sat(m0, Hq);
+  const float a = (float)(m1 - m0);
+  const float b = (float)(m2 - m0);
+  const float aa = a * a;
+  const float r2 = two_h2 - aa;
+  const float e = b + (b - a);
+  float D = three_h2 - 2.0f * ((aa + b * b) - a * b);
+  D = D > 0x1p-96f ? D : 0.0f;
+  const bool three = r2 > e * e;
+  const float R = three ? D : r2;
+  const float P = three ? a + b : a;
+  const float K = three ? third : 0.5f;
+  const float t = (P + sqrt_cr_nonneg(R)) * K;
+  const uint32_t tq = t >= 0.0f ? (uint32_t)__builtin_rintf(t) : 0u;
+  const uint32_t ut = __builtin_elementwise_add_sat(m0, tq);
+  const uint32_t u = u1 > m1 ? ut : u1;
+  return c < u ? c : u;
+}
+
 // Anisotropic difference form: bubble sort of (m, d, 1/d^2) triples, 2-D
 // root t = (a d0^2 + d0 d1 sqrt(d0^2 + d1^2 - a^2)) / (d0^2 + d1^2), 3-D root
 // t = (P + sqrt(P^2 - W Q)) / W with P = a w1 + b w2, Q = a^2 w1 + b^2 w2 - 1.
@@ -372,15 +412,23 @@ __device__ __forceinline__ double god_iso64_fminmax(double cur, double xm, doubl
 //                         slowness s per point: the point's spacings are
 //                         d_i s (isotropic: h s; anisotropic: d_i s and
 //                         weights w_i / s^2, W / s^2)
+//   FX_ISO                32-bit fixed-point values (god_fx_iso), isotropic
+//                         spacing, u = min(u, G(u)); the field's quantum
+//                         exponent is in its table entry
 // FP64 values cross column boundaries as two 32-bit halves, each in its own
 // epoch-tagged entry (two edge-wave lanes per value). The per-point inputs
 // (init, slowness) are "aux" streams: loaded with the field, plane by
 // plane, and kept in a per-thread LDS delay line until the thread's
 // hyperplane reaches them.
-template <bool F64> struct ValT { typedef float T; };
-template <> struct ValT<true> { typedef double T; };
+template <bool F64, bool FX> struct ValT { typedef float T; };
+template <> struct ValT<true, false> { typedef double T; };
+template <> struct ValT<false, true> { typedef uint32_t T; };
+// +inf of a value type (fixed point: UINT32_MAX)
+template <typename T> __device__ __forceinline__ T inf_of() { return (T)__builtin_inff(); }
+template <> __device__ __forceinline__ uint32_t inf_of<uint32_t>() { return 0xFFFFFFFFu; }
 template <int M> struct SMode {
   static constexpr bool F64 = PDFSK_MODE_IS_F64(M);
+  static constexpr bool FX = M == PDFSK_MODE_FX_ISO;
   static constexpr bool POL = M == PDFSK_MODE_F64_POL_ISO || M == PDFSK_MODE_F64_POL_ANISO ||
                               M == PDFSK_MODE_F64_POL_SPEED;
   static constexpr bool SPEED = M == PDFSK_MODE_F32_SPEED || M == PDFSK_MODE_F64_POL_SPEED;
@@ -388,7 +436,7 @@ template <int M> struct SMode {
   static constexpr int W = F64 ? 2 : 1;  // 32-bit words per value
   static constexpr int NAUX = (POL ? 1 : 0) + (SPEED ? 1 : 0);  // init, then slowness
   static constexpr int AUX_SLOW = POL ? 1 : 0;
-  typedef typename ValT<F64>::T T;
+  typedef typename ValT<F64, FX>::T T;
 };
 
 __device__ __forceinline__ float buf_ld(float, __amdgpu_buffer_rsrc_t r, uint32_t off) {
@@ -396,6 +444,12 @@ __device__ __forceinline__ float buf_ld(float, __amdgpu_buffer_rsrc_t r, uint32_
 }
 __device__ __forceinline__ double buf_ld(double, __amdgpu_buffer_rsrc_t r, uint32_t off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t buf_ld(uint32_t, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
 }
 __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
@@ -441,7 +495,7 @@ struct FieldDesc {
   int X, Y, Z;
   int nTx, nTy;
   uint32_t ex_off, ey_off;
-  int pad;
+  int fxk;  // fixed-point mode: quantum h 2^-fxk
 };
 
 // Launch state in device memory, advanced by the last workgroup of each
@@ -506,7 +560,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
 
   const int tid = threadIdx.x;
   const bool edge_wave = tid >= NT;  // wave-uniform: NT is a multiple of 64
-  const T INF = (T)__builtin_inff();
+  const T INF = inf_of<T>();
   const unsigned ncols = __hip_atomic_load(a.state + ST_NCOLS, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t epoch = __hip_atomic_load(a.state + ST_EPOCH, __ATOMIC_RELAXED,
@@ -570,6 +624,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       fd.nTy = __builtin_amdgcn_readfirstlane(fd.nTy);
       fd.ex_off = __builtin_amdgcn_readfirstlane(fd.ex_off);
       fd.ey_off = __builtin_amdgcn_readfirstlane(fd.ey_off);
+      fd.fxk = __builtin_amdgcn_readfirstlane(fd.fxk);
     }
     const int X = fd.X, Y = fd.Y, Z = fd.Z, nTy = fd.nTy;
     const int NL = Z + TX + TY - 2;  // steps 0..NL; the last one only stores
@@ -596,7 +651,8 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       const int k = isS ? ev : (isW ? ev - TX : 0);  // the served lane's skew (c or r)
       const bool up = isS ? cb > 0 : (isW && ca > 0);  // an upwind column exists
       // halo word of a boundary lane (+inf in the value's type)
-      const uint32_t infw = W == 1 ? 0x7F800000u : (ew ? 0x7FF00000u : 0u);
+      const uint32_t infw =
+          S::FX ? 0xFFFFFFFFu : (W == 1 ? 0x7F800000u : (ew ? 0x7FF00000u : 0u));
       // entry of step s: coff + s * cinc
       uint32_t coff = 0xFFFFFFF0u, cinc = 0u;
       if (isS && cb > 0) {
@@ -698,6 +754,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       };
       const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
       const bool dom = vx <= X && vy <= Y;
+      // fixed-point mode: the spacing in quanta (2^fxk) and its exact squares
+      const uint32_t fxq = S::FX ? 1u << fd.fxk : 0u;
+      const float fxh = (float)fxq;
+      const float fx_two_h2 = 2.0f * (fxh * fxh), fx_three_h2 = 3.0f * (fxh * fxh);
       const int skew = tx + ty;
       const int lo = G::cell(tx, ty);  // ring cell of this thread
 
@@ -864,6 +924,8 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
             u = god_diff_aniso<float>(c, xm < xp ? xm : xp, ym < yp ? ym : yp, zm < zp ? zm : zp,
                                       dd, ww, a.sp.W * r);
           }
+        } else if constexpr (S::FX) {
+          u = god_fx_iso(c, xm, xp, ym, yp, zm, zp, fxq, fx_two_h2, fx_three_h2, a.sp.third);
         } else if constexpr (!S::F64) {
           u = god32<S::ISO>(c, xm, xp, ym, yp, zm, zp, a.sp);
         } else if constexpr (M == PDFSK_MODE_F64_POL_SPEED) {
@@ -1073,6 +1135,7 @@ struct JudgeLane {
   int Z;
   bool in, hxm, hxp, hym, hyp;     // judged lane; neighbour inside the field
   double htol;                     // hmin 2^-21
+  double fb, fd, ftol;             // fixed-point fields: base, quantum, tolerance
 };
 // The operands of one point: its value, its six neighbours' and its init.
 struct JudgeOps {
@@ -1151,9 +1214,43 @@ __device__ __forceinline__ unsigned judge_plane(const JudgeOps &o, const JudgeLa
   return (unsigned)__popcll(__ballot(bad));
 }
 
+// A point of a fixed-point field (value = base + q d, q = UINT32_MAX for
+// +inf; the operand words hold q): the per-axis minima and the sort are
+// taken on q (the decoding is increasing), the FP64 isotropic solve on the
+// decoded values; the point fails when |value - e| > ftol (equal
+// infinities pass, NaN fails). Decoding is base + (double)q * d without
+// contraction, as the fixed-to-FP32 conversion computes it.
+__device__ __forceinline__ unsigned judge_plane_fx(const JudgeOps &o, const JudgeLane &L, int z,
+                                                   double h, uint32_t *flags) {
+#pragma clang fp contract(off)
+  const uint32_t QI = 0xFFFFFFFFu;
+  const uint32_t zm = z > 0 ? __float_as_uint(o.zm) : QI;
+  const uint32_t zp = z + 1 < L.Z ? __float_as_uint(o.zp) : QI;
+  const uint32_t xm = L.hxm ? __float_as_uint(o.xm) : QI, xp = L.hxp ? __float_as_uint(o.xp) : QI;
+  const uint32_t ym = L.hym ? __float_as_uint(o.ym) : QI, yp = L.hyp ? __float_as_uint(o.yp) : QI;
+  const uint32_t mx = xm < xp ? xm : xp, my = ym < yp ? ym : yp, mz = zm < zp ? zm : zp;
+  const uint32_t lo = mx < my ? mx : my, hi = mx > my ? mx : my;
+  const uint32_t m0 = lo < mz ? lo : mz, mid = lo > mz ? lo : mz;
+  const uint32_t m1 = hi < mid ? hi : mid, m2 = hi > mid ? hi : mid;
+  const double inf = __builtin_inf();
+  auto dec = [&](uint32_t q) -> double { return q == QI ? inf : L.fb + (double)q * L.fd; };
+  const double e = god_iso64_sorted((double)o.ini, dec(m0), dec(m1), dec(m2), h, 2.0 * (h * h),
+                                    3.0 * (h * h), 1.0 / 3.0);
+  const double dv = dec(__float_as_uint(o.cur));
+  const bool bad = L.in && !(dv == e) && !(__builtin_fabs(dv - e) <= L.ftol);
+  if (bad && flags) {
+    const uint32_t p = (uint32_t)z * L.XY + L.col;
+    typedef __attribute__((address_space(1))) uint32_t gu32;
+    __hip_atomic_fetch_or((gu32 *)(flags + (p >> 5)), 1u << (p & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return (unsigned)__popcll(__ballot(bad));
+}
+
 // Fields of a batch: field f owns tiles [tab[f].tile0, tab[f+1].tile0)
-// (tab[nf].tile0 = total); flags optional per field.
-template <bool ISO, bool SPEED>
+// (tab[nf].tile0 = total); flags optional per field. FX: fixed-point fields
+// (isotropic, unit speed; dx is the spacing).
+template <bool ISO, bool SPEED, bool FX = false>
 __global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__restrict__ tab,
                                                    int nf, double dx, double dy, double dz,
                                                    unsigned long long *counts) {
@@ -1203,18 +1300,27 @@ __global__ __launch_bounds__(256) void k_judge_f32(const pdfsk_judge_field *__re
   L.oyp = L.hyp ? (uint32_t)X : 0u;
   const double hmin = dx < dy ? (dx < dz ? dx : dz) : (dy < dz ? dy : dz);
   L.htol = hmin * 0x1.0p-21;
+  L.fb = fd.fx_base;
+  L.fd = fd.fx_d;
+  L.ftol = fd.fx_tol;
   unsigned long long nbad = 0;
+  auto judge = [&](const JudgeOps &o, int zz) -> unsigned {
+    if constexpr (FX)
+      return judge_plane_fx(o, L, zz, dx, fd.flags);
+    else
+      return judge_plane<ISO, SPEED>(o, L, zz, dx, dy, dz, fd.flags);
+  };
   // Two operand sets, alternately loaded one plane ahead of the plane being
   // judged: the loads of plane z + 1 are in flight while plane z is solved.
   JudgeOps A = judge_load<SPEED>(v, init, slow, L, z0), B;
   int z = z0;
   for (; z + 1 < z1; z += 2) {
     B = judge_load<SPEED>(v, init, slow, L, z + 1);
-    nbad += judge_plane<ISO, SPEED>(A, L, z, dx, dy, dz, fd.flags);
+    nbad += judge(A, z);
     A = judge_load<SPEED>(v, init, slow, L, z + 2);
-    nbad += judge_plane<ISO, SPEED>(B, L, z + 1, dx, dy, dz, fd.flags);
+    nbad += judge(B, z + 1);
   }
-  if (z < z1) nbad += judge_plane<ISO, SPEED>(A, L, z, dx, dy, dz, fd.flags);
+  if (z < z1) nbad += judge(A, z);
   if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(counts + f, nbad);
 }
 
@@ -1295,6 +1401,53 @@ __global__ void k_convert_batch(const pdfsk_conv *__restrict__ tab, int dir) {
       t.f64[p] = (double)t.f32[p];
     else
       t.f32[p] = (float)t.f64[p];
+  }
+}
+
+// Fixed-point storage. Order key of an FP32 value: non-negative values map
+// to their bits with the sign bit set, negative values to their inverted
+// bits, so unsigned key order is value order (-inf < ... < -0 < +0 < ... <
+// +inf); NaN is skipped. Field f is handled by the workgroups of row
+// blockIdx.y.
+__device__ __forceinline__ uint32_t fx_key(float v) {
+  const uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__global__ void k_fx_min_batch(const pdfsk_fx *__restrict__ tab) {
+  const pdfsk_fx t = tab[blockIdx.y];
+  uint32_t k = 0xFFFFFFFFu;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const float v = t.f[p];
+    const uint32_t kv = v == v ? fx_key(v) : 0xFFFFFFFFu;
+    k = kv < k ? kv : k;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t o2 = (uint32_t)__shfl_xor((int)k, o);
+    k = o2 < k ? o2 : k;
+  }
+  if ((threadIdx.x & 63) == 0 && k != 0xFFFFFFFFu) atomicMin(t.key, k);
+}
+// q = rint((v - base) / d) below UINT32_MAX, else UINT32_MAX (and for NaN,
+// +inf); contraction off so that the oracle's C gives the same bits
+__global__ void k_fx_encode_batch(const pdfsk_fx *__restrict__ tab) {
+#pragma clang fp contract(off)
+  const pdfsk_fx t = tab[blockIdx.y];
+  uint32_t *q = (uint32_t *)t.f;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const double r = __builtin_rint(((double)t.f[p] - t.base) / t.d);
+    q[p] = r < 4294967295.0 ? (uint32_t)r : 0xFFFFFFFFu;
+  }
+}
+__global__ void k_fx_decode_batch(const pdfsk_fx *__restrict__ tab) {
+#pragma clang fp contract(off)
+  const pdfsk_fx t = tab[blockIdx.y];
+  const uint32_t *q = (const uint32_t *)t.f;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const uint32_t v = q[p];
+    t.f[p] = v == 0xFFFFFFFFu ? __builtin_inff() : (float)(t.base + (double)v * t.d);
   }
 }
 
@@ -1516,6 +1669,7 @@ static auto with_mode(int mode, F f) {
     case PDFSK_MODE_F64_POL_ANISO:
       return f(std::integral_constant<int, PDFSK_MODE_F64_POL_ANISO>());
     case PDFSK_MODE_F32_SPEED: return f(std::integral_constant<int, PDFSK_MODE_F32_SPEED>());
+    case PDFSK_MODE_FX_ISO: return f(std::integral_constant<int, PDFSK_MODE_FX_ISO>());
     default: return f(std::integral_constant<int, PDFSK_MODE_F64_POL_SPEED>());
   }
 }
@@ -1638,6 +1792,15 @@ extern "C" int pdfsk_judge_batch_f32(const pdfsk_judge_field *tab, int nf, long 
   return hip_rc(hipGetLastError());
 }
 
+extern "C" int pdfsk_judge_batch_fx(const pdfsk_judge_field *tab, int nf, long ntiles, double h,
+                                    unsigned long long *counts, hipStream_t st) {
+  if (nf < 1 || ntiles < 1 || ntiles >= (1L << 31) - 8) return -1;
+  const long g = (ntiles + 7) / 8 * 8;  // a multiple of the 8 XCDs
+  hipLaunchKernelGGL((k_judge_f32<true, false, true>), dim3((unsigned)g), dim3(256), 0, st, tab,
+                     nf, h, h, h, counts);
+  return hip_rc(hipGetLastError());
+}
+
 extern "C" int pdfsk_pack_f64(const double *padded, float *dense, int X, int Y, int Z,
                               hipStream_t st) {
   hipLaunchKernelGGL(k_pack_f64, dim3(grid_for((long)X * Y * Z, 256)), dim3(256), 0, st, padded,
@@ -1705,6 +1868,40 @@ extern "C" int pdfsk_convert_batch(const pdfsk_conv *tab, int nf, long long nmax
   hipLaunchKernelGGL(k_convert_batch, dim3((unsigned)gx, (unsigned)nf), dim3(256), 0, st, tab,
                      dir);
   return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_fx_bits(int X, int Y, int Z) {
+  const long long s = (long long)X + Y + Z;
+  int c = 0;  // ceil(log2(s))
+  while ((1ll << c) < s) c++;
+  return 32 - c;
+}
+
+extern "C" float pdfsk_fx_key_value(uint32_t key) {
+  const uint32_t b = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+  float v;
+  memcpy(&v, &b, sizeof v);
+  return v;
+}
+
+static int fx_launch(void (*kern)(const pdfsk_fx *), const pdfsk_fx *tab, int nf,
+                     long long nmax, hipStream_t st) {
+  if (nf < 1 || nf > 65535 || nmax < 1) return -1;
+  long long gx = (nmax + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  hipLaunchKernelGGL(kern, dim3((unsigned)gx, (unsigned)nf), dim3(256), 0, st, tab);
+  return hip_rc(hipGetLastError());
+}
+extern "C" int pdfsk_fx_min_batch(const pdfsk_fx *tab, int nf, long long nmax, hipStream_t st) {
+  return fx_launch(k_fx_min_batch, tab, nf, nmax, st);
+}
+extern "C" int pdfsk_fx_encode_batch(const pdfsk_fx *tab, int nf, long long nmax,
+                                     hipStream_t st) {
+  return fx_launch(k_fx_encode_batch, tab, nf, nmax, st);
+}
+extern "C" int pdfsk_fx_decode_batch(const pdfsk_fx *tab, int nf, long long nmax,
+                                     hipStream_t st) {
+  return fx_launch(k_fx_decode_batch, tab, nf, nmax, st);
 }
 
 extern "C" int pdfsk_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st) {

@@ -776,16 +776,18 @@ int pdfs_solve_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, float *field, const float 
     sp.judge_runs += sw.judge_runs;
     sp.fallbacks = 1;
     sp.ms_warm = wms;
-    sp.ms_total = sp.ms_sweeps + sp.ms_judge + sp.ms_polish + wms;
+    sp.ms_convert += sw.ms_convert;
+    sp.ms_total = sp.ms_sweeps + sp.ms_judge + sp.ms_polish + sp.ms_convert + wms;
     if (st) *st = sp;
     return rc;
   }
-  if (oo.polish) {
+  /* fields swept in fixed point need no polish (pdfs_opts.storage) */
+  if (oo.polish && sw.storage != PDFS_STORE_FIXED) {
     const int dims[3] = {x, y, z};
     rc = pdfsi_polish(c, 1, &field, &init, NULL, dims, dx, dy, dz, &oo, &sw);
     if (rc && rc != PDFS_ENOTCONV) return rc;
   }
-  sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + wms;
+  sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + sw.ms_convert + wms;
   if (st) *st = sw;
   return rc;
 }
@@ -855,8 +857,8 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
   }
   sw.fallbacks = 0;
   sw.judge_flagged = 0;
-  /* polish of the accepted fields */
-  if (oo.polish && nbad < nf) {
+  /* polish of the accepted fields (not of fields swept in fixed point) */
+  if (oo.polish && nbad < nf && sw.storage != PDFS_STORE_FIXED) {
     rc = pdfsi_polish(c, nf, fields, gi, NULL, dims, dx, dy, dz, &oo, &sw);
     if (rc && rc != PDFS_ENOTCONV) goto out;
   }
@@ -900,11 +902,12 @@ int pdfs_solve_batch_warm_f32_dev(pdfs_ctx *c, pdfs_net *n, int nf, float *const
     if (sp.polish_maxdiff > sw.polish_maxdiff) sw.polish_maxdiff = sp.polish_maxdiff;
     sw.polish_skipped += sp.polish_skipped;
     sw.ms_polish += sp.ms_polish;
+    sw.ms_convert += sp.ms_convert;
     sw.fallbacks = nbad;
   }
   if (rc == PDFS_OK) rc = rc_pol;
   sw.ms_warm = wms;
-  sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + wms;
+  sw.ms_total = sw.ms_sweeps + sw.ms_judge + sw.ms_polish + sw.ms_convert + wms;
   if (st) *st = sw;
   for (int f = 0; f < nf; f++) {
     if (field_passes) field_passes[f] = passes[f];

@@ -95,7 +95,7 @@ __global__ void k_na_setup(unsigned char *base, float *f, int X, int Y, int Z, i
     fd->nTy = nTy;
     fd->ex_off = 0;
     fd->ey_off = ey_off;
-    fd->pad = 0;
+    fd->fxk = 0;
     state[4] = (unsigned)nc;                 // column tickets
     if (state[1] == 0u) state[1] = 1u;       // epochs start at 1 (zeroed entries: 0)
     state[8] = 0u;                           // edge-wait timeout flag

@@ -49,6 +49,12 @@ struct pdfs_ctx {
   double *d_params;
   hipEvent_t ev0, ev1;
   hipEvent_t kev[18];   /* [0..15] per-ordering profile, [16..17] polish */
+  /* fixed-point solve in progress: per-field quantum exponents of the plan
+   * (NULL otherwise), conversion table (host staging, device), order keys */
+  const int *fxk;
+  pdfsk_fx *fx_host, *fx_dev;
+  uint32_t *fx_keys, *h_fx_keys;
+  int cap_fx;
 };
 
 static __thread char g_err[512];
@@ -137,6 +143,8 @@ void pdfs_default_opts(pdfs_opts *o)
   o->polish = 1;
   o->polish_tol = 2e-5;
   o->tile64 = -1;
+  o->storage = PDFS_STORE_FIXED;
+  o->judge_tol = 2e-6;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -193,6 +201,10 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->pbuf);
   hipFree(c->conv_dev);
   free(c->conv_host);
+  free(c->fx_host);
+  hipFree(c->fx_dev);
+  hipFree(c->fx_keys);
+  hipHostFree(c->h_fx_keys);
   free(c->h_streak);
   hipFree(c->bt.state);
   hipFree(c->d_u64);
@@ -436,7 +448,7 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     fd->nTy = (y + TY - 1) / TY;
     fd->ex_off = (uint32_t)ex;
     fd->ey_off = (uint32_t)ey;
-    fd->pad = 0;
+    fd->fxk = c->fxk ? c->fxk[f] : 0;
     size_t nc = (size_t)fd->nTx * fd->nTy;
     /* one row of face entries per column step (z + TX + TY - 1 steps) */
     ex += nc * (size_t)(z + TX + TY - 1) * TY * words;
@@ -924,10 +936,27 @@ int pdfs_sweep_f32_dev(pdfs_ctx *c, float *field, int x, int y, int z, double dx
 
 /* FP64 residual judge of n fields in one launch; flagged[i] = failing points
  * of field i. flags[i] (optional) receive one bit per point. */
+/* The judge of a batch; fxb (or NULL): the fields are fixed point, field i
+ * holding q of value = fxb[i] + q fxd[i], judged with tolerance fxtol[i]. */
+static int judge_batch_ex(pdfs_ctx *c, int n, const float *const *v, const float *const *init,
+                          const float *const *slow, const int *dims, uint32_t *const *flags,
+                          double dx, double dy, double dz, long long *flagged,
+                          const double *fxb, const double *fxd, const double *fxtol);
+
 int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *const *init,
                       const float *const *slow, const int *dims, uint32_t *const *flags,
                       double dx, double dy, double dz, long long *flagged)
 {
+  return judge_batch_ex(c, n, v, init, slow, dims, flags, dx, dy, dz, flagged, NULL, NULL, NULL);
+}
+
+static int judge_batch_ex(pdfs_ctx *c, int n, const float *const *v, const float *const *init,
+                          const float *const *slow, const int *dims, uint32_t *const *flags,
+                          double dx, double dy, double dz, long long *flagged,
+                          const double *fxb, const double *fxd, const double *fxtol)
+{
+  if (fxb && (slow || dx != dy || dy != dz))
+    return fail(PDFS_EINVAL, "judge: fixed-point fields need equal spacings and unit speed");
   if (!c || n < 1 || !v || !init || !dims) return fail(PDFS_EINVAL, "judge: bad arguments");
   if (n + 1 > c->cap_jt) {
     HIPCHK(hipStreamSynchronize(c->st));
@@ -964,6 +993,9 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
     t->Y = y;
     t->Z = z;
     t->tile0 = (int)tiles;
+    t->fx_base = fxb ? fxb[i] : 0.0;
+    t->fx_d = fxb ? fxd[i] : 0.0;
+    t->fx_tol = fxb ? fxtol[i] : 0.0;
     tiles += pdfsk_judge_tiles(x, y, z);
     if (tiles >= (1LL << 31) - 8) return fail(PDFS_EINVAL, "judge: batch too large");
     if (t->flags)
@@ -974,8 +1006,11 @@ int pdfsi_judge_batch(pdfs_ctx *c, int n, const float *const *v, const float *co
   HIPCHK(hipMemcpyAsync(c->jt_dev, c->jt_host, (size_t)(n + 1) * sizeof(pdfsk_judge_field),
                         hipMemcpyHostToDevice, c->st));
   KCHK(pdfsk_fill_u32((uint32_t *)c->jc_dev, 0u, (size_t)2 * n, c->st));
-  KCHK(pdfsk_judge_batch_f32(c->jt_dev, n, (long)tiles, dx, dy, dz, slow != NULL, c->jc_dev,
-                             c->st));
+  if (fxb)
+    KCHK(pdfsk_judge_batch_fx(c->jt_dev, n, (long)tiles, dx, c->jc_dev, c->st));
+  else
+    KCHK(pdfsk_judge_batch_f32(c->jt_dev, n, (long)tiles, dx, dy, dz, slow != NULL, c->jc_dev,
+                               c->st));
   HIPCHK(hipMemcpyAsync(c->jc_host, c->jc_dev, (size_t)n * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, c->st));
   /* also keeps the host table from being rewritten while its copy is pending */
@@ -1012,8 +1047,95 @@ int pdfs_judge_f32_dev(pdfs_ctx *c, const float *field, const float *init, int x
   return rc;
 }
 
+int pdfs_judge_fixed_dev(pdfs_ctx *c, const uint32_t *q, const float *init, int x, int y, int z,
+                         double h, double base, double d, double tol, uint32_t *flags,
+                         long long *flagged)
+{
+  if (!c || !q || !init || bad_h(h, h, h) || !(d > 0.0) || !(tol >= 0.0) || base != base)
+    return fail(PDFS_EINVAL, "pdfs_judge_fixed_dev: bad arguments");
+  const int dims[3] = {x, y, z};
+  const float *v = (const float *)q;
+  long long fl = 0;
+  int rc = judge_batch_ex(c, 1, &v, &init, NULL, dims, flags ? &flags : NULL, h, h, h, &fl, &base,
+                          &d, &tol);
+  if (rc == PDFS_OK && flagged) *flagged = fl;
+  return rc;
+}
+
 /* The FP32 batch solve without the spacing check (run_fsm and the CLI take
  * the caller's spacings as the reference does). */
+/* Fixed-point storage of a batch (pdfs_opts.storage). fx_encode finds each
+ * field's smallest non-NaN value (one reduction launch and a readback),
+ * picks the field's quantum d = h 2^-k, k = pdfsk_fx_bits, and converts the
+ * fields in place; *ok = 0 (nothing converted) when a field holds -inf.
+ * fx_decode converts them back (the device table of the encode is kept). */
+static int fx_encode(pdfs_ctx *c, int nf, float *const *fields, const int *dims, double h,
+                     int *k, double *base, double *d, int *ok)
+{
+  *ok = 0;
+  if (nf > c->cap_fx) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    free(c->fx_host);
+    hipFree(c->fx_dev);
+    hipFree(c->fx_keys);
+    hipHostFree(c->h_fx_keys);
+    c->fx_host = NULL;
+    c->fx_dev = NULL;
+    c->fx_keys = NULL;
+    c->h_fx_keys = NULL;
+    c->cap_fx = 0;
+    c->fx_host = (pdfsk_fx *)malloc((size_t)nf * sizeof(pdfsk_fx));
+    if (!c->fx_host) return fail(PDFS_ENOMEM, "fixed point: host alloc");
+    HIPCHK(hipMalloc((void **)&c->fx_dev, (size_t)nf * sizeof(pdfsk_fx)));
+    HIPCHK(hipMalloc((void **)&c->fx_keys, (size_t)nf * sizeof(uint32_t)));
+    HIPCHK(hipHostMalloc((void **)&c->h_fx_keys, (size_t)nf * sizeof(uint32_t), 0));
+    c->cap_fx = nf;
+  }
+  long long nmax = 0;
+  for (int f = 0; f < nf; f++) {
+    const long long n = (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
+    c->fx_host[f].f = fields[f];
+    c->fx_host[f].n = n;
+    c->fx_host[f].base = 0.0;
+    c->fx_host[f].d = 1.0;
+    c->fx_host[f].key = c->fx_keys + f;
+    if (n > nmax) nmax = n;
+  }
+  HIPCHK(hipMemcpyAsync(c->fx_dev, c->fx_host, (size_t)nf * sizeof(pdfsk_fx),
+                        hipMemcpyHostToDevice, c->st));
+  KCHK(pdfsk_fill_u32(c->fx_keys, 0xFFFFFFFFu, (size_t)nf, c->st));
+  KCHK(pdfsk_fx_min_batch(c->fx_dev, nf, nmax, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_fx_keys, c->fx_keys, (size_t)nf * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (int f = 0; f < nf; f++) {
+    const uint32_t key = c->h_fx_keys[f];
+    /* no non-NaN value, or all +inf: every point encodes as +inf */
+    double b = key == 0xFFFFFFFFu ? 0.0 : (double)pdfsk_fx_key_value(key);
+    if (b == -HUGE_VAL) return PDFS_OK;
+    if (b == HUGE_VAL) b = 0.0;
+    k[f] = pdfsk_fx_bits(dims[3 * f], dims[3 * f + 1], dims[3 * f + 2]);
+    base[f] = b;
+    d[f] = ldexp(h, -k[f]);
+    c->fx_host[f].base = b;
+    c->fx_host[f].d = d[f];
+  }
+  HIPCHK(hipMemcpyAsync(c->fx_dev, c->fx_host, (size_t)nf * sizeof(pdfsk_fx),
+                        hipMemcpyHostToDevice, c->st));
+  KCHK(pdfsk_fx_encode_batch(c->fx_dev, nf, nmax, c->st));
+  *ok = 1;
+  return PDFS_OK;
+}
+
+static int fx_decode(pdfs_ctx *c, int nf)
+{
+  long long nmax = 0;
+  for (int f = 0; f < nf; f++)
+    if (c->fx_host[f].n > nmax) nmax = c->fx_host[f].n;
+  KCHK(pdfsk_fx_decode_batch(c->fx_dev, nf, nmax, c->st));
+  return PDFS_OK;
+}
+
 static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *const *inits,
                      const float *const *slows, const int *dims, double dx, double dy,
                      double dz, const pdfs_opts *o, pdfs_stats *st, int *field_passes,
@@ -1094,10 +1216,8 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   }
   pdfs_sp32 sp;
   make_sp32(dx, dy, dz, &sp);
-  const int m32 = speed ? PDFSK_MODE_F32_SPEED : mode32(&sp);
+  int m32 = speed ? PDFSK_MODE_F32_SPEED : mode32(&sp);
   int rc;
-  if ((rc = plan_build_ex(c, nf, (void *const *)fields, NULL, slows, dims, tile, 0, 1)))
-    return rc;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
   /* per-field change count of the current pass (summed over its segments),
    * and the points the field's last judge flagged */
@@ -1114,8 +1234,14 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   const float **ji = (const float **)malloc((size_t)nf * sizeof(float *));
   const float **js = (const float **)malloc((size_t)nf * sizeof(float *));
   long long *jfl = (long long *)malloc((size_t)nf * sizeof(long long));
+  /* fixed-point storage: per-field quantum exponent, base, quantum and judge
+   * tolerance; [3 nf ..]: the same three for the fields judged together */
+  int *fxk = (int *)malloc((size_t)nf * sizeof(int));
+  double *fxv = (double *)malloc((size_t)6 * nf * sizeof(double));
   if (!delta || !pdel || !lastfl || !active || !certified || !jidx || !jdone || !jd || !jv ||
-      !ji || !js || !jfl) {
+      !ji || !js || !jfl || !fxk || !fxv) {
+    free(fxk);
+    free(fxv);
     free(delta);
     free(pdel);
     free(lastfl);
@@ -1130,6 +1256,27 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     free(jfl);
     return fail(PDFS_ENOMEM, "batch: host alloc");
   }
+  /* fixed-point storage (pdfs_opts.storage): encode in place, sweep in the
+   * fixed-point mode, judge the fixed-point values, decode at the end */
+  int fixed = 0, decoded = 0;
+  double *fxb = fxv, *fxd = fxv + nf, *fxt = fxv + 2 * nf;
+  double *jfb = fxv + 3 * nf, *jfd = fxv + 4 * nf, *jft = fxv + 5 * nf;
+  if (o->storage == PDFS_STORE_FIXED && !speed && o->fixed_passes <= 0 && sp.iso) {
+    HIPGO(hipEventRecord(c->kev[16], c->st));
+    if ((rc = fx_encode(c, nf, fields, dims, dx, fxk, fxb, fxd, &fixed))) goto out;
+    HIPGO(hipEventRecord(c->kev[17], c->st));
+    if (fixed) {
+      HIPGO(hipEventSynchronize(c->kev[17]));
+      float tc = 0.f;
+      hipEventElapsedTime(&tc, c->kev[16], c->kev[17]);
+      s.ms_convert = tc;
+      m32 = PDFSK_MODE_FX_ISO;
+      c->fxk = fxk;
+      for (int f = 0; f < nf; f++) fxt[f] = o->judge_tol * dx + 4.0 * fxd[f];
+    }
+  }
+  s.storage = fixed ? PDFS_STORE_FIXED : PDFS_STORE_F32;
+  if ((rc = plan_build_ex(c, nf, (void *const *)fields, NULL, slows, dims, tile, 0, 1))) goto out;
   for (int f = 0; f < nf; f++) {
     active[f] = 1;
     lastfl[f] = -1;  /* not judged yet */
@@ -1258,14 +1405,17 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
           jv[nj] = fields[f];
           ji[nj] = inits[f];
           js[nj] = speed ? slows[f] : NULL;
+          jfb[nj] = fxb[f];
+          jfd[nj] = fxd[f];
+          jft[nj] = fxt[f];
           memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
           nj++;
         }
       }
       /* all fields to certify in one judge launch */
       if (nj > 0) {
-        if ((rc = pdfsi_judge_batch(c, nj, jv, ji, speed ? js : NULL, jd, NULL, dx, dy, dz,
-                                    jfl)))
+        if ((rc = judge_batch_ex(c, nj, jv, ji, speed ? js : NULL, jd, NULL, dx, dy, dz, jfl,
+                                 fixed ? jfb : NULL, jfd, jft)))
           break;
         s.judge_runs += nj;
         for (int k = 0; k < nj; k++) {
@@ -1304,7 +1454,8 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   s.ms_sweep_kernels = kms;
   /* FP64 polish of the converged FP32 fields (they keep their FP32 tile
    * choice; the plan is rebuilt for the FP64 sweeps) */
-  if ((rc == PDFS_OK || rc == PDFS_ENOTCONV) && o->polish && inits && o->fixed_passes <= 0) {
+  if ((rc == PDFS_OK || rc == PDFS_ENOTCONV) && o->polish && inits && o->fixed_passes <= 0 &&
+      !fixed) {
     const int rc0 = rc;
     rc = pdfsi_polish(c, nf, fields, inits, slows, dims, dx, dy, dz, o, &s);
     if (rc == PDFS_OK) rc = rc0;
@@ -1320,11 +1471,14 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
       jv[nj] = fields[f];
       ji[nj] = inits[f];
       js[nj] = speed ? slows[f] : NULL;
+      jfb[nj] = fxb[f];
+      jfd[nj] = fxd[f];
+      jft[nj] = fxt[f];
       memcpy(jd + 3 * nj, dims + 3 * f, 3 * sizeof(int));
       nj++;
     }
-    if (nj > 0 && (rc = pdfsi_judge_batch(c, nj, jv, ji, speed ? js : NULL, jd, NULL, dx, dy, dz,
-                                          jfl)) == PDFS_OK) {
+    if (nj > 0 && (rc = judge_batch_ex(c, nj, jv, ji, speed ? js : NULL, jd, NULL, dx, dy, dz,
+                                       jfl, fixed ? jfb : NULL, jfd, jft)) == PDFS_OK) {
       for (int k = 0; k < nj; k++) {
         s.judge_flagged += jfl[k];
         if (field_flagged) field_flagged[jidx[k]] = jfl[k];
@@ -1338,7 +1492,22 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     s.ms_judge = tj_before + t;
     s.judge_runs += judged;
   }
-  s.ms_total = s.ms_sweeps + s.ms_judge + s.ms_polish;
+  if (fixed) {
+    /* back to FP32 (also after an error, so the caller's fields hold FP32) */
+    int rd;
+    if (hipEventRecord(c->kev[16], c->st) != hipSuccess || (rd = fx_decode(c, nf)) ||
+        hipEventRecord(c->kev[17], c->st) != hipSuccess ||
+        hipEventSynchronize(c->kev[17]) != hipSuccess) {
+      if (rc == PDFS_OK || rc == PDFS_ENOTCONV)
+        rc = fail(PDFS_EHIP, "fixed point: decode failed");
+    } else {
+      float tc = 0.f;
+      hipEventElapsedTime(&tc, c->kev[16], c->kev[17]);
+      s.ms_convert += tc;
+    }
+    decoded = 1;
+  }
+  s.ms_total = s.ms_sweeps + s.ms_judge + s.ms_polish + s.ms_convert;
   if (st) *st = s;
   if (field_flagged)
     for (int f = 0; f < nf; f++)
@@ -1346,6 +1515,10 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   if (rc == PDFS_OK && o->fixed_passes <= 0 && nactive > 0)
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, s.passes);
 out:
+  if (fixed && !decoded) fx_decode(c, nf);
+  c->fxk = NULL;
+  free(fxk);
+  free(fxv);
   free(delta);
   free(pdel);
   free(lastfl);

@@ -86,6 +86,21 @@ def lib():
         L.orc_judge_f32_speed.argtypes = [_fp, _fp, C.c_void_p, _I, _I, _I, _D, _D, _D,
                                           C.c_void_p]
         L.orc_judge_f32_speed.restype = C.c_long
+        _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+        L.orc_fx_bits.argtypes = [_I, _I, _I]
+        L.orc_fx_bits.restype = _I
+        L.orc_fx_min.argtypes = [_fp, C.c_size_t]
+        L.orc_fx_min.restype = C.c_float
+        L.orc_fx_encode.argtypes = [_fp, _up, C.c_size_t, _D, _D]
+        L.orc_fx_decode.argtypes = [_up, _fp, C.c_size_t, _D, _D]
+        L.orc_godunov_fx.argtypes = [C.c_uint32] * 4 + [_I]
+        L.orc_godunov_fx.restype = C.c_uint32
+        L.orc_sweep_fx.argtypes = [_up, _I, _I, _I, _I, _I]
+        L.orc_sweep_fx.restype = C.c_long
+        L.orc_solve_fx.argtypes = [_up, _I, _I, _I, _I, _I, C.POINTER(C.c_long)]
+        L.orc_solve_fx.restype = _I
+        L.orc_judge_fx.argtypes = [_up, _fp, _I, _I, _I, _D, _D, _D, _D, C.c_void_p]
+        L.orc_judge_fx.restype = C.c_long
         _lib = L
     return _lib
 
@@ -299,3 +314,76 @@ def random_field(X, Y, Z, seed, p_src=0.02, p_far=0.3, hi=20.0):
     v[r > 1.0 - p_far] = 1e6
     a[1:-1, 1:-1, 1:-1] = v
     return a
+
+
+# ---- fixed-point storage ----
+FX_INF = 0xFFFFFFFF
+
+
+def fx_bits(X, Y, Z):
+    return lib().orc_fx_bits(X, Y, Z)
+
+
+def fx_params(v32, h=1.0):
+    """(k, base, d) of a start field (FP32 array, any shape): base = its
+    smallest non-NaN value (0 when it has none or only +inf), d = h 2^-k."""
+    Z, Y, X = v32.shape if v32.ndim == 3 else (1, 1, v32.size)
+    flat = np.ascontiguousarray(v32, dtype=np.float32).ravel()
+    b = float(lib().orc_fx_min(flat, flat.size))
+    if not np.isfinite(b):
+        b = 0.0
+    k = fx_bits(X, Y, Z)
+    return k, b, float(np.ldexp(h, -k))
+
+
+def fx_encode(v32, base, d):
+    flat = np.ascontiguousarray(v32, dtype=np.float32).ravel()
+    q = np.empty(flat.size, np.uint32)
+    lib().orc_fx_encode(flat, q, flat.size, base, d)
+    return q.reshape(v32.shape)
+
+
+def fx_decode(q, base, d):
+    flat = np.ascontiguousarray(q, dtype=np.uint32).ravel()
+    v = np.empty(flat.size, np.float32)
+    lib().orc_fx_decode(flat, v, flat.size, base, d)
+    return v.reshape(q.shape)
+
+
+def fx_padded(q_interior):
+    """Padded uint32 array (ghosts +inf) around an interior [Z, Y, X]."""
+    Z, Y, X = q_interior.shape
+    a = np.full((Z + 2, Y + 2, X + 2), FX_INF, np.uint32)
+    a[1:-1, 1:-1, 1:-1] = q_interior
+    return a
+
+
+def sweep_fx(q, sn, k):
+    """One fixed-point ordering on a padded uint32 array, in place; returns
+    the points changed."""
+    return lib().orc_sweep_fx(q, *dims(q), k, sn)
+
+
+def solve_fx(q, k, max_passes=100):
+    ch = C.c_long()
+    p = lib().orc_solve_fx(q, *dims(q), k, max_passes, C.byref(ch))
+    return p, ch.value
+
+
+def solve_fixed(start, h=1.0, max_passes=100):
+    """The GPU's fixed-point solve of a dense FP32 start field [Z, Y, X]:
+    base, quantum, encode, passes until one changes nothing, decode. Returns
+    (FP32 result, passes, (k, base, d), padded uint32 result)."""
+    start = np.ascontiguousarray(start, dtype=np.float32)
+    k, b, d = fx_params(start, h)
+    q = fx_padded(fx_encode(start, b, d))
+    p, _ = solve_fx(q, k, max_passes)
+    return fx_decode(interior(q).copy(), b, d), p, (k, b, d), q
+
+
+def judge_fx(q, init32, h, base, d, tol, want_flags=False):
+    """q: padded uint32, init32: padded float32."""
+    flags = np.zeros(q.shape, np.uint8) if want_flags else None
+    n = lib().orc_judge_fx(q, init32, *dims(q), h, base, d, tol,
+                           flags.ctypes.data if want_flags else None)
+    return (n, flags) if want_flags else n

@@ -21,6 +21,9 @@ import pdfs_amd
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import nn_ref as R  # noqa: E402
 
+# these tests check the FP32-storage path (and its polish) against the FP32 oracle
+S32 = pdfs_amd.PDFS_STORE_F32
+
 pytestmark = pytest.mark.gpu
 
 FWD_TOL = 3e-2
@@ -174,7 +177,7 @@ def test_warm_solve_good_bound_certifies_without_fallback(ctx, net, dev):
     net.set_params(_random_params(41))
     net.set_calibration(1.0 / 32, 0.0, 1000.0)   # w = min(init, 1000 h): above everything
     f = init.clone()
-    st = ctx.solve_warm_f32(net, f, init)
+    st = ctx.solve_warm_f32(net, f, init, storage=S32)
     err = np.abs(f.cpu().numpy().astype(np.float64) - O.interior(ref64)).max()
     assert st["fallbacks"] == 0 and st["judge_flagged"] == 0
     assert err <= 1e-4
@@ -189,7 +192,7 @@ def test_warm_solve_bad_bound_falls_back(ctx, net, dev):
     init = torch.from_numpy(np.ascontiguousarray(O.interior(a)).astype(np.float32)).to(dev)
     net.set_calibration(1.0 / 32, 0.0, 0.5)
     f = init.clone()
-    st = ctx.solve_warm_f32(net, f, init)
+    st = ctx.solve_warm_f32(net, f, init, storage=S32)
     assert st["fallbacks"] == 1 and st["judge_flagged"] == 0
     assert np.array_equal(f.cpu().numpy().view(np.uint32), O.interior(ref32).view(np.uint32))
 
@@ -207,9 +210,9 @@ def test_trained_weights_warm_start_is_an_upper_bound(ctx, dev):
         init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
         ctx.generate(kind, 7000 + kind, X, Y, Z, 1.0, dense=init)
         plain = init.clone()
-        sp = ctx.solve_f32(plain, init)
+        sp = ctx.solve_f32(plain, init, storage=S32)
         f = init.clone()
-        st = ctx.solve_warm_f32(n, f, init)
+        st = ctx.solve_warm_f32(n, f, init, storage=S32)
         w = torch.empty_like(init)
         n.warm_start(init, w)
         below = int((w < plain).sum())
@@ -280,12 +283,12 @@ def test_trained_warm_start_bounds_far_corners_at_512(ctx, dev):
             init = torch.empty((S, S, S), dtype=torch.float32, device=dev)
             ctx.generate(pdfs_amd.GEN_SPHERES, seed, S, S, S, 1.0, dense=init)
             u = init.clone()
-            ctx.solve_f32(u, None, judge=0)
+            ctx.solve_f32(u, None, judge=0, storage=S32)
             w = torch.empty_like(init)
             n.warm_start(init, w)
             assert int((w < u).sum()) == 0
             f = init.clone()
-            st = ctx.solve_warm_f32(n, f, init)
+            st = ctx.solve_warm_f32(n, f, init, storage=S32)
             assert st["fallbacks"] == 0 and st["judge_flagged"] == 0
             del init, u, w, f
     finally:
@@ -320,7 +323,7 @@ def test_trained_network_gain_over_untrained_and_plain(ctx, dev):
             init = torch.empty((S, S, S), dtype=torch.float32, device=dev)
             ctx.generate_ex(kind, seed, S, S, S, 1.0, nobj, p, q, dense=init)
             u = init.clone()
-            sp = ctx.solve_f32(u, init)
+            sp = ctx.solve_f32(u, init, storage=S32)
             unk = init >= 1e6
             err = {}
             for name, net in (("trained", trained), ("untrained", untrained)):
@@ -341,7 +344,7 @@ def test_trained_network_gain_over_untrained_and_plain(ctx, dev):
             cov = unk & (w < init)
             beyond = 1.0 - float(cov.sum()) / float(unk.sum())
             f = init.clone()
-            sw = ctx.solve_warm_f32(trained, f, init)
+            sw = ctx.solve_warm_f32(trained, f, init, storage=S32)
             print(f"kind {kind}: mean |d - u| trained {err['trained']:.2f} h, untrained "
                   f"{err['untrained']:.2f} h; warm start slack mean {float((w - u)[cov].mean()):.1f} h "
                   f"on the covered cells, {beyond:.4f} beyond the cutoff "

@@ -13,6 +13,9 @@ import pdfs_amd
 
 pytestmark = pytest.mark.gpu
 
+# these tests check the FP32-storage path (and its polish) against the FP32 oracle
+S32 = pdfs_amd.PDFS_STORE_F32
+
 # FP32 vs FP64 oracle tolerance for the small fields here (distances < 64 h):
 # storage rounding random walk, a few hundred FP32 ulps at most.
 F32_VS_F64_TOL = 1e-4
@@ -104,7 +107,7 @@ def test_f32_converged_bitwise_and_vs_f64(ctx, dev, gen):
     t = _dense(a, dev)
     init = t.clone()
     # the FP32 fixed point itself (no polish): bit-exact against the oracle
-    st = ctx.solve_f32(t, init, polish=0)
+    st = ctx.solve_f32(t, init, polish=0, storage=S32)
     got = t.cpu().numpy()
     assert st["passes"] == p32 and st["polish_passes"] == 0
     assert np.array_equal(_bits32(got), _bits32(O.interior(ref32)))
@@ -116,7 +119,7 @@ def test_f32_converged_bitwise_and_vs_f64(ctx, dev, gen):
     # changes nothing by more than polish_tol = 1e-5 h: within that of the
     # FP64 solution (plus the final FP32 rounding)
     t2 = init.clone()
-    st2 = ctx.solve_f32(t2, init)
+    st2 = ctx.solve_f32(t2, init, storage=S32)
     err2 = np.abs(t2.cpu().numpy().astype(np.float64) - O.interior(ref64))
     print(f"{gen}: polished max={err2.max():.3e} (polish passes {st2['polish_passes']})")
     assert st2["passes"] == p32 and st2["polish_passes"] >= 1
@@ -193,12 +196,12 @@ def test_host_padded_solve_f32_matches_oracle():
     ref32 = a.astype(np.float32)
     O.solve_f32(ref32)
     b = a.copy()
-    st = pdfs_amd.solve(b, precision=pdfs_amd.PDFS_F32, polish=0)
+    st = pdfs_amd.solve(b, precision=pdfs_amd.PDFS_F32, polish=0, storage=S32)
     assert np.array_equal(O.interior(b), O.interior(ref32).astype(np.float64))
     # default: FP32 solve + FP64 polish, the oracle's bits
     want, _, _ = O.solve_f32_polished(a)
     c = a.copy()
-    pdfs_amd.solve(c, precision=pdfs_amd.PDFS_F32)
+    pdfs_amd.solve(c, precision=pdfs_amd.PDFS_F32, storage=S32)
     assert np.array_equal(O.interior(c), O.interior(want).astype(np.float64))
     assert np.all(np.isinf(b[0])) and np.all(np.isinf(b[:, 0])) and np.all(np.isinf(b[:, :, -1]))
     assert st["judge_flagged"] == 0
@@ -234,7 +237,7 @@ def test_batch_uneven_fields_bitwise(ctx, dev, graph, tile):
         wpasses.append(p)
     ts = [_dense(a, dev) for a in fields]
     inits = [t.clone() for t in ts]
-    st, passes, ms = ctx.solve_batch_f32(ts, inits, graph=graph, tile=tile, polish=0)
+    st, passes, ms = ctx.solve_batch_f32(ts, inits, graph=graph, tile=tile, polish=0, storage=S32)
     for i, (t, w) in enumerate(zip(ts, wants)):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w)), i
     assert passes == wpasses
@@ -242,7 +245,7 @@ def test_batch_uneven_fields_bitwise(ctx, dev, graph, tile):
     assert all(m > 0 for m in ms)
     # a second solve through the same (graph-cached) plan state stays exact
     ts2 = [_dense(a, dev) for a in fields]
-    ctx.solve_batch_f32(ts2, None, graph=graph, tile=tile, judge=0)
+    ctx.solve_batch_f32(ts2, None, graph=graph, tile=tile, judge=0, storage=S32)
     for t, w in zip(ts2, wants):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
 
@@ -257,7 +260,7 @@ def test_batch_autotune_same_result(ctx, dev):
         O.solve_f32(r)
         wants.append(O.interior(r))
     ts = [_dense(a, dev) for a in fields]
-    st, passes, ms = ctx.solve_batch_f32(ts, None, tile=pdfs_amd.TILE_AUTO, judge=0)
+    st, passes, ms = ctx.solve_batch_f32(ts, None, tile=pdfs_amd.TILE_AUTO, judge=0, storage=S32)
     assert st["tile"] in pdfs_amd.TILES
     for t, w in zip(ts, wants):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
@@ -279,8 +282,8 @@ def test_certified_stop_fewer_passes_within_tolerance(ctx, dev, gen):
     t = _dense(a, dev)
     init = t.clone()
     plain = t.clone()
-    sp = ctx.solve_f32(plain, init)
-    sc = ctx.solve_f32(t, init, certify=1)
+    sp = ctx.solve_f32(plain, init, storage=S32)
+    sc = ctx.solve_f32(t, init, certify=1, storage=S32)
     got = t.cpu().numpy()
     err = np.abs(got.astype(np.float64) - O.interior(ref64))
     print(f"{gen}: plain {sp['passes']} passes, certified {sc['passes']} "
@@ -301,12 +304,12 @@ def test_certified_batch_matches_single(ctx, dev):
     singles, sp = [], []
     for a in fields:
         t = _dense(a, dev)
-        st = ctx.solve_f32(t, t.clone(), certify=1)
+        st = ctx.solve_f32(t, t.clone(), certify=1, storage=S32)
         singles.append(t.cpu().numpy())
         sp.append(st["passes"])
     ts = [_dense(a, dev) for a in fields]
     inits = [t.clone() for t in ts]
-    st, passes, _ = ctx.solve_batch_f32(ts, inits, certify=1)
+    st, passes, _ = ctx.solve_batch_f32(ts, inits, certify=1, storage=S32)
     assert passes == sp
     for t, w in zip(ts, singles):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
@@ -336,7 +339,7 @@ def test_f32_no_sources_field_unchanged(ctx, dev):
     changed, the judge passes."""
     t = torch.full((20, 24, 28), 1.0e6, dtype=torch.float32, device=dev)
     init = t.clone()
-    st = ctx.solve_f32(t, init)
+    st = ctx.solve_f32(t, init, storage=S32)
     assert st["passes"] == 1 and st["last_changed"] == 0 and st["judge_flagged"] == 0
     assert torch.equal(t, init)
 
@@ -368,7 +371,7 @@ def test_certified_stop_at_half_passes(ctx, dev, gen):
                 break
     assert stop is not None
     f = init.clone()
-    st = ctx.solve_f32(f, init, certify=1, polish=0)
+    st = ctx.solve_f32(f, init, certify=1, polish=0, storage=S32)
     print(f"{gen}: certified at ordering {stop} ({mid} mid-pass judges), passes {st['passes']}")
     assert st["orderings"] == stop and st["passes"] == (stop + 7) // 8
     assert torch.equal(f.view(torch.int32), ref.view(torch.int32))
@@ -416,7 +419,7 @@ def test_converged_field_skips_rest_of_last_pass(ctx, dev, tile):
     assert all(r >= 1 for r in runs[first - 1:])
     assert first > 8 and first % 8 != 0 and passes == (first + 7) // 8 + 1  # mid-pass fixed point
     f = init.clone()
-    st = ctx.solve_f32(f, init, judge=0, tile=tile, polish=0)
+    st = ctx.solve_f32(f, init, judge=0, tile=tile, polish=0, storage=S32)
     assert torch.equal(f.view(torch.int32), ref.view(torch.int32))
     assert st["passes"] == passes
     assert st["orderings"] == first
@@ -426,7 +429,7 @@ def test_converged_field_skips_rest_of_last_pass(ctx, dev, tile):
     # that reached the fixed point. The plain stop's pass after it, which
     # changes nothing, is counted in passes but not launched.
     gs = [init.clone(), init.clone()]
-    sb, pb, _ = ctx.solve_batch_f32(gs, None, judge=0, tile=tile)
+    sb, pb, _ = ctx.solve_batch_f32(gs, None, judge=0, tile=tile, storage=S32)
     assert pb == [passes, passes] and sb["passes"] == passes and sb["last_changed"] == 0
     assert sb["orderings"] == 8 * ((first + 7) // 8)
     for g in gs:
@@ -443,7 +446,7 @@ def test_batch_with_single_point_field(ctx, dev):
         O.solve_f32(r)
         wants.append(O.interior(r))
     ts = [_dense(a, dev) for a in fields]
-    ctx.solve_batch_f32(ts, [t.clone() for t in ts], polish=0)
+    ctx.solve_batch_f32(ts, [t.clone() for t in ts], polish=0, storage=S32)
     for t, w in zip(ts, wants):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w))
 
@@ -460,15 +463,15 @@ def test_tuner_skips_tiles_whose_edges_do_not_fit(ctx, dev):
         ctx.generate(pdfs_amd.GEN_SPHERES, 900 + i, n, n, n, 1.0, dense=t)
         inits.append(t)
     fields = [t.clone() for t in inits]
-    st, passes, _ = ctx.solve_batch_f32(fields, None, tile=pdfs_amd.TILE_AUTO, judge=0)
+    st, passes, _ = ctx.solve_batch_f32(fields, None, tile=pdfs_amd.TILE_AUTO, judge=0, storage=S32)
     assert st["tile"] not in (pdfs_amd.TILE_16x8, pdfs_amd.TILE_16x8_2CU)
     for i in (0, K // 2, K - 1):
         f = inits[i].clone()
-        sp = ctx.solve_f32(f, None, judge=0, tile=pdfs_amd.TILE_32x16)
+        sp = ctx.solve_f32(f, None, judge=0, tile=pdfs_amd.TILE_32x16, storage=S32)
         assert sp["passes"] == passes[i]
         assert torch.equal(f.view(torch.int32), fields[i].view(torch.int32))
     with pytest.raises(pdfs_amd.PdfsError):
-        ctx.solve_batch_f32([t.clone() for t in inits], None, tile=pdfs_amd.TILE_16x8, judge=0)
+        ctx.solve_batch_f32([t.clone() for t in inits], None, tile=pdfs_amd.TILE_16x8, judge=0, storage=S32)
     del inits, fields
     torch.cuda.empty_cache()
 
@@ -483,7 +486,7 @@ def test_largest_f32_field_properties(ctx, dev):
     init = torch.empty((Z, Y, X), dtype=torch.float32, device=dev)
     ctx.generate(pdfs_amd.GEN_SPHERES, 77, X, Y, Z, 1.0, dense=init)
     f = init.clone()
-    st = ctx.solve_f32(f, init, certify=1)
+    st = ctx.solve_f32(f, init, certify=1, storage=S32)
     print(f"largest field: passes {st['passes']} ms {st['ms_total']:.1f}")
     assert st["judge_flagged"] == 0
     # beyond the FP64 column limit (2^28 points): left unpolished, counted
@@ -496,7 +499,7 @@ def test_largest_f32_field_properties(ctx, dev):
     torch.cuda.empty_cache()
     big = torch.empty((2, 1 << 14, 1 << 14), dtype=torch.float32, device=dev)  # 2^29 points
     with pytest.raises(pdfs_amd.PdfsError):
-        ctx.solve_f32(big, None, judge=0)
+        ctx.solve_f32(big, None, judge=0, storage=S32)
 
 
 @pytest.mark.parametrize("shape,h", [((17, 13, 11), (1.0, 1.0, 1.0)),
@@ -566,11 +569,11 @@ def test_certified_batch_retires_fields_mid_pass(ctx, dev):
                         (10.0, 60.0) if i % 2 == 0 else (0.0, 0.0), dense=u)
         inits.append(u)
     fields = [x.clone() for x in inits]
-    st, passes, _ = ctx.solve_batch_f32(fields, inits, certify=1, graph=1)
+    st, passes, _ = ctx.solve_batch_f32(fields, inits, certify=1, graph=1, storage=S32)
     assert st["judge_flagged"] == 0
     for i, x in enumerate(inits):
         f = x.clone()
-        s1 = ctx.solve_f32(f, x, certify=1)
+        s1 = ctx.solve_f32(f, x, certify=1, storage=S32)
         assert passes[i] == s1["passes"], (i, passes, s1)
         assert torch.equal(f.view(torch.int32), fields[i].view(torch.int32)), i
     del inits, fields, t
@@ -591,7 +594,7 @@ def test_two_contexts_in_two_threads(dev):
     want = []
     for t in inits:
         f = t.clone()
-        alone.solve_f32(f, t)
+        alone.solve_f32(f, t, storage=S32)
         want.append(f)
     alone.close()
     results, errors = {}, []
@@ -606,7 +609,7 @@ def test_two_contexts_in_two_threads(dev):
             for k in range(6):
                 f = inits[0].clone()
                 t0 = time.perf_counter()
-                st = c.solve_f32(f, inits[0])
+                st = c.solve_f32(f, inits[0], storage=S32)
                 spans["single"].append((t0, time.perf_counter()))
                 results[("single", k)] = (f, st["judge_flagged"])
         except Exception as e:  # reported by the main thread
@@ -621,7 +624,7 @@ def test_two_contexts_in_two_threads(dev):
             for k in range(6):
                 fs = [inits[1].clone(), inits[2].clone()]
                 t0 = time.perf_counter()
-                st, _, _ = c.solve_batch_f32(fs, [inits[1], inits[2]])
+                st, _, _ = c.solve_batch_f32(fs, [inits[1], inits[2]], storage=S32)
                 spans["batch"].append((t0, time.perf_counter()))
                 results[("batch", k)] = (fs, st["judge_flagged"])
         except Exception as e:
@@ -677,9 +680,9 @@ def test_contexts_release_their_device_memory(dev):
         c = pdfs_amd.Context()
         try:
             f.copy_(big)
-            c.solve_f32(f, big)
+            c.solve_f32(f, big, storage=S32)
             fs = [init.clone(), init.clone()]
-            c.solve_batch_f32(fs, [init, init], certify=1, graph=1)
+            c.solve_batch_f32(fs, [init, init], certify=1, graph=1, storage=S32)
             pad.copy_(pad0)
             c.solve_ref64(pad, passes=1)
             del fs

@@ -22,6 +22,9 @@ import oracle_ctypes as O
 import pdfs_amd
 from pdfs_amd import workloads
 
+# these tests check the FP32-storage path (and its polish) against the FP32 oracle
+S32 = pdfs_amd.PDFS_STORE_F32
+
 pytestmark = pytest.mark.gpu
 
 # max |FP32 result - FP64 solution| in units of h
@@ -52,13 +55,13 @@ def test_ref64_column_tiles_bitwise(tile, h):
         O.sweep_ref64(want1, sn, h)
     got1 = a.copy()
     print(f"ref64 column tile={tile} h={h} one pass", flush=True)
-    pdfs_amd.solve(got1, h=h, precision=pdfs_amd.PDFS_F64_REF, fixed_passes=1, tile64=tile)
+    pdfs_amd.solve(got1, h=h, precision=pdfs_amd.PDFS_F64_REF, fixed_passes=1, tile64=tile, storage=S32)
     assert np.array_equal(want1.view(np.uint64), got1.view(np.uint64))
     want = a.copy()
     p_o, _ = O.solve_ref64(want, h=h, tol=1e-12)
     got = a.copy()
     print(f"ref64 column tile={tile} h={h} converged", flush=True)
-    st = pdfs_amd.solve(got, h=h, precision=pdfs_amd.PDFS_F64_REF, tol=1e-12, tile64=tile)
+    st = pdfs_amd.solve(got, h=h, precision=pdfs_amd.PDFS_F64_REF, tol=1e-12, tile64=tile, storage=S32)
     assert st["passes"] == p_o
     assert np.array_equal(want.view(np.uint64), got.view(np.uint64))
 
@@ -90,7 +93,7 @@ def test_polish_bitwise_vs_oracle(ctx, dev, shape, h):
     want, p32, pp = O.solve_f32_polished(a, h=h)
     t = _dense(a, dev)
     print(f"polish {shape} h={h}", flush=True)
-    st = ctx.solve_f32(t, t.clone(), h=h)
+    st = ctx.solve_f32(t, t.clone(), h=h, storage=S32)
     assert st["passes"] == p32 and st["polish_passes"] == pp, (st["passes"], p32,
                                                                st["polish_passes"], pp)
     assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(O.interior(want)))
@@ -113,7 +116,7 @@ def test_polish_tiles_and_batch_equal_single(ctx, dev, tile):
         wants.append(O.interior(w))
     ts = [_dense(a, dev) for a in fields]
     print(f"polish batch tile64={tile}", flush=True)
-    st, passes, _ = ctx.solve_batch_f32(ts, [t.clone() for t in ts], tile64=tile)
+    st, passes, _ = ctx.solve_batch_f32(ts, [t.clone() for t in ts], tile64=tile, storage=S32)
     for i, (t, w) in enumerate(zip(ts, wants)):
         assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(w)), i
     assert st["judge_flagged"] == 0 and st["polish_passes"] >= 1
@@ -132,7 +135,7 @@ def test_polish_raises_values_below_the_solution(ctx, dev):
     init = _dense(a, dev)
     # FP32 min(u, G) passes cannot raise the planted region (they lower the
     # points downwind of it instead); the polish lifts all of them back
-    st = ctx.solve_f32(t, init)
+    st = ctx.solve_f32(t, init, storage=S32)
     got = t.cpu().numpy().astype(np.float64)
     err = np.abs(got - O.interior(u64)).max()
     print(f"planted region: polish passes {st['polish_passes']} max err {err:.3e}", flush=True)
@@ -158,17 +161,17 @@ def _modes_within_tolerance(ctx, dev, init, u64, label):
         f = init.clone()
         print(f"{label}: {mode}", flush=True)
         if mode == "raw":
-            st = ctx.solve_f32(f, init, polish=0)
+            st = ctx.solve_f32(f, init, polish=0, storage=S32)
         elif mode == "plain":
-            st = ctx.solve_f32(f, init)
+            st = ctx.solve_f32(f, init, storage=S32)
         elif mode == "certified":
-            st = ctx.solve_f32(f, init, certify=1)
+            st = ctx.solve_f32(f, init, certify=1, storage=S32)
         else:
             if not os.path.exists(pdfs_amd.WEIGHTS_PATH):
                 continue
             net = pdfs_amd.Net(ctx, path=pdfs_amd.WEIGHTS_PATH)
             try:
-                st = ctx.solve_warm_f32(net, f, init)
+                st = ctx.solve_warm_f32(net, f, init, storage=S32)
             finally:
                 net.close()
             assert st["fallbacks"] == 0
@@ -224,7 +227,7 @@ def test_c3_terrain_warm_start_bound_and_tolerance(ctx, dev):
         net.warm_start(init, w)
         below = int((w.double() < u64 - 1e-3).sum())
         f = init.clone()
-        st = ctx.solve_warm_f32(net, f, init)
+        st = ctx.solve_warm_f32(net, f, init, storage=S32)
     finally:
         net.close()
     emax, erms = _err(f, u64)
@@ -253,10 +256,10 @@ def test_c4_batch_equals_single_solves(ctx, dev, mode):
         print(f"c4 {mode}: batch", flush=True)
         if mode == "warm":
             st, passes, fb = ctx.solve_batch_warm_f32(net, fields, inits, graph=1,
-                                                      tile=pdfs_amd.TILE_AUTO)
+                                                      tile=pdfs_amd.TILE_AUTO, storage=S32)
         else:
             st, passes, _ = ctx.solve_batch_f32(fields, inits, graph=1, tile=pdfs_amd.TILE_AUTO,
-                                                certify=int(mode == "certified"))
+                                                certify=int(mode == "certified"), storage=S32)
             fb = [0] * len(fields)
         print(f"c4 {mode}: passes max {max(passes)}, fallbacks {sum(fb)}, polish passes "
               f"{st['polish_passes']}", flush=True)
@@ -264,10 +267,10 @@ def test_c4_batch_equals_single_solves(ctx, dev, mode):
         for i, (t, f) in enumerate(zip(inits, fields)):
             g = t.clone()
             if mode == "warm":
-                s1 = ctx.solve_warm_f32(net, g, t)
+                s1 = ctx.solve_warm_f32(net, g, t, storage=S32)
                 assert s1["fallbacks"] == fb[i], i
             else:
-                s1 = ctx.solve_f32(g, t, certify=int(mode == "certified"))
+                s1 = ctx.solve_f32(g, t, certify=int(mode == "certified"), storage=S32)
             assert s1["passes"] == passes[i], (i, s1["passes"], passes[i])
             assert torch.equal(g.view(torch.int32), f.view(torch.int32)), (i, specs[i][:2])
     finally:

@@ -22,6 +22,9 @@ import pdfs_amd
 
 pytestmark = pytest.mark.gpu
 
+# these tests check the FP32-storage path (and its polish) against the FP32 oracle
+S32 = pdfs_amd.PDFS_STORE_F32
+
 F32_VS_F64_TOL = 1e-4
 SPACINGS = [(1.0, 1.0, 1.0), (0.5, 0.7, 1.3)]
 SPEED_TILES = tuple(t for t in pdfs_amd.TILES if t != pdfs_amd.TILE_32x16)
@@ -80,7 +83,7 @@ def test_speed_unit_and_doubled_slowness(ctx, dev, h):
         t1, t2 = _dense(a, dev), _dense(a, dev)
         print(f"speed s={sc} h={h}", flush=True)
         ctx.solve_speed_f32(t1, torch.full_like(t1, sc), h=h, polish=0)
-        ctx.solve_f32(t2, h=tuple(sc * v for v in h), polish=0)
+        ctx.solve_f32(t2, h=tuple(sc * v for v in h), polish=0, storage=S32)
         assert torch.equal(t1.view(torch.int32), t2.view(torch.int32)), sc
 
 

@@ -4,14 +4,15 @@ sweep kernel's achieved HBM bandwidth against the 8 TB/s peak.
 Workload (the `value`): multi-sphere distance field, 512^3, h = 1, 64 seeded
 spheres of radius U(4,32) (interior 0, shell within 1.5h exact distance,
 rest 1e6), generated on the GPU. A step = reset the device field from the
-resident init (device copy) + one converged plain FSM solve (FP32 passes of
-the 8 orderings until a pass changes no value, then the FP64 polish pass that
-brings the result within 1e-4 h of the FP64 solution) + the FP64 residual
-judge.
+resident init (device copy) + one converged plain FSM solve (the default:
+passes of the 8 orderings over the field held in 32-bit fixed point, FP32
+local solves, until a pass changes no value; within 1e-4 h of the FP64
+solution with no polish) + the FP64 residual judge.
 
-Also reported (in "extra"): the certified stop (passes until the FP64 judge
-passes) and the learned warm start on the same field and on a fresh
-256x256x128 terrain field; a batch of 512^3 sphere fields (this field and
+Also reported (in "extra"): FP32 storage (its fixed point alone, and with
+the FP64 polish that brings it within the bound); the certified stop (passes
+until the FP64 judge passes) and the learned warm start on the same field and
+on a fresh 256x256x128 terrain field; a batch of 512^3 sphere fields (this field and
 the next seeds) in shared persistent launches, plain, certified and
 warm-started, against the single-field plain rate; the 512-field uneven batch
 solved in shared persistent launches (plain, certified, warm-started) against
@@ -135,6 +136,8 @@ def main():
     result = field.clone()
 
     extra = {"passes": st["passes"], "orderings": st["orderings"],
+             "storage": "fixed point" if st["storage"] == pdfs_amd.PDFS_STORE_FIXED else "fp32",
+             "ms_convert": round(pst["ms_convert"], 3),
              "polish_passes": st["polish_passes"], "polish_orderings": st["polish_orderings"],
              "ms_polish": round(pst["ms_polish"], 3), "ms_fp32_sweeps": round(pst["ms_sweeps"], 3),
              "judge_flagged": int(st["judge_flagged"]), "ms_judge": round(pst["ms_judge"], 4),
@@ -167,7 +170,7 @@ def main():
         extra["ref64_passes"] = ref_passes
         extra["ref64_seconds"] = round(t64, 4)
         extra["ref64_ms_per_ordering"] = round(rst["ms_sweeps"] / max(1, rst["orderings"]), 4)
-        log(f"[bench] FP32 (+ polish) vs FP64: max {extra['f32_vs_f64_max_err']:.3e} "
+        log(f"[bench] result vs FP64: max {extra['f32_vs_f64_max_err']:.3e} "
             f"rms {extra['f32_vs_f64_rms_err']:.3e}")
         del err
     net = None
@@ -220,30 +223,35 @@ def main():
         "dtype": "f32",
         "data": "synthetic (GPU generator, seeded)",
         "config": {"workload": f"plain FSM, multi-sphere distance {n}^3, h=1, 64 spheres r~U(4,32), "
-                               "FP32 until a pass changes nothing, then the FP64 polish pass, "
-                               "+ FP64 judge",
-                   "grid": [X, Y, Z], "seed": args.seed, "solver": "plain", "tile": st["tile"]},
+                               "FP32 local solves on the field held in 32-bit fixed point until a "
+                               "pass changes nothing, + FP64 judge",
+                   "grid": [X, Y, Z], "seed": args.seed, "solver": "plain", "tile": st["tile"],
+                   "storage": extra["storage"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(),
-                     "kernel": "k_sweep_col (FP32, mode 0)",
+                     "kernel": ("k_sweep_col (fixed point, mode 7)"
+                                if st["storage"] == pdfs_amd.PDFS_STORE_FIXED
+                                else "k_sweep_col (FP32, mode 0)"),
                      "avg_launch_ms": round(avg_launch_ms, 4),
                      "algorithmic_bytes_per_launch": round(algo_bytes),
                      "launches": launches, "orderings_swept": pst["orderings"]},
         "cpu_baseline": cpu,
         "extra": extra,
     }
-    # the other kernel of the solve: the FP64 polish sweep (mode 3)
-    out["extra"]["roofline_polish"] = {
-        "bound": "hbm", "achieved": round(pol_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(pol_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("polish"),
-        "kernel": "k_sweep_col (FP64 polish, mode 3)",
-        "avg_launch_ms": round(pol_ms, 4), "algorithmic_bytes_per_launch": round(pol_bytes),
-        "launches": pst["polish_orderings"], "ms_total": round(pst["ms_polish_kernels"], 3)}
-    # the dominant kernel by time carries the top-level roofline
-    if pst["ms_polish_kernels"] > pst["ms_sweep_kernels"]:
-        out["roofline"], out["extra"]["roofline_polish"] = (
-            out["extra"]["roofline_polish"], out["roofline"])
+    # the other kernel of a solve with FP32 storage: the FP64 polish sweep
+    # (mode 3); the fixed-point default runs no polish
+    if pst["polish_orderings"] > 0:
+        out["extra"]["roofline_polish"] = {
+            "bound": "hbm", "achieved": round(pol_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(pol_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("polish"),
+            "kernel": "k_sweep_col (FP64 polish, mode 3)",
+            "avg_launch_ms": round(pol_ms, 4), "algorithmic_bytes_per_launch": round(pol_bytes),
+            "launches": pst["polish_orderings"], "ms_total": round(pst["ms_polish_kernels"], 3)}
+        # the dominant kernel by time carries the top-level roofline
+        if pst["ms_polish_kernels"] > pst["ms_sweep_kernels"]:
+            out["roofline"], out["extra"]["roofline_polish"] = (
+                out["extra"]["roofline_polish"], out["roofline"])
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -345,6 +353,8 @@ def solve_modes(ctx, net, init, field, tile, plain_ms, u64=None):
     judge result, fallbacks (best of 3 for the timed modes), and with the
     FP64 solution u64 each mode's max / RMS error against it (in h)."""
     import torch
+
+    import pdfs_amd
     out = {}
 
     def err(d):
@@ -373,15 +383,23 @@ def solve_modes(ctx, net, init, field, tile, plain_ms, u64=None):
     else:
         field.copy_(init)
         sp = ctx.solve_f32(field, init, tile=tile)
-    out["plain"] = err({"passes": sp["passes"], "ms": round(plain_ms, 3),
-                        "polish_passes": sp["polish_passes"], "ms_polish": round(sp["ms_polish"], 3),
+    S32 = pdfs_amd.PDFS_STORE_F32
+    out["plain"] = err({"passes": sp["passes"], "orderings": sp["orderings"],
+                        "ms": round(plain_ms, 3), "ms_convert": round(sp["ms_convert"], 3),
                         "judge_flagged": int(sp["judge_flagged"])})
-    rms, sr = timed(lambda: ctx.solve_f32(field, init, tile=tile, polish=0))
+    pms, sq = timed(lambda: ctx.solve_f32(field, init, tile=tile, storage=S32))
+    out["plain_fp32_storage_polished"] = err(
+        {"passes": sq["passes"], "orderings": sq["orderings"], "ms": round(pms, 3),
+         "polish_passes": sq["polish_passes"], "ms_polish": round(sq["ms_polish"], 3),
+         "judge_flagged": int(sq["judge_flagged"]),
+         "note": "FP32 storage, then the FP64 polish"})
+    rms, sr = timed(lambda: ctx.solve_f32(field, init, tile=tile, storage=S32, polish=0))
     out["plain_fp32_only"] = err({"passes": sr["passes"], "ms": round(rms, 3),
                                   "judge_flagged": int(sr["judge_flagged"]),
-                                  "note": "the FP32 fixed point without the polish"})
+                                  "note": "the FP32-storage fixed point without the polish"})
     cms, sc = timed(lambda: ctx.solve_f32(field, init, tile=tile, certify=1))
-    out["certified"] = err({"passes": sc["passes"], "ms": round(cms, 3),
+    out["certified"] = err({"passes": sc["passes"], "orderings": sc["orderings"],
+                            "ms": round(cms, 3),
                             "judge_runs": sc["judge_runs"], "polish_passes": sc["polish_passes"],
                             "judge_flagged": int(sc["judge_flagged"]),
                             "speedup_vs_plain": round(plain_ms / cms, 3)})
@@ -501,11 +519,18 @@ def parity_leg(ctx, args):
         r64 = ctx.solve_ref64(pad, passes=0, tol=1e-12)
         t64 = time.perf_counter() - t
         init = dense.clone()
-        s32 = ctx.solve_f32(dense, init, judge=0, polish=0)
+        S32 = pdfs_amd.PDFS_STORE_F32
+        s32 = ctx.solve_f32(dense, init, judge=0, polish=0, storage=S32)
         err = (dense.double() - pad[1:-1, 1:-1, 1:-1]).abs()
         dense.copy_(init)
-        ctx.solve_f32(dense, init, judge=0)
+        ctx.solve_f32(dense, init, judge=0, storage=S32)
         perr = float((dense.double() - pad[1:-1, 1:-1, 1:-1]).abs().max())
+        dense.copy_(init)
+        ctx.solve_f32(dense, init, judge=0)
+        ferr = (dense.double() - pad[1:-1, 1:-1, 1:-1]).abs()
+        perr = {"f32_polished_vs_f64_max_err": perr,
+                "fixed_vs_f64_max_err": float(ferr.max()),
+                "fixed_vs_f64_rms_err": float(ferr.pow(2).mean().sqrt())}
         return pad, r64, t64, s32, float(err.max()), float(err.pow(2).mean().sqrt()), perr
 
     X = Y = Z = 64
@@ -514,7 +539,7 @@ def parity_leg(ctx, args):
     c0 = {"grid": [X, Y, Z], "seed": 12345, "gpu_f64_passes": r64["passes"],
           "gpu_f64_seconds": round(t64, 4), "sha256": hashlib.sha256(host.tobytes()).hexdigest(),
           "f32_passes": s32["passes"], "f32_only_vs_f64_max_err": emax,
-          "f32_only_vs_f64_rms_err": erms, "f32_polished_vs_f64_max_err": perr}
+          "f32_only_vs_f64_rms_err": erms, **perr}
     so = os.path.join(ROOT, "oracle", "_ref", "libpdfs_ref.so")
     if os.path.exists(so):
         ref = torch.empty_like(pad)
@@ -530,8 +555,7 @@ def parity_leg(ctx, args):
                                                  (5.0, 40.0), (10.0, 60.0))
     out["c1"] = {"grid": [X, Y, Z], "seed": 777, "gpu_f64_passes": r64["passes"],
                  "gpu_f64_seconds": round(t64, 3), "f32_passes": s32["passes"],
-                 "f32_only_vs_f64_max_err": emax, "f32_only_vs_f64_rms_err": erms,
-                 "f32_polished_vs_f64_max_err": perr}
+                 "f32_only_vs_f64_max_err": emax, "f32_only_vs_f64_rms_err": erms, **perr}
     if os.path.exists(so) and not args.no_cpu:
         # the reference core on one host core, two passes of the same init
         # (the converged solve takes about a minute there), against two

@@ -114,6 +114,9 @@ typedef struct {
                       other solves and for a start field holding -inf. */
   double judge_tol;/* fixed-point storage: judge tolerance in units of h (default 2e-6; the
                       certified stop at 512^3 leaves about 10x this much error) */
+  int stop_quanta; /* fixed-point storage, plain stop: a field ends on a pass that changes no
+                      value by more than this many quanta (default 4; 0: a pass that changes
+                      nothing, the FP32 storage rule) */
 } pdfs_opts;
 
 typedef struct {

@@ -818,8 +818,9 @@ uint32_t orc_godunov_fx(uint32_t cur, uint32_t mx, uint32_t my, uint32_t mz, int
   return cur < u ? cur : u;
 }
 
-long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn)
+long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn, uint32_t *maxdrop)
 {
+  uint32_t md = 0;
   int sx, sy, sz;
   order_dirs(sn, &sx, &sy, &sz);
   size_t extX = (size_t)X + 2, extXY = extX * (size_t)(Y + 2);
@@ -836,22 +837,30 @@ long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn)
         uint32_t mz = ORC_MIN(q[id - extXY], q[id + extXY]);
         uint32_t u = orc_godunov_fx(q[id], mx, my, mz, k);
         changed += u != q[id];
+        if (q[id] - u > md) md = q[id] - u;
         q[id] = u;
       }
     }
   }
+  if (maxdrop) *maxdrop = md;
   return changed;
 }
 
-int orc_solve_fx(uint32_t *q, int X, int Y, int Z, int k, int max_passes, long *last_changed)
+int orc_solve_fx(uint32_t *q, int X, int Y, int Z, int k, int stop_quanta, int max_passes,
+                 long *last_changed)
 {
   int passes = 0;
   long changed = 0;
   while (passes < max_passes) {
     changed = 0;
-    for (int sn = 1; sn <= 8; sn++) changed += orc_sweep_fx(q, X, Y, Z, k, sn);
+    uint32_t pmd = 0;
+    for (int sn = 1; sn <= 8; sn++) {
+      uint32_t md;
+      changed += orc_sweep_fx(q, X, Y, Z, k, sn, &md);
+      if (md > pmd) pmd = md;
+    }
     passes++;
-    if (changed == 0) break;
+    if (changed == 0 || pmd <= (uint32_t)stop_quanta) break;
   }
   if (last_changed) *last_changed = changed;
   return passes;

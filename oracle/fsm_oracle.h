@@ -41,12 +41,13 @@
  *                        update fails this); point-source axis values n h.
  *   orc_godunov_fx / orc_sweep_fx / orc_solve_fx (fixed-point storage)
  *                        1-D updates exact (axis values n 2^k quanta),
- *                        2-D / 3-D equal-minima roots the nearest quantum to
- *                        h/sqrt2, h/sqrt3, general roots within 1.5 quanta
- *                        of an independent root finder, +inf saturation;
+ *                        2-D / 3-D equal-minima roots h/sqrt2, h/sqrt3 to
+ *                        half a quantum plus the FP32 rounding of the root
+ *                        offset, general roots within 1.5 quanta (plus that
+ *                        rounding) of an independent root finder, +inf
+ *                        saturation; shift of the start field;
  *                        converged fields within 5e-5 h of the FP64 solve
- *                        (orc_solve_d64); base-shift invariance; the order
- *                        key and encode / decode against numpy.
+ *                        (orc_solve_d64); encode / decode against numpy.
  *   orc_judge_fx         a converged fixed-point field passes, planted
  *                        defects of a few quanta beyond the tolerance fail.
  */
@@ -158,10 +159,13 @@ void orc_fx_encode(const float *v, uint32_t *q, size_t n, double base, double d)
 void orc_fx_decode(const uint32_t *q, float *v, size_t n, double base, double d);
 /* local solve on quanta: Hq = 2^k */
 uint32_t orc_godunov_fx(uint32_t cur, uint32_t mx, uint32_t my, uint32_t mz, int k);
-/* one ordering in place; returns the points changed */
-long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn);
-/* passes until a pass changes nothing (or max_passes); returns passes run */
-int orc_solve_fx(uint32_t *q, int X, int Y, int Z, int k, int max_passes, long *last_changed);
+/* one ordering in place; returns the points changed and (maxdrop, or NULL)
+ * the largest drop old - new of an update, in quanta */
+long orc_sweep_fx(uint32_t *q, int X, int Y, int Z, int k, int sn, uint32_t *maxdrop);
+/* passes until a pass changes nothing or none of its drops exceeds
+ * stop_quanta (or max_passes); returns passes run */
+int orc_solve_fx(uint32_t *q, int X, int Y, int Z, int k, int stop_quanta, int max_passes,
+                 long *last_changed);
 /* FP64 residual judge of a fixed-point field (spacing h): a point fails when
  * |value - min(init, G64(value))| > tol; returns the failing points. */
 long orc_judge_fx(const uint32_t *q, const float *init, int X, int Y, int Z, double h,

@@ -125,7 +125,7 @@ class Opts(C.Structure):
                 ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int),
                 ("certify", C.c_int), ("certify_from", C.c_int), ("polish", C.c_int),
                 ("polish_tol", C.c_double), ("tile64", C.c_int), ("storage", C.c_int),
-                ("judge_tol", C.c_double)]
+                ("judge_tol", C.c_double), ("stop_quanta", C.c_int)]
 
 
 class Stats(C.Structure):

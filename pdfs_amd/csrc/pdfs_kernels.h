@@ -112,7 +112,8 @@ typedef struct {
   unsigned *streak;      /* [nfields][2]: counter value after the last ordering, and the
                             number of consecutive orderings that changed nothing */
   int nfields;
-  unsigned long long *maxdiff; /* [nfields] FP64 modes: max |old - new| bits (accumulate) */
+  unsigned long long *maxdiff; /* [nfields] FP64 modes: max |old - new| bits; fixed point: the
+                                  largest drop in quanta, as an integer (accumulate) */
   unsigned long long *maxdiff_last; /* [nfields] or NULL: ordering 8 accumulates here instead */
   unsigned *err;         /* bounded-spin timeout flag */
   uint64_t *edge_x, *edge_y;        /* face parts (index -PDFSK_EDGE_PAD) of one allocation

@@ -864,6 +864,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       T zm = INF;
       unsigned nchg = 0;
       T md = (T)0;  // FP64 modes: largest |old - new| of this thread's updates
+      uint32_t mdq = 0u;  // fixed point: largest drop (quanta) of this thread's updates
       // uniform slots: plane l+3 is written at step l, plane l-(TX+TY-2) is
       // stored at step l
       int slot_w = 3 % R, slot_s = (R - (TX + TY - 2) % R) % R;
@@ -963,6 +964,11 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
           md = (act && u != c && dd > md) ? dd : md;
         }
         nchg += (act && u != c) ? 1u : 0u;
+        if constexpr (S::FX) {
+          // u <= c (a min update); lanes off the field's z range are masked
+          const uint32_t dq = act ? (uint32_t)(c - u) : 0u;
+          mdq = dq > mdq ? dq : mdq;
+        }
         // lanes outside the field keep their cells at +inf (the ghost
         // values their neighbours read); inactive lanes inside it write a
         // value no active lane reads before the slot is refilled
@@ -1015,6 +1021,15 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
           b = t > b ? t : b;
         }
         if ((tid & 63) == 0 && b) atomicMax(a.maxdiff + fi, b);
+      }
+      if constexpr (S::FX) {
+        // wave max of the drop in quanta (an integer in the 64-bit slot)
+        uint32_t b = mdq;
+        for (int o = 32; o > 0; o >>= 1) {
+          const uint32_t t = (uint32_t)__shfl_xor((int)b, o);
+          b = t > b ? t : b;
+        }
+        if ((tid & 63) == 0 && b) atomicMax(a.maxdiff + fi, (unsigned long long)b);
       }
     }
     __syncthreads();
@@ -1706,7 +1721,7 @@ extern "C" int pdfsk_col_max_resident(int variant, int mode) {
 extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *sp,
                                  const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st) {
   if (sn < 1 || sn > 8 || grid < 1 || !pdfsk_tile_supports(variant, mode)) return -1;
-  if (PDFSK_MODE_IS_F64(mode) && !b->maxdiff) return -1;
+  if ((PDFSK_MODE_IS_F64(mode) || mode == PDFSK_MODE_FX_ISO) && !b->maxdiff) return -1;
   static_assert(sizeof(FieldDesc) == sizeof(pdfsk_field), "field table layout");
   // Ordering sn as (+,+,+) in virtual coordinates: physical direction per
   // axis (mpi_openacc/src/phi3D_fsm.c:206-218 reflections x{4,8} y{2,6}

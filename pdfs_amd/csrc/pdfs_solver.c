@@ -145,6 +145,7 @@ void pdfs_default_opts(pdfs_opts *o)
   o->tile64 = -1;
   o->storage = PDFS_STORE_FIXED;
   o->judge_tol = 2e-6;
+  o->stop_quanta = 4;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -510,7 +511,9 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     memset(c->prev_changed, 0, (size_t)nf * sizeof(unsigned));
   }
   c->bt.maxdiff_last = c->bt.maxdiff + c->cap_md;
-  KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)4 * c->cap_md, c->st));
+  /* a re-tile mid-pass keeps the pass's largest changes as it keeps the
+   * change counters */
+  if (!keep) KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)4 * c->cap_md, c->st));
   c->bt.nfields = nf;
   for (int g = 0; g < 3; g++)
     if (c->gexec[g]) {
@@ -585,8 +588,18 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
 /* Per-field points changed since the last read; returns the timeout flag.
  * maxdiff (FP64 plans, or NULL): per-field largest change since the last
  * read, and last (or NULL) that of ordering 8 alone; both are then reset. */
+static int read_changed_md3(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff,
+                            double *last, int as_int);
 static int read_changed_md2(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff,
                             double *last)
+{
+  return read_changed_md3(c, delta, err, maxdiff, last, 0);
+}
+
+/* as_int: the maxdiff slots hold integers (fixed point: quanta), not FP64
+ * bits */
+static int read_changed_md3(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff,
+                            double *last, int as_int)
 {
   HIPCHK(hipMemcpyAsync(c->h_changed, c->bt.changed, (size_t)c->nf * sizeof(unsigned),
                         hipMemcpyDeviceToHost, c->st));
@@ -608,8 +621,13 @@ static int read_changed_md2(pdfs_ctx *c, long long *delta, unsigned *err, double
     c->prev_changed[f] = c->h_changed[f];
     if (maxdiff) {
       double a, b;
-      memcpy(&a, &c->h_maxdiff[f], sizeof(double));
-      memcpy(&b, &c->h_maxdiff[c->cap_md + f], sizeof(double));
+      if (as_int) {
+        a = (double)c->h_maxdiff[f];
+        b = (double)c->h_maxdiff[c->cap_md + f];
+      } else {
+        memcpy(&a, &c->h_maxdiff[f], sizeof(double));
+        memcpy(&b, &c->h_maxdiff[c->cap_md + f], sizeof(double));
+      }
       maxdiff[f] = b > a ? b : a;
       if (last) last[f] = b;
     }
@@ -1237,7 +1255,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   /* fixed-point storage: per-field quantum exponent, base, quantum and judge
    * tolerance; [3 nf ..]: the same three for the fields judged together */
   int *fxk = (int *)malloc((size_t)nf * sizeof(int));
-  double *fxv = (double *)malloc((size_t)6 * nf * sizeof(double));
+  double *fxv = (double *)malloc((size_t)8 * nf * sizeof(double));
   if (!delta || !pdel || !lastfl || !active || !certified || !jidx || !jdone || !jd || !jv ||
       !ji || !js || !jfl || !fxk || !fxv) {
     free(fxk);
@@ -1261,6 +1279,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   int fixed = 0, decoded = 0;
   double *fxb = fxv, *fxd = fxv + nf, *fxt = fxv + 2 * nf;
   double *jfb = fxv + 3 * nf, *jfd = fxv + 4 * nf, *jft = fxv + 5 * nf;
+  /* fixed point: the largest drop (quanta) of the current pass, per field,
+   * and of the last segment read back */
+  double *pmq = fxv + 6 * nf, *fmd = fxv + 7 * nf;
   if (o->storage == PDFS_STORE_FIXED && !speed && o->fixed_passes <= 0 && sp.iso) {
     HIPGO(hipEventRecord(c->kev[16], c->st));
     if ((rc = fx_encode(c, nf, fields, dims, dx, fxk, fxb, fxd, &fixed))) goto out;
@@ -1305,6 +1326,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
                lastfl[f] * 1024 <= (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
     const int nseg = halves ? 2 : 1;
     for (int f = 0; f < nf; f++) pdel[f] = 0;
+    for (int f = 0; f < nf; f++) pmq[f] = 0.0;
     for (int g = 0; g < nseg && nactive > 0 && rc == PDFS_OK; g++) {
       const int sn0 = halves ? 1 + 4 * g : 1, sn1 = halves ? 4 + 4 * g : 8;
       const int nord = sn1 - sn0 + 1, last_seg = g == nseg - 1;
@@ -1347,7 +1369,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
       }
       HIPGO(hipEventRecord(c->ev1, c->st));
       unsigned err = 0;
-      if ((rc = read_changed(c, delta, &err))) break;
+      if ((rc = fixed ? read_changed_md3(c, delta, &err, fmd, NULL, 1)
+                      : read_changed(c, delta, &err)))
+        break;
       float t = 0.f;
       hipEventElapsedTime(&t, c->ev0, c->ev1);
       ms = t;
@@ -1374,9 +1398,13 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
         jdone[f] = 0;
         if (!active[f]) continue;
         pdel[f] += delta[f];
+        if (fixed && fmd[f] > pmq[f]) pmq[f] = fmd[f];
         last_total += pdel[f];
         if (field_passes) field_passes[f] = p + 1;
-        jdone[f] = last_seg && o->fixed_passes <= 0 && pdel[f] == 0;
+        /* plain stop: a pass that changed nothing, or in fixed point none of
+         * its values by more than stop_quanta quanta */
+        jdone[f] = last_seg && o->fixed_passes <= 0 &&
+                   (pdel[f] == 0 || (fixed && pmq[f] <= (double)o->stop_quanta));
         {
           /* the device skipped the orderings of this segment that followed
            * the field's PDFSK_CONVERGED_RUN-th consecutive unchanged one */

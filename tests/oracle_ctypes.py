@@ -95,9 +95,9 @@ def lib():
         L.orc_fx_decode.argtypes = [_up, _fp, C.c_size_t, _D, _D]
         L.orc_godunov_fx.argtypes = [C.c_uint32] * 4 + [_I]
         L.orc_godunov_fx.restype = C.c_uint32
-        L.orc_sweep_fx.argtypes = [_up, _I, _I, _I, _I, _I]
+        L.orc_sweep_fx.argtypes = [_up, _I, _I, _I, _I, _I, C.c_void_p]
         L.orc_sweep_fx.restype = C.c_long
-        L.orc_solve_fx.argtypes = [_up, _I, _I, _I, _I, _I, C.POINTER(C.c_long)]
+        L.orc_solve_fx.argtypes = [_up, _I, _I, _I, _I, _I, _I, C.POINTER(C.c_long)]
         L.orc_solve_fx.restype = _I
         L.orc_judge_fx.argtypes = [_up, _fp, _I, _I, _I, _D, _D, _D, _D, C.c_void_p]
         L.orc_judge_fx.restype = C.c_long
@@ -358,26 +358,29 @@ def fx_padded(q_interior):
     return a
 
 
-def sweep_fx(q, sn, k):
+def sweep_fx(q, sn, k, want_drop=False):
     """One fixed-point ordering on a padded uint32 array, in place; returns
-    the points changed."""
-    return lib().orc_sweep_fx(q, *dims(q), k, sn)
+    the points changed (and the largest drop in quanta)."""
+    md = C.c_uint32()
+    n = lib().orc_sweep_fx(q, *dims(q), k, sn, C.addressof(md))
+    return (n, md.value) if want_drop else n
 
 
-def solve_fx(q, k, max_passes=100):
+def solve_fx(q, k, max_passes=100, stop_quanta=4):
     ch = C.c_long()
-    p = lib().orc_solve_fx(q, *dims(q), k, max_passes, C.byref(ch))
+    p = lib().orc_solve_fx(q, *dims(q), k, stop_quanta, max_passes, C.byref(ch))
     return p, ch.value
 
 
-def solve_fixed(start, h=1.0, max_passes=100):
+def solve_fixed(start, h=1.0, max_passes=100, stop_quanta=4):
     """The GPU's fixed-point solve of a dense FP32 start field [Z, Y, X]:
-    base, quantum, encode, passes until one changes nothing, decode. Returns
-    (FP32 result, passes, (k, base, d), padded uint32 result)."""
+    base, quantum, encode, passes until one changes no value by more than
+    stop_quanta quanta, decode. Returns (FP32 result, passes, (k, base, d),
+    padded uint32 result)."""
     start = np.ascontiguousarray(start, dtype=np.float32)
     k, b, d = fx_params(start, h)
     q = fx_padded(fx_encode(start, b, d))
-    p, _ = solve_fx(q, k, max_passes)
+    p, _ = solve_fx(q, k, max_passes, stop_quanta)
     return fx_decode(interior(q).copy(), b, d), p, (k, b, d), q
 
 

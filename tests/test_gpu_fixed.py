@@ -65,6 +65,22 @@ def test_fixed_converged_bitwise_vs_oracle(ctx, dev, shape, h, kind):
     assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(want))
 
 
+@pytest.mark.parametrize("stop", [0, 1, 4, 50])
+def test_fixed_stop_rule_matches_oracle(ctx, dev, stop):
+    """The plain stop (a pass whose largest drop is at most stop_quanta
+    quanta; 0: a pass that changes nothing) ends on the oracle's pass, with
+    its bits, on fields that need several passes."""
+    for shape, kind in (((33, 40, 20), "spheres"), ((64, 64, 64), "random"),
+                        ((90, 20, 70), "spheres")):
+        start = _start(kind, *shape, 3 + stop)
+        want, p_or, _, _ = O.solve_fixed(start, stop_quanta=stop)
+        t = _dev(start, dev)
+        st = ctx.solve_f32(t, None, judge=0, stop_quanta=stop)
+        print(f"stop {stop} {kind} {shape}: passes gpu {st['passes']} oracle {p_or}", flush=True)
+        assert st["passes"] == p_or, (shape, st["passes"], p_or)
+        assert np.array_equal(_bits32(t.cpu().numpy()), _bits32(want)), shape
+
+
 @pytest.mark.parametrize("tile", pdfs_amd.TILES)
 def test_fixed_per_pass_bitwise_every_tile(ctx, dev, tile):
     """One and two passes (capped solves) on a ragged random field equal the

@@ -644,3 +644,142 @@ def test_speed_judge_uses_the_slowness():
     w = v.copy()
     w[7, 6, 9] -= np.float32(1e-3)
     assert O.judge_f32_speed(w, init, s32, h) >= 1
+
+
+# ------------------------------------------------------- fixed-point storage
+FXI = 0xFFFFFFFF
+
+
+def test_fx_bits_range_bound():
+    """k = 32 - ceil(log2(X + Y + Z)): the largest converged offset, at most
+    (X + Y + Z - 3) h, stays below 2^32 - 1 quanta, and one more bit would
+    not (the quantum is the finest that always fits)."""
+    for X, Y, Z in [(1, 1, 1), (5, 4, 3), (64, 64, 64), (256, 256, 128), (512, 512, 512),
+                    (812, 812, 812), (1000, 1, 1), (31, 1, 1), (32, 1, 1)]:
+        k = O.fx_bits(X, Y, Z)
+        s = X + Y + Z
+        assert (s - 3) * 2 ** k < FXI
+        assert s * 2 ** (k + 1) > 2 ** 32
+    assert O.fx_bits(512, 512, 512) == 21 and O.fx_bits(256, 256, 128) == 22
+
+
+@pytest.mark.parametrize("k", [21, 24, 30])
+def test_godunov_fx_closed_forms(k):
+    """1-D updates are exact (m + 2^k); equal minima in 2-D / 3-D give
+    m + h/sqrt2, m + h/sqrt3 (h = 2^k quanta) to half a quantum plus the FP32
+    rounding of the root offset (the nearest quantum up to k = 23); +inf
+    neighbours and saturation."""
+    g = O.lib().orc_godunov_fx
+    H = 2 ** k
+    for m in (0, 12345, 3 * 2 ** 29):
+        if m + 2 * H >= FXI:
+            continue
+        assert g(FXI, m, FXI, FXI, k) == m + H
+        for n in (2, 3):
+            t = H / math.sqrt(n)
+            got = g(FXI, m, m, FXI if n == 2 else m, k) - m
+            assert abs(got - t) <= 0.5 + 2 * t * 2.0 ** -24, (k, n, got, t)
+            if k <= 23:
+                assert got == round(t)
+        # cur below the candidate wins
+        assert g(m, m, m, m, k) == m
+    assert g(77, FXI, FXI, FXI, k) == 77          # no finite neighbour
+    assert g(FXI, FXI - 5, FXI, FXI, k) == FXI    # saturates at +inf
+
+
+@pytest.mark.parametrize("k", [21, 24])
+def test_godunov_fx_general_roots_vs_numpy(k):
+    """Random sorted minima within 2 h: the fixed-point update is the
+    independent root (numpy.roots) to 1.5 quanta plus FP32 rounding of the
+    root offset; a dropped term or wrong branch rule is off by many quanta."""
+    g = O.lib().orc_godunov_fx
+    H = 2 ** k
+    rng = np.random.default_rng(17)
+    for it in range(4000):
+        base = int(rng.integers(0, 2 ** 31))
+        off = [int(v) for v in rng.uniform(0, 2.0 * H, 3)]
+        m = [base + o for o in off]
+        if it % 7 == 0:
+            m[int(rng.integers(3))] = FXI
+        got = g(FXI, m[0], m[1], m[2], k)
+        fin = [(v - base) / H if v != FXI else math.inf for v in m]
+        want = godunov_brute(fin[0], fin[1], fin[2], 1.0, 1.0, 1.0) * H + base
+        t = want - min(x for x in m if x != FXI)
+        assert abs(got - want) <= 1.5 + 4 * t * 2.0 ** -24, (m, got, want)
+
+
+def test_fx_encode_decode_round_trip():
+    rng = np.random.default_rng(2)
+    v = rng.uniform(-40, 900, 5000).astype(np.float32)
+    v[:5] = [np.inf, np.nan, 1e6, -40.0, 0.0]
+    k = O.fx_bits(512, 512, 512)
+    base = float(np.nanmin(v[np.isfinite(v)]))
+    assert float(O.lib().orc_fx_min(np.ascontiguousarray(v), v.size)) == base
+    d = 2.0 ** -k
+    q = O.fx_encode(v, base, d)
+    assert q[0] == FXI and q[1] == FXI and q[2] == FXI   # +inf, NaN, beyond the range
+    r = np.rint((v.astype(np.float64) - base) / d)
+    ok = np.isfinite(r) & (r < FXI)
+    assert np.array_equal(q[ok], r[ok].astype(np.uint32))
+    back = O.fx_decode(q, base, d)
+    assert np.all(np.isinf(back[~ok]))
+    assert np.all(np.abs(back[ok].astype(np.float64) - v[ok]) <=
+                  d / 2 + np.abs(v[ok]) * 2.0 ** -24)
+
+
+@pytest.mark.parametrize("h", [1.0, 0.37])
+def test_fixed_solve_close_to_fp64_and_point_axis_exact(h):
+    """Converged fixed-point solves are within 5e-5 h of the FP64 difference
+    form's solution (a different arithmetic), on spheres and a random field;
+    along the axes through a point source the values are n h exactly."""
+    for a in (O.gen_spheres(70, 53, 41, 9, h=h, n=6, r=(2.0, 8.0)),
+              O.random_field(48, 40, 36, 3)):
+        if h != 1.0:
+            a = a.copy()
+            a[1:-1, 1:-1, 1:-1] *= h
+        start = np.ascontiguousarray(O.interior(a)).astype(np.float32)
+        ref = start.astype(np.float64)
+        pad = np.full((ref.shape[0] + 2, ref.shape[1] + 2, ref.shape[2] + 2), np.inf)
+        pad[1:-1, 1:-1, 1:-1] = ref
+        O.solve_d64(pad, h=(h, h, h), tol=0.0)
+        got, p, _, _ = O.solve_fixed(start, h=h)
+        err = np.abs(got.astype(np.float64) - O.interior(pad)).max()
+        assert err <= 5e-5 * h, err
+    X, Y, Z = 37, 29, 23
+    src = (11, 7, 5)   # (z, y, x)
+    start = np.full((Z, Y, X), 1e6, np.float32)
+    start[src] = 0.0
+    got, _, (k, base, d), q = O.solve_fixed(start, h=h)
+    qi = O.interior(q)
+    for n in range(1, X - src[2]):
+        assert qi[src[0], src[1], src[2] + n] == n * 2 ** k
+    for n in range(1, Z - src[0]):
+        assert qi[src[0] + n, src[1], src[2]] == n * 2 ** k
+
+
+def test_fixed_solve_shift_invariant():
+    """Adding a constant to the start field (values on a 2^-8 grid, so the
+    shifted FP32 values are exact) adds it to every stored result: the base
+    takes it and the quanta are equal."""
+    start = np.ascontiguousarray(O.interior(O.gen_spheres(40, 33, 29, 5, n=5, r=(2.0, 6.0))))
+    start = (np.round(start * 256.0) / 256.0).astype(np.float32)
+    _, _, (k, b0, d), q0 = O.solve_fixed(start)
+    _, _, (_, b1, _), q1 = O.solve_fixed(start - np.float32(37.25))
+    assert b1 == b0 - 37.25
+    assert np.array_equal(q0, q1)
+
+
+def test_judge_fx_converged_passes_planted_fails():
+    start = np.ascontiguousarray(O.interior(O.gen_spheres(30, 26, 22, 8, n=4, r=(2.0, 5.0))))
+    start = start.astype(np.float32)
+    _, _, (k, base, d), q = O.solve_fixed(start)
+    init = np.full(q.shape, np.inf, np.float32)
+    init[1:-1, 1:-1, 1:-1] = start
+    tol = 2e-6 + 4 * d
+    assert O.judge_fx(q, init, 1.0, base, d, tol) == 0
+    bad = q.copy()
+    bad[10, 10, 10] -= np.uint32(int(tol / d) + 10)   # too low: below its own update
+    n, fl = O.judge_fx(bad, init, 1.0, base, d, tol, want_flags=True)
+    assert n >= 1 and fl[10, 10, 10] == 1
+    # a quantum size that disagrees with the stored values (a wrong d) fails
+    assert O.judge_fx(q, init, 1.0, base, 1.01 * d, tol) > 0

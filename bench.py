@@ -229,7 +229,8 @@ def main():
                    "storage": extra["storage"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(),
+                     "traffic": measured_traffic(
+                         "fixed" if st["storage"] == pdfs_amd.PDFS_STORE_FIXED else "sweep"),
                      "kernel": ("k_sweep_col (fixed point, mode 7)"
                                 if st["storage"] == pdfs_amd.PDFS_STORE_FIXED
                                 else "k_sweep_col (FP32, mode 0)"),
@@ -319,11 +320,12 @@ def profile_batch(args):
 
 
 def measured_traffic(kernel="sweep"):
-    """HBM bytes per launch of the FP32 sweep (kernel="sweep") or the FP64
-    polish (kernel="polish") on this workload from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's
-    reading for a plain copy of known size), or None."""
-    p = os.path.join(ROOT, "profiles", "k_sweep_col_spheres512.json")
+    """HBM bytes per launch on this workload from the committed rocprofv3 PMC
+    summaries (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's reading
+    for a plain copy of known size): the fixed-point sweep (kernel="fixed"),
+    the FP32-storage sweep ("sweep") or the FP64 polish ("polish"); or None."""
+    name = "k_sweep_col_fixed_spheres512.json" if kernel == "fixed" else "k_sweep_col_spheres512.json"
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     with open(p) as f:

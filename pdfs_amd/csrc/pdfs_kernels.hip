@@ -168,40 +168,43 @@ __device__ __forceinline__ float god_iso32(float c, float xm, float xp, float ym
 
 // Fixed-point isotropic solve. Values are unsigned multiples of a quantum
 // (the field's h 2^-k); UINT32_MAX is +inf. The per-axis minima and their
-// sort are exact integer min / max; the differences a = m1 - m0 and
-// b = m2 - m0 go to FP32 (exact below 2^24 quanta), and the root offset t of
-// god_iso32's difference form is evaluated in FP32 in units of the quantum
-// (Hq = 2^k, 2 Hq^2 and 3 Hq^2 exact) and rounded to the nearest quantum;
-// u = m0 + t saturates at +inf. The 1-D candidate m0 + Hq is exact, so the
-// stored values carry no rounding from their magnitude: the quantum bounds
-// what one update adds, wherever the value lies. Lanes whose root candidate
-// is not selected may compute a NaN t; it is converted as 0.
+// sort are exact integer min / max (min3 / med3 / max3); the differences
+// a = m1 - m0 and b = m2 - m0 go to FP32 (exact below 2^24 quanta), and the
+// root offset t of god_iso32's difference form is evaluated in FP32 in units
+// of the quantum (Hq = 2^k, 2 Hq^2 and 3 Hq^2 exact) and rounded to the
+// nearest quantum; u = m0 + t saturates at +inf. The 1-D candidate m0 + Hq
+// is exact, so the stored values carry no rounding from their magnitude:
+// the quantum bounds what one update adds, wherever the value lies.
+// The radicand is clamped at 0: the 3-D one like god_iso32's (a nonzero
+// radicand of integers in quanta is at least 1, so god_iso32's flush below
+// 2^-96 changes nothing here), the 2-D one only where it is negative, which
+// means a > h, so u1 <= m1 and the root is not selected. t is then finite,
+// non-negative and below 2^32 (P < 2^33, K <= 1/2), and converts exactly.
 __device__ __forceinline__ uint32_t god_fx_iso(uint32_t c, uint32_t xm, uint32_t xp,
                                                uint32_t ym, uint32_t yp, uint32_t zm,
                                                uint32_t zp, uint32_t Hq, float two_h2,
                                                float three_h2, float third) {
 #pragma clang fp contract(off)
-  const uint32_t mx = xm < xp ? xm : xp, my = ym < yp ? ym : yp, mz = zm < zp ? zm : zp;
-  const uint32_t lo = mx < my ? mx : my, hi = mx > my ? mx : my;
-  const uint32_t m0 = lo < mz ? lo : mz;
-  const uint32_t mid = lo > mz ? lo : mz;
-  const uint32_t m1 = hi < mid ? hi : mid;
-  const uint32_t m2 = hi > mid ? hi : mid;
+  const uint32_t mx = __builtin_elementwise_min(xm, xp);
+  const uint32_t my = __builtin_elementwise_min(ym, yp);
+  const uint32_t mz = __builtin_elementwise_min(zm, zp);
+  const uint32_t lo = __builtin_elementwise_min(mx, my), hi = __builtin_elementwise_max(mx, my);
+  const uint32_t m0 = __builtin_elementwise_min(lo, mz);
+  const uint32_t m2 = __builtin_elementwise_max(hi, mz);
+  const uint32_t m1 = __builtin_elementwise_max(lo, __builtin_elementwise_min(hi, mz));
   const uint32_t u1 = __builtin_elementwise_add_sat(m0, Hq);
   const float a = (float)(m1 - m0);
   const float b = (float)(m2 - m0);
   const float aa = a * a;
   const float r2 = two_h2 - aa;
   const float e = b + (b - a);
-  float D = three_h2 - 2.0f * ((aa + b * b) - a * b);
-  D = D > 0x1p-96f ? D : 0.0f;
+  const float D = three_h2 - 2.0f * ((aa + b * b) - a * b);
   const bool three = r2 > e * e;
-  const float R = three ? D : r2;
+  const float R = __builtin_elementwise_maximum(three ? D : r2, 0.0f);
   const float P = three ? a + b : a;
   const float K = three ? third : 0.5f;
   const float t = (P + sqrt_cr_nonneg(R)) * K;
-  const uint32_t tq = t >= 0.0f ? (uint32_t)__builtin_rintf(t) : 0u;
-  const uint32_t ut = __builtin_elementwise_add_sat(m0, tq);
+  const uint32_t ut = __builtin_elementwise_add_sat(m0, (uint32_t)__builtin_rintf(t));
   const uint32_t u = u1 > m1 ? ut : u1;
   return c < u ? c : u;
 }

@@ -117,6 +117,11 @@ typedef struct {
   int stop_quanta; /* fixed-point storage, plain stop: a field ends on a pass that changes no
                       value by more than this many quanta (default 4; 0: a pass that changes
                       nothing, the FP32 storage rule) */
+  int overlap;     /* FP32 path (default 1): sweep a pass (or half pass) in one persistent launch
+                      whose orderings overlap (a column starts ordering k + 1 once it and its
+                      four neighbours have finished ordering k), when every field's extents are
+                      multiples of the column tile and the edge regions fit; else one launch
+                      per ordering. Results are the same bits. */
 } pdfs_opts;
 
 typedef struct {
@@ -148,6 +153,7 @@ typedef struct {
   long long polish_point_updates; /* interior points x polish orderings executed */
   double ms_polish_kernels;  /* profile: sum of per-launch polish kernel times */
   int storage;               /* PDFS_STORE_FIXED if the fields were swept in fixed point */
+  int sweep_launches;        /* sweep kernel launches of the solve (the polish's excluded) */
   double ms_convert;         /* fixed point: FP32 <-> fixed conversions (with the base readback) */
 } pdfs_stats;
 

@@ -125,7 +125,7 @@ class Opts(C.Structure):
                 ("quiet", C.c_int), ("profile", C.c_int), ("graph", C.c_int),
                 ("certify", C.c_int), ("certify_from", C.c_int), ("polish", C.c_int),
                 ("polish_tol", C.c_double), ("tile64", C.c_int), ("storage", C.c_int),
-                ("judge_tol", C.c_double), ("stop_quanta", C.c_int)]
+                ("judge_tol", C.c_double), ("stop_quanta", C.c_int), ("overlap", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -138,7 +138,7 @@ class Stats(C.Structure):
                 ("polish_orderings", C.c_int), ("polish_maxdiff", C.c_double),
                 ("ms_polish", C.c_double), ("polish_skipped", C.c_int),
                 ("polish_point_updates", C.c_longlong), ("ms_polish_kernels", C.c_double),
-                ("storage", C.c_int), ("ms_convert", C.c_double)]
+                ("storage", C.c_int), ("sweep_launches", C.c_int), ("ms_convert", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -45,6 +45,17 @@ int pdfsi_batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *cons
 void pdfsi_make_sp32(double dx, double dy, double dz, pdfs_sp32 *s);
 /* default column variant of an FP32 sweep of nf fields (x, y of the first) */
 int pdfsi_default_tile(int nf, int x, int y);
+/* Ticket order of an overlapped segment (orderings sn0..sn1 in one launch):
+ * nf fields with tnx[f] x tny[f] column tiles and fz[f] planes; nc active
+ * columns cols[3 i] = (field, a, b) in the virtual frame; TX x TY tiles;
+ * grid concurrent columns. Writes nc (sn1 - sn0 + 1) tickets of 5 ints
+ * (slot, field, a, b, physical column: the field's base plus pa * tny + pb)
+ * in a topological order of their dependencies (upwind in the slot; itself
+ * and its physical neighbours in the slot before; all of the slot two
+ * before), earliest simulated ready time first. Returns the tickets written
+ * (nc (sn1 - sn0 + 1) when complete), -1 when out of memory. Host only. */
+int pdfsi_ov_order(int nf, const int *tnx, const int *tny, const int *fz, int nc,
+                   const int *cols, int TX, int TY, int sn0, int sn1, int grid, int *out);
 
 #ifdef __cplusplus
 }

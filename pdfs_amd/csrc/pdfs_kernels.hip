@@ -521,6 +521,14 @@ struct ColArgs {
   unsigned *err;
   pdfs_sp32 sp;
   pdfs_sp64 sp64;
+  // overlapped pass (k_sweep_col<..., true>): tickets of orderings 0..ov_n-1
+  // of the launch, 5 ints each (ordering slot, field, a, b, physical column)
+  int ov_n, ov_nt;          // orderings in the launch, tickets
+  int ov_dirs;              // bits 3 o + {0, 1, 2}: reflections x, y, z of ordering slot o
+  int ov_ncolp;             // physical columns of the plan (stride of ov_done per slot)
+  uint32_t ov_rx, ov_ry;    // bytes between the two edge regions (x part, y part)
+  unsigned *ov_done;        // [8][ov_ncolp]: epoch of the last ordering each column finished
+  unsigned *ov_cnt;         // [8]: columns finished per ordering slot of this launch
 };
 
 
@@ -539,7 +547,34 @@ struct ColGeom {
   static constexpr int cell(int x, int y) { return (y + 1) * PW + x + 1; }
 };
 
-template <int TX, int TY, int U, int M>
+// Bounded wait until a 32-bit word (written with relaxed agent-scope
+// atomics) holds at least `target` (wrapping compare), or the shared error
+// flag is raised (2 s timeout, like wait_edge).
+__device__ __noinline__ void wait_at_least(const unsigned *p, unsigned target, unsigned *err) {
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  unsigned n = 0;
+  while ((int)(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+    __builtin_amdgcn_s_sleep(8);
+    if ((++n & 15u) == 0u) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+        atomicOr(err, 1u);
+        break;
+      }
+    }
+  }
+}
+
+// OV: one launch sweeps several consecutive orderings of a pass ("slots"),
+// their columns interleaved in one ticket order. A column starts ordering
+// slot o once (1) its upwind columns of o publish their edge entries (the
+// start gate and edge waits, as in a single ordering), (2) it and its four
+// neighbours have finished slot o - 1 (done flags: release by the producer,
+// acquire here, so their plane stores are visible across the XCDs), and (3)
+// every column has finished slot o - 2, whose edge region slot o reuses.
+// Tickets are ordered so that every dependency holds an earlier ticket,
+// which keeps the persistent grid deadlock free.
+template <int TX, int TY, int U, int M, bool OV = false>
 __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
   using S = SMode<M>;
   using T = typename S::T;
@@ -564,8 +599,9 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
   const int tid = threadIdx.x;
   const bool edge_wave = tid >= NT;  // wave-uniform: NT is a multiple of 64
   const T INF = inf_of<T>();
-  const unsigned ncols = __hip_atomic_load(a.state + ST_NCOLS, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned ncols = OV ? (unsigned)a.ov_nt
+                           : __hip_atomic_load(a.state + ST_NCOLS, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t epoch = __hip_atomic_load(a.state + ST_EPOCH, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
   const unsigned tbase = __hip_atomic_load(a.state + ST_BASE, __ATOMIC_RELAXED,
@@ -593,8 +629,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     // buffer operation a waterfall loop)
     const int col = __builtin_amdgcn_readfirstlane(s_col);
     if ((unsigned)col >= ncols) break;
-    const int fi = __builtin_amdgcn_readfirstlane(a.cols[3 * col]);
-    {
+    constexpr int CW = OV ? 5 : 3;  // ints per ticket
+    const int oslot = OV ? __builtin_amdgcn_readfirstlane(a.cols[CW * col]) : 0;
+    const int fi = __builtin_amdgcn_readfirstlane(a.cols[CW * col + (OV ? 1 : 0)]);
+    if (!OV) {
       // skip the columns of a converged field (its whole field skips, so no
       // column waits on one of them); the barrier keeps thread 0 from
       // taking the next ticket before every wave has read this one
@@ -605,8 +643,14 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         continue;
       }
     }
-    const int ca = __builtin_amdgcn_readfirstlane(a.cols[3 * col + 1]);
-    const int cb = __builtin_amdgcn_readfirstlane(a.cols[3 * col + 2]);
+    const int ca = __builtin_amdgcn_readfirstlane(a.cols[CW * col + (OV ? 2 : 1)]);
+    const int cb = __builtin_amdgcn_readfirstlane(a.cols[CW * col + (OV ? 3 : 2)]);
+    // the ordering this ticket sweeps: reflections, epoch, edge region
+    const int fx = OV ? (a.ov_dirs >> (3 * oslot)) & 1 : a.fx;
+    const int fy = OV ? (a.ov_dirs >> (3 * oslot + 1)) & 1 : a.fy;
+    const int fz = OV ? (a.ov_dirs >> (3 * oslot + 2)) & 1 : a.fz;
+    const uint32_t ep = epoch + (uint32_t)oslot;
+    const uint32_t rgx = OV && (oslot & 1) ? a.ov_rx : 0u, rgy = OV && (oslot & 1) ? a.ov_ry : 0u;
     FieldDesc fd = a.fields[fi];
     {
       const uint64_t fp = (uint64_t)fd.f;
@@ -641,8 +685,32 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     // s + TX - 1 for every r: a step's entries are written together and read
     // together.
     constexpr uint32_t PADB = PDFSK_EDGE_PAD * 8u;
-    const uint32_t exb = PADB + fd.ex_off * 8u, eyb = ybase + PADB + fd.ey_off * 8u;
+    const uint32_t exb = PADB + fd.ex_off * 8u + rgx, eyb = ybase + PADB + fd.ey_off * 8u + rgy;
     const int self = ca * nTy + cb;
+    // OV: this column's physical tile and flag index; wait for the slot's
+    // dependencies (edge-wave lanes 0..5), then make the producers' plane
+    // stores visible to every wave of this workgroup
+    const int colp = OV ? __builtin_amdgcn_readfirstlane(a.cols[CW * col + 4]) : 0;
+    if (OV && oslot > 0) {
+      const int nTx = __builtin_amdgcn_readfirstlane(fd.nTx);
+      const int pa = fx ? nTx - 1 - ca : ca, pb = fy ? nTy - 1 - cb : cb;
+      if (edge_wave) {
+        const int e = tid - NT;
+        const unsigned *prev = a.ov_done + (size_t)(oslot - 1) * (unsigned)a.ov_ncolp;
+        int q = -1;
+        if (e == 0) q = colp;
+        else if (e == 1 && pa > 0) q = colp - nTy;
+        else if (e == 2 && pa + 1 < nTx) q = colp + nTy;
+        else if (e == 3 && pb > 0) q = colp - 1;
+        else if (e == 4 && pb + 1 < nTy) q = colp + 1;
+        if (q >= 0) wait_at_least(prev + q, ep - 1u, a.err);
+        // wave 0 of every column of slot o - 2 counted 64
+        if (e == 5 && oslot >= 2)
+          wait_at_least(a.ov_cnt + (oslot - 2), 64u * (ncols / (unsigned)a.ov_n), a.err);
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
 
     if (edge_wave) {
       // ---- edge wave: value lane v < TX serves column c = v of the south /
@@ -694,15 +762,15 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         const int gl = Z - 1 < GATE ? Z - 1 : GATE;
         if (up && (unsigned)(gl - k) < (unsigned)Z) {
           const uint64_t *p = a.edge_x + (coff + (uint32_t)gl * cinc) / 8u;
-          if ((uint32_t)(ld_edge(p) >> 32) != epoch) wait_edge<8>(p, epoch, a.err);
+          if ((uint32_t)(ld_edge(p) >> 32) != ep) wait_edge<8>(p, ep, a.err);
         }
       }
       // consume the entry of step s (loaded UE steps earlier) into the halo
       // cell; a stale entry (its upwind column is behind) is waited for
       auto consume = [&](const int s, const u32x2 q) __attribute__((always_inline)) {
         uint32_t v = q.x;
-        if (__builtin_expect(up && (unsigned)(s - k) < (unsigned)Z && q.y != epoch, 0))
-          v = wait_edge<1>(a.edge_x + (coff + (uint32_t)s * cinc) / 8u, epoch, a.err);
+        if (__builtin_expect(up && (unsigned)(s - k) < (unsigned)Z && q.y != ep, 0))
+          v = wait_edge<1>(a.edge_x + (coff + (uint32_t)s * cinc) / 8u, ep, a.err);
         lds32[(hcl + hs) * W + ew] = up ? v : infw;
         hs = next(hs);
       };
@@ -711,7 +779,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       auto publish = [&](const int lp) __attribute__((always_inline)) {
         const uint32_t v = lds32[(pcl + ps) * W + ew];
         ps = next(ps);
-        const u32x2 q = {v, epoch};
+        const u32x2 q = {v, ep};
         const bool live = (isS || isW) && (unsigned)(lp - pk) < (unsigned)Z;
         __builtin_amdgcn_raw_buffer_store_b64(q, re, opq(live ? poff : 0xFFFFFFF0u), 0, PDFSK_SC1);
         poff += pinc;
@@ -746,13 +814,13 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       // ---- compute waves
       const int tx = tid % TX, ty = tid / TX;
       const uint32_t XYb = (uint32_t)X * (uint32_t)Y * (uint32_t)sizeof(T);
-      const uint32_t dZ = a.fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
+      const uint32_t dZ = fz ? (uint32_t)(0u - XYb) : XYb;  // byte step of virtual z + 1
       const __amdgpu_buffer_rsrc_t fr = __builtin_amdgcn_make_buffer_rsrc(
           fd.f, (short)0, (int)(XYb * (uint32_t)Z), 0x00020000);
       // byte offset of virtual plane vz (negative / >= N*4 wrap to out of range)
-      auto zb = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XYb; };
+      auto zb = [&](int vz) -> uint32_t { return (uint32_t)(fz ? Z - vz : vz - 1) * XYb; };
       auto xyi = [&](int vx, int vy) -> uint32_t {
-        const int px = a.fx ? X - vx : vx - 1, py = a.fy ? Y - vy : vy - 1;
+        const int px = fx ? X - vx : vx - 1, py = fy ? Y - vy : vy - 1;
         return (uint32_t)(py * X + px);
       };
       const int vx = ca * TX + tx + 1, vy = cb * TY + ty + 1;
@@ -803,8 +871,8 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         ir[k] = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(S::POL && k == 0 ? fd.init : fd.slow), (short)0, (int)(XY4 * (uint32_t)Z),
             0x00020000);
-      const uint32_t dZI = a.fz ? (uint32_t)(0u - XY4) : XY4;
-      auto zbi = [&](int vz) -> uint32_t { return (uint32_t)(a.fz ? Z - vz : vz - 1) * XY4; };
+      const uint32_t dZI = fz ? (uint32_t)(0u - XY4) : XY4;
+      auto zbi = [&](int vz) -> uint32_t { return (uint32_t)(fz ? Z - vz : vz - 1) * XY4; };
       const uint32_t lxyi = dom ? xyi(vx, vy) * 4u : 0x80000000u;
       uint32_t ldi = zbi(3) + lxyi;
 
@@ -939,9 +1007,9 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
             u = god_iso64_fminmax((double)ini, xm, xp, ym, yp, zm, zp, hs, 2.0 * h2, 3.0 * h2,
                                   a.sp64.third);
           } else {
-            const double mx = a.fx ? (xp < xm ? xp : xm) : (xm < xp ? xm : xp);
-            const double my = a.fy ? (yp < ym ? yp : ym) : (ym < yp ? ym : yp);
-            const double mz = a.fz ? (zp < zm ? zp : zm) : (zm < zp ? zm : zp);
+            const double mx = fx ? (xp < xm ? xp : xm) : (xm < xp ? xm : xp);
+            const double my = fy ? (yp < ym ? yp : ym) : (ym < yp ? ym : yp);
+            const double mz = fz ? (zp < zm ? zp : zm) : (zm < zp ? zm : zp);
             const double r = 1.0 / (sl * sl);
             const double dd[3] = {a.sp64.d[0] * sl, a.sp64.d[1] * sl, a.sp64.d[2] * sl};
             const double ww[3] = {a.sp64.w[0] * r, a.sp64.w[1] * r, a.sp64.w[2] * r};
@@ -953,9 +1021,9 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         } else {
           // per-axis minima in the physical order of the reference's
           // min(lower index, higher index) (mpi_openacc/src/utilities.h:35)
-          const T mx = a.fx ? (xp < xm ? xp : xm) : (xm < xp ? xm : xp);
-          const T my = a.fy ? (yp < ym ? yp : ym) : (ym < yp ? ym : yp);
-          const T mz = a.fz ? (zp < zm ? zp : zm) : (zm < zp ? zm : zp);
+          const T mx = fx ? (xp < xm ? xp : xm) : (xm < xp ? xm : xp);
+          const T my = fy ? (yp < ym ? yp : ym) : (ym < yp ? ym : yp);
+          const T mz = fz ? (zp < zm ? zp : zm) : (zm < zp ? zm : zp);
           if constexpr (M == PDFSK_MODE_F64_REF)
             u = god_ref64(c, mx, my, mz, a.sp64.d[0], a.sp64.d[1], a.sp64.d[2]);
           else
@@ -1035,8 +1103,23 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         if ((tid & 63) == 0 && b) atomicMax(a.maxdiff + fi, (unsigned long long)b);
       }
     }
+    // OV: every wave's plane stores have completed before the barrier; one
+    // agent-scope release then makes them visible across the XCDs ahead of
+    // the done flag that the next ordering's columns wait for
+    if (OV) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);
+    // The release is wave-uniform (wave 0, every lane: the same flag value,
+    // 64 counts per column). A lane-0-only tail here let the compiler split
+    // wave 0 around the loop's barriers: lanes 1..63 re-entered the loop
+    // top's barrier before lane 0 took the next ticket, so every wave read
+    // the stale ticket again, without end.
+    if (OV && __builtin_amdgcn_readfirstlane(tid) == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(a.ov_done + (size_t)oslot * (unsigned)a.ov_ncolp + colp, ep,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(a.ov_cnt + oslot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   // The last workgroup to finish advances the launch state for the next
   // launch and each field's run of orderings without change. (Release: this
@@ -1057,8 +1140,9 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
       a.streak[2 * f] = cur;
       a.streak[2 * f + 1] = cur == last ? (run < 0xFFFFu ? run + 1u : run) : 0u;
     }
+    if (OV && tid < 8) a.ov_cnt[tid] = 0u;
     if (tid == 0) {
-      a.state[ST_EPOCH] = epoch + 1u;
+      a.state[ST_EPOCH] = epoch + (OV ? (uint32_t)a.ov_n : 1u);
       a.state[ST_BASE] = tbase + ncols + gridDim.x;
       a.state[ST_DONE] = 0u;
     }
@@ -1759,6 +1843,61 @@ extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const p
       return 0;
     });
   });
+  return hip_rc(hipGetLastError());
+}
+
+extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, const int *otab,
+                                    int nt, int ncolp, unsigned *done, unsigned *cnt,
+                                    size_t rx_bytes, size_t ry_bytes, int mode,
+                                    const pdfs_sp32 *sp, int variant, int grid, hipStream_t st) {
+  if (sn0 < 1 || sn1 > 8 || sn1 < sn0 || grid < 1 || nt < 1 || !otab || !done || !cnt ||
+      !pdfsk_tile_supports(variant, mode) ||
+      (mode != PDFSK_MODE_F32_ISO && mode != PDFSK_MODE_F32_ANISO && mode != PDFSK_MODE_FX_ISO) ||
+      (mode == PDFSK_MODE_FX_ISO && !b->maxdiff))
+    return -1;
+  ColArgs a;
+  memset(&a, 0, sizeof a);
+  a.fields = (const FieldDesc *)b->fields;
+  a.cols = otab;
+  a.state = b->state;
+  a.edge_x = b->edge_x;
+  a.edge_y = b->edge_y;
+  a.edge_x_bytes = (uint32_t)b->edge_x_bytes;
+  a.edge_y_bytes = (uint32_t)b->edge_y_bytes;
+  a.changed = b->changed;
+  a.streak = b->streak;
+  a.maxdiff = b->maxdiff;
+  a.nfields = b->nfields;
+  a.err = b->err;
+  if (sp) a.sp = *sp;
+  a.ov_n = sn1 - sn0 + 1;
+  a.ov_nt = nt;
+  a.ov_ncolp = ncolp;
+  a.ov_rx = (uint32_t)rx_bytes;
+  a.ov_ry = (uint32_t)ry_bytes;
+  a.ov_done = done;
+  a.ov_cnt = cnt;
+  for (int sn = sn0; sn <= sn1; sn++) {
+    const int desc = (sn == 2 || sn == 5 || sn == 7 || sn == 8);
+    const int rx = (sn == 4 || sn == 8), ry = (sn == 2 || sn == 6), rz = (sn == 3 || sn == 7);
+    const int o = sn - sn0;
+    a.ov_dirs |= ((desc ^ rx) | ((desc ^ ry) << 1) | ((desc ^ rz) << 2)) << (3 * o);
+  }
+  auto go = [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    return with_tile<M, int>(kVariants[variant], 0, [&](auto shape) {
+      using Sh = decltype(shape);
+      hipLaunchKernelGGL((k_sweep_col<Sh::TX, Sh::TY, queue_depth<M>(), M, true>), dim3(grid),
+                         dim3(Sh::TX * Sh::TY + 64), 0, st, a);
+      return 0;
+    });
+  };
+  if (mode == PDFSK_MODE_FX_ISO)
+    go(std::integral_constant<int, PDFSK_MODE_FX_ISO>());
+  else if (mode == PDFSK_MODE_F32_ISO)
+    go(std::integral_constant<int, PDFSK_MODE_F32_ISO>());
+  else
+    go(std::integral_constant<int, PDFSK_MODE_F32_ANISO>());
   return hip_rc(hipGetLastError());
 }
 

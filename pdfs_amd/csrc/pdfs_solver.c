@@ -49,6 +49,18 @@ struct pdfs_ctx {
   double *d_params;
   hipEvent_t ev0, ev1;
   hipEvent_t kev[18];   /* [0..15] per-ordering profile, [16..17] polish */
+  /* overlapped passes (pdfsk_sweep_batch_ov): whether the current plan may
+   * use them (its fields tile exactly and its edge allocation holds two
+   * regions), whether the solve in progress does, ticket tables of the
+   * segments 1-8, 1-4 and 5-8 for the current active set, done flags */
+  int ov_ok, ov_use, last_ov;
+  long long launches;  /* sweep launches since the context was created */
+  int *ov_tab[3];
+  int ov_nt[3], ov_cap[3], ov_valid[3];
+  unsigned *ov_done;
+  size_t ov_done_cap;
+  int ov_ncolp;
+  size_t ov_rx, ov_ry;
   /* fixed-point solve in progress: per-field quantum exponents of the plan
    * (NULL otherwise), conversion table (host staging, device), order keys */
   const int *fxk;
@@ -146,6 +158,7 @@ void pdfs_default_opts(pdfs_opts *o)
   o->storage = PDFS_STORE_FIXED;
   o->judge_tol = 2e-6;
   o->stop_quanta = 4;
+  o->overlap = 1;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -160,7 +173,7 @@ pdfs_ctx *pdfs_ctx_create(void)
   pdfs_ctx *c = (pdfs_ctx *)calloc(1, sizeof *c);
   if (!c) return NULL;
   if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
-      hipMalloc((void **)&c->bt.state, 16 * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc((void **)&c->bt.state, 32 * sizeof(unsigned)) != hipSuccess ||
       hipMalloc((void **)&c->d_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_err, 16 * sizeof(unsigned), 0) != hipSuccess ||
       hipHostMalloc((void **)&c->h_pin64, 4 * sizeof(unsigned long long), 0) != hipSuccess ||
@@ -174,7 +187,7 @@ pdfs_ctx *pdfs_ctx_create(void)
   c->bt.err = c->bt.state + 8;
   /* epochs start at 1 (zeroed edge entries carry epoch 0) and only grow for
    * the lifetime of the context */
-  unsigned init[16] = {0, 1, 0, 0, 0};
+  unsigned init[32] = {0, 1, 0, 0, 0};
   if (hipMemcpy(c->bt.state, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->d_u64, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
     free(c);
@@ -202,6 +215,8 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->pbuf);
   hipFree(c->conv_dev);
   free(c->conv_host);
+  for (int i = 0; i < 3; i++) hipFree(c->ov_tab[i]);
+  hipFree(c->ov_done);
   free(c->fx_host);
   hipFree(c->fx_dev);
   hipFree(c->fx_keys);
@@ -380,7 +395,238 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
   HIPCHK(hipMemcpyAsync(c->bt.state + 4, &n, sizeof n, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->ncols = t;
+  for (int i = 0; i < 3; i++) c->ov_valid[i] = 0;
   return PDFS_OK;
+}
+
+/* ---- overlapped passes --------------------------------------------------
+ * One launch sweeps the orderings sn0..sn1 of a pass (pdfsk_sweep_batch_ov).
+ * Its tickets (slot, field, a, b, physical column) follow a simulated
+ * schedule: columns take Z + TX + TY - 2 steps, start TX (TY) +
+ * PDFS_TICKET_EDGE_STEPS steps after their upwind neighbour in the slot and
+ * after the slot before has finished them and their physical neighbours,
+ * on as many concurrent columns as the grid holds. Tickets are emitted in a
+ * topological order of those dependencies (Kahn, earliest simulated ready
+ * time first; the whole slot o - 2 before any ticket of slot o), so every
+ * ticket's dependencies hold earlier tickets. */
+typedef struct {
+  double key;
+  int id;
+} ov_item;
+
+static void ov_push(ov_item *h, int *n, double key, int id)
+{
+  int i = (*n)++;
+  while (i > 0) {
+    const int p = (i - 1) / 2;
+    if (h[p].key < key || (h[p].key == key && h[p].id < id)) break;
+    h[i] = h[p];
+    i = p;
+  }
+  h[i].key = key;
+  h[i].id = id;
+}
+
+static ov_item ov_pop(ov_item *h, int *n)
+{
+  const ov_item top = h[0], last = h[--(*n)];
+  int i = 0;
+  for (;;) {
+    int l = 2 * i + 1, m = i;
+    ov_item best = last;
+    if (l < *n && (h[l].key < best.key || (h[l].key == best.key && h[l].id < best.id))) {
+      best = h[l];
+      m = l;
+    }
+    if (l + 1 < *n &&
+        (h[l + 1].key < best.key || (h[l + 1].key == best.key && h[l + 1].id < best.id))) {
+      best = h[l + 1];
+      m = l + 1;
+    }
+    if (m == i) break;
+    h[i] = h[m];
+    i = m;
+  }
+  if (*n > 0) h[i] = last;
+  return top;
+}
+
+static void ov_dirs(int sn, int *fx, int *fy)
+{
+  const int desc = (sn == 2 || sn == 5 || sn == 7 || sn == 8);
+  *fx = desc ^ (sn == 4 || sn == 8);
+  *fy = desc ^ (sn == 2 || sn == 6);
+}
+
+int pdfsi_ov_order(int nf, const int *tnx, const int *tny, const int *fz, int nc,
+                   const int *cols, int TX, int TY, int sn0, int sn1, int grid, int *out)
+{
+  const int ns = sn1 - sn0 + 1, nt = ns * nc;
+  const double LAG = PDFS_TICKET_EDGE_STEPS;
+  /* column index of (f, a, b) in the active list; physical column bases */
+  int *cbase = (int *)malloc((size_t)(nf + 1) * sizeof(int));
+  int *pbase = (int *)malloc((size_t)(nf + 1) * sizeof(int));
+  int *cidx = NULL, *left = (int *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(int));
+  double *ready = (double *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+  double *slotend = (double *)calloc((size_t)ns, sizeof(double));
+  int *emitted = (int *)calloc((size_t)ns, sizeof(int));
+  ov_item *heap = (ov_item *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(ov_item));
+  ov_item *slots = (ov_item *)malloc((size_t)(grid > 0 ? grid : 1) * sizeof(ov_item));
+  int ne = -1;
+  if (!cbase || !pbase || !left || !ready || !slotend || !emitted || !heap || !slots) goto done;
+  cbase[0] = pbase[0] = 0;
+  for (int f = 0; f < nf; f++) {
+    pbase[f + 1] = pbase[f] + tnx[f] * tny[f];
+    cbase[f + 1] = pbase[f + 1];
+  }
+  cidx = (int *)malloc((size_t)(pbase[nf] > 0 ? pbase[nf] : 1) * sizeof(int));
+  if (!cidx) goto done;
+  for (int i = 0; i < pbase[nf]; i++) cidx[i] = -1;
+  for (int i = 0; i < nc; i++) {
+    const int f = cols[3 * i], a = cols[3 * i + 1], b = cols[3 * i + 2];
+    cidx[cbase[f] + a * tny[f] + b] = i;
+  }
+  /* dependency counts: upwind in the slot, the slot before (self and
+   * physical neighbours) */
+  for (int o = 0; o < ns; o++) {
+    int fx, fy, gx, gy;
+    ov_dirs(sn0 + o, &fx, &fy);
+    if (o > 0) ov_dirs(sn0 + o - 1, &gx, &gy);
+    for (int i = 0; i < nc; i++) {
+      const int f = cols[3 * i], a = cols[3 * i + 1], b = cols[3 * i + 2];
+      const int nTx = tnx[f], nTy = tny[f];
+      int d = (a > 0) + (b > 0);
+      if (o > 0) {
+        const int pa = fx ? nTx - 1 - a : a, pb = fy ? nTy - 1 - b : b;
+        d += 1 + (pa > 0) + (pa + 1 < nTx) + (pb > 0) + (pb + 1 < nTy);
+      }
+      left[o * nc + i] = d;
+    }
+  }
+  int nh = 0, nslot = 0;
+  for (int g = 0; g < grid; g++) ov_push(slots, &nslot, 0.0, g);
+  for (int i = 0; i < nc; i++)
+    if (left[i] == 0) ov_push(heap, &nh, 0.0, i);
+  ne = 0;
+  while (nh > 0) {
+    const ov_item it = ov_pop(heap, &nh);
+    const int t = it.id, o = t / nc, i = t % nc;
+    const int f = cols[3 * i], a = cols[3 * i + 1], b = cols[3 * i + 2];
+    const int nTx = tnx[f], nTy = tny[f];
+    ov_item sl = ov_pop(slots, &nslot);
+    const double st0 = it.key > sl.key ? it.key : sl.key;
+    const double fin = st0 + (double)(fz[f] + TX + TY - 2);
+    ov_push(slots, &nslot, fin, sl.id);
+    if (fin > slotend[o]) slotend[o] = fin;
+    int fx, fy;
+    ov_dirs(sn0 + o, &fx, &fy);
+    const int pa = fx ? nTx - 1 - a : a, pb = fy ? nTy - 1 - b : b;
+    out[5 * ne] = o;
+    out[5 * ne + 1] = f;
+    out[5 * ne + 2] = a;
+    out[5 * ne + 3] = b;
+    out[5 * ne + 4] = pbase[f] + pa * nTy + pb;
+    ne++;
+    emitted[o]++;
+    /* dependents: downwind in the slot */
+    for (int k = 0; k < 2; k++) {
+      const int a2 = a + (k == 0), b2 = b + (k == 1);
+      if (a2 >= nTx || b2 >= nTy) continue;
+      const int j = o * nc + cidx[cbase[f] + a2 * nTy + b2];
+      const double r = st0 + (k == 0 ? TX : TY) + LAG;
+      if (r > ready[j]) ready[j] = r;
+      if (--left[j] == 0 && (o < 2 || emitted[o - 2] == nc))
+        ov_push(heap, &nh, ready[j], j);
+    }
+    /* dependents in the next slot: the tiles at and next to this one */
+    if (o + 1 < ns) {
+      int hx, hy;
+      ov_dirs(sn0 + o + 1, &hx, &hy);
+      static const int dq[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+      for (int k = 0; k < 5; k++) {
+        const int qa = pa + dq[k][0], qb = pb + dq[k][1];
+        if (qa < 0 || qa >= nTx || qb < 0 || qb >= nTy) continue;
+        const int va = hx ? nTx - 1 - qa : qa, vb = hy ? nTy - 1 - qb : qb;
+        const int j = (o + 1) * nc + cidx[cbase[f] + va * nTy + vb];
+        if (fin > ready[j]) ready[j] = fin;
+        if (--left[j] == 0 && (o + 1 < 2 || emitted[o - 1] == nc))
+          ov_push(heap, &nh, ready[j], j);
+      }
+    }
+    /* slot o complete: the slot two later may start */
+    if (emitted[o] == nc && o + 2 < ns)
+      for (int j = (o + 2) * nc; j < (o + 3) * nc; j++) {
+        if (slotend[o] > ready[j]) ready[j] = slotend[o];
+        if (left[j] == 0) ov_push(heap, &nh, ready[j], j);
+      }
+  }
+done:
+  free(cbase);
+  free(pbase);
+  free(cidx);
+  free(left);
+  free(ready);
+  free(slotend);
+  free(emitted);
+  free(heap);
+  free(slots);
+  return ne;
+}
+
+static int ov_build(pdfs_ctx *c, int seg, int sn0, int sn1, int grid)
+{
+  int TX, TY;
+  pdfsk_tile_dims(c->tile, &TX, &TY);
+  const int ns = sn1 - sn0 + 1, nc = c->ncols, nt = ns * nc;
+  int *tnx = (int *)malloc((size_t)(c->nf > 0 ? c->nf : 1) * 3 * sizeof(int));
+  int *out = (int *)malloc((size_t)5 * (nt > 0 ? nt : 1) * sizeof(int));
+  int rc = PDFS_OK;
+  if (!tnx || !out) {
+    rc = fail(PDFS_ENOMEM, "overlap plan: host alloc");
+    goto done;
+  }
+  for (int f = 0; f < c->nf; f++) {
+    tnx[f] = c->h_fields[f].nTx;
+    tnx[c->nf + f] = c->h_fields[f].nTy;
+    tnx[2 * c->nf + f] = c->h_fields[f].Z;
+  }
+  const int ne = pdfsi_ov_order(c->nf, tnx, tnx + c->nf, tnx + 2 * c->nf, nc, c->h_cols, TX, TY,
+                                sn0, sn1, grid, out);
+  if (ne != nt) {
+    rc = fail(PDFS_EINVAL, "overlap plan: %d of %d tickets ordered", ne, nt);
+    goto done;
+  }
+  if (nt > c->ov_cap[seg]) {
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+      rc = fail(PDFS_EHIP, "overlap plan: synchronize");
+      goto done;
+    }
+    hipFree(c->ov_tab[seg]);
+    c->ov_tab[seg] = NULL;
+    c->ov_cap[seg] = 0;
+    if (hipMalloc((void **)&c->ov_tab[seg], (size_t)5 * nt * sizeof(int)) != hipSuccess) {
+      rc = fail(PDFS_ENOMEM, "overlap plan: device alloc");
+      goto done;
+    }
+    c->ov_cap[seg] = nt;
+  }
+  if (hipMemcpyAsync(c->ov_tab[seg], out, (size_t)5 * nt * sizeof(int), hipMemcpyHostToDevice,
+                     c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess) {
+    rc = fail(PDFS_EHIP, "overlap plan: upload");
+    goto done;
+  }
+  c->ov_nt[seg] = nt;
+  c->ov_valid[seg] = 1;
+  /* a captured launch of this segment holds the old table's size */
+  if (c->gexec[seg]) {
+    hipGraphExecDestroy(c->gexec[seg]);
+    c->gexec[seg] = NULL;
+  }
+done:
+  free(tnx);
+  free(out);
+  return rc;
 }
 
 /* Face entries of a batch's edge allocation for a tile (one row per column
@@ -455,6 +701,31 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     ex += nc * (size_t)(z + TX + TY - 1) * TY * words;
     ey += nc * (size_t)(z + TX + TY - 1) * TX * words;
     ncols += (int)nc;
+  }
+  /* overlapped passes need every field to tile exactly (each ordering then
+   * tiles the same physical columns) and a second edge region */
+  int ov = words == 1;
+  for (int f = 0; f < nf && ov; f++)
+    ov = dims[3 * f] % TX == 0 && dims[3 * f + 1] % TY == 0;
+  if (ov && (2 * (ex + ey) + 4 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31)) ov = 0;
+  c->ov_ok = ov;
+  c->ov_rx = ov ? ex * 8 : 0;
+  c->ov_ry = ov ? ey * 8 : 0;
+  for (int i = 0; i < 3; i++) c->ov_valid[i] = 0;
+  if (ov) {
+    ex *= 2;
+    ey *= 2;
+    if ((size_t)8 * ncols > c->ov_done_cap) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      hipFree(c->ov_done);
+      c->ov_done = NULL;
+      c->ov_done_cap = 0;
+      HIPCHK(hipMalloc((void **)&c->ov_done, (size_t)8 * ncols * sizeof(unsigned)));
+      /* epochs start at 1: zeroed flags are never taken for a finished slot */
+      HIPCHK(hipMemsetAsync(c->ov_done, 0, (size_t)8 * ncols * sizeof(unsigned), c->st));
+      c->ov_done_cap = (size_t)8 * ncols;
+    }
+    c->ov_ncolp = ncols;
   }
   if (ex > c->cap_ex || ey > c->cap_ey) {
     /* one allocation: the x-face part, then the y-face part, each with a
@@ -538,12 +809,64 @@ static int launch_grid(pdfs_ctx *c, int mode)
 /* Orderings sn0..sn1 of the current plan in a sweep mode, optionally
  * profiled per launch or replayed from a captured graph (a whole pass, or
  * one of its halves). */
+static int ov_build(pdfs_ctx *c, int seg, int sn0, int sn1, int grid);
+
 static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_sp64 *sp64,
                          int sn0, int sn1, int profile, int graph, double *kernel_ms)
 {
   int grid = launch_grid(c, mode);
   if (grid <= 0) return fail(PDFS_EHIP, "column kernel cannot be resident");
   const int gi = sn0 == 1 && sn1 == 8 ? 0 : sn0 == 1 && sn1 == 4 ? 1 : sn0 == 5 && sn1 == 8 ? 2 : -1;
+  /* overlapped: the segment's orderings in one launch */
+  const int ov = c->ov_use && c->ov_ok && gi >= 0 &&
+                 (mode == PDFSK_MODE_F32_ISO || mode == PDFSK_MODE_F32_ANISO ||
+                  mode == PDFSK_MODE_FX_ISO);
+  c->last_ov = ov;
+  if (ov) {
+    int rc;
+    if (!c->ov_valid[gi] && (rc = ov_build(c, gi, sn0, sn1, grid))) return rc;
+    if (graph && !profile) {
+      const int key = 1000 + mode;
+      if (!c->gexec[gi] || c->g_tile[gi] != c->tile || c->g_mode[gi] != key ||
+          c->g_grid[gi] != grid) {
+        if (c->gexec[gi]) {
+          hipGraphExecDestroy(c->gexec[gi]);
+          c->gexec[gi] = NULL;
+        }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        if (pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
+                                 c->ov_done, c->bt.state + 16, c->ov_rx, c->ov_ry, mode, sp,
+                                 c->tile, grid, c->st)) {
+          hipStreamEndCapture(c->st, &g);
+          return fail(PDFS_EHIP, "launch during capture failed");
+        }
+        HIPCHK(hipStreamEndCapture(c->st, &g));
+        hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, NULL, NULL, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(PDFS_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+        c->g_tile[gi] = c->tile;
+        c->g_mode[gi] = key;
+        c->g_grid[gi] = grid;
+      }
+      HIPCHK(hipGraphLaunch(c->gexec[gi], c->st));
+      c->launches++;
+      return PDFS_OK;
+    }
+    if (profile) HIPCHK(hipEventRecord(c->kev[0], c->st));
+    KCHK(pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
+                              c->ov_done, c->bt.state + 16, c->ov_rx, c->ov_ry, mode, sp,
+                              c->tile, grid, c->st));
+    c->launches++;
+    if (profile) {
+      HIPCHK(hipEventRecord(c->kev[1], c->st));
+      HIPCHK(hipEventSynchronize(c->kev[1]));
+      float tk = 0.f;
+      hipEventElapsedTime(&tk, c->kev[0], c->kev[1]);
+      *kernel_ms += tk;
+    }
+    return PDFS_OK;
+  }
   if (graph && gi >= 0 && !profile) {
     if (!c->gexec[gi] || c->g_tile[gi] != c->tile || c->g_mode[gi] != mode ||
         c->g_grid[gi] != grid) {
@@ -567,11 +890,13 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
       c->g_grid[gi] = grid;
     }
     HIPCHK(hipGraphLaunch(c->gexec[gi], c->st));
+    c->launches += sn1 - sn0 + 1;
     return PDFS_OK;
   }
   for (int sn = sn0; sn <= sn1; sn++) {
     if (profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1)], c->st));
     KCHK(pdfsk_sweep_batch(&c->bt, sn, mode, sp, sp64, c->tile, grid, c->st));
+    c->launches++;
     if (profile) HIPCHK(hipEventRecord(c->kev[2 * (sn - 1) + 1], c->st));
   }
   if (profile) {
@@ -1298,6 +1623,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   }
   s.storage = fixed ? PDFS_STORE_FIXED : PDFS_STORE_F32;
   if ((rc = plan_build_ex(c, nf, (void *const *)fields, NULL, slows, dims, tile, 0, 1))) goto out;
+  /* whole and half passes as one launch each where the plan allows it */
+  c->ov_use = o->overlap && !speed;
+  const long long launches0 = c->launches;
   for (int f = 0; f < nf; f++) {
     active[f] = 1;
     lastfl[f] = -1;  /* not judged yet */
@@ -1411,6 +1739,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
           const unsigned run = c->h_streak[2 * f + 1];
           long long skipped =
               run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
+          if (c->last_ov) skipped = 0; /* an overlapped launch skips nothing */
           if (skipped > nord) skipped = nord;
           s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
           if (nf == 1) s.orderings -= (int)skipped;
@@ -1476,6 +1805,8 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
     if (rc) break;
   }
   if (counted_passes > s.passes) s.passes = counted_passes;
+  s.sweep_launches = (int)(c->launches - launches0);
+  c->ov_use = 0;
   s.tile = c->tile;
   s.last_changed = last_total;
   s.ms_sweeps = ms - s.ms_judge;

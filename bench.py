@@ -111,17 +111,15 @@ def main():
 
     # rooflines of the two kernels of a solve, from per-launch HIP events on
     # the solver's stream in one profiled solve.
-    # FP32 column sweep: every ordering of a pass that runs is a launch. The
-    # field reaches its fixed point at its last swept ordering (orderings
-    # counts the swept ones); the rest of that pass's launches skip its
-    # columns, and the plain stop's pass after it is counted but not
-    # launched: 8 * ceil(orderings / 8) launches. Algorithmic bytes: 8 B per
-    # executed point-update (FP32 read + write), averaged over the same
-    # launches as the time (and as rocprofv3's per-dispatch average).
+    # Column sweep (fixed point or FP32 storage): the solver counts its
+    # launches (an overlapped launch sweeps a pass, or half a pass, of
+    # orderings; otherwise one launch per ordering). Algorithmic bytes: 8 B
+    # per executed point-update (32-bit read + write), averaged over the
+    # same launches as the time (and as rocprofv3's per-dispatch average).
     # FP64 polish sweep: 20 B per point-update (FP64 read + write, FP32 init
-    # read), over its launches.
+    # read), one launch per polish ordering.
     pst = step(profile=1)
-    launches = 8 * ((pst["orderings"] + 7) // 8)
+    launches = sweep_launches(pst)
     avg_launch_ms = pst["ms_sweep_kernels"] / launches
     algo_bytes = 8.0 * pst["point_updates"] / launches
     achieved = algo_bytes / (avg_launch_ms * 1e-3) / 1e9
@@ -302,7 +300,7 @@ def profile_batch(args):
     for f, t in zip(fields, inits):
         f.copy_(t)
     st, passes, _ = ctx.solve_batch_f32(fields, inits, tile=pdfs_amd.TILE_AUTO, judge=0, profile=1)
-    launches = 8 * ((st["orderings"] + 7) // 8)
+    launches = sweep_launches(st)
     sweep_ms = st["ms_sweep_kernels"] / max(1, launches)
     pol_n = max(1, st["polish_orderings"])
     pol_ms = st["ms_polish_kernels"] / pol_n
@@ -317,6 +315,13 @@ def profile_batch(args):
            "ms_sweeps": round(st["ms_sweeps"], 3), "ms_polish": round(st["ms_polish"], 3)}
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def sweep_launches(st):
+    """Launches of the column sweep (fixed point or FP32 storage) in a solve:
+    the solver's count of sweep launches less the FP64 polish launches (one
+    per polish ordering)."""
+    return max(1, st["sweep_launches"] - st["polish_orderings"])
 
 
 def measured_traffic(kernel="sweep"):
@@ -466,7 +471,7 @@ def speed_leg(ctx, init, field, args):
     raw = field.clone()
     field.copy_(init)
     pst = ctx.solve_speed_f32(field, s, init=init, tile=args.tile, profile=1)
-    launches = 8 * ((pst["orderings"] + 7) // 8)
+    launches = sweep_launches(pst)
     out.update({"ms": round(ms, 3), "fields_per_s": round(1000.0 / ms, 3), "passes": st["passes"],
                 "orderings": st["orderings"], "polish_passes": st["polish_passes"],
                 "judge_flagged": int(st["judge_flagged"]), "tile": st["tile"],

@@ -73,6 +73,12 @@ enum { PDFS_F64_REF = 0, /* FP64, reference arithmetic and schedule */
 /* Storage of the FP32 path's fields while they are swept (opts.storage). */
 enum { PDFS_STORE_F32 = 0,   /* FP32 values (then the FP64 polish, opts.polish) */
        PDFS_STORE_FIXED = 1 }; /* 32-bit fixed point (default; see pdfs_opts.storage) */
+/* Overlapped passes (opts.overlap). */
+enum { PDFS_OVERLAP_OFF = 0,    /* one launch per ordering */
+       PDFS_OVERLAP_AUTO = 1,   /* where it pays: few fields, long critical path (default) */
+       PDFS_OVERLAP_ALWAYS = 2, /* wherever the fields tile exactly */
+       PDFS_OV_AUTO_FIELDS = 2, /* AUTO: at most this many active fields, */
+       PDFS_OV_AUTO_DEPTH = 6 };/* and critical path >= this/10 of the steps per workgroup */
 
 typedef struct {
   int precision;   /* PDFS_F64_REF or PDFS_F32 */
@@ -117,11 +123,15 @@ typedef struct {
   int stop_quanta; /* fixed-point storage, plain stop: a field ends on a pass that changes no
                       value by more than this many quanta (default 4; 0: a pass that changes
                       nothing, the FP32 storage rule) */
-  int overlap;     /* FP32 path (default 1): sweep a pass (or half pass) in one persistent launch
-                      whose orderings overlap (a column starts ordering k + 1 once it and its
-                      four neighbours have finished ordering k), when every field's extents are
-                      multiples of the column tile and the edge regions fit; else one launch
-                      per ordering. Results are the same bits. */
+  int overlap;     /* FP32 path, PDFS_OVERLAP_*: sweep a pass (or half pass) in one persistent
+                      launch whose orderings overlap (a column starts ordering k + 1 once it
+                      and its four neighbours have finished ordering k). Possible when every
+                      field's extents are multiples of the column tile and the edge regions
+                      fit. AUTO (the default) takes it for at most PDFS_OV_AUTO_FIELDS
+                      active fields whose longest critical path through an ordering is at
+                      least PDFS_OV_AUTO_DEPTH / 10 of the ordering's column steps per
+                      resident workgroup (one 512^3 field: 1.7); ALWAYS wherever possible;
+                      else one launch per ordering. Results are the same bits. */
 } pdfs_opts;
 
 typedef struct {

@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libpdfs.so")
 PDFS_OK, PDFS_EINVAL, PDFS_EHIP, PDFS_ENOMEM, PDFS_ETIMEOUT, PDFS_ENOTCONV = 0, -1, -2, -3, -4, -5
 PDFS_F64_REF, PDFS_F32 = 0, 1
 PDFS_STORE_F32, PDFS_STORE_FIXED = 0, 1
+PDFS_OVERLAP_OFF, PDFS_OVERLAP_AUTO, PDFS_OVERLAP_ALWAYS = 0, 1, 2
+PDFS_OV_AUTO_FIELDS, PDFS_OV_AUTO_DEPTH = 2, 6
 GEN_POINT, GEN_TERRAIN, GEN_SPHERES = 0, 1, 2
 TILE_16x16, TILE_32x8, TILE_16x8, TILE_AUTO = 0, 1, 2, -2
 TILE_32x16, TILE_16x16_1CU, TILE_16x8_2CU, TILE_8x8, TILE_16x4 = 3, 4, 5, 6, 7

@@ -57,6 +57,10 @@ int pdfsi_default_tile(int nf, int x, int y);
 int pdfsi_ov_order(int nf, const int *tnx, const int *tny, const int *fz, int nc,
                    const int *cols, int TX, int TY, int sn0, int sn1, int grid, int *out);
 
+/* Workgroups of the column kernel (tile variant, sweep mode) resident on
+ * the device at once: the persistent grid of a launch. Needs the GPU. */
+int pdfsi_col_resident(int tile, int mode);
+
 #ifdef __cplusplus
 }
 #endif

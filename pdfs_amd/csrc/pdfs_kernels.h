@@ -166,11 +166,13 @@ int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *s
  * its upwind columns in the slot, itself and its 4 physical neighbours in
  * the slot before, and all columns of the slot two before. done: [8][ncolp]
  * epochs (zeroed once, never reset); cnt: [8] counters, zero between
- * launches (the launch leaves them zero). The edge allocation holds two
+ * launches (the launch leaves them zero); chg: [8][nfields] per-slot change
+ * counters, likewise zero between launches, from which the launch leaves
+ * each field's streak as one launch per ordering would. The edge allocation holds two
  * regions, the second rx_bytes (x part) / ry_bytes (y part) after the first;
  * slot o uses region o % 2. Epochs advance by the number of slots. */
 int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, const int *otab, int nt,
-                         int ncolp, unsigned *done, unsigned *cnt, size_t rx_bytes,
+                         int ncolp, unsigned *done, unsigned *cnt, unsigned *chg, size_t rx_bytes,
                          size_t ry_bytes, int mode, const pdfs_sp32 *sp, int variant, int grid,
                          hipStream_t st);
 

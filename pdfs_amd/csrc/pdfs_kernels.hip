@@ -529,6 +529,7 @@ struct ColArgs {
   uint32_t ov_rx, ov_ry;    // bytes between the two edge regions (x part, y part)
   unsigned *ov_done;        // [8][ov_ncolp]: epoch of the last ordering each column finished
   unsigned *ov_cnt;         // [8]: columns finished per ordering slot of this launch
+  unsigned *ov_chg;         // [8][nfields]: columns that changed a value, per slot and field
 };
 
 
@@ -1108,7 +1109,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     // the done flag that the next ordering's columns wait for
     if (OV) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (tid == 0 && s_changed) atomicAdd(a.changed + fi, s_changed);
+    if (tid == 0 && s_changed) {
+      atomicAdd(a.changed + fi, s_changed);
+      if (OV) atomicAdd(a.ov_chg + oslot * a.nfields + fi, 1u);
+    }
     // The release is wave-uniform (wave 0, every lane: the same flag value,
     // 64 counts per column). A lane-0-only tail here let the compiler split
     // wave 0 around the loop's barriers: lanes 1..63 re-entered the loop
@@ -1138,7 +1142,23 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
           __hip_atomic_load(a.changed + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned last = a.streak[2 * f], run = a.streak[2 * f + 1];
       a.streak[2 * f] = cur;
-      a.streak[2 * f + 1] = cur == last ? (run < 0xFFFFu ? run + 1u : run) : 0u;
+      if (!OV) {
+        a.streak[2 * f + 1] = cur == last ? (run < 0xFFFFu ? run + 1u : run) : 0u;
+      } else {
+        // the run one launch per ordering would leave: the slots without a
+        // change after the last slot with one, or the whole launch added
+        unsigned trail = 0u;
+        bool any = false;
+        for (int o = a.ov_n - 1; o >= 0; o--) {
+          unsigned *p = a.ov_chg + o * a.nfields + f;
+          const unsigned c = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (c) any = true;
+          if (!any) trail++;
+          *p = 0u;
+        }
+        const unsigned add = run + (unsigned)a.ov_n;
+        a.streak[2 * f + 1] = any ? trail : (add < 0xFFFFu ? add : 0xFFFFu);
+      }
     }
     if (OV && tid < 8) a.ov_cnt[tid] = 0u;
     if (tid == 0) {
@@ -1847,7 +1867,7 @@ extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const p
 }
 
 extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, const int *otab,
-                                    int nt, int ncolp, unsigned *done, unsigned *cnt,
+                                    int nt, int ncolp, unsigned *done, unsigned *cnt, unsigned *chg,
                                     size_t rx_bytes, size_t ry_bytes, int mode,
                                     const pdfs_sp32 *sp, int variant, int grid, hipStream_t st) {
   if (sn0 < 1 || sn1 > 8 || sn1 < sn0 || grid < 1 || nt < 1 || !otab || !done || !cnt ||
@@ -1877,6 +1897,7 @@ extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, cons
   a.ov_ry = (uint32_t)ry_bytes;
   a.ov_done = done;
   a.ov_cnt = cnt;
+  a.ov_chg = chg;
   for (int sn = sn0; sn <= sn1; sn++) {
     const int desc = (sn == 2 || sn == 5 || sn == 7 || sn == 8);
     const int rx = (sn == 4 || sn == 8), ry = (sn == 2 || sn == 6), rz = (sn == 3 || sn == 7);

@@ -23,6 +23,7 @@ struct pdfs_ctx {
   pdfsk_field *h_fields;           /* host copy of the field table */
   int *h_cols;
   int nf, ncols, tile, cap_f, cap_cols;
+  int nact; /* fields with columns in the plan */
   int words;                       /* 32-bit words per value of the current plan (1 or 2) */
   unsigned *h_changed, *prev_changed; /* pinned readback, last values */
   unsigned *h_streak;              /* per field: counter, run of unchanged orderings */
@@ -57,7 +58,11 @@ struct pdfs_ctx {
   long long launches;  /* sweep launches since the context was created */
   int *ov_tab[3];
   int ov_nt[3], ov_cap[3], ov_valid[3];
+  int *ov_key[3];  /* inputs of each segment's table (tile, grid, orderings, fields, columns) */
+  int ov_key_n[3], ov_key_cap[3];
   unsigned *ov_done;
+  unsigned *ov_chg;  /* [8][nf] per-slot change counters of an overlapped launch */
+  size_t ov_chg_cap;
   size_t ov_done_cap;
   int ov_ncolp;
   size_t ov_rx, ov_ry;
@@ -158,7 +163,7 @@ void pdfs_default_opts(pdfs_opts *o)
   o->storage = PDFS_STORE_FIXED;
   o->judge_tol = 2e-6;
   o->stop_quanta = 4;
-  o->overlap = 1;
+  o->overlap = PDFS_OVERLAP_AUTO;
 }
 
 static int create_events(hipEvent_t *ev, int n)
@@ -215,8 +220,12 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->pbuf);
   hipFree(c->conv_dev);
   free(c->conv_host);
-  for (int i = 0; i < 3; i++) hipFree(c->ov_tab[i]);
+  for (int i = 0; i < 3; i++) {
+    hipFree(c->ov_tab[i]);
+    free(c->ov_key[i]);
+  }
   hipFree(c->ov_done);
+  hipFree(c->ov_chg);
   free(c->fx_host);
   hipFree(c->fx_dev);
   hipFree(c->fx_keys);
@@ -364,9 +373,12 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
   int TX = 0, TY = 0;
   pdfsk_tile_dims(c->tile, &TX, &TY);
   const int wx = TX + PDFS_TICKET_EDGE_STEPS, wy = TY + PDFS_TICKET_EDGE_STEPS;
-  int n_all = 0;
+  int n_all = 0, nact = 0;
   for (int f = 0; f < c->nf; f++)
-    if (!active || active[f]) n_all += c->h_fields[f].nTx * c->h_fields[f].nTy;
+    if (!active || active[f]) {
+      n_all += c->h_fields[f].nTx * c->h_fields[f].nTy;
+      nact += c->h_fields[f].nTx * c->h_fields[f].nTy > 0;
+    }
   col_key *keys = (col_key *)malloc((size_t)(n_all > 0 ? n_all : 1) * sizeof(col_key));
   if (!keys) return fail(PDFS_ENOMEM, "plan: host alloc");
   int t = 0;
@@ -395,6 +407,7 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
   HIPCHK(hipMemcpyAsync(c->bt.state + 4, &n, sizeof n, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->ncols = t;
+  c->nact = nact;
   for (int i = 0; i < 3; i++) c->ov_valid[i] = 0;
   return PDFS_OK;
 }
@@ -573,11 +586,53 @@ done:
   return ne;
 }
 
+/* The inputs a segment's ticket table is made from, in `key` (NULL: only
+ * the length): tile, grid, orderings, per-field tiles and depths, the
+ * active columns in ticket order. */
+static int ov_key_fill(const pdfs_ctx *c, int sn0, int sn1, int grid, int *key)
+{
+  const int n = 6 + 3 * c->nf + 3 * c->ncols;
+  if (!key) return n;
+  key[0] = c->tile;
+  key[1] = grid;
+  key[2] = sn0;
+  key[3] = sn1;
+  key[4] = c->nf;
+  key[5] = c->ncols;
+  for (int f = 0; f < c->nf; f++) {
+    key[6 + 3 * f] = c->h_fields[f].nTx;
+    key[7 + 3 * f] = c->h_fields[f].nTy;
+    key[8 + 3 * f] = c->h_fields[f].Z;
+  }
+  memcpy(key + 6 + 3 * c->nf, c->h_cols, (size_t)3 * c->ncols * sizeof(int));
+  return n;
+}
+
 static int ov_build(pdfs_ctx *c, int seg, int sn0, int sn1, int grid)
 {
   int TX, TY;
   pdfsk_tile_dims(c->tile, &TX, &TY);
   const int ns = sn1 - sn0 + 1, nc = c->ncols, nt = ns * nc;
+  /* the same inputs as the table already uploaded: keep it (repeated solves
+   * of fields of the same shapes order their tickets once) */
+  const int kn = ov_key_fill(c, sn0, sn1, grid, NULL);
+  if (kn > c->ov_key_cap[seg]) {
+    free(c->ov_key[seg]);
+    c->ov_key_n[seg] = c->ov_key_cap[seg] = 0;
+    if (!(c->ov_key[seg] = (int *)malloc((size_t)kn * sizeof(int))))
+      return fail(PDFS_ENOMEM, "overlap plan: host alloc");
+    c->ov_key_cap[seg] = kn;
+  }
+  int *key = (int *)malloc((size_t)kn * sizeof(int));
+  if (!key) return fail(PDFS_ENOMEM, "overlap plan: host alloc");
+  ov_key_fill(c, sn0, sn1, grid, key);
+  if (c->ov_key_n[seg] == kn && c->ov_nt[seg] == nt &&
+      !memcmp(key, c->ov_key[seg], (size_t)kn * sizeof(int))) {
+    free(key);
+    c->ov_valid[seg] = 1;
+    return PDFS_OK;
+  }
+  c->ov_key_n[seg] = 0;
   int *tnx = (int *)malloc((size_t)(c->nf > 0 ? c->nf : 1) * 3 * sizeof(int));
   int *out = (int *)malloc((size_t)5 * (nt > 0 ? nt : 1) * sizeof(int));
   int rc = PDFS_OK;
@@ -618,15 +673,51 @@ static int ov_build(pdfs_ctx *c, int seg, int sn0, int sn1, int grid)
   }
   c->ov_nt[seg] = nt;
   c->ov_valid[seg] = 1;
+  memcpy(c->ov_key[seg], key, (size_t)kn * sizeof(int));
+  c->ov_key_n[seg] = kn;
   /* a captured launch of this segment holds the old table's size */
   if (c->gexec[seg]) {
     hipGraphExecDestroy(c->gexec[seg]);
     c->gexec[seg] = NULL;
   }
 done:
+  free(key);
   free(tnx);
   free(out);
   return rc;
+}
+
+/* Whether the automatic choice takes overlapped passes for the active plan:
+ * at most PDFS_OV_AUTO_FIELDS fields, and the longest field's critical path
+ * through one ordering (its upwind column lags plus one column) at least
+ * PDFS_OV_AUTO_DEPTH (in tenths) of the ordering's column steps per resident
+ * workgroup. Overlap then fills the drain of one ordering with the start of
+ * the next; with more work per workgroup, or many fields, it does not pay
+ * (measured: one 512^3 field 1.38x kernel time, two 1.13x, eight 0.95x;
+ * 32 fields of 128^3 0.65x). */
+static int ov_auto(const pdfs_ctx *c, int grid)
+{
+  int TX, TY;
+  pdfsk_tile_dims(c->tile, &TX, &TY);
+  const double lx = TX + PDFS_TICKET_EDGE_STEPS, ly = TY + PDFS_TICKET_EDGE_STEPS;
+  int nact, last = -1;
+  double work = 0.0, cp = 0.0;
+  for (int i = 0; i < c->ncols; i++) {
+    const int f = c->h_cols[3 * i];
+    const pdfsk_field *fd = &c->h_fields[f];
+    const double col = fd->Z + TX + TY - 2;
+    work += col;
+    if (f != last) {
+      /* the columns of a field need not be adjacent in ticket order */
+      last = f;
+      const double p = (fd->nTx - 1) * lx + (fd->nTy - 1) * ly + col;
+      if (p > cp) cp = p;
+    }
+  }
+  nact = c->nact;
+  if (nact == 0 || nact > PDFS_OV_AUTO_FIELDS) return 0;
+  const double slots = grid < c->ncols ? grid : c->ncols;
+  return 10.0 * cp * slots >= (double)PDFS_OV_AUTO_DEPTH * work;
 }
 
 /* Face entries of a batch's edge allocation for a tile (one row per column
@@ -726,6 +817,16 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
       c->ov_done_cap = (size_t)8 * ncols;
     }
     c->ov_ncolp = ncols;
+    if ((size_t)8 * nf > c->ov_chg_cap) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      hipFree(c->ov_chg);
+      c->ov_chg = NULL;
+      c->ov_chg_cap = 0;
+      HIPCHK(hipMalloc((void **)&c->ov_chg, (size_t)8 * nf * sizeof(unsigned)));
+      c->ov_chg_cap = (size_t)8 * nf;
+    }
+    /* zero between launches (each launch leaves them zero) */
+    KCHK(pdfsk_fill_u32(c->ov_chg, 0u, (size_t)8 * nf, c->st));
   }
   if (ex > c->cap_ex || ey > c->cap_ey) {
     /* one allocation: the x-face part, then the y-face part, each with a
@@ -799,6 +900,12 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
   return plan_build_ex(c, nf, (void *const *)fields, NULL, NULL, dims, tile, 0, 1);
 }
 
+int pdfsi_col_resident(int tile, int mode)
+{
+  if (tile < 0 || tile >= PDFSK_TILE_COUNT || mode < 0 || mode >= PDFSK_MODE_COUNT) return -1;
+  return pdfsk_col_max_resident(tile, mode);
+}
+
 static int launch_grid(pdfs_ctx *c, int mode)
 {
   int grid = c->resident[c->tile][mode];
@@ -817,10 +924,11 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
   int grid = launch_grid(c, mode);
   if (grid <= 0) return fail(PDFS_EHIP, "column kernel cannot be resident");
   const int gi = sn0 == 1 && sn1 == 8 ? 0 : sn0 == 1 && sn1 == 4 ? 1 : sn0 == 5 && sn1 == 8 ? 2 : -1;
-  /* overlapped: the segment's orderings in one launch */
+  /* overlapped: the segment's orderings in one launch (automatic: ov_auto) */
   const int ov = c->ov_use && c->ov_ok && gi >= 0 &&
                  (mode == PDFSK_MODE_F32_ISO || mode == PDFSK_MODE_F32_ANISO ||
-                  mode == PDFSK_MODE_FX_ISO);
+                  mode == PDFSK_MODE_FX_ISO) &&
+                 (c->ov_use == PDFS_OVERLAP_ALWAYS || ov_auto(c, grid));
   c->last_ov = ov;
   if (ov) {
     int rc;
@@ -836,7 +944,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         if (pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
-                                 c->ov_done, c->bt.state + 16, c->ov_rx, c->ov_ry, mode, sp,
+                                 c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode, sp,
                                  c->tile, grid, c->st)) {
           hipStreamEndCapture(c->st, &g);
           return fail(PDFS_EHIP, "launch during capture failed");
@@ -855,7 +963,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
     }
     if (profile) HIPCHK(hipEventRecord(c->kev[0], c->st));
     KCHK(pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
-                              c->ov_done, c->bt.state + 16, c->ov_rx, c->ov_ry, mode, sp,
+                              c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode, sp,
                               c->tile, grid, c->st));
     c->launches++;
     if (profile) {
@@ -1624,7 +1732,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   s.storage = fixed ? PDFS_STORE_FIXED : PDFS_STORE_F32;
   if ((rc = plan_build_ex(c, nf, (void *const *)fields, NULL, slows, dims, tile, 0, 1))) goto out;
   /* whole and half passes as one launch each where the plan allows it */
-  c->ov_use = o->overlap && !speed;
+  c->ov_use = speed ? 0 : o->overlap;
   const long long launches0 = c->launches;
   for (int f = 0; f < nf; f++) {
     active[f] = 1;

@@ -11,6 +11,7 @@ import oracle_ctypes as O
 import pdfs_amd
 
 pytestmark = pytest.mark.gpu
+ALWAYS, OFF = pdfs_amd.PDFS_OVERLAP_ALWAYS, pdfs_amd.PDFS_OVERLAP_OFF
 
 
 def _start(X, Y, Z, seed, kind="spheres"):
@@ -34,8 +35,8 @@ def test_overlap_equals_per_ordering_launches(ctx, dev, tile, shape, storage):
     X, Y, Z = shape
     start = _start(X, Y, Z, 31 + X)
     a, b = _dev(start, dev), _dev(start, dev)
-    sa = ctx.solve_f32(a, None, judge=0, tile=tile, storage=storage, polish=0, overlap=1)
-    sb = ctx.solve_f32(b, None, judge=0, tile=tile, storage=storage, polish=0, overlap=0)
+    sa = ctx.solve_f32(a, None, judge=0, tile=tile, storage=storage, polish=0, overlap=ALWAYS)
+    sb = ctx.solve_f32(b, None, judge=0, tile=tile, storage=storage, polish=0, overlap=OFF)
     print(f"{shape} tile {tile} storage {storage}: passes {sa['passes']} / {sb['passes']}, "
           f"launches {sa['sweep_launches']} / {sb['sweep_launches']}", flush=True)
     assert sa["passes"] == sb["passes"]
@@ -56,8 +57,8 @@ def test_overlap_batch_equals_per_ordering(ctx, dev, graph, certify):
     fb = [_dev(s, dev) for s in starts]
     inits = [_dev(s, dev) for s in starts]
     kw = dict(tile=pdfs_amd.TILE_32x16, graph=graph, certify=certify)
-    sa, pa, _ = ctx.solve_batch_f32(fa, inits, overlap=1, **kw)
-    sb, pb, _ = ctx.solve_batch_f32(fb, inits, overlap=0, **kw)
+    sa, pa, _ = ctx.solve_batch_f32(fa, inits, overlap=ALWAYS, **kw)
+    sb, pb, _ = ctx.solve_batch_f32(fb, inits, overlap=OFF, **kw)
     print(f"batch graph {graph} certify {certify}: passes {pa} / {pb}, launches "
           f"{sa['sweep_launches']} / {sb['sweep_launches']}", flush=True)
     assert pa == pb and sa["judge_flagged"] == 0
@@ -73,13 +74,74 @@ def test_overlap_repeated_solves_and_unaligned_fallback(ctx, dev):
     ref = None
     for r in range(4):
         t = _dev(start, dev)
-        st = ctx.solve_f32(t, None, judge=0, tile=pdfs_amd.TILE_16x16, overlap=1)
+        st = ctx.solve_f32(t, None, judge=0, tile=pdfs_amd.TILE_16x16, overlap=ALWAYS)
         if ref is None:
             ref = t.clone()
         assert torch.equal(t.view(torch.int32), ref.view(torch.int32)), r
     odd = _start(50, 37, 20, 3)
     t = _dev(odd, dev)
-    st = ctx.solve_f32(t, None, judge=0, tile=pdfs_amd.TILE_16x16, overlap=1)
+    st = ctx.solve_f32(t, None, judge=0, tile=pdfs_amd.TILE_16x16, overlap=ALWAYS)
     assert st["sweep_launches"] >= 8 * (st["passes"] - 1)
     want, _, _, _ = O.solve_fixed(odd)
     assert np.array_equal(t.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+def test_auto_takes_overlap_for_one_field_not_for_many(ctx, dev):
+    """The default (PDFS_OVERLAP_AUTO) overlaps the passes of one field whose
+    ordering is dominated by its critical path, and keeps one launch per
+    ordering for a batch of more fields than PDFS_OV_AUTO_FIELDS."""
+    start = _start(64, 64, 40, 5)
+    t = _dev(start, dev)
+    st = ctx.solve_f32(t, None, judge=0, tile=pdfs_amd.TILE_32x16)
+    assert st["sweep_launches"] * 4 <= st["orderings"]
+    want, _, _, _ = O.solve_fixed(start)
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    n = pdfs_amd.PDFS_OV_AUTO_FIELDS + 1
+    fs = [_dev(_start(64, 64, 40, 6 + i), dev) for i in range(n)]
+    st3, _, _ = ctx.solve_batch_f32(fs, [f.clone() for f in fs], tile=pdfs_amd.TILE_32x16)
+    assert st3["sweep_launches"] >= st3["orderings"]
+
+
+def test_ticket_tables_follow_the_shapes(ctx, dev):
+    """Alternating shapes on one context (the ticket table is kept while the
+    shapes repeat and rebuilt when they change): every solve gives the bits
+    of one launch per ordering."""
+    shapes = [(64, 64, 40), (64, 64, 24), (64, 64, 40), (32, 96, 40), (64, 64, 40)]
+    for i, s in enumerate(shapes):
+        start = _start(*s, 70 + i)
+        a, b = _dev(start, dev), _dev(start, dev)
+        sa = ctx.solve_f32(a, None, judge=0, tile=pdfs_amd.TILE_16x16, overlap=ALWAYS)
+        sb = ctx.solve_f32(b, None, judge=0, tile=pdfs_amd.TILE_16x16, overlap=OFF)
+        assert sa["passes"] == sb["passes"], (i, s)
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32)), (i, s)
+
+
+@pytest.mark.parametrize("certify", [0, 1])
+def test_overlap_keeps_the_unchanged_ordering_streak(ctx, dev, certify):
+    """A point source in a corner is exact after the pass's first ordering,
+    so the orderings after it change nothing: one launch per ordering counts
+    that streak and ends the field by the plain stop (its last pass counted,
+    not launched) before the judge would run. An overlapped launch leaves
+    the same streak, so both give the same passes and bits, alone and in a
+    batch beside a field that needs more passes."""
+    X, Y, Z = 32, 32, 32
+    corner = np.full((Z, Y, X), np.inf, dtype=np.float32)
+    corner[0, 0, 0] = 0.0
+    other = _start(64, 64, 40, 12)
+    res = {}
+    for ov in (ALWAYS, OFF):
+        t = _dev(corner, dev)
+        s1 = ctx.solve_f32(t, _dev(corner, dev), certify=certify, tile=pdfs_amd.TILE_16x16,
+                           overlap=ov)
+        fs = [_dev(corner, dev), _dev(other, dev)]
+        sb, pb, _ = ctx.solve_batch_f32(fs, [f.clone() for f in fs], certify=certify,
+                                        tile=pdfs_amd.TILE_16x16, overlap=ov)
+        res[ov] = (s1["passes"], t, pb, fs, s1["sweep_launches"], sb["sweep_launches"])
+    a, b = res[ALWAYS], res[OFF]
+    print(f"certify {certify}: single passes {a[0]} / {b[0]}, batch {a[2]} / {b[2]}, "
+          f"launches {a[4]} / {b[4]}, {a[5]} / {b[5]}", flush=True)
+    assert a[0] == b[0] and a[2] == b[2]
+    assert a[4] < b[4] and a[5] < b[5]
+    assert torch.equal(a[1].view(torch.int32), b[1].view(torch.int32))
+    for x, y in zip(a[3], b[3]):
+        assert torch.equal(x.view(torch.int32), y.view(torch.int32))

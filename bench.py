@@ -318,10 +318,10 @@ def profile_batch(args):
 
 
 def sweep_launches(st):
-    """Launches of the column sweep (fixed point or FP32 storage) in a solve:
-    the solver's count of sweep launches less the FP64 polish launches (one
-    per polish ordering)."""
-    return max(1, st["sweep_launches"] - st["polish_orderings"])
+    """Launches of the column sweep (fixed point or FP32 storage) in a solve,
+    as the solver counts them (the FP64 polish's launches are not among
+    them)."""
+    return max(1, st["sweep_launches"])
 
 
 def measured_traffic(kernel="sweep"):

@@ -123,9 +123,11 @@ typedef struct {
   int stop_quanta; /* fixed-point storage, plain stop: a field ends on a pass that changes no
                       value by more than this many quanta (default 4; 0: a pass that changes
                       nothing, the FP32 storage rule) */
-  int overlap;     /* FP32 path, PDFS_OVERLAP_*: sweep a pass (or half pass) in one persistent
-                      launch whose orderings overlap (a column starts ordering k + 1 once it
-                      and its four neighbours have finished ordering k). Possible when every
+  int overlap;     /* PDFS_OVERLAP_* (FP32 path, fixed point, speed fields, and the FP64
+                      reference arithmetic of pdfs_solve; not the FP64 polish): sweep a pass
+                      (or half pass) in one persistent launch whose orderings overlap (a
+                      column starts ordering k + 1 once it and its four neighbours have
+                      finished ordering k). Possible when every
                       field's extents are multiples of the column tile and the edge regions
                       fit. AUTO (the default) takes it for at most PDFS_OV_AUTO_FIELDS
                       active fields whose longest critical path through an ordering is at

@@ -175,6 +175,8 @@ def lib():
         L.pdfs_ctx_create.argtypes = []
         L.pdfs_ctx_create.restype = vp
         L.pdfs_ctx_destroy.argtypes = [vp]
+        L.pdfsi_set_overlap64.argtypes = [vp, I]
+        L.pdfsi_set_overlap64.restype = I
         L.pdfs_last_error.restype = C.c_char_p
         L.pdfs_solve_f32_dev.argtypes = [vp, vp, vp, I, I, I, D, D, D, C.POINTER(Opts),
                                          C.POINTER(Stats)]

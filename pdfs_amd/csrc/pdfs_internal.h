@@ -57,6 +57,11 @@ int pdfsi_default_tile(int nf, int x, int y);
 int pdfsi_ov_order(int nf, const int *tnx, const int *tny, const int *fz, int nc,
                    const int *cols, int TX, int TY, int sn0, int sn1, int grid, int *out);
 
+/* Overlapped passes (PDFS_OVERLAP_*) of the FP64 reference-arithmetic
+ * solves that take no options (pdfs_solve_ref64_dev, run_fsm,
+ * pdfs_calc_dist_field); PDFS_OVERLAP_AUTO when the context is made. */
+int pdfsi_set_overlap64(pdfs_ctx *c, int overlap);
+
 /* Workgroups of the column kernel (tile variant, sweep mode) resident on
  * the device at once: the persistent grid of a launch. Needs the GPU. */
 int pdfsi_col_resident(int tile, int mode);

@@ -158,9 +158,9 @@ int pdfsk_col_max_resident(int variant, int mode);
 int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *sp,
                       const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st);
 
-/* Orderings sn0..sn1 of a pass in one persistent launch (FP32 modes and the
- * fixed-point mode; every field's extents multiples of the tile, so each
- * ordering tiles the same physical columns). otab: nt tickets of 5 ints
+/* Orderings sn0..sn1 of a pass in one persistent launch (modes F32_ISO,
+ * F32_ANISO, FX_ISO, F64_REF and F32_SPEED; every field's extents
+ * multiples of the tile, so each ordering tiles the same physical columns). otab: nt tickets of 5 ints
  * (ordering slot sn - sn0, field, virtual tile a, b, physical column index
  * in [0, ncolp)) in an order where every ticket's dependencies come first:
  * its upwind columns in the slot, itself and its 4 physical neighbours in
@@ -173,8 +173,8 @@ int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *s
  * slot o uses region o % 2. Epochs advance by the number of slots. */
 int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, const int *otab, int nt,
                          int ncolp, unsigned *done, unsigned *cnt, unsigned *chg, size_t rx_bytes,
-                         size_t ry_bytes, int mode, const pdfs_sp32 *sp, int variant, int grid,
-                         hipStream_t st);
+                         size_t ry_bytes, int mode, const pdfs_sp32 *sp, const pdfs_sp64 *sp64,
+                         int variant, int grid, hipStream_t st);
 
 /* One ordering sn of the reference hyperplane schedule, FP64, on the padded
  * array, one launch per level (reference arithmetic, bitwise parity).

@@ -1869,12 +1869,15 @@ extern "C" int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const p
 extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, const int *otab,
                                     int nt, int ncolp, unsigned *done, unsigned *cnt, unsigned *chg,
                                     size_t rx_bytes, size_t ry_bytes, int mode,
-                                    const pdfs_sp32 *sp, int variant, int grid, hipStream_t st) {
-  if (sn0 < 1 || sn1 > 8 || sn1 < sn0 || grid < 1 || nt < 1 || !otab || !done || !cnt ||
-      !pdfsk_tile_supports(variant, mode) ||
-      (mode != PDFSK_MODE_F32_ISO && mode != PDFSK_MODE_F32_ANISO && mode != PDFSK_MODE_FX_ISO) ||
-      (mode == PDFSK_MODE_FX_ISO && !b->maxdiff))
+                                    const pdfs_sp32 *sp, const pdfs_sp64 *sp64, int variant,
+                                    int grid, hipStream_t st) {
+  if (sn0 < 1 || sn1 > 8 || sn1 < sn0 || grid < 1 || nt < 1 || !otab || !done || !cnt || !chg ||
+      !pdfsk_tile_supports(variant, mode))
     return -1;
+  if (mode != PDFSK_MODE_F32_ISO && mode != PDFSK_MODE_F32_ANISO && mode != PDFSK_MODE_FX_ISO &&
+      mode != PDFSK_MODE_F64_REF && mode != PDFSK_MODE_F32_SPEED)
+    return -1;
+  if ((PDFSK_MODE_IS_F64(mode) || mode == PDFSK_MODE_FX_ISO) && !b->maxdiff) return -1;
   ColArgs a;
   memset(&a, 0, sizeof a);
   a.fields = (const FieldDesc *)b->fields;
@@ -1890,6 +1893,7 @@ extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, cons
   a.nfields = b->nfields;
   a.err = b->err;
   if (sp) a.sp = *sp;
+  if (sp64) a.sp64 = *sp64;
   a.ov_n = sn1 - sn0 + 1;
   a.ov_nt = nt;
   a.ov_ncolp = ncolp;
@@ -1917,8 +1921,12 @@ extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, cons
     go(std::integral_constant<int, PDFSK_MODE_FX_ISO>());
   else if (mode == PDFSK_MODE_F32_ISO)
     go(std::integral_constant<int, PDFSK_MODE_F32_ISO>());
-  else
+  else if (mode == PDFSK_MODE_F32_ANISO)
     go(std::integral_constant<int, PDFSK_MODE_F32_ANISO>());
+  else if (mode == PDFSK_MODE_F64_REF)
+    go(std::integral_constant<int, PDFSK_MODE_F64_REF>());
+  else
+    go(std::integral_constant<int, PDFSK_MODE_F32_SPEED>());
   return hip_rc(hipGetLastError());
 }
 

@@ -24,6 +24,7 @@ struct pdfs_ctx {
   int *h_cols;
   int nf, ncols, tile, cap_f, cap_cols;
   int nact; /* fields with columns in the plan */
+  int overlap64; /* opts.overlap of the FP64 entry points without options (default AUTO) */
   int words;                       /* 32-bit words per value of the current plan (1 or 2) */
   unsigned *h_changed, *prev_changed; /* pinned readback, last values */
   unsigned *h_streak;              /* per field: counter, run of unchanged orderings */
@@ -177,6 +178,7 @@ pdfs_ctx *pdfs_ctx_create(void)
 {
   pdfs_ctx *c = (pdfs_ctx *)calloc(1, sizeof *c);
   if (!c) return NULL;
+  c->overlap64 = PDFS_OVERLAP_AUTO;
   if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
       hipMalloc((void **)&c->bt.state, 32 * sizeof(unsigned)) != hipSuccess ||
       hipMalloc((void **)&c->d_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
@@ -795,7 +797,7 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
   }
   /* overlapped passes need every field to tile exactly (each ordering then
    * tiles the same physical columns) and a second edge region */
-  int ov = words == 1;
+  int ov = 1;
   for (int f = 0; f < nf && ov; f++)
     ov = dims[3 * f] % TX == 0 && dims[3 * f + 1] % TY == 0;
   if (ov && (2 * (ex + ey) + 4 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31)) ov = 0;
@@ -900,6 +902,13 @@ static int plan_build(pdfs_ctx *c, int nf, float *const *fields, const int *dims
   return plan_build_ex(c, nf, (void *const *)fields, NULL, NULL, dims, tile, 0, 1);
 }
 
+int pdfsi_set_overlap64(pdfs_ctx *c, int overlap)
+{
+  if (!c || overlap < PDFS_OVERLAP_OFF || overlap > PDFS_OVERLAP_ALWAYS) return PDFS_EINVAL;
+  c->overlap64 = overlap;
+  return PDFS_OK;
+}
+
 int pdfsi_col_resident(int tile, int mode)
 {
   if (tile < 0 || tile >= PDFSK_TILE_COUNT || mode < 0 || mode >= PDFSK_MODE_COUNT) return -1;
@@ -927,7 +936,8 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
   /* overlapped: the segment's orderings in one launch (automatic: ov_auto) */
   const int ov = c->ov_use && c->ov_ok && gi >= 0 &&
                  (mode == PDFSK_MODE_F32_ISO || mode == PDFSK_MODE_F32_ANISO ||
-                  mode == PDFSK_MODE_FX_ISO) &&
+                  mode == PDFSK_MODE_FX_ISO || mode == PDFSK_MODE_F64_REF ||
+                  mode == PDFSK_MODE_F32_SPEED) &&
                  (c->ov_use == PDFS_OVERLAP_ALWAYS || ov_auto(c, grid));
   c->last_ov = ov;
   if (ov) {
@@ -944,7 +954,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         if (pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
-                                 c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode, sp,
+                                 c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode, sp, sp64,
                                  c->tile, grid, c->st)) {
           hipStreamEndCapture(c->st, &g);
           return fail(PDFS_EHIP, "launch during capture failed");
@@ -963,7 +973,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
     }
     if (profile) HIPCHK(hipEventRecord(c->kev[0], c->st));
     KCHK(pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
-                              c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode, sp,
+                              c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode, sp, sp64,
                               c->tile, grid, c->st));
     c->launches++;
     if (profile) {
@@ -1109,7 +1119,7 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
                          const float *const *inits, const float *const *slows, const int *dims,
                          const pdfs_sp64 *sp64,
                          int tile, int fixed_passes, double tol, int stop_on_last, int maxp,
-                         int graph, int profile, pdfs_stats *st, int *field_passes,
+                         int graph, int profile, int overlap, pdfs_stats *st, int *field_passes,
                          double *field_maxdiff)
 {
   double kms = 0.0;
@@ -1117,6 +1127,11 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
   memset(&s, 0, sizeof s);
   int rc = plan_build_ex(c, nf, (void *const *)fields, inits, slows, dims, tile, 0, 2);
   if (rc) return rc;
+  /* overlapped passes for the reference arithmetic; a stop on the last
+   * ordering's change (the polish) needs that ordering's own maximum, which
+   * an overlapped launch does not keep */
+  c->ov_use = mode == PDFSK_MODE_F64_REF && !stop_on_last ? overlap : 0;
+  const long long launches0 = c->launches;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
   double *md = (double *)calloc((size_t)2 * nf, sizeof(double));
   double *md8 = md ? md + nf : NULL;
@@ -1151,6 +1166,7 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
       const long long nfp = (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
       const unsigned run = c->h_streak[2 * f + 1];
       long long skipped = run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
+      if (c->last_ov) skipped = 0; /* an overlapped launch skips nothing */
       if (skipped > 8) skipped = 8;
       s.point_updates -= skipped * nfp;
       fp[f] = p + 1;
@@ -1192,6 +1208,8 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
   }
   s.tile = tile;
   s.ms_sweep_kernels = kms;
+  s.sweep_launches = (int)(c->launches - launches0);
+  c->ov_use = 0;
   if (rc == PDFS_OK && nactive > 0)
     rc = fail(PDFS_ENOTCONV, "%d field(s) not converged after %d passes", nactive, maxp);
 out:
@@ -1272,7 +1290,7 @@ static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *c
     const int mode = slows ? PDFSK_MODE_F64_POL_SPEED
                            : sp64->iso ? PDFSK_MODE_F64_POL_ISO : PDFSK_MODE_F64_POL_ANISO;
     rc = sweep64_batch(c, mode, k, f64, inits, slows, dims, sp64, tile, 0, tol, 1, maxp, graph,
-                       profile, ps, NULL, NULL);
+                       profile, PDFS_OVERLAP_OFF, ps, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) goto out;
     const int rc0 = rc;
     rc = PDFS_OK;
@@ -1732,7 +1750,7 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
   s.storage = fixed ? PDFS_STORE_FIXED : PDFS_STORE_F32;
   if ((rc = plan_build_ex(c, nf, (void *const *)fields, NULL, slows, dims, tile, 0, 1))) goto out;
   /* whole and half passes as one launch each where the plan allows it */
-  c->ov_use = speed ? 0 : o->overlap;
+  c->ov_use = o->overlap;
   const long long launches0 = c->launches;
   for (int f = 0; f < nf; f++) {
     active[f] = 1;
@@ -2069,7 +2087,7 @@ static int col64_ok(pdfs_ctx *c, const double *padded, int x, int y, int z, int 
  * with a slowness field, or otherwise, on the per-level kernel. */
 static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, int y, int z,
                        double dx, double dy, double dz, int passes, double tol, int maxp,
-                       int tile, pdfs_stats *st)
+                       int tile, int overlap, pdfs_stats *st)
 {
   pdfs_stats s;
   memset(&s, 0, sizeof s);
@@ -2086,7 +2104,7 @@ static int solve_ref64(pdfs_ctx *c, double *padded, const double *slow, int x, i
     make_sp64(dx, dy, dz, &sp64);
     KCHK(pdfsk_pack64(padded, dn, x, y, z, c->st));
     rc = sweep64_batch(c, PDFSK_MODE_F64_REF, 1, &dn, NULL, NULL, dims, &sp64, tile,
-                       passes > 0 ? passes : 0, tol, 0, maxp, 0, 0, &s, NULL, NULL);
+                       passes > 0 ? passes : 0, tol, 0, maxp, 0, 0, overlap, &s, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) return rc;
     KCHK(pdfsk_unpack64(dn, padded, x, y, z, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -2132,7 +2150,7 @@ int pdfs_solve_ref64_dev(pdfs_ctx *c, double *padded, int x, int y, int z, doubl
   if (!c || !padded || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_ref64_dev: bad arguments");
   return solve_ref64(c, padded, NULL, x, y, z, dx, dy, dz, passes, tol, 1000, tile64_of(NULL),
-                     st);
+                     c->overlap64, st);
 }
 
 int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, int x, int y,
@@ -2142,7 +2160,7 @@ int pdfs_solve_speed64_dev(pdfs_ctx *c, double *padded, const double *slowness, 
   if (!c || !padded || !slowness || x < 1 || y < 1 || z < 1 || bad_h(dx, dy, dz))
     return fail(PDFS_EINVAL, "pdfs_solve_speed64_dev: bad arguments");
   return solve_ref64(c, padded, slowness, x, y, z, dx, dy, dz, passes, tol, 1000,
-                     tile64_of(NULL), st);
+                     tile64_of(NULL), c->overlap64, st);
 }
 
 /* ---- host padded FP64 entry points ---- */
@@ -2190,7 +2208,7 @@ static int solve_host(double *distance, int x, int y, int z, double dx, double d
   }
   if (o->precision == PDFS_F64_REF) {
     rc = solve_ref64(c, d, NULL, x, y, z, dx, dy, dz, o->fixed_passes > 0 ? o->fixed_passes : 0,
-                     o->tol, o->max_passes, tile64_of(o), st);
+                     o->tol, o->max_passes, tile64_of(o), o->overlap, st);
     if (rc && rc != PDFS_ENOTCONV) goto out;
   } else {
     if (hipMalloc((void **)&f, n * sizeof(float)) != hipSuccess ||
@@ -2418,7 +2436,7 @@ int pdfs_calc_dist_field(Phi *pf, int passes, int precision)
   }
   if (precision == PDFS_F64_REF) {
     if ((rc = solve_ref64(c, d, NULL, x, y, z, pf->dx, pf->dy, pf->dz, passes, 1e-12, 1000,
-                          tile64_of(NULL), NULL)))
+                          tile64_of(NULL), c->overlap64, NULL)))
       goto out;
   } else {
     pdfs_opts o;

@@ -145,3 +145,66 @@ def test_overlap_keeps_the_unchanged_ordering_streak(ctx, dev, certify):
     assert torch.equal(a[1].view(torch.int32), b[1].view(torch.int32))
     for x, y in zip(a[3], b[3]):
         assert torch.equal(x.view(torch.int32), y.view(torch.int32))
+
+
+def _set64(ctx, v):
+    assert pdfs_amd.lib().pdfsi_set_overlap64(ctx.c, v) == 0
+
+
+@pytest.mark.parametrize("shape,h", [((64, 48, 40), (1.0, 1.0, 1.0)),
+                                     ((48, 64, 24), (1.0, 0.7, 1.3))])
+def test_overlap_fp64_reference_arithmetic(ctx, dev, shape, h):
+    """The FP64 reference-arithmetic solve (the path behind run_fsm) with
+    overlapped passes: after one and two passes and converged, the bits of
+    the oracle's sweeps, and of one launch per ordering, in fewer launches."""
+    X, Y, Z = shape
+    a = O.gen_spheres(X, Y, Z, seed=X + Z, n=5, r=(2.0, 6.0))
+    try:
+        for passes in (1, 2, 0):
+            want = a.copy()
+            if passes:
+                for _ in range(passes):
+                    for sn in range(1, 9):
+                        O.sweep_ref64(want, sn, h)
+            else:
+                p_or, _ = O.solve_ref64(want, h, tol=1e-12)
+            got = {}
+            for ov in (ALWAYS, OFF):
+                _set64(ctx, ov)
+                t = torch.from_numpy(a.copy()).to(dev)
+                st = ctx.solve_ref64(t, passes=passes, tol=1e-12, h=h)
+                got[ov] = (t.cpu().numpy(), st)
+            print(f"{shape} h={h} passes={passes}: launches {got[ALWAYS][1]['sweep_launches']}"
+                  f" / {got[OFF][1]['sweep_launches']}", flush=True)
+            assert got[ALWAYS][1]["sweep_launches"] < got[OFF][1]["sweep_launches"]
+            assert got[ALWAYS][1]["passes"] == got[OFF][1]["passes"]
+            if not passes:
+                assert got[ALWAYS][1]["passes"] == p_or
+            for ov in (ALWAYS, OFF):
+                assert np.array_equal(got[ov][0].view(np.uint64), want.view(np.uint64)), (passes, ov)
+    finally:
+        _set64(ctx, pdfs_amd.PDFS_OVERLAP_AUTO)
+
+
+def test_overlap_speed_path(ctx, dev):
+    """The FP32 speed sweep (a slowness per point) with overlapped passes,
+    then its FP64 speed polish: the bits and passes of one launch per
+    ordering."""
+    X, Y, Z = 64, 48, 40
+    start = _start(X, Y, Z, 21)
+    z = np.arange(Z, dtype=np.float32)[:, None, None]
+    x = np.arange(X, dtype=np.float32)[None, None, :]
+    slow = (1.0 + 0.5 * np.sin(z / 7.0) + 0.25 * np.cos(x / 11.0)).astype(np.float32)
+    slow = np.broadcast_to(slow, (Z, Y, X)).copy()
+    s = _dev(slow, dev)
+    res = {}
+    for ov in (ALWAYS, OFF):
+        t = _dev(start, dev)
+        st = ctx.solve_speed_f32(t, s, init=_dev(start, dev), tile=pdfs_amd.TILE_32x8, overlap=ov)
+        res[ov] = (t, st)
+    a, b = res[ALWAYS], res[OFF]
+    print(f"speed: passes {a[1]['passes']} / {b[1]['passes']}, launches "
+          f"{a[1]['sweep_launches']} / {b[1]['sweep_launches']}", flush=True)
+    assert a[1]["passes"] == b[1]["passes"] and a[1]["polish_passes"] == b[1]["polish_passes"]
+    assert a[1]["sweep_launches"] < b[1]["sweep_launches"]
+    assert torch.equal(a[0].view(torch.int32), b[0].view(torch.int32))

@@ -58,6 +58,9 @@ def main():
                     help="skip the spatially varying speed leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run plain 1 GiB copies (for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--profile-ref64", action="store_true",
+                    help="only run the FP64 reference-arithmetic solve of the --size^3 sphere field "
+                         "(the path behind run_fsm), twice, for a rocprofv3 run; prints its statistics")
     ap.add_argument("--profile-batch", choices=("512", "c4"),
                     help="only run one batch's plain solves (8 x 512^3 spheres, or the c4 batch of "
                          "--batch fields) for a rocprofv3 run; prints their statistics")
@@ -67,6 +70,9 @@ def main():
         return
     if args.profile_batch:
         profile_batch(args)
+        return
+    if args.profile_ref64:
+        profile_ref64(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -269,6 +275,36 @@ def copy_calibration():
         torch.add(a, 0.0, out=b)
     torch.cuda.synchronize()
     print(f"copy calibration: {n * 4} bytes read and written per kernel", flush=True)
+
+
+def profile_ref64(args):
+    """The FP64 reference-arithmetic solve (column pipeline, overlapped passes
+    where the field tiles exactly) of the sphere field, until a pass changes
+    nothing by more than 1e-12, twice; one JSON line with the second solve's
+    passes, launches and times. Algorithmic bytes: 16 B per point-update."""
+    import torch
+
+    import pdfs_amd
+    n = args.size
+    ctx = pdfs_amd.Context()
+    padded = torch.empty((n + 2, n + 2, n + 2), dtype=torch.float64, device="cuda:0")
+    ctx.generate(pdfs_amd.GEN_SPHERES, args.seed, n, n, n, 1.0, padded=padded)
+    work = torch.empty_like(padded)
+    for _ in range(2):
+        work.copy_(padded)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.solve_ref64(work, passes=0, tol=1e-12)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    launches = max(1, st["sweep_launches"])
+    out = {"workload": f"FP64 reference arithmetic, spheres {n}^3, seed {args.seed}",
+           "seconds": round(dt, 4), "passes": st["passes"], "orderings": st["orderings"],
+           "sweep_launches": launches, "ms_sweeps": round(st["ms_sweeps"], 3),
+           "algorithmic_bytes_per_launch": round(16.0 * st["point_updates"] / launches),
+           "gbs_over_ms_sweeps": round(16.0 * st["point_updates"] / (st["ms_sweeps"] * 1e-3) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+    ctx.close()
 
 
 def profile_batch(args):

@@ -1535,15 +1535,39 @@ __device__ __forceinline__ uint32_t fx_key(float v) {
   const uint32_t b = __float_as_uint(v);
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
+// The conversion kernels move 16 bytes per lane where the field is 16-byte
+// aligned (four values; elementwise the same operations), then the rest
+// one value per lane.
+__device__ __forceinline__ uint32_t fx_key_valid(float v) {
+  return v == v ? fx_key(v) : 0xFFFFFFFFu;
+}
 __global__ void k_fx_min_batch(const pdfsk_fx *__restrict__ tab) {
   const pdfsk_fx t = tab[blockIdx.y];
-  uint32_t k = 0xFFFFFFFFu;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
-       p += (long long)gridDim.x * blockDim.x) {
-    const float v = t.f[p];
-    const uint32_t kv = v == v ? fx_key(v) : 0xFFFFFFFFu;
-    k = kv < k ? kv : k;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  uint32_t k = 0xFFFFFFFFu, k2 = 0xFFFFFFFFu;
+  long long head = 0;
+  if (((uintptr_t)t.f & 15u) == 0) {
+    const float4 *f4 = (const float4 *)t.f;
+    const long long n4 = t.n >> 2;
+    long long p = tid;
+    // two loads in flight per lane
+    for (; p + stride < n4; p += 2 * stride) {
+      const float4 a = f4[p], b = f4[p + stride];
+      k = min(k, min(min(fx_key_valid(a.x), fx_key_valid(a.y)),
+                     min(fx_key_valid(a.z), fx_key_valid(a.w))));
+      k2 = min(k2, min(min(fx_key_valid(b.x), fx_key_valid(b.y)),
+                       min(fx_key_valid(b.z), fx_key_valid(b.w))));
+    }
+    if (p < n4) {
+      const float4 a = f4[p];
+      k = min(k, min(min(fx_key_valid(a.x), fx_key_valid(a.y)),
+                     min(fx_key_valid(a.z), fx_key_valid(a.w))));
+    }
+    head = n4 << 2;
   }
+  for (long long p = head + tid; p < t.n; p += stride) k = min(k, fx_key_valid(t.f[p]));
+  k = min(k, k2);
   for (int o = 32; o > 0; o >>= 1) {
     const uint32_t o2 = (uint32_t)__shfl_xor((int)k, o);
     k = o2 < k ? o2 : k;
@@ -1552,25 +1576,56 @@ __global__ void k_fx_min_batch(const pdfsk_fx *__restrict__ tab) {
 }
 // q = rint((v - base) / d) below UINT32_MAX, else UINT32_MAX (and for NaN,
 // +inf); contraction off so that the oracle's C gives the same bits
-__global__ void k_fx_encode_batch(const pdfsk_fx *__restrict__ tab) {
+__device__ __forceinline__ uint32_t fx_enc1(float v, double base, double d) {
 #pragma clang fp contract(off)
+  const double r = __builtin_rint(((double)v - base) / d);
+  return r < 4294967295.0 ? (uint32_t)r : 0xFFFFFFFFu;
+}
+__device__ __forceinline__ float fx_dec1(uint32_t v, double base, double d) {
+#pragma clang fp contract(off)
+  return v == 0xFFFFFFFFu ? __builtin_inff() : (float)(base + (double)v * d);
+}
+__global__ void k_fx_encode_batch(const pdfsk_fx *__restrict__ tab) {
   const pdfsk_fx t = tab[blockIdx.y];
   uint32_t *q = (uint32_t *)t.f;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
-       p += (long long)gridDim.x * blockDim.x) {
-    const double r = __builtin_rint(((double)t.f[p] - t.base) / t.d);
-    q[p] = r < 4294967295.0 ? (uint32_t)r : 0xFFFFFFFFu;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long head = 0;
+  if (((uintptr_t)t.f & 15u) == 0) {
+    const long long n4 = t.n >> 2;
+    for (long long p = tid; p < n4; p += stride) {
+      const float4 a = ((const float4 *)t.f)[p];
+      uint4 o;
+      o.x = fx_enc1(a.x, t.base, t.d);
+      o.y = fx_enc1(a.y, t.base, t.d);
+      o.z = fx_enc1(a.z, t.base, t.d);
+      o.w = fx_enc1(a.w, t.base, t.d);
+      ((uint4 *)q)[p] = o;
+    }
+    head = n4 << 2;
   }
+  for (long long p = head + tid; p < t.n; p += stride) q[p] = fx_enc1(t.f[p], t.base, t.d);
 }
 __global__ void k_fx_decode_batch(const pdfsk_fx *__restrict__ tab) {
-#pragma clang fp contract(off)
   const pdfsk_fx t = tab[blockIdx.y];
   const uint32_t *q = (const uint32_t *)t.f;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < t.n;
-       p += (long long)gridDim.x * blockDim.x) {
-    const uint32_t v = q[p];
-    t.f[p] = v == 0xFFFFFFFFu ? __builtin_inff() : (float)(t.base + (double)v * t.d);
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long head = 0;
+  if (((uintptr_t)t.f & 15u) == 0) {
+    const long long n4 = t.n >> 2;
+    for (long long p = tid; p < n4; p += stride) {
+      const uint4 a = ((const uint4 *)q)[p];
+      float4 o;
+      o.x = fx_dec1(a.x, t.base, t.d);
+      o.y = fx_dec1(a.y, t.base, t.d);
+      o.z = fx_dec1(a.z, t.base, t.d);
+      o.w = fx_dec1(a.w, t.base, t.d);
+      ((float4 *)t.f)[p] = o;
+    }
+    head = n4 << 2;
   }
+  for (long long p = head + tid; p < t.n; p += stride) t.f[p] = fx_dec1(q[p], t.base, t.d);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {

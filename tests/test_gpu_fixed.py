@@ -336,3 +336,24 @@ def test_c4_fixed_batch_equals_single_solves(ctx, dev, mode):
             assert np.array_equal(_bits32(fields[i].cpu().numpy()), _bits32(want)), i
     del inits, fields
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_unaligned_field_pointer(ctx, dev, offset):
+    """A field that starts 4, 8 or 12 bytes into its allocation (the
+    conversions then take their one-value-per-lane path for all of it) gives
+    the bits of the same field 16-byte aligned, and of the oracle."""
+    X, Y, Z = 37, 29, 23
+    start = _start("spheres", X, Y, Z, 90 + offset)
+    n = X * Y * Z
+    buf = torch.empty(n + 4, dtype=torch.float32, device=dev)
+    t = buf[offset:offset + n].view(Z, Y, X)
+    t.copy_(torch.from_numpy(start))
+    assert t.data_ptr() % 16 != 0
+    a = _dev(start, dev)
+    sa = ctx.solve_f32(t, None, judge=0)
+    sb = ctx.solve_f32(a, None, judge=0)
+    assert sa["storage"] == FIXED and sa["passes"] == sb["passes"]
+    assert torch.equal(t.view(torch.int32), a.view(torch.int32))
+    want, _, _, _ = O.solve_fixed(start)
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), want.view(np.uint32))

@@ -224,6 +224,14 @@ typedef struct {
 } pdfsk_conv;
 int pdfsk_convert_batch(const pdfsk_conv *tab, int nf, long long nmax, int dir, hipStream_t st);
 
+/* After a sweep segment: changed[nf], err[0], streak[2 nf] and (md != NULL)
+ * md[f], md[cap_md + f] copied to pinned host memory (visible once the
+ * stream is synchronized), then md[f], md[cap_md + f] zeroed. One launch. */
+int pdfsk_read_counters(const unsigned *changed, const unsigned *err, const unsigned *streak,
+                        unsigned long long *md, int nf, int cap_md, unsigned *h_changed,
+                        unsigned *h_err, unsigned *h_streak, unsigned long long *h_md,
+                        hipStream_t st);
+
 /* Layout conversions: padded FP64 <-> dense FP32 and dense copies. */
 int pdfsk_pack_f64(const double *padded, float *dense, int X, int Y, int Z,
                    hipStream_t st);

@@ -1628,6 +1628,28 @@ __global__ void k_fx_decode_batch(const pdfsk_fx *__restrict__ tab) {
   for (long long p = head + tid; p < t.n; p += stride) t.f[p] = fx_dec1(q[p], t.base, t.d);
 }
 
+// The counters the host reads after a sweep segment, in one launch: each
+// field's changed count, streak pair and (with md) largest changes, written
+// to pinned host memory; the largest changes are then zeroed for the next
+// segment.
+__global__ void k_read_counters(const unsigned *__restrict__ changed, const unsigned *err,
+                                const unsigned *__restrict__ streak, unsigned long long *md,
+                                int nf, int cap_md, unsigned *h_changed, unsigned *h_err,
+                                unsigned *h_streak, unsigned long long *h_md) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+    h_changed[f] = changed[f];
+    h_streak[2 * f] = streak[2 * f];
+    h_streak[2 * f + 1] = streak[2 * f + 1];
+    if (md) {
+      h_md[f] = md[f];
+      h_md[cap_md + f] = md[cap_md + f];
+      md[f] = 0ull;
+      md[cap_md + f] = 0ull;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) h_err[0] = err[0];
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -2143,6 +2165,16 @@ extern "C" int pdfsk_fx_encode_batch(const pdfsk_fx *tab, int nf, long long nmax
 extern "C" int pdfsk_fx_decode_batch(const pdfsk_fx *tab, int nf, long long nmax,
                                      hipStream_t st) {
   return fx_launch(k_fx_decode_batch, tab, nf, nmax, st);
+}
+
+extern "C" int pdfsk_read_counters(const unsigned *changed, const unsigned *err,
+                                   const unsigned *streak, unsigned long long *md, int nf,
+                                   int cap_md, unsigned *h_changed, unsigned *h_err,
+                                   unsigned *h_streak, unsigned long long *h_md, hipStream_t st) {
+  if (nf < 1 || (md && cap_md < nf)) return -1;
+  hipLaunchKernelGGL(k_read_counters, dim3(grid_for(nf, 256)), dim3(256), 0, st, changed, err,
+                     streak, md, nf, cap_md, h_changed, h_err, h_streak, h_md);
+  return hip_rc(hipGetLastError());
 }
 
 extern "C" int pdfsk_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st) {

@@ -232,7 +232,7 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->fx_dev);
   hipFree(c->fx_keys);
   hipHostFree(c->h_fx_keys);
-  free(c->h_streak);
+  hipHostFree(c->h_streak);
   hipFree(c->bt.state);
   hipFree(c->d_u64);
   hipFree(c->jt_dev);
@@ -756,15 +756,18 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
   if (nf > c->nf || !c->h_fields) {
     free(c->h_fields);
     free(c->prev_changed);
-    free(c->h_streak);
+    hipHostFree(c->h_streak);
     c->h_fields = (pdfsk_field *)calloc((size_t)nf, sizeof(pdfsk_field));
     c->prev_changed = (unsigned *)calloc((size_t)nf, sizeof(unsigned));
-    c->h_streak = (unsigned *)calloc((size_t)2 * nf, sizeof(unsigned));
+    c->h_streak = NULL;
     hipHostFree(c->h_changed);
     c->h_changed = NULL;
-    if (!c->h_fields || !c->prev_changed || !c->h_streak ||
-        hipHostMalloc((void **)&c->h_changed, (size_t)nf * sizeof(unsigned), 0) != hipSuccess)
+    if (!c->h_fields || !c->prev_changed ||
+        hipHostMalloc((void **)&c->h_changed, (size_t)nf * sizeof(unsigned), 0) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_streak, (size_t)2 * nf * sizeof(unsigned), 0) !=
+            hipSuccess)
       return fail(PDFS_ENOMEM, "plan: host alloc");
+    memset(c->h_streak, 0, (size_t)2 * nf * sizeof(unsigned));
   }
   const long long maxpts = words == 2 ? PDFSK_MAX_POINTS64 : PDFSK_MAX_POINTS;
   size_t ex = 0, ey = 0;
@@ -1044,20 +1047,11 @@ static int read_changed_md2(pdfs_ctx *c, long long *delta, unsigned *err, double
 static int read_changed_md3(pdfs_ctx *c, long long *delta, unsigned *err, double *maxdiff,
                             double *last, int as_int)
 {
-  HIPCHK(hipMemcpyAsync(c->h_changed, c->bt.changed, (size_t)c->nf * sizeof(unsigned),
-                        hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemcpyAsync(c->h_err, c->bt.err, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemcpyAsync(c->h_streak, c->bt.streak, (size_t)2 * c->nf * sizeof(unsigned),
-                        hipMemcpyDeviceToHost, c->st));
-  if (maxdiff) {
-    HIPCHK(hipMemcpyAsync(c->h_maxdiff, c->bt.maxdiff,
-                          (size_t)c->nf * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          c->st));
-    HIPCHK(hipMemcpyAsync(c->h_maxdiff + c->cap_md, c->bt.maxdiff_last,
-                          (size_t)c->nf * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          c->st));
-    KCHK(pdfsk_fill_u32((uint32_t *)c->bt.maxdiff, 0u, (size_t)4 * c->cap_md, c->st));
-  }
+  /* one launch writes the counters into pinned host memory (and zeroes the
+   * largest changes, which maxdiff_last follows at cap_md) */
+  KCHK(pdfsk_read_counters(c->bt.changed, c->bt.err, c->bt.streak,
+                           maxdiff ? c->bt.maxdiff : NULL, c->nf, c->cap_md, c->h_changed,
+                           c->h_err, c->h_streak, c->h_maxdiff, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   for (int f = 0; f < c->nf; f++) {
     delta[f] = (long long)(unsigned)(c->h_changed[f] - c->prev_changed[f]);

@@ -208,3 +208,27 @@ def test_overlap_speed_path(ctx, dev):
     assert a[1]["passes"] == b[1]["passes"] and a[1]["polish_passes"] == b[1]["polish_passes"]
     assert a[1]["sweep_launches"] < b[1]["sweep_launches"]
     assert torch.equal(a[0].view(torch.int32), b[0].view(torch.int32))
+
+
+@pytest.mark.parametrize("shape,tile", [((32, 16, 40), pdfs_amd.TILE_32x16),
+                                        ((16, 80, 24), pdfs_amd.TILE_16x16),
+                                        ((128, 8, 12), pdfs_amd.TILE_16x8),
+                                        ((64, 32, 1), pdfs_amd.TILE_32x16),
+                                        ((48, 48, 2), pdfs_amd.TILE_16x16)])
+def test_overlap_degenerate_shapes(ctx, dev, shape, tile):
+    """One column per ordering, one-tile-wide strips, a single plane and two
+    planes: overlapped passes give the bits and passes of one launch per
+    ordering and of the oracle."""
+    X, Y, Z = shape
+    start = _start(X, Y, Z, 3 * X + Z, "random")
+    a, b = _dev(start, dev), _dev(start, dev)
+    sa = ctx.solve_f32(a, None, judge=0, tile=tile, overlap=ALWAYS)
+    sb = ctx.solve_f32(b, None, judge=0, tile=tile, overlap=OFF)
+    print(f"{shape}: passes {sa['passes']} / {sb['passes']}, launches {sa['sweep_launches']} / "
+          f"{sb['sweep_launches']}", flush=True)
+    assert sa["passes"] == sb["passes"]
+    assert sa["sweep_launches"] < sb["sweep_launches"]
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+    want, p_or, _, _ = O.solve_fixed(start)
+    assert sa["passes"] == p_or
+    assert np.array_equal(a.cpu().numpy().view(np.uint32), want.view(np.uint32))

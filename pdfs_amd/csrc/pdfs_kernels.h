@@ -159,18 +159,19 @@ int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *s
                       const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st);
 
 /* Orderings sn0..sn1 of a pass in one persistent launch (modes F32_ISO,
- * F32_ANISO, FX_ISO, F64_REF and F32_SPEED; every field's extents
- * multiples of the tile, so each ordering tiles the same physical columns). otab: nt tickets of 5 ints
- * (ordering slot sn - sn0, field, virtual tile a, b, physical column index
- * in [0, ncolp)) in an order where every ticket's dependencies come first:
- * its upwind columns in the slot, itself and its 4 physical neighbours in
- * the slot before, and all columns of the slot two before. done: [8][ncolp]
- * epochs (zeroed once, never reset); cnt: [8] counters, zero between
- * launches (the launch leaves them zero); chg: [8][nfields] per-slot change
- * counters, likewise zero between launches, from which the launch leaves
- * each field's streak as one launch per ordering would. The edge allocation holds two
- * regions, the second rx_bytes (x part) / ry_bytes (y part) after the first;
- * slot o uses region o % 2. Epochs advance by the number of slots. */
+ * F32_ANISO, FX_ISO, F64_REF and F32_SPEED; every field's extents multiples
+ * of the tile, so each ordering tiles the same physical columns). otab: nt
+ * tickets of 5 ints (ordering slot sn - sn0, field, virtual tile a, b,
+ * physical column index in [0, ncolp)) in an order where every ticket's
+ * dependencies come first: its upwind columns in the slot, itself and its 4
+ * physical neighbours in the slot before, and all columns of the slot two
+ * before. done: [8][ncolp] epochs (zeroed once, never reset); cnt: [8]
+ * counters, zero between launches (the launch leaves them zero); chg:
+ * [8][nfields] per-slot change counters, likewise zero between launches,
+ * from which the launch leaves each field's streak as one launch per
+ * ordering would. The edge allocation holds two regions, the second rx_bytes
+ * (x part) / ry_bytes (y part) after the first; slot o uses region o % 2.
+ * Epochs advance by the number of slots. */
 int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, const int *otab, int nt,
                          int ncolp, unsigned *done, unsigned *cnt, unsigned *chg, size_t rx_bytes,
                          size_t ry_bytes, int mode, const pdfs_sp32 *sp, const pdfs_sp64 *sp64,

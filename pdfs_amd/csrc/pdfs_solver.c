@@ -799,8 +799,9 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     ncols += (int)nc;
   }
   /* overlapped passes need every field to tile exactly (each ordering then
-   * tiles the same physical columns) and a second edge region */
-  int ov = 1;
+   * tiles the same physical columns) and a second edge region; the FP64
+   * polish (FP64 words with an init stream) never overlaps */
+  int ov = words == 1 || !inits;
   for (int f = 0; f < nf && ov; f++)
     ov = dims[3 * f] % TX == 0 && dims[3 * f + 1] % TY == 0;
   if (ov && (2 * (ex + ey) + 4 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31)) ov = 0;

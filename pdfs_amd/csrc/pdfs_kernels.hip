@@ -365,12 +365,16 @@ __device__ __noinline__ uint32_t wait_edge(const uint64_t *p, uint32_t epoch, un
 // Overlapped passes: one ordering's critical path matters less when the next
 // ordering fills its drain, so a deeper queue pays (512^3: 17.56 against
 // 18.01 ms of sweeps with depth 2; smaller single fields within 0.5%). With
-// one launch per ordering, depth 3 costs c4's short columns 32%.
+// one launch per ordering, depth 3 costs c4's short columns 32%. Depth 4,
+// and start-gate slack 1 or 3 at depth 3, measured no better.
 #ifndef PDFSK_UE_OV
 #define PDFSK_UE_OV 3
 #endif
 #ifndef PDFSK_GATE_EXTRA
 #define PDFSK_GATE_EXTRA 2
+#endif
+#ifndef PDFSK_GATE_EXTRA_OV
+#define PDFSK_GATE_EXTRA_OV PDFSK_GATE_EXTRA
 #endif
 
 // Isotropic FP64 difference form with its own sort: the operations of
@@ -589,7 +593,8 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
   constexpr int W = S::W;
   // queue depths: plane loads U steps ahead (HBM latency), edge entries UE
   // steps ahead; GATE = start lag beyond the inherent skew
-  constexpr int UE = OV ? PDFSK_UE_OV : PDFSK_UE, GATE = UE + PDFSK_GATE_EXTRA;
+  constexpr int UE = OV ? PDFSK_UE_OV : PDFSK_UE;
+  constexpr int GATE = UE + (OV ? PDFSK_GATE_EXTRA_OV : PDFSK_GATE_EXTRA);
   using G = ColGeom<TX, TY, W>;
   constexpr int NT = G::NT, R = G::R, PW = G::PW, PS = G::PS;
   static_assert(NT % 64 == 0 && W * (TX + TY) <= 64, "compute waves and edge-wave lanes");

@@ -158,9 +158,11 @@ int pdfsk_col_max_resident(int variant, int mode);
 int pdfsk_sweep_batch(const pdfsk_batch *b, int sn, int mode, const pdfs_sp32 *sp,
                       const pdfs_sp64 *sp64, int variant, int grid, hipStream_t st);
 
-/* Orderings sn0..sn1 of a pass in one persistent launch (modes F32_ISO,
- * F32_ANISO, FX_ISO, F64_REF and F32_SPEED; every field's extents multiples
- * of the tile, so each ordering tiles the same physical columns). otab: nt
+/* Orderings sn0..sn1 of a pass in one persistent launch (any sweep mode;
+ * every field's extents multiples of the tile, so each ordering tiles the
+ * same physical columns). FP64 modes: maxdiff takes every slot's largest
+ * change; a launch that ends at ordering 8 also puts its last slot's into
+ * maxdiff_last. otab: nt
  * tickets of 5 ints (ordering slot sn - sn0, field, virtual tile a, b,
  * physical column index in [0, ncolp)) in an order where every ticket's
  * dependencies come first: its upwind columns in the slot, itself and its 4

@@ -541,6 +541,8 @@ struct ColArgs {
   unsigned *ov_done;        // [8][ov_ncolp]: epoch of the last ordering each column finished
   unsigned *ov_cnt;         // [8]: columns finished per ordering slot of this launch
   unsigned *ov_chg;         // [8][nfields]: columns that changed a value, per slot and field
+  unsigned long long *ov_md_last;  // FP64 modes, a launch that ends at ordering 8: its largest
+                                   // change per field (also counted in maxdiff), else NULL
 };
 
 
@@ -1104,7 +1106,10 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
           const unsigned long long t = __shfl_xor(b, o);
           b = t > b ? t : b;
         }
-        if ((tid & 63) == 0 && b) atomicMax(a.maxdiff + fi, b);
+        if ((tid & 63) == 0 && b) {
+          atomicMax(a.maxdiff + fi, b);
+          if (OV && a.ov_md_last && oslot == a.ov_n - 1) atomicMax(a.ov_md_last + fi, b);
+        }
       }
       if constexpr (S::FX) {
         // wave max of the drop in quanta (an integer in the 64-bit slot)
@@ -1963,9 +1968,6 @@ extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, cons
   if (sn0 < 1 || sn1 > 8 || sn1 < sn0 || grid < 1 || nt < 1 || !otab || !done || !cnt || !chg ||
       !pdfsk_tile_supports(variant, mode))
     return -1;
-  if (mode != PDFSK_MODE_F32_ISO && mode != PDFSK_MODE_F32_ANISO && mode != PDFSK_MODE_FX_ISO &&
-      mode != PDFSK_MODE_F64_REF && mode != PDFSK_MODE_F32_SPEED)
-    return -1;
   if ((PDFSK_MODE_IS_F64(mode) || mode == PDFSK_MODE_FX_ISO) && !b->maxdiff) return -1;
   ColArgs a;
   memset(&a, 0, sizeof a);
@@ -1991,6 +1993,7 @@ extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, cons
   a.ov_done = done;
   a.ov_cnt = cnt;
   a.ov_chg = chg;
+  a.ov_md_last = sn1 == 8 ? b->maxdiff_last : nullptr;
   for (int sn = sn0; sn <= sn1; sn++) {
     const int desc = (sn == 2 || sn == 5 || sn == 7 || sn == 8);
     const int rx = (sn == 4 || sn == 8), ry = (sn == 2 || sn == 6), rz = (sn == 3 || sn == 7);
@@ -2006,16 +2009,7 @@ extern "C" int pdfsk_sweep_batch_ov(const pdfsk_batch *b, int sn0, int sn1, cons
       return 0;
     });
   };
-  if (mode == PDFSK_MODE_FX_ISO)
-    go(std::integral_constant<int, PDFSK_MODE_FX_ISO>());
-  else if (mode == PDFSK_MODE_F32_ISO)
-    go(std::integral_constant<int, PDFSK_MODE_F32_ISO>());
-  else if (mode == PDFSK_MODE_F32_ANISO)
-    go(std::integral_constant<int, PDFSK_MODE_F32_ANISO>());
-  else if (mode == PDFSK_MODE_F64_REF)
-    go(std::integral_constant<int, PDFSK_MODE_F64_REF>());
-  else
-    go(std::integral_constant<int, PDFSK_MODE_F32_SPEED>());
+  with_mode(mode, [&](auto m) { return go(m); });
   return hip_rc(hipGetLastError());
 }
 

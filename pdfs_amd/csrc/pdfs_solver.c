@@ -13,6 +13,20 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* A ticket table of the overlapped passes (device, nt tickets of 5 ints, cap
+ * allocated), the inputs it was ordered from (key: tile, grid, orderings,
+ * fields, columns) and when it was last used. */
+typedef struct {
+  int *tab;
+  int nt, cap;
+  int *key;
+  int key_n, key_cap;
+  long long used;
+} ov_table;
+/* tables kept per segment: the plans a solve alternates between (its sweep
+ * and its polish, a batch as fields drop out) order their tickets once */
+#define PDFS_OV_TABLES 4
+
 struct pdfs_ctx {
   int dev;
   hipStream_t st;
@@ -57,10 +71,9 @@ struct pdfs_ctx {
    * segments 1-8, 1-4 and 5-8 for the current active set, done flags */
   int ov_ok, ov_use, last_ov;
   long long launches;  /* sweep launches since the context was created */
-  int *ov_tab[3];
-  int ov_nt[3], ov_cap[3], ov_valid[3];
-  int *ov_key[3];  /* inputs of each segment's table (tile, grid, orderings, fields, columns) */
-  int ov_key_n[3], ov_key_cap[3];
+  ov_table ov_cache[3][PDFS_OV_TABLES]; /* per segment (1-8, 1-4, 5-8) */
+  int ov_cur[3], ov_valid[3];           /* the table of the current plan, if valid */
+  long long ov_clock;
   unsigned *ov_done;
   unsigned *ov_chg;  /* [8][nf] per-slot change counters of an overlapped launch */
   size_t ov_chg_cap;
@@ -179,6 +192,7 @@ pdfs_ctx *pdfs_ctx_create(void)
   pdfs_ctx *c = (pdfs_ctx *)calloc(1, sizeof *c);
   if (!c) return NULL;
   c->overlap64 = PDFS_OVERLAP_AUTO;
+  for (int i = 0; i < 3; i++) c->ov_cur[i] = -1;
   if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
       hipMalloc((void **)&c->bt.state, 32 * sizeof(unsigned)) != hipSuccess ||
       hipMalloc((void **)&c->d_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
@@ -222,10 +236,11 @@ void pdfs_ctx_destroy(pdfs_ctx *c)
   hipFree(c->pbuf);
   hipFree(c->conv_dev);
   free(c->conv_host);
-  for (int i = 0; i < 3; i++) {
-    hipFree(c->ov_tab[i]);
-    free(c->ov_key[i]);
-  }
+  for (int i = 0; i < 3; i++)
+    for (int e = 0; e < PDFS_OV_TABLES; e++) {
+      hipFree(c->ov_cache[i][e].tab);
+      free(c->ov_cache[i][e].key);
+    }
   hipFree(c->ov_done);
   hipFree(c->ov_chg);
   free(c->fx_host);
@@ -615,73 +630,85 @@ static int ov_build(pdfs_ctx *c, int seg, int sn0, int sn1, int grid)
   int TX, TY;
   pdfsk_tile_dims(c->tile, &TX, &TY);
   const int ns = sn1 - sn0 + 1, nc = c->ncols, nt = ns * nc;
-  /* the same inputs as the table already uploaded: keep it (repeated solves
-   * of fields of the same shapes order their tickets once) */
   const int kn = ov_key_fill(c, sn0, sn1, grid, NULL);
-  if (kn > c->ov_key_cap[seg]) {
-    free(c->ov_key[seg]);
-    c->ov_key_n[seg] = c->ov_key_cap[seg] = 0;
-    if (!(c->ov_key[seg] = (int *)malloc((size_t)kn * sizeof(int))))
-      return fail(PDFS_ENOMEM, "overlap plan: host alloc");
-    c->ov_key_cap[seg] = kn;
-  }
   int *key = (int *)malloc((size_t)kn * sizeof(int));
   if (!key) return fail(PDFS_ENOMEM, "overlap plan: host alloc");
   ov_key_fill(c, sn0, sn1, grid, key);
-  if (c->ov_key_n[seg] == kn && c->ov_nt[seg] == nt &&
-      !memcmp(key, c->ov_key[seg], (size_t)kn * sizeof(int))) {
-    free(key);
-    c->ov_valid[seg] = 1;
-    return PDFS_OK;
-  }
-  c->ov_key_n[seg] = 0;
-  int *tnx = (int *)malloc((size_t)(c->nf > 0 ? c->nf : 1) * 3 * sizeof(int));
-  int *out = (int *)malloc((size_t)5 * (nt > 0 ? nt : 1) * sizeof(int));
-  int rc = PDFS_OK;
-  if (!tnx || !out) {
-    rc = fail(PDFS_ENOMEM, "overlap plan: host alloc");
-    goto done;
-  }
-  for (int f = 0; f < c->nf; f++) {
-    tnx[f] = c->h_fields[f].nTx;
-    tnx[c->nf + f] = c->h_fields[f].nTy;
-    tnx[2 * c->nf + f] = c->h_fields[f].Z;
-  }
-  const int ne = pdfsi_ov_order(c->nf, tnx, tnx + c->nf, tnx + 2 * c->nf, nc, c->h_cols, TX, TY,
-                                sn0, sn1, grid, out);
-  if (ne != nt) {
-    rc = fail(PDFS_EINVAL, "overlap plan: %d of %d tickets ordered", ne, nt);
-    goto done;
-  }
-  if (nt > c->ov_cap[seg]) {
-    if (hipStreamSynchronize(c->st) != hipSuccess) {
-      rc = fail(PDFS_EHIP, "overlap plan: synchronize");
+  ov_table *tabs = c->ov_cache[seg];
+  int e = -1, built = 0, rc = PDFS_OK;
+  int *tnx = NULL, *out = NULL;
+  /* a table ordered from the same inputs: use it */
+  for (int i = 0; i < PDFS_OV_TABLES && e < 0; i++)
+    if (tabs[i].key_n == kn && tabs[i].nt == nt &&
+        !memcmp(key, tabs[i].key, (size_t)kn * sizeof(int)))
+      e = i;
+  if (e < 0) {
+    /* else order the tickets into the least recently used table (its
+     * upload follows, on the stream, any launch still reading it) */
+    e = 0;
+    for (int i = 1; i < PDFS_OV_TABLES; i++)
+      if (tabs[i].used < tabs[e].used) e = i;
+    ov_table *t = &tabs[e];
+    t->key_n = 0;
+    built = 1;
+    if (kn > t->key_cap) {
+      free(t->key);
+      t->key_cap = 0;
+      if (!(t->key = (int *)malloc((size_t)kn * sizeof(int)))) {
+        rc = fail(PDFS_ENOMEM, "overlap plan: host alloc");
+        goto done;
+      }
+      t->key_cap = kn;
+    }
+    tnx = (int *)malloc((size_t)(c->nf > 0 ? c->nf : 1) * 3 * sizeof(int));
+    out = (int *)malloc((size_t)5 * (nt > 0 ? nt : 1) * sizeof(int));
+    if (!tnx || !out) {
+      rc = fail(PDFS_ENOMEM, "overlap plan: host alloc");
       goto done;
     }
-    hipFree(c->ov_tab[seg]);
-    c->ov_tab[seg] = NULL;
-    c->ov_cap[seg] = 0;
-    if (hipMalloc((void **)&c->ov_tab[seg], (size_t)5 * nt * sizeof(int)) != hipSuccess) {
-      rc = fail(PDFS_ENOMEM, "overlap plan: device alloc");
+    for (int f = 0; f < c->nf; f++) {
+      tnx[f] = c->h_fields[f].nTx;
+      tnx[c->nf + f] = c->h_fields[f].nTy;
+      tnx[2 * c->nf + f] = c->h_fields[f].Z;
+    }
+    const int ne = pdfsi_ov_order(c->nf, tnx, tnx + c->nf, tnx + 2 * c->nf, nc, c->h_cols, TX,
+                                  TY, sn0, sn1, grid, out);
+    if (ne != nt) {
+      rc = fail(PDFS_EINVAL, "overlap plan: %d of %d tickets ordered", ne, nt);
       goto done;
     }
-    c->ov_cap[seg] = nt;
+    if (nt > t->cap) {
+      if (hipStreamSynchronize(c->st) != hipSuccess) {
+        rc = fail(PDFS_EHIP, "overlap plan: synchronize");
+        goto done;
+      }
+      hipFree(t->tab);
+      t->tab = NULL;
+      t->cap = 0;
+      if (hipMalloc((void **)&t->tab, (size_t)5 * nt * sizeof(int)) != hipSuccess) {
+        rc = fail(PDFS_ENOMEM, "overlap plan: device alloc");
+        goto done;
+      }
+      t->cap = nt;
+    }
+    if (hipMemcpyAsync(t->tab, out, (size_t)5 * nt * sizeof(int), hipMemcpyHostToDevice,
+                       c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess) {
+      rc = fail(PDFS_EHIP, "overlap plan: upload");
+      goto done;
+    }
+    t->nt = nt;
+    memcpy(t->key, key, (size_t)kn * sizeof(int));
+    t->key_n = kn;
   }
-  if (hipMemcpyAsync(c->ov_tab[seg], out, (size_t)5 * nt * sizeof(int), hipMemcpyHostToDevice,
-                     c->st) != hipSuccess ||
-      hipStreamSynchronize(c->st) != hipSuccess) {
-    rc = fail(PDFS_EHIP, "overlap plan: upload");
-    goto done;
-  }
-  c->ov_nt[seg] = nt;
-  c->ov_valid[seg] = 1;
-  memcpy(c->ov_key[seg], key, (size_t)kn * sizeof(int));
-  c->ov_key_n[seg] = kn;
-  /* a captured launch of this segment holds the old table's size */
-  if (c->gexec[seg]) {
+  tabs[e].used = ++c->ov_clock;
+  /* a captured launch of this segment holds another table's pointer and size */
+  if ((built || e != c->ov_cur[seg]) && c->gexec[seg]) {
     hipGraphExecDestroy(c->gexec[seg]);
     c->gexec[seg] = NULL;
   }
+  c->ov_cur[seg] = e;
+  c->ov_valid[seg] = 1;
 done:
   free(key);
   free(tnx);
@@ -799,9 +826,8 @@ static int plan_build_ex(pdfs_ctx *c, int nf, void *const *fields, const float *
     ncols += (int)nc;
   }
   /* overlapped passes need every field to tile exactly (each ordering then
-   * tiles the same physical columns) and a second edge region; the FP64
-   * polish (FP64 words with an init stream) never overlaps */
-  int ov = words == 1 || !inits;
+   * tiles the same physical columns) and a second edge region */
+  int ov = 1;
   for (int f = 0; f < nf && ov; f++)
     ov = dims[3 * f] % TX == 0 && dims[3 * f + 1] % TY == 0;
   if (ov && (2 * (ex + ey) + 4 * PDFSK_EDGE_PAD) * 8 >= (1ull << 31)) ov = 0;
@@ -935,10 +961,7 @@ static int ov_build(pdfs_ctx *c, int seg, int sn0, int sn1, int grid);
  * overlapped launches in `mode` (automatic: ov_auto). */
 static int ov_takes(pdfs_ctx *c, int mode, int grid)
 {
-  return c->ov_use && c->ov_ok && grid > 0 &&
-         (mode == PDFSK_MODE_F32_ISO || mode == PDFSK_MODE_F32_ANISO ||
-          mode == PDFSK_MODE_FX_ISO || mode == PDFSK_MODE_F64_REF ||
-          mode == PDFSK_MODE_F32_SPEED) &&
+  return c->ov_use && c->ov_ok && grid > 0 && mode >= 0 && mode < PDFSK_MODE_COUNT &&
          (c->ov_use == PDFS_OVERLAP_ALWAYS || ov_auto(c, grid));
 }
 
@@ -954,6 +977,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
   if (ov) {
     int rc;
     if (!c->ov_valid[gi] && (rc = ov_build(c, gi, sn0, sn1, grid))) return rc;
+    const ov_table *tb = &c->ov_cache[gi][c->ov_cur[gi]];
     if (graph && !profile) {
       const int key = 1000 + mode;
       if (!c->gexec[gi] || c->g_tile[gi] != c->tile || c->g_mode[gi] != key ||
@@ -964,7 +988,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
         }
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        if (pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
+        if (pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, tb->tab, tb->nt, c->ov_ncolp,
                                  c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry,
                                  mode, sp, sp64, c->tile, grid, c->st)) {
           hipStreamEndCapture(c->st, &g);
@@ -983,7 +1007,7 @@ static int run_orderings(pdfs_ctx *c, int mode, const pdfs_sp32 *sp, const pdfs_
       return PDFS_OK;
     }
     if (profile) HIPCHK(hipEventRecord(c->kev[0], c->st));
-    KCHK(pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, c->ov_tab[gi], c->ov_nt[gi], c->ov_ncolp,
+    KCHK(pdfsk_sweep_batch_ov(&c->bt, sn0, sn1, tb->tab, tb->nt, c->ov_ncolp,
                               c->ov_done, c->bt.state + 16, c->ov_chg, c->ov_rx, c->ov_ry, mode,
                               sp, sp64, c->tile, grid, c->st));
     c->launches++;
@@ -1129,10 +1153,9 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
   memset(&s, 0, sizeof s);
   int rc = plan_build_ex(c, nf, (void *const *)fields, inits, slows, dims, tile, 0, 2);
   if (rc) return rc;
-  /* overlapped passes for the reference arithmetic; a stop on the last
-   * ordering's change (the polish) needs that ordering's own maximum, which
-   * an overlapped launch does not keep */
-  c->ov_use = mode == PDFSK_MODE_F64_REF && !stop_on_last ? overlap : 0;
+  /* overlapped passes (whole passes end at ordering 8, so the polish's stop
+   * on the last ordering's change reads that slot's own maximum) */
+  c->ov_use = overlap;
   const long long launches0 = c->launches;
   long long *delta = (long long *)calloc((size_t)nf, sizeof(long long));
   double *md = (double *)calloc((size_t)2 * nf, sizeof(double));
@@ -1248,7 +1271,8 @@ static int pbuf_layout(pdfs_ctx *c, int n, const long long *npts, size_t *off)
 /* FP64 polish of one group of fields that fits one FP64 plan. */
 static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *const *inits,
                         const float *const *slows, const int *dims, const pdfs_sp64 *sp64,
-                        double tol, int maxp, int tile, int graph, int profile, pdfs_stats *ps)
+                        double tol, int maxp, int tile, int graph, int profile, int overlap,
+                        pdfs_stats *ps)
 {
   long long *np = (long long *)malloc((size_t)k * sizeof(long long));
   size_t *off = (size_t *)malloc((size_t)k * sizeof(size_t));
@@ -1292,7 +1316,7 @@ static int polish_group(pdfs_ctx *c, int k, float *const *fields, const float *c
     const int mode = slows ? PDFSK_MODE_F64_POL_SPEED
                            : sp64->iso ? PDFSK_MODE_F64_POL_ISO : PDFSK_MODE_F64_POL_ANISO;
     rc = sweep64_batch(c, mode, k, f64, inits, slows, dims, sp64, tile, 0, tol, 1, maxp, graph,
-                       profile, PDFS_OVERLAP_OFF, ps, NULL, NULL);
+                       profile, overlap, ps, NULL, NULL);
     if (rc && rc != PDFS_ENOTCONV) goto out;
     const int rc0 = rc;
     rc = PDFS_OK;
@@ -1353,7 +1377,7 @@ int pdfsi_polish(pdfs_ctx *c, int nf, float *const *fields, const float *const *
       pdfs_stats ps;
       memset(&ps, 0, sizeof ps);
       const int r = polish_group(c, k, gf, gi, slows ? gs : NULL, gd, &sp64, tol, maxp, tile,
-                                 o->graph, o->profile, &ps);
+                                 o->graph, o->profile, o->overlap, &ps);
       if (ps.passes > s->polish_passes) s->polish_passes = ps.passes;
       s->polish_orderings += ps.orderings;
       s->polish_point_updates += ps.point_updates;

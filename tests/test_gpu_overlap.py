@@ -232,3 +232,31 @@ def test_overlap_degenerate_shapes(ctx, dev, shape, tile):
     want, p_or, _, _ = O.solve_fixed(start)
     assert sa["passes"] == p_or
     assert np.array_equal(a.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (1.0, 0.8, 1.25)])
+def test_overlap_fp64_polish(ctx, dev, h):
+    """FP32 storage then the FP64 polish (modes 3/4), both overlapped: the
+    polish stops on its last ordering's largest change, which the overlapped
+    launch keeps apart. Same bits and passes as one launch per ordering, and
+    (isotropic) as the oracle's FP32 solve and polish."""
+    X, Y, Z = 64, 48, 40
+    padded = O.gen_spheres(X, Y, Z, seed=77, n=5, r=(2.0, 6.0)).astype(np.float32).astype(np.float64)
+    start = np.ascontiguousarray(O.interior(padded)).astype(np.float32)
+    res = {}
+    for ov in (ALWAYS, OFF):
+        t = _dev(start, dev)
+        st = ctx.solve_f32(t, _dev(start, dev), h=h, judge=0, storage=pdfs_amd.PDFS_STORE_F32,
+                           tile=pdfs_amd.TILE_16x16, overlap=ov)
+        res[ov] = (t, st)
+    a, b = res[ALWAYS], res[OFF]
+    print(f"polish h={h}: passes {a[1]['passes']} / {b[1]['passes']}, polish passes "
+          f"{a[1]['polish_passes']} / {b[1]['polish_passes']}", flush=True)
+    assert a[1]["polish_passes"] >= 1
+    assert a[1]["passes"] == b[1]["passes"] and a[1]["polish_passes"] == b[1]["polish_passes"]
+    assert a[1]["sweep_launches"] < b[1]["sweep_launches"]
+    assert torch.equal(a[0].view(torch.int32), b[0].view(torch.int32))
+    if h == (1.0, 1.0, 1.0):
+        want, p32, ppol = O.solve_f32_polished(padded)
+        assert ppol == a[1]["polish_passes"]
+        assert np.array_equal(a[0].cpu().numpy().view(np.uint32), O.interior(want).view(np.uint32))

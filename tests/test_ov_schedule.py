@@ -1,5 +1,6 @@
 """CPU checks of the overlapped-pass ticket order (pdfsi_ov_order, the host
-function that orders the tickets of k_sweep_col<..., true>).
+function that orders the tickets of k_sweep_col<..., true>), and of the
+argument checks of its launcher.
 
 The kernel's waits are restated here from its source: a ticket (slot o,
 field f, virtual tile a, b, physical column p) waits for the edge entries of
@@ -135,3 +136,41 @@ def test_order_follows_the_schedule():
     assert first[1] < last[0]
     for o in range(2, 8):
         assert first[o] > last[o - 2]
+
+
+class _Batch(C.Structure):
+    # pdfsk_batch (pdfs_amd/csrc/pdfs_kernels.h)
+    _fields_ = [("fields", C.c_void_p), ("cols", C.c_void_p), ("state", C.c_void_p),
+                ("changed", C.c_void_p), ("streak", C.c_void_p), ("nfields", C.c_int),
+                ("maxdiff", C.c_void_p), ("maxdiff_last", C.c_void_p), ("err", C.c_void_p),
+                ("edge_x", C.c_void_p), ("edge_y", C.c_void_p), ("edge_x_bytes", C.c_size_t),
+                ("edge_y_bytes", C.c_size_t)]
+
+
+def test_overlapped_launcher_rejects_bad_arguments():
+    """pdfsk_sweep_batch_ov refuses, before any HIP call, orderings outside
+    1..8, an empty or missing ticket table, missing flag or counter arrays,
+    a tile the mode cannot run, and an FP64 or fixed-point mode without its
+    change maxima. (Accepted arguments are not launched here: no GPU.)"""
+    L = pdfs_amd.lib()
+    f = L.pdfsk_sweep_batch_ov
+    vp = C.c_void_p
+    f.argtypes = [C.POINTER(_Batch), C.c_int, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp,
+                  C.c_size_t, C.c_size_t, C.c_int, vp, vp, C.c_int, C.c_int, vp]
+    f.restype = C.c_int
+    b = _Batch()
+    b.nfields = 1
+    b.maxdiff = 4096
+    dummy = 8192
+
+    def call(sn0=1, sn1=8, otab=dummy, nt=8, done=dummy, cnt=dummy, chg=dummy, mode=0,
+             tile=pdfs_amd.TILE_32x16, grid=1):
+        return f(C.byref(b), sn0, sn1, otab, nt, 1, done, cnt, chg, 0, 0, mode, None, None,
+                 tile, grid, None)
+
+    assert call(sn0=0) == -1 and call(sn1=9) == -1 and call(sn0=5, sn1=4) == -1
+    assert call(otab=None) == -1 and call(nt=0) == -1 and call(grid=0) == -1
+    assert call(done=None) == -1 and call(cnt=None) == -1 and call(chg=None) == -1
+    assert call(mode=5, tile=pdfs_amd.TILE_32x16) == -1  # the speed sweep has no 32x16 variant
+    b.maxdiff = None
+    assert call(mode=7) == -1 and call(mode=2, tile=pdfs_amd.TILE_8x8) == -1

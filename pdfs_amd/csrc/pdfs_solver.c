@@ -439,22 +439,28 @@ static int plan_upload_cols(pdfs_ctx *c, const int *active)
  * topological order of those dependencies (Kahn, earliest simulated ready
  * time first; the whole slot o - 2 before any ticket of slot o), so every
  * ticket's dependencies hold earlier tickets. */
+/* Simulated times are whole column steps: integer keys, ties broken by id. */
 typedef struct {
-  double key;
+  long long key;
   int id;
 } ov_item;
 
-static void ov_push(ov_item *h, int *n, double key, int id)
+static inline int ov_less(ov_item a, ov_item b)
 {
+  return a.key < b.key || (a.key == b.key && a.id < b.id);
+}
+
+static void ov_push(ov_item *h, int *n, long long key, int id)
+{
+  const ov_item it = {key, id};
   int i = (*n)++;
   while (i > 0) {
     const int p = (i - 1) / 2;
-    if (h[p].key < key || (h[p].key == key && h[p].id < id)) break;
+    if (ov_less(h[p], it)) break;
     h[i] = h[p];
     i = p;
   }
-  h[i].key = key;
-  h[i].id = id;
+  h[i] = it;
 }
 
 static ov_item ov_pop(ov_item *h, int *n)
@@ -462,18 +468,10 @@ static ov_item ov_pop(ov_item *h, int *n)
   const ov_item top = h[0], last = h[--(*n)];
   int i = 0;
   for (;;) {
-    int l = 2 * i + 1, m = i;
-    ov_item best = last;
-    if (l < *n && (h[l].key < best.key || (h[l].key == best.key && h[l].id < best.id))) {
-      best = h[l];
-      m = l;
-    }
-    if (l + 1 < *n &&
-        (h[l + 1].key < best.key || (h[l + 1].key == best.key && h[l + 1].id < best.id))) {
-      best = h[l + 1];
-      m = l + 1;
-    }
-    if (m == i) break;
+    const int l = 2 * i + 1;
+    if (l >= *n) break;
+    const int m = l + 1 < *n && ov_less(h[l + 1], h[l]) ? l + 1 : l;
+    if (!ov_less(h[m], last)) break;
     h[i] = h[m];
     i = m;
   }
@@ -492,95 +490,85 @@ int pdfsi_ov_order(int nf, const int *tnx, const int *tny, const int *fz, int nc
                    const int *cols, int TX, int TY, int sn0, int sn1, int grid, int *out)
 {
   const int ns = sn1 - sn0 + 1, nt = ns * nc;
-  const double LAG = PDFS_TICKET_EDGE_STEPS;
+  const long long LAG = PDFS_TICKET_EDGE_STEPS;
+  if (ns < 1 || ns > 8 || nc < 0 || grid < 1) return -1;
+  /* x / y reflections of each slot's ordering */
+  int dfx[8], dfy[8];
+  for (int o = 0; o < ns; o++) ov_dirs(sn0 + o, &dfx[o], &dfy[o]);
   /* column index of (f, a, b) in the active list; physical column bases */
-  int *cbase = (int *)malloc((size_t)(nf + 1) * sizeof(int));
   int *pbase = (int *)malloc((size_t)(nf + 1) * sizeof(int));
   int *cidx = NULL, *left = (int *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(int));
-  double *ready = (double *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
-  double *slotend = (double *)calloc((size_t)ns, sizeof(double));
-  int *emitted = (int *)calloc((size_t)ns, sizeof(int));
+  long long *ready = (long long *)calloc((size_t)(nt > 0 ? nt : 1), sizeof(long long));
+  long long slotend[8] = {0};
+  int emitted[8] = {0};
   ov_item *heap = (ov_item *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(ov_item));
-  ov_item *slots = (ov_item *)malloc((size_t)(grid > 0 ? grid : 1) * sizeof(ov_item));
+  ov_item *slots = (ov_item *)malloc((size_t)grid * sizeof(ov_item));
   int ne = -1;
-  if (!cbase || !pbase || !left || !ready || !slotend || !emitted || !heap || !slots) goto done;
-  cbase[0] = pbase[0] = 0;
-  for (int f = 0; f < nf; f++) {
-    pbase[f + 1] = pbase[f] + tnx[f] * tny[f];
-    cbase[f + 1] = pbase[f + 1];
-  }
+  if (!pbase || !left || !ready || !heap || !slots) goto done;
+  pbase[0] = 0;
+  for (int f = 0; f < nf; f++) pbase[f + 1] = pbase[f] + tnx[f] * tny[f];
   cidx = (int *)malloc((size_t)(pbase[nf] > 0 ? pbase[nf] : 1) * sizeof(int));
   if (!cidx) goto done;
   for (int i = 0; i < pbase[nf]; i++) cidx[i] = -1;
   for (int i = 0; i < nc; i++) {
     const int f = cols[3 * i], a = cols[3 * i + 1], b = cols[3 * i + 2];
-    cidx[cbase[f] + a * tny[f] + b] = i;
+    cidx[pbase[f] + a * tny[f] + b] = i;
   }
   /* dependency counts: upwind in the slot, the slot before (self and
    * physical neighbours) */
-  for (int o = 0; o < ns; o++) {
-    int fx, fy, gx, gy;
-    ov_dirs(sn0 + o, &fx, &fy);
-    if (o > 0) ov_dirs(sn0 + o - 1, &gx, &gy);
+  for (int o = 0; o < ns; o++)
     for (int i = 0; i < nc; i++) {
       const int f = cols[3 * i], a = cols[3 * i + 1], b = cols[3 * i + 2];
       const int nTx = tnx[f], nTy = tny[f];
       int d = (a > 0) + (b > 0);
       if (o > 0) {
-        const int pa = fx ? nTx - 1 - a : a, pb = fy ? nTy - 1 - b : b;
+        const int pa = dfx[o] ? nTx - 1 - a : a, pb = dfy[o] ? nTy - 1 - b : b;
         d += 1 + (pa > 0) + (pa + 1 < nTx) + (pb > 0) + (pb + 1 < nTy);
       }
       left[o * nc + i] = d;
     }
-  }
   int nh = 0, nslot = 0;
-  for (int g = 0; g < grid; g++) ov_push(slots, &nslot, 0.0, g);
+  for (int g = 0; g < grid; g++) ov_push(slots, &nslot, 0, g);
   for (int i = 0; i < nc; i++)
-    if (left[i] == 0) ov_push(heap, &nh, 0.0, i);
+    if (left[i] == 0) ov_push(heap, &nh, 0, i);
   ne = 0;
   while (nh > 0) {
     const ov_item it = ov_pop(heap, &nh);
-    const int t = it.id, o = t / nc, i = t % nc;
+    const int t = it.id, o = t / nc, i = t - o * nc;
     const int f = cols[3 * i], a = cols[3 * i + 1], b = cols[3 * i + 2];
-    const int nTx = tnx[f], nTy = tny[f];
-    ov_item sl = ov_pop(slots, &nslot);
-    const double st0 = it.key > sl.key ? it.key : sl.key;
-    const double fin = st0 + (double)(fz[f] + TX + TY - 2);
+    const int nTx = tnx[f], nTy = tny[f], cb = pbase[f];
+    const ov_item sl = ov_pop(slots, &nslot);
+    const long long st0 = it.key > sl.key ? it.key : sl.key;
+    const long long fin = st0 + (fz[f] + TX + TY - 2);
     ov_push(slots, &nslot, fin, sl.id);
     if (fin > slotend[o]) slotend[o] = fin;
-    int fx, fy;
-    ov_dirs(sn0 + o, &fx, &fy);
-    const int pa = fx ? nTx - 1 - a : a, pb = fy ? nTy - 1 - b : b;
+    const int pa = dfx[o] ? nTx - 1 - a : a, pb = dfy[o] ? nTy - 1 - b : b;
     out[5 * ne] = o;
     out[5 * ne + 1] = f;
     out[5 * ne + 2] = a;
     out[5 * ne + 3] = b;
-    out[5 * ne + 4] = pbase[f] + pa * nTy + pb;
+    out[5 * ne + 4] = cb + pa * nTy + pb;
     ne++;
     emitted[o]++;
     /* dependents: downwind in the slot */
     for (int k = 0; k < 2; k++) {
       const int a2 = a + (k == 0), b2 = b + (k == 1);
       if (a2 >= nTx || b2 >= nTy) continue;
-      const int j = o * nc + cidx[cbase[f] + a2 * nTy + b2];
-      const double r = st0 + (k == 0 ? TX : TY) + LAG;
+      const int j = o * nc + cidx[cb + a2 * nTy + b2];
+      const long long r = st0 + (k == 0 ? TX : TY) + LAG;
       if (r > ready[j]) ready[j] = r;
-      if (--left[j] == 0 && (o < 2 || emitted[o - 2] == nc))
-        ov_push(heap, &nh, ready[j], j);
+      if (--left[j] == 0 && (o < 2 || emitted[o - 2] == nc)) ov_push(heap, &nh, ready[j], j);
     }
     /* dependents in the next slot: the tiles at and next to this one */
     if (o + 1 < ns) {
-      int hx, hy;
-      ov_dirs(sn0 + o + 1, &hx, &hy);
       static const int dq[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
       for (int k = 0; k < 5; k++) {
         const int qa = pa + dq[k][0], qb = pb + dq[k][1];
         if (qa < 0 || qa >= nTx || qb < 0 || qb >= nTy) continue;
-        const int va = hx ? nTx - 1 - qa : qa, vb = hy ? nTy - 1 - qb : qb;
-        const int j = (o + 1) * nc + cidx[cbase[f] + va * nTy + vb];
+        const int va = dfx[o + 1] ? nTx - 1 - qa : qa, vb = dfy[o + 1] ? nTy - 1 - qb : qb;
+        const int j = (o + 1) * nc + cidx[cb + va * nTy + vb];
         if (fin > ready[j]) ready[j] = fin;
-        if (--left[j] == 0 && (o + 1 < 2 || emitted[o - 1] == nc))
-          ov_push(heap, &nh, ready[j], j);
+        if (--left[j] == 0 && (o + 1 < 2 || emitted[o - 1] == nc)) ov_push(heap, &nh, ready[j], j);
       }
     }
     /* slot o complete: the slot two later may start */
@@ -591,13 +579,10 @@ int pdfsi_ov_order(int nf, const int *tnx, const int *tny, const int *fz, int nc
       }
   }
 done:
-  free(cbase);
   free(pbase);
   free(cidx);
   free(left);
   free(ready);
-  free(slotend);
-  free(emitted);
   free(heap);
   free(slots);
   return ne;

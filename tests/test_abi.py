@@ -132,3 +132,44 @@ def test_reference_host_program_links_run_fsm_from_libpdfs():
     line = next(l for l in ldd.splitlines() if "libpdfs.so" in l)
     assert "not found" not in line and os.path.samefile(line.split("=>")[1].split()[0],
                                                         pdfs_amd.LIB_PATH)
+
+
+def test_opts_stats_layouts_and_enums_match_the_header(tmp_path):
+    """The ctypes mirrors of pdfs_opts and pdfs_stats have the header's size
+    and field offsets, and the enum values the module exports equal the
+    header's (a C program built against include/pdfs.h prints them)."""
+    import ctypes as C
+    import re
+    import subprocess
+
+    opts = pdfs_amd.Opts
+    stats = pdfs_amd.Stats
+    header = open(os.path.join(ROOT, "include", "pdfs.h")).read()
+    enums = sorted(set(re.findall(r"\b(PDFS_[A-Z0-9_]+)\s*=", header)))
+    enums = [e for e in enums if hasattr(pdfs_amd, e)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pdfs.h"', "int main(void) {"]
+    lines.append('  printf("opts %zu\\n", sizeof(pdfs_opts));')
+    for name, _ in opts._fields_:
+        lines.append(f'  printf("opts.{name} %zu\\n", offsetof(pdfs_opts, {name}));')
+    lines.append('  printf("stats %zu\\n", sizeof(pdfs_stats));')
+    for name, _ in stats._fields_:
+        lines.append(f'  printf("stats.{name} %zu\\n", offsetof(pdfs_stats, {name}));')
+    for e in enums:
+        lines.append(f'  printf("{e} %d\\n", (int){e});')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    assert int(got["opts"]) == C.sizeof(opts)
+    assert int(got["stats"]) == C.sizeof(stats)
+    for name, _ in opts._fields_:
+        assert int(got[f"opts.{name}"]) == getattr(opts, name).offset, name
+    for name, _ in stats._fields_:
+        assert int(got[f"stats.{name}"]) == getattr(stats, name).offset, name
+    assert "PDFS_OVERLAP_AUTO" in enums and "PDFS_OV_AUTO_FIELDS" in enums
+    for e in enums:
+        assert int(got[e]) == getattr(pdfs_amd, e), e

@@ -145,8 +145,14 @@ def test_opts_stats_layouts_and_enums_match_the_header(tmp_path):
     opts = pdfs_amd.Opts
     stats = pdfs_amd.Stats
     header = open(os.path.join(ROOT, "include", "pdfs.h")).read()
-    enums = sorted(set(re.findall(r"\b(PDFS_[A-Z0-9_]+)\s*=", header)))
-    enums = [e for e in enums if hasattr(pdfs_amd, e)]
+    # the module keeps each name, or drops its PDFS_ prefix (GEN_*, TILE_AUTO)
+    pyname = {}
+    for e in sorted(set(re.findall(r"\b(PDFS_[A-Z0-9_]+)\s*=", header))):
+        for n in (e, e[len("PDFS_"):]):
+            if hasattr(pdfs_amd, n):
+                pyname[e] = n
+                break
+    enums = sorted(pyname)
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pdfs.h"', "int main(void) {"]
     lines.append('  printf("opts %zu\\n", sizeof(pdfs_opts));')
     for name, _ in opts._fields_:
@@ -171,5 +177,6 @@ def test_opts_stats_layouts_and_enums_match_the_header(tmp_path):
     for name, _ in stats._fields_:
         assert int(got[f"stats.{name}"]) == getattr(stats, name).offset, name
     assert "PDFS_OVERLAP_AUTO" in enums and "PDFS_OV_AUTO_FIELDS" in enums
+    assert "PDFS_TILE_AUTO" in enums and "PDFS_GEN_SPHERES" in enums
     for e in enums:
-        assert int(got[e]) == getattr(pdfs_amd, e), e
+        assert int(got[e]) == getattr(pdfs_amd, pyname[e]), e

@@ -608,6 +608,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
   __shared__ float iring[NA ? NA * R * NT : 1];
   __shared__ int s_col;
   __shared__ unsigned s_changed;
+  __shared__ unsigned s_skip;
   T *const ring = lds;
   uint32_t *const lds32 = (uint32_t *)lds;
 
@@ -637,6 +638,7 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
     if (tid == 0) {
       s_col = (int)(atomicAdd(a.state + ST_TICKET, 1u) - tbase);
       s_changed = 0;
+      s_skip = 0;
     }
     __syncthreads();
     // the ticket and everything derived from it are wave-uniform: keep them
@@ -719,15 +721,26 @@ __global__ __launch_bounds__(TX *TY + 64) void k_sweep_col(ColArgs a) {
         else if (e == 3 && pb > 0) q = colp - 1;
         else if (e == 4 && pb + 1 < nTy) q = colp + 1;
         if (q >= 0) wait_at_least(prev + q, ep - 1u, a.err);
-        // wave 0 of every column of slot o - 2 counted 64
-        if (e == 5 && oslot >= 2)
+        // wave 0 of every column of slot o - 2 counted 64; that slot's
+        // change counts are then final (each column adds to them before its
+        // release). A field that slot changed nothing in is at its fixed
+        // point: its columns skip this slot (every one of them reads the
+        // same count, so no column waits on a skipped one's edges).
+        if (e == 5 && oslot >= 2) {
           wait_at_least(a.ov_cnt + (oslot - 2), 64u * (ncols / (unsigned)a.ov_n), a.err);
+          s_skip = __hip_atomic_load(a.ov_chg + (oslot - 2) * a.nfields + fi, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        }
       }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    const bool skip_col = OV && oslot >= 2 && __builtin_amdgcn_readfirstlane(s_skip) != 0u;
 
-    if (edge_wave) {
+    if (skip_col) {
+      // at the fixed point: nothing to sweep; the tail still reports the
+      // column done
+    } else if (edge_wave) {
       // ---- edge wave: value lane v < TX serves column c = v of the south /
       // north faces, value lane TX + r row r of the west / east faces; lane
       // e = v * W + w moves word w of its value; other lanes idle

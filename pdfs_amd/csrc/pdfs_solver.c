@@ -1176,7 +1176,7 @@ static int sweep64_batch(pdfs_ctx *c, int mode, int nf, double *const *fields,
       const long long nfp = (long long)dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
       const unsigned run = c->h_streak[2 * f + 1];
       long long skipped = run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
-      if (c->last_ov) skipped = 0; /* an overlapped launch skips nothing */
+      if (c->last_ov) skipped = run >= 2 ? (run < 8u ? run : 8) - 2 : 0;
       if (skipped > 8) skipped = 8;
       s.point_updates -= skipped * nfp;
       fp[f] = p + 1;
@@ -1880,7 +1880,9 @@ static int batch_f32(pdfs_ctx *c, int nf, float *const *fields, const float *con
           const unsigned run = c->h_streak[2 * f + 1];
           long long skipped =
               run > PDFSK_CONVERGED_RUN ? (long long)(run - PDFSK_CONVERGED_RUN) : 0;
-          if (c->last_ov) skipped = 0; /* an overlapped launch skips nothing */
+          /* an overlapped launch skips the slots from two after its first
+           * unchanged one */
+          if (c->last_ov) skipped = run >= 2 ? (run < (unsigned)nord ? run : nord) - 2 : 0;
           if (skipped > nord) skipped = nord;
           s.point_updates -= skipped * dims[3 * f] * dims[3 * f + 1] * dims[3 * f + 2];
           if (nf == 1) s.orderings -= (int)skipped;

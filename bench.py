@@ -429,7 +429,15 @@ def solve_modes(ctx, net, init, field, tile, plain_ms, u64=None):
     S32 = pdfs_amd.PDFS_STORE_F32
     out["plain"] = err({"passes": sp["passes"], "orderings": sp["orderings"],
                         "ms": round(plain_ms, 3), "ms_convert": round(sp["ms_convert"], 3),
+                        "sweep_launches": sp["sweep_launches"],
                         "judge_flagged": int(sp["judge_flagged"])})
+    # the same solve with one launch per ordering (no overlapped passes)
+    oms, so = timed(lambda: ctx.solve_f32(field, init, tile=tile,
+                                          overlap=pdfs_amd.PDFS_OVERLAP_OFF))
+    out["plain_one_launch_per_ordering"] = err(
+        {"passes": so["passes"], "ms": round(oms, 3), "sweep_launches": so["sweep_launches"],
+         "judge_flagged": int(so["judge_flagged"]),
+         "overlapped_speedup": round(oms / plain_ms, 3)})
     pms, sq = timed(lambda: ctx.solve_f32(field, init, tile=tile, storage=S32))
     out["plain_fp32_storage_polished"] = err(
         {"passes": sq["passes"], "orderings": sq["orderings"], "ms": round(pms, 3),

@@ -133,7 +133,8 @@ typedef struct {
                       active fields whose longest critical path through an ordering is at
                       least PDFS_OV_AUTO_DEPTH / 10 of the ordering's column steps per
                       resident workgroup (one 512^3 field: 1.7); ALWAYS wherever possible;
-                      else one launch per ordering. Results are the same bits. */
+                      else one launch per ordering. Results are the same bits. Values
+                      other than OFF and ALWAYS act as AUTO. */
 } pdfs_opts;
 
 typedef struct {

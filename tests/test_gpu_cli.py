@@ -53,8 +53,12 @@ def test_prepost_kernels_bitwise(op):
 
 
 @pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.75, 1.25)])
-def test_calc_dist_field_matches_oracle(h):
-    loc, dist = random_cli_input(13, 11, 9, 21)
+@pytest.mark.parametrize("shape", [(13, 11, 9), (24, 16, 10)])
+def test_calc_dist_field_matches_oracle(h, shape):
+    """The CLI pipeline on the GPU against the oracle pipeline, bit for bit.
+    24 x 16 tiles the FP64 path's 8 x 8 columns exactly, so its pass runs as
+    one overlapped launch."""
+    loc, dist = random_cli_input(*shape, 21)
     L, D = _padded(loc, dist)
     want = oracle_pipeline(L, D, 1, h)
     got = D.copy()
@@ -85,20 +89,21 @@ REF_HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__
 
 @pytest.mark.skipif(not os.path.exists(REF_HOST), reason="reference host program not built")
 @pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.75, 1.25)])
-def test_reference_host_program_linked_to_libpdfs(tmp_path, h):
+@pytest.mark.parametrize("shape", [(12, 9, 7), (16, 16, 8)])
+def test_reference_host_program_linked_to_libpdfs(tmp_path, h, shape):
     """The drop-in as the reference builds it: single_gpu_cuda/src/{main,
     phi3D,vti_parser,file_writer}.c compiled unmodified (oracle/Makefile
     `dropin`) and linked against lib/libpdfs.so in place of the nvcc object
     (single_gpu_cuda/Makefile:19-29). Its run_fsm call (phi3D.c:119) goes
     through this build's C-ABI: the output file equals bin/PDFS's and the
     oracle pipeline's byte for byte, and stdout has the reference's lines."""
-    loc, dist = random_cli_input(12, 9, 7, 47)
+    loc, dist = random_cli_input(*shape, 47)
     inp = str(tmp_path / "field.vti")
     write_input_vti(inp, loc, dist, h)
     r = subprocess.run([REF_HOST, inp, str(tmp_path / "ref")], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "Parallel FSM time:" in r.stdout and "x: 12\tdx:" in r.stdout
+    assert "Parallel FSM time:" in r.stdout and f"x: {shape[0]}\tdx:" in r.stdout
     q = subprocess.run([pdfs_amd.CLI_PATH, inp, str(tmp_path / "ours")], capture_output=True,
                        text=True, timeout=120)
     assert q.returncode == 0, q.stdout + q.stderr

@@ -1571,38 +1571,45 @@ __device__ __forceinline__ uint32_t fx_key(float v) {
 __device__ __forceinline__ uint32_t fx_key_valid(float v) {
   return v == v ? fx_key(v) : 0xFFFFFFFFu;
 }
-__global__ void k_fx_min_batch(const pdfsk_fx *__restrict__ tab) {
+__device__ __forceinline__ uint32_t fx_key_min4(float4 a) {
+  return min(min(fx_key_valid(a.x), fx_key_valid(a.y)), min(fx_key_valid(a.z), fx_key_valid(a.w)));
+}
+// Four 16-byte loads in flight per lane; the wave minima meet in LDS, so one
+// atomic per workgroup reaches the field's key.
+__global__ __launch_bounds__(256) void k_fx_min_batch(const pdfsk_fx *__restrict__ tab) {
+  __shared__ uint32_t s_min[4];
   const pdfsk_fx t = tab[blockIdx.y];
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  uint32_t k = 0xFFFFFFFFu, k2 = 0xFFFFFFFFu;
+  uint32_t k0 = 0xFFFFFFFFu, k1 = 0xFFFFFFFFu, k2 = 0xFFFFFFFFu, k3 = 0xFFFFFFFFu;
   long long head = 0;
   if (((uintptr_t)t.f & 15u) == 0) {
     const float4 *f4 = (const float4 *)t.f;
     const long long n4 = t.n >> 2;
     long long p = tid;
-    // two loads in flight per lane
-    for (; p + stride < n4; p += 2 * stride) {
-      const float4 a = f4[p], b = f4[p + stride];
-      k = min(k, min(min(fx_key_valid(a.x), fx_key_valid(a.y)),
-                     min(fx_key_valid(a.z), fx_key_valid(a.w))));
-      k2 = min(k2, min(min(fx_key_valid(b.x), fx_key_valid(b.y)),
-                       min(fx_key_valid(b.z), fx_key_valid(b.w))));
+    for (; p + 3 * stride < n4; p += 4 * stride) {
+      const float4 a = f4[p], b = f4[p + stride], c = f4[p + 2 * stride],
+                   d = f4[p + 3 * stride];
+      k0 = min(k0, fx_key_min4(a));
+      k1 = min(k1, fx_key_min4(b));
+      k2 = min(k2, fx_key_min4(c));
+      k3 = min(k3, fx_key_min4(d));
     }
-    if (p < n4) {
-      const float4 a = f4[p];
-      k = min(k, min(min(fx_key_valid(a.x), fx_key_valid(a.y)),
-                     min(fx_key_valid(a.z), fx_key_valid(a.w))));
-    }
+    for (; p < n4; p += stride) k0 = min(k0, fx_key_min4(f4[p]));
     head = n4 << 2;
   }
-  for (long long p = head + tid; p < t.n; p += stride) k = min(k, fx_key_valid(t.f[p]));
-  k = min(k, k2);
+  for (long long p = head + tid; p < t.n; p += stride) k0 = min(k0, fx_key_valid(t.f[p]));
+  uint32_t k = min(min(k0, k1), min(k2, k3));
   for (int o = 32; o > 0; o >>= 1) {
     const uint32_t o2 = (uint32_t)__shfl_xor((int)k, o);
     k = o2 < k ? o2 : k;
   }
-  if ((threadIdx.x & 63) == 0 && k != 0xFFFFFFFFu) atomicMin(t.key, k);
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    k = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+    if (k != 0xFFFFFFFFu) atomicMin(t.key, k);
+  }
 }
 // q = rint((v - base) / d) below UINT32_MAX, else UINT32_MAX (and for NaN,
 // +inf); contraction off so that the oracle's C gives the same bits
